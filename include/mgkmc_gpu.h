@@ -1,0 +1,173 @@
+/*
+ * mgkmc_gpu.h -- C ABI of the MI355X (gfx950) kinetic Monte Carlo + FFT
+ * Lippmann-Schwinger hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (MGK_OK == 0); mgk_last_error() gives the message of the last failure on
+ * the calling thread.  A context (mgk_ctx) owns one HIP stream and all
+ * device memory of one grid/trajectory; calls on one context are not
+ * thread-safe, distinct contexts may be used from distinct threads.
+ *
+ * Tensor layout at this boundary is the reference's own: an array of N
+ * column-major D x D matrices (Eigen::Matrix2d / Matrix3d storage), i.e.
+ * 4 doubles per voxel in 2D ([xx, yx, xy, yy]) and 9 in 3D
+ * ([xx, yx, zx, xy, yy, zy, xz, yz, zz]).  Voxel order is the reference's:
+ * idx = x*ny + y (2D) and idx = x*ny*nz + y*nz + z (3D)
+ * (linear_elastic_solver.hpp:81, :357).  So
+ * reinterpret_cast<const double*>(std::vector<Eigen::Matrix2d>::data())
+ * can be passed directly.
+ */
+#ifndef MGKMC_GPU_H
+#define MGKMC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGK_ABI_VERSION 1
+
+enum mgk_status {
+    MGK_OK = 0,
+    MGK_E_ARG = 1,          /* invalid argument (null pointer, bad size, ...) */
+    MGK_E_HIP = 2,          /* a HIP runtime call failed */
+    MGK_E_UNSUPPORTED = 3,  /* valid for the reference, not built here (e.g. FFT length with a prime factor > 5) */
+    MGK_E_STATE = 4         /* call order violated (e.g. KMC call before mgk_kmc_init) */
+};
+
+enum mgk_plane_mode { MGK_PLANE_STRAIN = 0, MGK_PLANE_STRESS = 1 };
+
+/* Arithmetic of the Lippmann-Schwinger fixed point.
+ * MGK_FP64: the reference's basic scheme, every iteration in FP64.
+ * MGK_FP32_REFINE: FP32 inner iterations on the correction, FP64 residual
+ * and convergence test (iterative refinement). */
+enum mgk_precision { MGK_FP64 = 0, MGK_FP32_REFINE = 1 };
+
+typedef struct mgk_ctx mgk_ctx;
+
+/* Device-resident fields, for mgk_set_field / mgk_get_field.
+ * Sizes (doubles): EPS, SIG, EPS_PLASTIC, PREV_DIR: N*D*D (AoS col-major);
+ * Q, Q0: N*M; CATALOG: N*M*D*D; SOFT: N*2 ([g_p, g_t] per voxel,
+ * kmc_solver.hpp:139); LAST_EVENT_TIME: N; E, NU: N (read-only). */
+enum mgk_field {
+    MGK_FIELD_EPS = 0,
+    MGK_FIELD_SIG = 1,
+    MGK_FIELD_EPS_PLASTIC = 2,
+    MGK_FIELD_Q = 3,
+    MGK_FIELD_Q0 = 4,
+    MGK_FIELD_CATALOG = 5,
+    MGK_FIELD_SOFT = 6,
+    MGK_FIELD_LAST_EVENT_TIME = 7,
+    MGK_FIELD_PREV_DIR = 8,
+    MGK_FIELD_E = 9,
+    MGK_FIELD_NU = 10
+};
+
+/* Per-trajectory KMC physics on the device (the members of
+ * KmcSimulation2D/3D that update_barriers, the rate loop and
+ * apply_flip_soa_* read: kmc_solver.hpp:100-131, :1077-1108). */
+typedef struct mgk_kmc_params {
+    int    M;                            /* modes per voxel */
+    int    softening_directional;        /* softening_scheme == "directional" */
+    double softening_cap;
+    double jp;
+    double jt;
+    double neighbor_softening_fraction;
+    double tau;                          /* softening decay time (s); +inf when T == 0 (kmc_solver.hpp:207-214) */
+    double volume;                       /* pixel^3, also in 2D (kmc_solver.hpp:182) */
+    double temperature;                  /* K; <= 0 gives beta = +inf (kmc_solver.hpp:786) */
+    double nu0;                          /* attempt frequency (1/s) */
+    double stability_threshold;          /* eV */
+    int    instability_kmc;              /* instability_mode == "kmc": every mode is rated */
+    int    scale_rate_by_volume;
+} mgk_kmc_params;
+
+const char* mgk_last_error(void);
+int mgk_abi_version(void);
+
+/* Context lifecycle.  dims = 2 or 3; nz is ignored (taken as 1) for dims = 2.
+ * E and nu are N doubles in Pa and 1.  Replaces the per-call E/nu arguments
+ * of spectral_solver_2d/3d (linear_elastic_solver.hpp:146-160, :431-444) and
+ * the E_field/nu_field members of KmcSimulation2D/3D (kmc_solver.hpp:134-135). */
+int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode,
+               int precision, const double* E, const double* nu, mgk_ctx** out);
+int mgk_destroy(mgk_ctx* ctx);
+int mgk_set_precision(mgk_ctx* ctx, int precision);
+
+/* spectral_solver_2d / spectral_solver_3d
+ * (linear_elastic_solver.hpp:146-309, :431-593), with host arrays.
+ *   eps_bar      D*D  imposed macro strain
+ *   eps          N*D*D in/out; warm_start != 0 means "eps.size() == N" in the
+ *                reference (mean shifted to eps_bar), else it is filled with eps_bar
+ *   eps_plastic  N*D*D or NULL (empty vector in the reference)
+ *   sig_out      N*D*D or NULL
+ *   eps_macro, sig_macro  D*D outputs (may alias eps_bar)
+ *   iters_out    iterations run (== max_iter when not converged; the
+ *                reference continues silently, so this is not an error)
+ *   diff_out     last relative increment
+ * The Green operator argument of the reference is not passed: it is
+ * evaluated on the fly from the same mean medium (mean E, mean nu) that the
+ * reference's callers build it from (kmc_solver.hpp:259-274, main.cpp:555). */
+int mgk_ls_solve(mgk_ctx* ctx, const double* eps_bar, double* eps, int warm_start,
+                 const double* eps_plastic, double* sig_out,
+                 double* eps_macro, double* sig_macro,
+                 int max_iter, double tol, int* iters_out, double* diff_out);
+
+/* Device-resident fields. */
+int mgk_set_field(mgk_ctx* ctx, int which, const double* host);
+int mgk_get_field(mgk_ctx* ctx, int which, double* host);
+
+/* KMC state on the device: allocates Q, Q0, catalog, softening state,
+ * last_event_time, prev_strain_dir, eps_plastic (all zero; last_event_time
+ * = -inf), as the constructor does (kmc_solver.hpp:196-220). */
+int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* p);
+
+/* KmcSimulation*::elastic_run (kmc_solver.hpp:334-339, :1333-1338) on the
+ * device-resident eps/sig/eps_plastic: eps_macro is the imposed strain on
+ * entry and the field mean on return (the reference aliases the two). */
+int mgk_elastic_run(mgk_ctx* ctx, double* eps_macro, double* sig_macro,
+                    int max_iter, double tol, int* iters_out, double* diff_out);
+
+/* update_barriers + find_unstable_indices count + rate loop
+ * (kmc_solver.hpp:341-410, :412-421, :783-806; 3D :1340-1411, :1796-1818).
+ * Writes Q on the device.  total_rate is the FP64 sum of the rates of the
+ * included modes; n_unstable counts Q < stability_threshold; n_included
+ * counts the rated modes. */
+int mgk_update_rates(mgk_ctx* ctx, double time, double* total_rate,
+                     int* n_unstable, int* n_included);
+
+/* Event selection (kmc_solver.hpp:816-834, :1828-1849): with
+ * target = u2 * total_rate, the first included mode, in flat order, whose
+ * sequential FP64 prefix sum of rates is >= target (the last included mode if
+ * none is).  Uses the rates of the last mgk_update_rates.  The device decides
+ * from block sums and an in-block scan; when the target is within the
+ * rounding bound of a prefix boundary it reruns the reference's exact
+ * left-to-right sum (total and prefixes), so the index is the reference's.
+ * Outputs the flat mode index and Q of that mode (is_instab = q_sel <= thr). */
+int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel);
+
+/* apply_flip_soa_2d/3d + prev_strain_dir update + redraw upload
+ * (kmc_solver.hpp:423-462, :835-864; 3D :1424-1471, :1850-1875).
+ * new_catalog (n_modes*D*D, AoS) and new_Q0 (n_modes) may be NULL; when
+ * given they overwrite modes [first_mode, first_mode + n_modes) of the
+ * flipped voxel after the flip has used the old catalog entry. */
+int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time,
+                   const double* new_catalog, const double* new_Q0,
+                   int first_mode, int n_modes);
+
+/* find_unstable_indices (kmc_solver.hpp:412-421): flat indices with
+ * Q < stability_threshold in ascending order, and their Q values.
+ * Writes at most cap entries; n_out is the full count. */
+int mgk_find_unstable(mgk_ctx* ctx, int* idx_out, double* q_out, int cap, int* n_out);
+
+/* Stream synchronisation and timing of the context's stream (HIP events). */
+int mgk_sync(mgk_ctx* ctx);
+int mgk_timer_start(mgk_ctx* ctx);
+int mgk_timer_stop(mgk_ctx* ctx, double* ms);
+
+/* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */
+int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGKMC_GPU_H */

@@ -1,0 +1,851 @@
+// api.hip -- the C ABI of include/mgkmc_gpu.h: context lifecycle, the
+// Lippmann-Schwinger solve driver (batched enqueue, device-side convergence
+// flag, one host sync per batch), device-resident KMC state and the per-event
+// primitives.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/mgkmc_gpu.h"
+#include "common.hpp"
+#include "kmc_kernels.hpp"
+#include "ls_kernels.hpp"
+
+using namespace mgk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F> int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(MGK_E_HIP, e.what);
+    } catch (const ArgError& e) {
+        return fail(MGK_E_ARG, e.what);
+    } catch (const UnsupportedError& e) {
+        return fail(MGK_E_UNSUPPORTED, e.what);
+    } catch (const std::exception& e) {
+        return fail(MGK_E_HIP, e.what());
+    }
+}
+
+template <typename T> T* dalloc(size_t n) {
+    void* p = nullptr;
+    MGK_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+template <typename T> T* halloc(size_t n) {
+    void* p = nullptr;
+    MGK_HIP_CHECK(hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    return static_cast<T*>(p);
+}
+
+// exp(-2 pi i k / n), k < n, rounded from long double
+template <typename R> std::vector<R> twiddle_table(int n) {
+    std::vector<R> t(2 * (size_t)n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n; ++k) {
+        long double a = 2.0L * pi * (long double)k / (long double)n;
+        t[2 * k] = (R)cosl(a);
+        t[2 * k + 1] = (R)(-sinl(a));
+    }
+    return t;
+}
+
+void lame(int dims, int plane_mode, double E, double nu, double& lam, double& mu) {
+    // compute_lame_2d / compute_lame_3d (linear_elastic_solver.hpp:36-45, :315-319)
+    mu = E / (2.0 * (1.0 + nu));
+    if (dims == 2 && plane_mode == MGK_PLANE_STRESS) lam = E * nu / (1.0 - nu * nu);
+    else lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+}
+
+}  // namespace
+
+struct mgk_ctx {
+    Geom g;
+    double pixel = 1.0;
+    int plane_mode = MGK_PLANE_STRAIN;
+    int precision = MGK_FP64;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int device = 0;
+
+    // material
+    double *E = nullptr, *nu = nullptr, *lam = nullptr, *mu = nullptr;
+    float *dlam32 = nullptr, *dmu32 = nullptr;
+    double lam0 = 0, mu0 = 0, mean_mu = 0;
+    std::vector<double> h_mu;
+
+    // strain state (symmetric SoA) + uniform antisymmetric part on the host
+    double *eps = nullptr, *sig = nullptr, *epsp = nullptr;
+    bool epsp_set = false;
+    double W[3] = {0, 0, 0};   // 2D: w_xy; 3D: w_xy, w_xz, w_yz
+
+    // spectra and FP32 refinement buffers
+    void* spec = nullptr;
+    float *y32 = nullptr, *corr = nullptr;
+    double* partials = nullptr;
+    Ctl* ctl = nullptr;
+    Ctl* hctl = nullptr;
+    void *tw64[3] = {nullptr, nullptr, nullptr}, *tw32[3] = {nullptr, nullptr, nullptr};
+
+    // staging for AoS <-> SoA transfers
+    double* dstage = nullptr;
+    size_t dstage_n = 0;
+    double* hstage = nullptr;
+    size_t hstage_n = 0;
+
+    // statistics
+    int last_iters = 0, last_inner = 0, predicted = 4;
+    int inner_per_outer = 6;
+
+    // KMC
+    bool kmc = false;
+    mgk_kmc_params kp{};
+    KmcParams K{};
+    KmcCtl* kctl = nullptr;
+    KmcCtl* hkctl = nullptr;
+    double* kstage = nullptr;
+    double* hkstage = nullptr;
+    int* unst_idx = nullptr;
+    double* unst_q = nullptr;
+
+    ~mgk_ctx() { release(); }
+
+    void release() {
+        auto f = [](void* p) { if (p) (void)hipFree(p); };
+        auto fh = [](void* p) { if (p) (void)hipHostFree(p); };
+        f(E); f(nu); f(lam); f(mu); f(dlam32); f(dmu32); f(eps); f(sig); f(epsp); f(spec); f(y32); f(corr);
+        f(partials); f(ctl); f(dstage);
+        for (int a = 0; a < 3; ++a) { f(tw64[a]); f(tw32[a]); }
+        fh(hctl); fh(hstage);
+        f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.rates); f(K.bsum); f(K.bincl);
+        f(K.bunst); f(K.bunst_off); f(K.binc); f(kctl); f(kstage); f(unst_idx); f(unst_q);
+        fh(hkctl); fh(hkstage);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+        E = nullptr;
+        stream = nullptr;
+        ev0 = ev1 = nullptr;
+    }
+
+    int D() const { return g.dims; }
+    int full() const { return g.dims * g.dims; }
+
+    double* stage_dev(size_t n) {
+        if (n > dstage_n) {
+            if (dstage) MGK_HIP_CHECK(hipFree(dstage));
+            dstage = dalloc<double>(n);
+            dstage_n = n;
+        }
+        return dstage;
+    }
+    double* stage_host(size_t n) {
+        if (n > hstage_n) {
+            if (hstage) MGK_HIP_CHECK(hipHostFree(hstage));
+            hstage = halloc<double>(n);
+            hstage_n = n;
+        }
+        return hstage;
+    }
+
+    LsParams ls_params() const {
+        LsParams p{};
+        p.g = g;
+        p.eps = eps;
+        p.epsp = epsp_set ? epsp : nullptr;
+        p.lam = lam;
+        p.mu = mu;
+        p.sig = sig;
+        p.lam0 = lam0;
+        p.mu0 = mu0;
+        p.spec64 = spec;
+        p.spec32 = spec;
+        p.r32 = (precision == MGK_FP32_REFINE) ? y32 : nullptr;
+        p.y32 = y32;
+        p.corr = (precision == MGK_FP32_REFINE) ? corr : nullptr;
+        p.dlam32 = dlam32;
+        p.dmu32 = dmu32;
+        for (int a = 0; a < 3; ++a) {
+            (&p.tw_f64)[a] = tw64[a];
+            (&p.tw_f32)[a] = tw32[a];
+        }
+        p.ctl = ctl;
+        p.partials = partials;
+        return p;
+    }
+
+    // Symmetric / antisymmetric split of a column-major D x D matrix.
+    void split(const double* a, double* sym, double* skew) const {
+        const int Dd = D();
+        for (int i = 0; i < Dd; ++i)
+            for (int j = i; j < Dd; ++j) {
+                const int c = comp_of(Dd, i, j);
+                if (i == j) sym[c] = a[i + Dd * i];
+                else {
+                    sym[c] = 0.5 * (a[i + Dd * j] + a[j + Dd * i]);
+                    const int w = (Dd == 2) ? 0 : (c - 3);
+                    skew[w] = 0.5 * (a[i + Dd * j] - a[j + Dd * i]);
+                }
+            }
+    }
+    void join(const double* sym, const double* skew, double* a) const {
+        const int Dd = D();
+        for (int i = 0; i < Dd; ++i)
+            for (int j = 0; j < Dd; ++j) {
+                const int c = comp_of(Dd, i, j);
+                if (i == j) a[i + Dd * j] = sym[c];
+                else {
+                    const int w = (Dd == 2) ? 0 : (comp_of(Dd, i, j) - 3);
+                    a[i + Dd * j] = sym[c] + (i < j ? skew[w] : -skew[w]);
+                }
+            }
+    }
+    int nskew() const { return D() == 2 ? 1 : 3; }
+
+    // AoS (N x D x D, column-major) host array -> symmetric SoA device field.
+    // Returns the per-voxel antisymmetric parts through skew_out (may be null).
+    void upload_tensor(const double* host, double* dev, std::vector<double>* skew_out) {
+        const long N = g.N;
+        const int C = g.C, F = full(), ns = nskew();
+        double* h = stage_host((size_t)N * C);
+        if (skew_out) skew_out->assign((size_t)N * ns, 0.0);
+        double sym[6], sk[3] = {0, 0, 0};
+        for (long v = 0; v < N; ++v) {
+            split(host + v * F, sym, sk);
+            for (int c = 0; c < C; ++c) h[(size_t)c * N + v] = sym[c];
+            if (skew_out)
+                for (int w = 0; w < ns; ++w) (*skew_out)[(size_t)v * ns + w] = sk[w];
+        }
+        MGK_HIP_CHECK(hipMemcpyAsync(dev, h, sizeof(double) * N * C, hipMemcpyHostToDevice, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    // symmetric SoA device field -> AoS host array with per-voxel skew(v) added.
+    template <typename SkewFn> void download_tensor(const double* dev, double* host, SkewFn skew_of) {
+        const long N = g.N;
+        const int C = g.C, F = full(), ns = nskew();
+        double* h = stage_host((size_t)N * C);
+        MGK_HIP_CHECK(hipMemcpyAsync(h, dev, sizeof(double) * N * C, hipMemcpyDeviceToHost, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        double sym[6], sk[3];
+        for (long v = 0; v < N; ++v) {
+            for (int c = 0; c < C; ++c) sym[c] = h[(size_t)c * N + v];
+            skew_of(v, sk);
+            (void)ns;
+            join(sym, sk, host + v * F);
+        }
+    }
+
+    // The LS solve on the device-resident fields.  eps_bar_sym / norm terms
+    // describe the imposed strain; returns iterations.
+    int solve(const double* eps_bar_sym, double norm_extra0, double diff_extra0, double norm_extra, bool warm,
+              int max_iter, double tol, double* diff_out) {
+        (void)diff_extra0;
+        Ctl& c = *hctl;
+        std::memset(&c, 0, sizeof(Ctl));
+        c.max_iter = max_iter;
+        c.tol = tol;
+        c.norm_extra = norm_extra;
+        for (int k = 0; k < g.C; ++k) c.eps_bar[k] = eps_bar_sym[k];
+        (void)norm_extra0;
+        MGK_HIP_CHECK(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        LsParams p = ls_params();
+        ls_enqueue_start(p, warm, stream);
+        int enqueued = 0;
+        if (max_iter > 0) {
+            int batch = std::min(max_iter, std::max(2, predicted));
+            while (true) {
+                for (int b = 0; b < batch; ++b) {
+                    ls_enqueue_iteration(p, LS_FULL, stream);
+                    if (precision == MGK_FP32_REFINE)
+                        for (int j = 0; j < inner_per_outer; ++j) ls_enqueue_iteration(p, LS_DELTA, stream);
+                }
+                enqueued += batch;
+                MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+                MGK_HIP_CHECK(hipStreamSynchronize(stream));
+                if (hctl->done || enqueued >= max_iter) break;
+                batch = std::min(max_iter - enqueued, std::max(2, batch / 2));
+            }
+        } else if (warm) {
+            // no iteration: the reference still applies the warm-start shift (:181-191)
+            apply_shift_now();
+        }
+        ls_enqueue_final(p, stream);
+        MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        last_iters = hctl->iters;
+        last_inner = hctl->inner;
+        predicted = std::max(2, last_iters);
+        if (diff_out) *diff_out = hctl->diff;
+        return hctl->iters;
+    }
+
+    void apply_shift_now() {
+        // eps += shift; done on the host via staging (only for max_iter == 0)
+        MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        const long N = g.N;
+        double* h = stage_host((size_t)N * g.C);
+        MGK_HIP_CHECK(hipMemcpyAsync(h, eps, sizeof(double) * N * g.C, hipMemcpyDeviceToHost, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int c = 0; c < g.C; ++c)
+            for (long v = 0; v < N; ++v) h[(size_t)c * N + v] += hctl->shift[c];
+        MGK_HIP_CHECK(hipMemcpyAsync(eps, h, sizeof(double) * N * g.C, hipMemcpyHostToDevice, stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mgk_last_error(void) { return g_last_error.c_str(); }
+int mgk_abi_version(void) { return MGK_ABI_VERSION; }
+
+int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, int precision, const double* E,
+               const double* nu, mgk_ctx** out) {
+    return guarded([&]() -> int {
+        if (!out || !E || !nu) throw ArgError("null argument");
+        *out = nullptr;
+        if (dims != 2 && dims != 3) throw ArgError("dims must be 2 or 3");
+        if (dims == 2) nz = 1;
+        if (nx < 2 || ny < 2 || nz < 1 || (dims == 3 && nz < 2)) throw ArgError("grid too small");
+        if (precision != MGK_FP64 && precision != MGK_FP32_REFINE) throw ArgError("bad precision");
+        Geom g;
+        g.dims = dims;
+        g.nf = nx;
+        g.nm = (dims == 3) ? ny : 1;
+        g.nl = (dims == 3) ? nz : ny;
+        if (g.nl % 2 != 0) throw UnsupportedError("the contiguous axis length must be even");
+        for (int n : {g.nf, g.nl})
+            if (!ls_length_supported(n)) throw UnsupportedError("FFT length " + std::to_string(n) + " not built");
+        if (dims == 3 && !ls_length_supported(g.nm)) throw UnsupportedError("FFT length " + std::to_string(g.nm) + " not built");
+        g.nlh = g.nl / 2 + 1;
+        g.N = (long)g.nf * g.nm * g.nl;
+        g.NS = (long)g.nf * g.nm * g.nlh;
+        g.C = (dims == 2) ? 3 : 6;
+
+        mgk_ctx* c = new mgk_ctx();
+        try {
+            c->g = g;
+            c->pixel = pixel;
+            c->plane_mode = plane_mode;
+            c->precision = precision;
+            MGK_HIP_CHECK(hipGetDevice(&c->device));
+            MGK_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            MGK_HIP_CHECK(hipEventCreate(&c->ev0));
+            MGK_HIP_CHECK(hipEventCreate(&c->ev1));
+            const long N = g.N;
+            // material: mean medium and per-voxel Lame constants
+            // (linear_elastic_solver.hpp:163-178, :447-462)
+            double Es = 0.0, ns = 0.0;
+            for (long i = 0; i < N; ++i) { Es += E[i]; ns += nu[i]; }
+            lame(dims, plane_mode, Es / N, ns / N, c->lam0, c->mu0);
+            std::vector<double> hl(N), hm(N);
+            std::vector<float> hdl(N), hdm(N);
+            double msum = 0.0;
+            for (long i = 0; i < N; ++i) {
+                lame(dims, plane_mode, E[i], nu[i], hl[i], hm[i]);
+                hdl[i] = (float)(hl[i] - c->lam0);
+                hdm[i] = (float)(hm[i] - c->mu0);
+                msum += hm[i];
+            }
+            c->mean_mu = msum / N;
+            c->h_mu = hm;
+            c->E = dalloc<double>(N);
+            c->nu = dalloc<double>(N);
+            c->lam = dalloc<double>(N);
+            c->mu = dalloc<double>(N);
+            c->dlam32 = dalloc<float>(N);
+            c->dmu32 = dalloc<float>(N);
+            MGK_HIP_CHECK(hipMemcpy(c->E, E, N * sizeof(double), hipMemcpyHostToDevice));
+            MGK_HIP_CHECK(hipMemcpy(c->nu, nu, N * sizeof(double), hipMemcpyHostToDevice));
+            MGK_HIP_CHECK(hipMemcpy(c->lam, hl.data(), N * sizeof(double), hipMemcpyHostToDevice));
+            MGK_HIP_CHECK(hipMemcpy(c->mu, hm.data(), N * sizeof(double), hipMemcpyHostToDevice));
+            MGK_HIP_CHECK(hipMemcpy(c->dlam32, hdl.data(), N * sizeof(float), hipMemcpyHostToDevice));
+            MGK_HIP_CHECK(hipMemcpy(c->dmu32, hdm.data(), N * sizeof(float), hipMemcpyHostToDevice));
+            const int C = g.C;
+            c->eps = dalloc<double>((size_t)C * N);
+            c->sig = dalloc<double>((size_t)C * N);
+            c->epsp = dalloc<double>((size_t)C * N);
+            MGK_HIP_CHECK(hipMemset(c->eps, 0, sizeof(double) * C * N));
+            MGK_HIP_CHECK(hipMemset(c->sig, 0, sizeof(double) * C * N));
+            MGK_HIP_CHECK(hipMemset(c->epsp, 0, sizeof(double) * C * N));
+            c->spec = dalloc<double>((size_t)2 * C * g.NS);
+            c->y32 = dalloc<float>((size_t)C * N);
+            c->corr = dalloc<float>((size_t)C * N);
+            c->partials = dalloc<double>((size_t)ls_partial_slots(g) + 64);
+            c->ctl = dalloc<Ctl>(1);
+            c->hctl = halloc<Ctl>(1);
+            MGK_HIP_CHECK(hipMemset(c->ctl, 0, sizeof(Ctl)));
+            const int lens[3] = {g.nf, g.nm, g.nl};
+            for (int a = 0; a < 3; ++a) {
+                auto t64 = twiddle_table<double>(lens[a]);
+                auto t32 = twiddle_table<float>(lens[a]);
+                c->tw64[a] = dalloc<double>(t64.size());
+                c->tw32[a] = dalloc<float>(t32.size());
+                MGK_HIP_CHECK(hipMemcpy(c->tw64[a], t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+                MGK_HIP_CHECK(hipMemcpy(c->tw32[a], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+            }
+            MGK_HIP_CHECK(hipDeviceSynchronize());
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+        return MGK_OK;
+    });
+}
+
+int mgk_destroy(mgk_ctx* ctx) {
+    if (!ctx) return MGK_OK;
+    return guarded([&]() -> int {
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+        return MGK_OK;
+    });
+}
+
+int mgk_set_precision(mgk_ctx* ctx, int precision) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        if (precision != MGK_FP64 && precision != MGK_FP32_REFINE) throw ArgError("bad precision");
+        ctx->precision = precision;
+        return MGK_OK;
+    });
+}
+
+int mgk_ls_solve(mgk_ctx* ctx, const double* eps_bar, double* eps, int warm_start, const double* eps_plastic,
+                 double* sig_out, double* eps_macro, double* sig_macro, int max_iter, double tol, int* iters_out,
+                 double* diff_out) {
+    return guarded([&]() -> int {
+        if (!ctx || !eps_bar || !eps || !eps_macro || !sig_macro) throw ArgError("null argument");
+        if (max_iter < 0) throw ArgError("max_iter < 0");
+        mgk_ctx& c = *ctx;
+        const long N = c.g.N;
+        const int ns = c.nskew();
+        double bar_sym[6], bar_skew[3] = {0, 0, 0};
+        c.split(eps_bar, bar_sym, bar_skew);   // copy first: eps_macro may alias eps_bar
+        // plastic strain: symmetric part to the device, antisymmetric part kept here
+        std::vector<double> pskew;
+        if (eps_plastic) {
+            c.upload_tensor(eps_plastic, c.epsp, &pskew);
+            c.epsp_set = true;
+        } else {
+            c.epsp_set = false;
+        }
+        std::vector<double> eskew;
+        double norm_extra0 = 0.0, diff_extra0 = 0.0;
+        double norm_extra = 0.0;
+        for (int w = 0; w < ns; ++w) norm_extra += 2.0 * (double)N * bar_skew[w] * bar_skew[w];
+        if (warm_start) {
+            c.upload_tensor(eps, c.eps, &eskew);
+            // antisymmetric part after the warm-start shift: W_v + W_bar - mean(W)
+            double mw[3] = {0, 0, 0};
+            for (long v = 0; v < N; ++v)
+                for (int w = 0; w < ns; ++w) mw[w] += eskew[(size_t)v * ns + w];
+            for (int w = 0; w < ns; ++w) mw[w] /= N;
+            double extra_norm = 0.0, extra_diff = 0.0, uniform_norm = 0.0;
+            for (long v = 0; v < N; ++v)
+                for (int w = 0; w < ns; ++w) {
+                    const double sh = eskew[(size_t)v * ns + w] + bar_skew[w] - mw[w];
+                    extra_norm += 2.0 * sh * sh;
+                    const double d = bar_skew[w] - sh;
+                    extra_diff += 2.0 * d * d;
+                }
+            (void)uniform_norm;
+            norm_extra0 = extra_norm;
+            diff_extra0 = extra_diff;
+            const bool uniform = (extra_diff == 0.0);
+            if (!uniform)
+                throw ArgError("the antisymmetric part of a warm-start strain field must be uniform");
+        }
+        int it = c.solve(bar_sym, norm_extra0, diff_extra0, norm_extra, warm_start != 0, max_iter, tol, diff_out);
+        if (iters_out) *iters_out = it;
+        for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
+        // outputs
+        c.download_tensor(c.eps, eps, [&](long, double* sk) { for (int w = 0; w < ns; ++w) sk[w] = bar_skew[w]; });
+        const std::vector<double>& hm = c.h_mu;
+        if (sig_out) {
+            c.download_tensor(c.sig, sig_out, [&](long v, double* sk) {
+                for (int w = 0; w < ns; ++w) {
+                    const double pw = pskew.empty() ? 0.0 : pskew[(size_t)v * ns + w];
+                    sk[w] = 2.0 * hm[v] * (bar_skew[w] - pw);
+                }
+            });
+        }
+        double sk_sig[3] = {0, 0, 0};
+        for (int w = 0; w < ns; ++w) {
+            double s = 0.0;
+            for (long v = 0; v < N; ++v) {
+                const double pw = pskew.empty() ? 0.0 : pskew[(size_t)v * ns + w];
+                s += 2.0 * hm[v] * (bar_skew[w] - pw);
+            }
+            sk_sig[w] = s / N;
+        }
+        c.join(c.hctl->mean_eps, bar_skew, eps_macro);
+        c.join(c.hctl->mean_sig, sk_sig, sig_macro);
+        return MGK_OK;
+    });
+}
+
+static long field_len(const mgk_ctx& c, int which) {
+    const long N = c.g.N;
+    const int F = c.full(), M = c.kp.M;
+    switch (which) {
+        case MGK_FIELD_EPS: case MGK_FIELD_SIG: case MGK_FIELD_EPS_PLASTIC: case MGK_FIELD_PREV_DIR: return N * F;
+        case MGK_FIELD_Q: case MGK_FIELD_Q0: return N * M;
+        case MGK_FIELD_CATALOG: return N * M * F;
+        case MGK_FIELD_SOFT: return 2 * N;
+        case MGK_FIELD_LAST_EVENT_TIME: case MGK_FIELD_E: case MGK_FIELD_NU: return N;
+    }
+    return -1;
+}
+
+static bool kmc_field(int which) {
+    return which == MGK_FIELD_Q || which == MGK_FIELD_Q0 || which == MGK_FIELD_CATALOG || which == MGK_FIELD_SOFT ||
+           which == MGK_FIELD_LAST_EVENT_TIME || which == MGK_FIELD_PREV_DIR;
+}
+
+int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
+    return guarded([&]() -> int {
+        if (!ctx || !host) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (field_len(c, which) < 0) throw ArgError("unknown field");
+        if (kmc_field(which) && !c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const long N = c.g.N;
+        const int ns = c.nskew();
+        std::vector<double> sk;
+        switch (which) {
+            case MGK_FIELD_EPS: {
+                c.upload_tensor(host, c.eps, &sk);
+                for (int w = 0; w < ns; ++w) c.W[w] = sk.empty() ? 0.0 : sk[w];
+                for (long v = 0; v < N; ++v)
+                    for (int w = 0; w < ns; ++w)
+                        if (sk[(size_t)v * ns + w] != c.W[w]) throw ArgError("antisymmetric part of eps must be uniform");
+                break;
+            }
+            case MGK_FIELD_SIG: c.upload_tensor(host, c.sig, nullptr); break;
+            case MGK_FIELD_EPS_PLASTIC: {
+                c.upload_tensor(host, c.epsp, &sk);
+                for (double s : sk)
+                    if (s != 0.0) throw ArgError("eps_plastic must be symmetric on the device path");
+                c.epsp_set = true;
+                break;
+            }
+            case MGK_FIELD_PREV_DIR: c.upload_tensor(host, c.K.prev_dir, nullptr); break;
+            case MGK_FIELD_CATALOG: {
+                // N*M tensors -> [C][N*M]
+                const long nm = N * c.kp.M;
+                const int C = c.g.C, F = c.full();
+                std::vector<double> h((size_t)nm * C);
+                double sym[6], skw[3];
+                for (long i = 0; i < nm; ++i) {
+                    c.split(host + i * F, sym, skw);
+                    for (int k = 0; k < C; ++k) h[(size_t)k * nm + i] = sym[k];
+                }
+                MGK_HIP_CHECK(hipMemcpy(c.K.cat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+                break;
+            }
+            case MGK_FIELD_Q: MGK_HIP_CHECK(hipMemcpy(c.K.Q, host, N * c.kp.M * sizeof(double), hipMemcpyHostToDevice)); break;
+            case MGK_FIELD_Q0: MGK_HIP_CHECK(hipMemcpy(c.K.Q0, host, N * c.kp.M * sizeof(double), hipMemcpyHostToDevice)); break;
+            case MGK_FIELD_SOFT: {
+                std::vector<double> h(2 * N);
+                for (long v = 0; v < N; ++v) { h[v] = host[2 * v]; h[N + v] = host[2 * v + 1]; }
+                MGK_HIP_CHECK(hipMemcpy(c.K.soft, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+                break;
+            }
+            case MGK_FIELD_LAST_EVENT_TIME:
+                MGK_HIP_CHECK(hipMemcpy(c.K.last_t, host, N * sizeof(double), hipMemcpyHostToDevice));
+                break;
+            default: throw ArgError("field is read-only");
+        }
+        MGK_HIP_CHECK(hipDeviceSynchronize());
+        return MGK_OK;
+    });
+}
+
+int mgk_get_field(mgk_ctx* ctx, int which, double* host) {
+    return guarded([&]() -> int {
+        if (!ctx || !host) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (field_len(c, which) < 0) throw ArgError("unknown field");
+        if (kmc_field(which) && !c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        const long N = c.g.N;
+        const int ns = c.nskew();
+        const double zero[3] = {0, 0, 0};
+        switch (which) {
+            case MGK_FIELD_EPS:
+                c.download_tensor(c.eps, host, [&](long, double* sk) { for (int w = 0; w < ns; ++w) sk[w] = c.W[w]; });
+                break;
+            case MGK_FIELD_SIG:
+                c.download_tensor(c.sig, host, [&](long v, double* sk) {
+                    for (int w = 0; w < ns; ++w) sk[w] = 2.0 * c.h_mu[v] * c.W[w];
+                });
+                break;
+            case MGK_FIELD_EPS_PLASTIC:
+                c.download_tensor(c.epsp, host, [&](long, double* sk) { for (int w = 0; w < ns; ++w) sk[w] = zero[w]; });
+                break;
+            case MGK_FIELD_PREV_DIR:
+                c.download_tensor(c.K.prev_dir, host, [&](long, double* sk) { for (int w = 0; w < ns; ++w) sk[w] = zero[w]; });
+                break;
+            case MGK_FIELD_CATALOG: {
+                const long nm = N * c.kp.M;
+                const int C = c.g.C, F = c.full();
+                std::vector<double> h((size_t)nm * C);
+                MGK_HIP_CHECK(hipMemcpy(h.data(), c.K.cat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+                double sym[6];
+                for (long i = 0; i < nm; ++i) {
+                    for (int k = 0; k < C; ++k) sym[k] = h[(size_t)k * nm + i];
+                    c.join(sym, zero, host + i * F);
+                }
+                break;
+            }
+            case MGK_FIELD_Q: MGK_HIP_CHECK(hipMemcpy(host, c.K.Q, N * c.kp.M * sizeof(double), hipMemcpyDeviceToHost)); break;
+            case MGK_FIELD_Q0: MGK_HIP_CHECK(hipMemcpy(host, c.K.Q0, N * c.kp.M * sizeof(double), hipMemcpyDeviceToHost)); break;
+            case MGK_FIELD_SOFT: {
+                std::vector<double> h(2 * N);
+                MGK_HIP_CHECK(hipMemcpy(h.data(), c.K.soft, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+                for (long v = 0; v < N; ++v) { host[2 * v] = h[v]; host[2 * v + 1] = h[N + v]; }
+                break;
+            }
+            case MGK_FIELD_LAST_EVENT_TIME:
+                MGK_HIP_CHECK(hipMemcpy(host, c.K.last_t, N * sizeof(double), hipMemcpyDeviceToHost));
+                break;
+            case MGK_FIELD_E: MGK_HIP_CHECK(hipMemcpy(host, c.E, N * sizeof(double), hipMemcpyDeviceToHost)); break;
+            case MGK_FIELD_NU: MGK_HIP_CHECK(hipMemcpy(host, c.nu, N * sizeof(double), hipMemcpyDeviceToHost)); break;
+        }
+        return MGK_OK;
+    });
+}
+
+int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
+    return guarded([&]() -> int {
+        if (!ctx || !kp) throw ArgError("null argument");
+        if (kp->M < 1) throw ArgError("M must be >= 1");
+        mgk_ctx& c = *ctx;
+        const long N = c.g.N;
+        const long nm = N * kp->M;
+        if (nm > (long)std::numeric_limits<int>::max()) throw ArgError("N*M exceeds the int index range of the reference");
+        if (c.kmc) {
+            auto f = [](void* p) { if (p) (void)hipFree(p); };
+            f(c.K.Q); f(c.K.Q0); f(c.K.cat); f(c.K.soft); f(c.K.last_t); f(c.K.prev_dir); f(c.K.rates); f(c.K.bsum);
+            f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kctl); f(c.kstage); f(c.unst_idx); f(c.unst_q);
+            if (c.hkctl) (void)hipHostFree(c.hkctl);
+            if (c.hkstage) (void)hipHostFree(c.hkstage);
+        }
+        c.kp = *kp;
+        const int C = c.g.C;
+        KmcParams& K = c.K;
+        K = KmcParams{};
+        K.g = c.g;
+        KmcPhys& ph = K.ph;
+        ph.M = kp->M;
+        ph.directional = kp->softening_directional;
+        ph.instability_kmc = kp->instability_kmc;
+        ph.cap = kp->softening_cap;
+        ph.jp = kp->jp;
+        ph.jt = kp->jt;
+        ph.nfrac = kp->neighbor_softening_fraction;
+        ph.tau = kp->tau;
+        ph.tau_inf = std::isinf(kp->tau) ? 1 : 0;
+        ph.volume = kp->volume;
+        ph.eff_volume = kp->scale_rate_by_volume ? kp->volume : 1.0;   // kmc_solver.hpp:784
+        ph.nu0 = kp->nu0;
+        const double kB = 8.617e-5;                                       // kmc_solver.hpp:785
+        ph.beta = (kp->temperature > 0.0) ? (1.0 / (kB * kp->temperature)) : std::numeric_limits<double>::infinity();
+        ph.thr = kp->stability_threshold;
+        K.sig = c.sig;
+        K.eps_p = c.epsp;
+        K.Q = dalloc<double>(nm);
+        K.Q0 = dalloc<double>(nm);
+        K.cat = dalloc<double>((size_t)C * nm);
+        K.soft = dalloc<double>(2 * N);
+        K.last_t = dalloc<double>(N);
+        K.prev_dir = dalloc<double>((size_t)C * N);
+        K.rates = dalloc<double>(nm);
+        const int nb = kmc_nblocks(nm);
+        K.bsum = dalloc<double>(nb);
+        K.bincl = dalloc<double>(nb);
+        K.bunst = dalloc<int>(nb);
+        K.bunst_off = dalloc<int>(nb);
+        K.binc = dalloc<int>(nb);
+        c.kctl = dalloc<KmcCtl>(1);
+        c.hkctl = halloc<KmcCtl>(1);
+        K.kctl = c.kctl;
+        c.kstage = dalloc<double>((size_t)kp->M * (C + 1));
+        c.hkstage = halloc<double>((size_t)kp->M * (C + 1));
+        K.staging = c.kstage;
+        c.unst_idx = dalloc<int>(nm);
+        c.unst_q = dalloc<double>(nm);
+        K.unst_idx = c.unst_idx;
+        K.unst_q = c.unst_q;
+        MGK_HIP_CHECK(hipMemset(K.Q, 0, nm * sizeof(double)));
+        MGK_HIP_CHECK(hipMemset(K.Q0, 0, nm * sizeof(double)));
+        MGK_HIP_CHECK(hipMemset(K.cat, 0, (size_t)C * nm * sizeof(double)));
+        MGK_HIP_CHECK(hipMemset(K.soft, 0, 2 * N * sizeof(double)));
+        MGK_HIP_CHECK(hipMemset(K.prev_dir, 0, (size_t)C * N * sizeof(double)));
+        MGK_HIP_CHECK(hipMemset(c.kctl, 0, sizeof(KmcCtl)));
+        std::vector<double> ninf(N, -std::numeric_limits<double>::infinity());
+        MGK_HIP_CHECK(hipMemcpy(K.last_t, ninf.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        MGK_HIP_CHECK(hipMemset(c.epsp, 0, sizeof(double) * C * N));
+        c.epsp_set = true;
+        MGK_HIP_CHECK(hipDeviceSynchronize());
+        c.kmc = true;
+        return MGK_OK;
+    });
+}
+
+int mgk_elastic_run(mgk_ctx* ctx, double* eps_macro, double* sig_macro, int max_iter, double tol, int* iters_out,
+                    double* diff_out) {
+    return guarded([&]() -> int {
+        if (!ctx || !eps_macro || !sig_macro) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        const int ns = c.nskew();
+        double bar_sym[6], bar_skew[3] = {0, 0, 0};
+        c.split(eps_macro, bar_sym, bar_skew);
+        double norm_extra = 0.0;
+        for (int w = 0; w < ns; ++w) norm_extra += 2.0 * (double)c.g.N * bar_skew[w] * bar_skew[w];
+        // warm start always: the KMC fields are sized N (kmc_solver.hpp:197)
+        int it = c.solve(bar_sym, norm_extra, 0.0, norm_extra, true, max_iter, tol, diff_out);
+        if (iters_out) *iters_out = it;
+        for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
+        double sk_sig[3];
+        for (int w = 0; w < ns; ++w) sk_sig[w] = 2.0 * c.mean_mu * bar_skew[w];
+        c.join(c.hctl->mean_eps, bar_skew, eps_macro);
+        c.join(c.hctl->mean_sig, sk_sig, sig_macro);
+        return MGK_OK;
+    });
+}
+
+int mgk_update_rates(mgk_ctx* ctx, double time, double* total_rate, int* n_unstable, int* n_included) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        kmc_enqueue_rates(c.K, time, c.stream);
+        MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (total_rate) *total_rate = c.hkctl->total_rate;
+        if (n_unstable) *n_unstable = c.hkctl->n_unstable;
+        if (n_included) *n_included = c.hkctl->n_included;
+        return MGK_OK;
+    });
+}
+
+int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
+    return guarded([&]() -> int {
+        if (!ctx || !flat_idx) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        kmc_enqueue_select(c.K, u2, c.stream);
+        MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (c.hkctl->ambiguous) {
+            kmc_enqueue_select_fallback(c.K, c.stream);
+            MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+            MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        }
+        *flat_idx = c.hkctl->sel;
+        if (q_sel) *q_sel = c.hkctl->q_sel;
+        return MGK_OK;
+    });
+}
+
+int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_catalog, const double* new_Q0,
+                   int first_mode, int n_modes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const long nm = c.g.N * c.kp.M;
+        if (flat_idx < 0 || flat_idx >= nm) throw ArgError("flat index out of range");
+        if ((new_catalog || new_Q0) && (first_mode < 0 || n_modes < 0 || first_mode + n_modes > c.kp.M))
+            throw ArgError("redraw mode range out of range");
+        if (!new_catalog && !new_Q0) n_modes = 0;
+        const int C = c.g.C, F = c.full();
+        double* h = c.hkstage;
+        double skw[3];
+        if (new_catalog)
+            for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
+        if (new_Q0)
+            for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
+        if (n_modes > 0)
+            MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, h, sizeof(double) * n_modes * (C + 1), hipMemcpyHostToDevice, c.stream));
+        kmc_enqueue_flip(c.K, flat_idx, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
+        // the pinned staging buffer is reused by the next call: wait for the copy
+        if (n_modes > 0) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return MGK_OK;
+    });
+}
+
+int mgk_find_unstable(mgk_ctx* ctx, int* idx_out, double* q_out, int cap, int* n_out) {
+    return guarded([&]() -> int {
+        if (!ctx || !n_out) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        const int n = c.hkctl->n_unstable;   // from the last mgk_update_rates
+        *n_out = n;
+        if (n > 0) {
+            kmc_enqueue_unstable(c.K, c.stream);
+            const int m = std::min(n, cap);
+            if (m > 0 && idx_out) MGK_HIP_CHECK(hipMemcpyAsync(idx_out, c.unst_idx, m * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            if (m > 0 && q_out) MGK_HIP_CHECK(hipMemcpyAsync(q_out, c.unst_q, m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        }
+        return MGK_OK;
+    });
+}
+
+int mgk_sync(mgk_ctx* ctx) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        MGK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return MGK_OK;
+    });
+}
+
+int mgk_timer_start(mgk_ctx* ctx) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        MGK_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        return MGK_OK;
+    });
+}
+
+int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
+    return guarded([&]() -> int {
+        if (!ctx || !ms) throw ArgError("null argument");
+        MGK_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        MGK_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        float f = 0.f;
+        MGK_HIP_CHECK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+        *ms = f;
+        return MGK_OK;
+    });
+}
+
+int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        if (fp64_iters) *fp64_iters = ctx->last_iters;
+        if (fp32_iters) *fp32_iters = ctx->last_inner;
+        return MGK_OK;
+    });
+}
+
+}  // extern "C"

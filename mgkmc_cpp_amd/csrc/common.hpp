@@ -1,0 +1,81 @@
+// common.hpp -- shared types of the device library (geometry, control block,
+// error handling).  Host and device code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+namespace mgk {
+
+// Grid geometry of one trajectory.  The FFT axes are named by role:
+// F (first, x), M (middle, y in 3D, absent in 2D) and L (last, contiguous:
+// z in 3D, y in 2D), matching the reference's voxel order idx = x*ny + y and
+// idx = x*ny*nz + y*nz + z (linear_elastic_solver.hpp:81, :357).
+struct Geom {
+    int dims = 2;
+    int nf = 1, nm = 1, nl = 1;
+    int nlh = 1;       // nl/2 + 1 bins of the real-to-complex last axis
+    long N = 1;        // voxels
+    long NS = 1;       // complex bins of the half spectrum = nf*nm*nlh
+    int C = 3;         // symmetric components stored: 3 (2D) or 6 (3D)
+    __host__ __device__ long nlines() const { return (long)nf * nm; }
+};
+
+// Symmetric component order on the device.
+// 2D: 0 xx, 1 yy, 2 xy.   3D: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz.
+__host__ __device__ inline int comp_of(int dims, int i, int j) {
+    if (i == j) return i;
+    int a = i < j ? i : j, b = i < j ? j : i;
+    if (dims == 2) return 2;
+    return (a == 0 && b == 1) ? 3 : (a == 0 ? 4 : 5);
+}
+
+// Control block of one Lippmann-Schwinger solve, in device memory and
+// mirrored into pinned host memory at the end of every batch of iterations.
+struct Ctl {
+    int done;            // convergence reached or iteration cap hit
+    int iters;           // FP64 iterations run
+    int inner;           // FP32 inner iterations run (refinement path)
+    int corr_valid;      // the FP32 correction field holds a pending update
+    unsigned ticket;     // last-block ticket of the reduction
+    int max_iter;
+    int pad0, pad1;
+    double tol;
+    double diff;         // last relative increment
+    double norm_extra;   // sum over the uniform antisymmetric part of eps (host-known)
+    double shift[6];     // eps_bar - mean(eps): the warm-start shift of iteration 0
+    double eps_bar[6];   // imposed mean strain (symmetric part)
+    double mean_eps[6];  // outputs of the final pass
+    double mean_sig[6];
+    double diff_sq, norm_sq;
+};
+
+struct Status {
+    int code = 0;
+    std::string msg;
+};
+
+}  // namespace mgk
+
+#define MGK_HIP_CHECK(expr)                                                                 \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            throw mgk::HipError(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+        }                                                                                   \
+    } while (0)
+
+namespace mgk {
+struct HipError {
+    std::string what;
+    explicit HipError(std::string w) : what(std::move(w)) {}
+};
+struct ArgError {
+    std::string what;
+    explicit ArgError(std::string w) : what(std::move(w)) {}
+};
+struct UnsupportedError {
+    std::string what;
+    explicit UnsupportedError(std::string w) : what(std::move(w)) {}
+};
+}  // namespace mgk

@@ -1,0 +1,422 @@
+// kmc_kernels.hip -- per-event KMC kernels on gfx950: barriers and rates,
+// FP64 block sums, two-level event selection, flip, unstable-mode compaction.
+//
+// Rates span many decades (exp(-Q/kT)), so every sum is FP64.  Selection is
+// the first included mode whose sequential prefix sum reaches
+// target = u2 * total (kmc_solver.hpp:816-826).  The device computes block
+// sums and an in-block scan; when the target lies within a rounding bound of
+// a prefix boundary -- the only case where the parallel order could choose a
+// different mode than the reference's left-to-right sum -- it reports
+// "ambiguous" and the host runs the exact sequential pass instead.
+#include "common.hpp"
+#include "kmc_kernels.hpp"
+
+namespace mgk {
+
+template <typename T> __device__ __forceinline__ T wsum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double load_coh(const double* p) {
+    unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
+}
+__device__ __forceinline__ int load_coh(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// update_barriers + rates (kmc_solver.hpp:341-410, :783-806; 3D :1340-1411,
+// :1796-1818).  One lane per (voxel, mode), 1024 modes per block.
+// ---------------------------------------------------------------------------
+template <int D, int C>
+__global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
+    const Geom g = p.g;
+    const KmcPhys ph = p.ph;
+    const long nmodes = g.N * ph.M;
+    const long base = (long)blockIdx.x * kRateBlock;
+    double lsum = 0.0;
+    int lun = 0, linc = 0;
+    const double GPa_nm3_to_eV = 6.241509;
+#pragma unroll
+    for (int k = 0; k < kRateBlock / 256; ++k) {
+        const long i = base + k * 256 + threadIdx.x;
+        if (i >= nmodes) break;
+        const long v = i / ph.M;
+        // softening state of the voxel
+        const double g_t = p.soft[g.N + v];
+        double g_t_curr = 0.0;
+        if (g_t > 0.0) {
+            const double t_last = p.last_t[v];
+            if (ph.tau_inf) g_t_curr = g_t;
+            else if (t_last == -INFINITY) g_t_curr = 0.0;
+            else {
+                double dt = time - t_last;
+                if (dt < 0.0) dt = 0.0;
+                g_t_curr = g_t * exp(-dt / ph.tau);
+            }
+        }
+        double g_base = p.soft[v] + g_t_curr;
+        if (ph.cap > 0.0 && g_base > ph.cap) g_base = ph.cap;
+        double cat[C], s[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            cat[c] = p.cat[c * nmodes + i];
+            s[c] = p.sig[c * g.N + v];
+        }
+        double modifier = 1.0;
+        if (ph.directional) {
+            double dot = 0.0, nmq = 0.0, npq = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    const int c = comp_of(D, a, b);
+                    const double vm = cat[c], vp = p.prev_dir[c * g.N + v];
+                    dot += vm * vp;
+                    nmq += vm * vm;
+                    npq += vp * vp;
+                }
+            const double npv = sqrt(npq), nmv = sqrt(nmq);
+            if (!(npv < 1e-12 || nmv < 1e-12)) {
+                const double ct = dot / (nmv * npv);
+                modifier = ((1.0 + ct) * (1.0 + ct)) / 4.0;
+            }
+        }
+        const double g_eff = modifier * g_base;
+        double w_sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                const int c = comp_of(D, a, b);
+                w_sum += s[c] * cat[c];
+            }
+        const double w = 0.5 * ph.volume * (w_sum / 1e9) * GPa_nm3_to_eV;
+        const double q = p.Q0[i] * exp(-g_eff) - w;
+        p.Q[i] = q;
+        const bool incl = ph.instability_kmc || q > 0.0;
+        double r = -1.0;
+        if (incl) r = (q <= 0.0) ? ph.eff_volume * ph.nu0 : ph.eff_volume * ph.nu0 * exp(-q * ph.beta);
+        p.rates[i] = r;
+        if (incl) { lsum += r; linc++; }
+        if (q < ph.thr) lun++;
+    }
+    __shared__ double sd[4];
+    __shared__ int su[4], si[4];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    lsum = wsum(lsum);
+    lun = wsum(lun);
+    linc = wsum(linc);
+    if (lane == 0) { sd[wid] = lsum; su[wid] = lun; si[wid] = linc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        p.bsum[blockIdx.x] = (sd[0] + sd[1]) + (sd[2] + sd[3]);
+        p.bunst[blockIdx.x] = su[0] + su[1] + su[2] + su[3];
+        p.binc[blockIdx.x] = si[0] + si[1] + si[2] + si[3];
+        __threadfence();
+        unsigned t = atomicAdd(&p.kctl->ticket, 1u);
+        s_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // last block: inclusive prefix of block sums (chunked, fixed order), and
+    // exclusive prefix of unstable counts.
+    const int nb = gridDim.x;
+    const int chunk = (nb + 255) / 256;
+    const int b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
+    double csum = 0.0;
+    int cun = 0, cinc = 0;
+    for (int b = b0; b < b1; ++b) {
+        csum += load_coh(&p.bsum[b]);
+        cun += load_coh(&p.bunst[b]);
+        cinc += load_coh(&p.binc[b]);
+    }
+    __shared__ double sc[256];
+    __shared__ int scu[256], sci[256];
+    sc[threadIdx.x] = csum;
+    scu[threadIdx.x] = cun;
+    sci[threadIdx.x] = cinc;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 256 chunk totals
+    for (int off = 1; off < 256; off <<= 1) {
+        double a = 0.0;
+        int au = 0, ai = 0;
+        if ((int)threadIdx.x >= off) { a = sc[threadIdx.x - off]; au = scu[threadIdx.x - off]; ai = sci[threadIdx.x - off]; }
+        __syncthreads();
+        sc[threadIdx.x] += a;
+        scu[threadIdx.x] += au;
+        sci[threadIdx.x] += ai;
+        __syncthreads();
+    }
+    double run = (threadIdx.x > 0) ? sc[threadIdx.x - 1] : 0.0;
+    int runu = (threadIdx.x > 0) ? scu[threadIdx.x - 1] : 0;
+    for (int b = b0; b < b1; ++b) {
+        run += load_coh(&p.bsum[b]);
+        p.bincl[b] = run;
+        p.bunst_off[b] = runu;
+        runu += load_coh(&p.bunst[b]);
+    }
+    __shared__ double s_total;
+    // the total is the last inclusive prefix, so that target <= bincl[nb-1]
+    if (b0 < nb && b1 == nb) s_total = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tot = s_total;
+        KmcCtl* kc = p.kctl;
+        kc->total_rate = tot;
+        kc->n_unstable = scu[255];
+        kc->n_included = sci[255];
+        kc->nblocks = nb;
+        kc->sel = -1;
+        kc->ambiguous = 0;
+        kc->ticket = 0;
+        __threadfence();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Selection: block by binary search over inclusive block prefixes, mode by an
+// in-block scan; a rounding-bound guard flags targets too close to call.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
+    const long nmodes = p.g.N * p.ph.M;
+    KmcCtl* kc = p.kctl;
+    const int nb = kc->nblocks;
+    const double total = kc->total_rate;
+    const double target = u2 * total;
+    __shared__ int s_b;
+    __shared__ double s_scan[kRateBlock];
+    __shared__ double s_tot[256];
+    __shared__ int s_first;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {   // first b with bincl[b] >= target
+            int mid = (lo + hi) / 2;
+            if (p.bincl[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+        s_b = lo;
+        s_first = kRateBlock;
+    }
+    __syncthreads();
+    const int b = s_b;
+    const long base = (long)b * kRateBlock;
+    const double prev = (b > 0) ? p.bincl[b - 1] : 0.0;
+    // 4 consecutive modes per lane, local inclusive prefix
+    double loc[4];
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long i = base + threadIdx.x * 4 + k;
+        double r = (i < nmodes) ? p.rates[i] : -1.0;
+        run += (r >= 0.0) ? r : 0.0;
+        loc[k] = run;
+    }
+    s_tot[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        double a = ((int)threadIdx.x >= off) ? s_tot[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        s_tot[threadIdx.x] += a;
+        __syncthreads();
+    }
+    const double excl = (threadIdx.x > 0) ? s_tot[threadIdx.x - 1] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int li = threadIdx.x * 4 + k;
+        const long i = base + li;
+        const double S = prev + (excl + loc[k]);
+        s_scan[li] = S;
+        const double r = (i < nmodes) ? p.rates[i] : -1.0;
+        if (r >= 0.0 && S >= target) atomicMin(&s_first, li);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double u = 1.1102230246251565e-16;   // 2^-53
+    int sel = -1, amb = 1;
+    if (s_first < kRateBlock) {
+        const int li = s_first;
+        const long i = base + li;
+        const double S = s_scan[li];
+        const double Sp = (li > 0) ? s_scan[li - 1] : prev;
+        // reference prefix error <= i u S (left-to-right sum); device <= (log2 nb + 1024) u S;
+        // target error <= nmodes u total
+        const double mS = 2.0 * ((double)i + 2048.0) * u * S + 2.0 * (double)nmodes * u * total;
+        const double mP = 2.0 * ((double)i + 2048.0) * u * Sp + 2.0 * (double)nmodes * u * total;
+        sel = (int)i;
+        amb = (S - target <= mS) || (target - Sp <= mP) ? 1 : 0;
+    }
+    kc->u2 = u2;
+    kc->target = target;
+    kc->sel = sel;
+    kc->ambiguous = amb;
+    kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+    __threadfence();
+}
+
+// Exact left-to-right pass of the reference (single lane): the total in flat
+// order, target = u2 * total, then the first prefix >= target.
+__global__ void k_select_seq(KmcParams p) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    KmcCtl* kc = p.kctl;
+    const long nmodes = p.g.N * p.ph.M;
+    double tot = 0.0;
+    long last = -1;
+    for (long i = 0; i < nmodes; ++i) {
+        const double r = p.rates[i];
+        if (r >= 0.0) { tot += r; last = i; }
+    }
+    const double target = kc->u2 * tot;
+    double cur = 0.0;
+    long sel = last;
+    for (long i = 0; i < nmodes; ++i) {
+        const double r = p.rates[i];
+        if (r < 0.0) continue;
+        cur += r;
+        if (cur >= target) { sel = i; break; }
+    }
+    kc->target = target;
+    kc->sel = (int)sel;
+    kc->ambiguous = 0;
+    kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+    __threadfence();
+}
+
+// ---------------------------------------------------------------------------
+// apply_flip_soa_2d/3d + prev_strain_dir + redraw upload (kmc_solver.hpp:423-462,
+// :835-864; 3D :1424-1471, :1850-1875).  The neighbour loop runs in the
+// reference's order on one lane: periodic wraps on small grids hit the same
+// voxel several times and each hit accumulates.
+// ---------------------------------------------------------------------------
+template <int D, int C>
+__global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n_modes, int has_cat, int has_q0) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Geom g = p.g;
+    const KmcPhys ph = p.ph;
+    const long nmodes = g.N * ph.M;
+    const long v = flat / ph.M;
+    double c[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) c[k] = p.cat[k * nmodes + flat];
+#pragma unroll
+    for (int k = 0; k < C; ++k) p.eps_p[k * g.N + v] += c[k];
+    double sum_sq;
+    if constexpr (D == 2) {
+        const double e11 = c[0], e22 = c[1], e12 = c[2];
+        sum_sq = (e12 * e12) + (e22 * e22 + e11 * e11 + (e11 - e22) * (e11 - e22)) / 6.0;
+    } else {
+        const double e11 = c[0], e22 = c[1], e33 = c[2], e12 = c[3], e13 = c[4], e23 = c[5];
+        sum_sq = (e12 * e12 + e23 * e23 + e13 * e13) +
+                 ((e22 - e33) * (e22 - e33) + (e33 - e11) * (e33 - e11) + (e11 - e22) * (e11 - e22)) / 6.0;
+    }
+    double gp = p.soft[v] + ph.jp * sum_sq;
+    if (ph.cap > 0.0 && gp > ph.cap) gp = ph.cap;
+    p.soft[v] = gp;
+    p.soft[g.N + v] = ph.jt * sum_sq;
+    p.last_t[v] = time;
+    if (ph.nfrac > 0.0) {
+        int x, y, z = 0;
+        const int nx = g.nf, ny = (D == 2) ? g.nl : g.nm, nz = (D == 2) ? 1 : g.nl;
+        if (D == 2) { y = (int)(v % ny); x = (int)(v / ny); }
+        else { z = (int)(v % nz); const long t = v / nz; y = (int)(t % ny); x = (int)(t / ny); }
+        const int zr = (D == 3) ? 1 : 0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -zr; dz <= zr; ++dz) {
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny;
+                    const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
+                    const long nb = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
+                    double gn = p.soft[nb] + ph.nfrac * ph.jp * sum_sq;
+                    if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
+                    p.soft[nb] = gn;
+                    p.soft[g.N + nb] += ph.nfrac * ph.jt * sum_sq;
+                }
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) p.prev_dir[k * g.N + v] = c[k];
+    for (int j = 0; j < n_modes; ++j) {
+        const long mi = v * ph.M + first_mode + j;
+        if (has_cat)
+            for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = p.staging[j * C + k];
+        if (has_q0) p.Q0[mi] = p.staging[n_modes * C + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// find_unstable_indices (kmc_solver.hpp:412-421): ascending flat indices with
+// Q < threshold, compacted with the block offsets of the rate pass.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unstable(KmcParams p) {
+    const long nmodes = p.g.N * p.ph.M;
+    const long base = (long)blockIdx.x * kRateBlock;
+    __shared__ int s_cnt[256];
+    int flags[4], cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long i = base + threadIdx.x * 4 + k;
+        flags[k] = (i < nmodes && p.Q[i] < p.ph.thr) ? 1 : 0;
+        cnt += flags[k];
+    }
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int a = ((int)threadIdx.x >= off) ? s_cnt[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_cnt[threadIdx.x] += a;
+        __syncthreads();
+    }
+    int pos = p.bunst_off[blockIdx.x] + ((threadIdx.x > 0) ? s_cnt[threadIdx.x - 1] : 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!flags[k]) continue;
+        const long i = base + threadIdx.x * 4 + k;
+        p.unst_idx[pos] = (int)i;
+        p.unst_q[pos] = p.Q[i];
+        pos++;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlock); }
+
+void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s) {
+    const int nb = kmc_nblocks(p.g.N * p.ph.M);
+    if (p.g.dims == 2) hipLaunchKernelGGL((k_rates<2, 3>), dim3(nb), dim3(256), 0, s, p, time);
+    else hipLaunchKernelGGL((k_rates<3, 6>), dim3(nb), dim3(256), 0, s, p, time);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s) {
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, s, p, u2);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_select_fallback(const KmcParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_select_seq, dim3(1), dim3(64), 0, s, p);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
+                      int has_q0, hipStream_t s) {
+    if (p.g.dims == 2)
+        hipLaunchKernelGGL((k_flip<2, 3>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0);
+    else
+        hipLaunchKernelGGL((k_flip<3, 6>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_unstable(const KmcParams& p, hipStream_t s) {
+    const int nb = kmc_nblocks(p.g.N * p.ph.M);
+    hipLaunchKernelGGL(k_unstable, dim3(nb), dim3(256), 0, s, p);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace mgk

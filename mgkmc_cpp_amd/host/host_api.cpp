@@ -1,0 +1,276 @@
+// host_api.cpp -- C façade (include/mgkmc_host.h) over the drop-in
+// KmcSimulation2D/3D classes, for language bindings and benchmarks.
+#include <chrono>
+#include <cstring>
+#include <ctime>
+#include <iomanip>
+#include <memory>
+#include <sstream>
+#include <string>
+
+#include "../../include/mgkmc_gpu.h"
+#include "../../include/mgkmc_host.h"
+#include "kmc_solver.hpp"
+
+// get_current_timestamp (declared by kmc_solver.hpp:22, defined in main.cpp:18-24)
+std::string get_current_timestamp() {
+    auto now = std::chrono::system_clock::now();
+    auto t = std::chrono::system_clock::to_time_t(now);
+    std::stringstream ss;
+    ss << std::put_time(std::localtime(&t), "%Y-%m-%d %H:%M:%S");
+    return ss.str();
+}
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F> int guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return MGK_E_ARG;
+    }
+}
+
+BarrierGenerator barrier_from(const mgk_sim_config& c) {
+    BarrierGenerator b;
+    static const char* names[] = {"gaussian", "rayleigh", "modified_rayleigh", "modified_rayleigh_with_exponential",
+                                  "none"};
+    b.type = names[(c.barrier_type >= 0 && c.barrier_type <= 4) ? c.barrier_type : 4];
+    b.mean = c.b_mean;
+    b.std_val = c.b_std;
+    b.min_cutoff = c.b_min_cutoff;
+    b.max_cutoff = c.b_max_cutoff;
+    b.loc = c.b_loc;
+    b.scale = c.b_scale;
+    b.epsilon = c.b_epsilon;
+    b.ratio = c.b_ratio;
+    return b;
+}
+}  // namespace
+
+struct mgk_sim {
+    int dims = 2;
+    std::unique_ptr<KmcSimulation2D> s2;
+    std::unique_ptr<KmcSimulation3D> s3;
+    template <typename F> auto with(F&& f) { return dims == 2 ? f(*s2) : f(*s3); }
+};
+
+extern "C" {
+
+const char* mgk_host_last_error(void) { return g_err.c_str(); }
+
+void mgk_sim_config_defaults(mgk_sim_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    // constructor defaults (kmc_solver.hpp:165-181) and main.cpp's YAML defaults (:279-317)
+    c->dims = 2;
+    c->nx = c->ny = c->nz = 32;
+    c->pixel = 1.0;
+    c->plane_mode = MGK_PLANE_STRAIN;
+    c->M = 20;
+    c->gamma0 = 0.14;
+    c->barrier_type = MGK_BARRIER_GAUSSIAN;
+    c->b_mean = 2.0;
+    c->b_std = 0.6;
+    c->b_min_cutoff = 0.1;
+    c->b_max_cutoff = -1.0;
+    c->b_loc = 1.0;
+    c->b_scale = 0.5;
+    c->b_epsilon = 0.1;
+    c->b_ratio = 0.8;
+    c->softening_directional = 0;
+    c->softening_cap = 2.0;
+    c->jp = 10.0;
+    c->jt = 30.0;
+    c->neighbor_softening_fraction = 0.0;
+    c->q_act_temp = 0.37;
+    c->temperature = 0.0;
+    c->strain_rate = 1.0;
+    c->stability_threshold = 0.0;
+    c->nu0 = 1e13;
+    c->instability_kmc = 0;
+    c->scale_rate_by_volume = 1;
+    c->redraw_directions = 1;
+    c->redraw_barriers = 1;
+    c->use_3d_barriers = 0;
+    c->seed = 42;
+    c->catalog_mode = MGK_CATALOG_RANDOM;
+    c->catalog_magnitude = 0.1;
+    c->ls_max_iter = 200;
+    c->ls_tol = 1e-6;
+    c->precision = MGK_FP64;
+}
+
+void mgk_run_config_defaults(mgk_run_config* r) {
+    std::memset(r, 0, sizeof(*r));
+    r->n_global_steps = 500;
+    r->step_size = 1e-4;
+    r->drv_i = r->drv_j = 0;
+    r->mixed_tol = 1e6;   // 1 MPa (main.cpp:280, :390)
+    r->mixed_max_iter = 50;
+    r->max_kmc_steps_pct = 0.3;
+    r->max_events = -1;
+}
+
+int mgk_sim_create(const mgk_sim_config* c, const double* E, const double* nu, mgk_sim** out) {
+    return guard([&]() -> int {
+        if (!c || !E || !nu || !out) throw std::invalid_argument("null argument");
+        const long N = (long)c->nx * c->ny * (c->dims == 3 ? c->nz : 1);
+        std::vector<double> Ev(E, E + N), nv(nu, nu + N);
+        auto s = std::make_unique<mgk_sim>();
+        s->dims = c->dims;
+        const BarrierGenerator bg = barrier_from(*c);
+        const std::string scheme = c->softening_directional ? "directional" : "isotropic";
+        const std::string imode = c->instability_kmc ? "kmc" : "cascade";
+        const bool fixed = c->catalog_mode == MGK_CATALOG_FIXED;
+        if (c->dims == 2) {
+            s->s2 = std::make_unique<KmcSimulation2D>(
+                c->nx, c->ny, c->M, c->gamma0, Ev, nv, c->pixel, bg, scheme, c->softening_cap, c->jp, c->jt,
+                c->neighbor_softening_fraction, c->q_act_temp, "output", c->temperature, c->strain_rate,
+                c->stability_threshold, c->nu0, c->plane_mode == MGK_PLANE_STRESS ? "plane_stress" : "plane_strain",
+                false, 5, 100, imode, "none", c->scale_rate_by_volume != 0, c->redraw_directions != 0,
+                c->redraw_barriers != 0, c->seed, c->precision, fixed, c->catalog_magnitude);
+        } else if (c->dims == 3) {
+            s->s3 = std::make_unique<KmcSimulation3D>(
+                c->nx, c->ny, c->nz, c->M, c->gamma0, Ev, nv, c->pixel, bg, scheme, c->softening_cap, c->jp, c->jt,
+                c->neighbor_softening_fraction, c->q_act_temp, "output", c->temperature, c->strain_rate,
+                c->stability_threshold, c->nu0, false, 5, 100, imode, "none", c->scale_rate_by_volume != 0,
+                c->redraw_directions != 0, c->redraw_barriers != 0, c->use_3d_barriers != 0, c->seed, c->precision,
+                fixed, c->catalog_magnitude);
+        } else {
+            throw std::invalid_argument("dims must be 2 or 3");
+        }
+        s->with([&](auto& sim) {
+            sim.ls_tol = c->ls_tol;
+            sim.ls_max_iter = c->ls_max_iter;
+            sim.record_events = true;
+            return 0;
+        });
+        *out = s.release();
+        return MGK_OK;
+    });
+}
+
+int mgk_sim_destroy(mgk_sim* s) {
+    delete s;
+    return MGK_OK;
+}
+
+static void fill_stats(const mgk_host::RunCounters& k, double time, mgk_run_stats* st) {
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->kmc_events = k.kmc_events;
+    st->cascades = k.cascades;
+    st->cascade_flips = k.cascade_flips;
+    st->elastic_steps = k.elastic_steps;
+    st->ls_solves = k.ls_solves;
+    st->ls_iters = k.ls_iters;
+    st->ls_iters_fp32 = k.ls_iters_fp32;
+    st->terminated = k.terminated ? 1 : 0;
+    st->event_cap_reached = k.event_cap_reached ? 1 : 0;
+    st->time = time;
+}
+
+int mgk_sim_run(mgk_sim* s, const mgk_run_config* r, mgk_run_stats* stats) {
+    return guard([&]() -> int {
+        std::map<std::pair<int, int>, double> targets;
+        for (int t = 0; t < r->n_targets && t < 9; ++t) targets[{r->target_i[t], r->target_j[t]}] = r->target_val[t];
+        return s->with([&](auto& sim) {
+            sim.max_events = r->max_events;
+            sim.run_simulation(r->n_global_steps, r->step_size, {r->drv_i, r->drv_j}, targets, r->mixed_tol,
+                               r->mixed_max_iter, "none", "checkpoint", "none", true, false, false,
+                               "summary_log.txt", false, false, r->max_kmc_steps_pct);
+            fill_stats(sim.counters, sim.time, stats);
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_event_steps(mgk_sim* s, int n, int drv_i, int drv_j, mgk_run_stats* stats) {
+    return guard([&]() -> int {
+        return s->with([&](auto& sim) {
+            sim.counters = mgk_host::RunCounters{};
+            sim.event_steps(n, {drv_i, drv_j});
+            fill_stats(sim.counters, sim.time, stats);
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_elastic_run(mgk_sim* s) {
+    return guard([&]() -> int { return s->with([&](auto& sim) { return sim.elastic_run(); }); });
+}
+
+int mgk_sim_get_field(mgk_sim* s, int which, double* host) {
+    return guard([&]() -> int {
+        return s->with([&](auto& sim) {
+            mgk_host::check(mgk_sync(sim.ctx), "sync");
+            mgk_host::check(mgk_get_field(sim.ctx, which, host), "mgk_get_field");
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_set_field(mgk_sim* s, int which, const double* host) {
+    return guard([&]() -> int {
+        return s->with([&](auto& sim) {
+            mgk_host::check(mgk_set_field(sim.ctx, which, host), "mgk_set_field");
+            const long N = sim.N();
+            if (which == MGK_FIELD_Q0) std::memcpy(sim.Q0.data(), host, sizeof(double) * N * sim.M);
+            if (which == MGK_FIELD_CATALOG) std::memcpy(sim.catalog[0].data(), host, sizeof(sim.catalog[0]) * N * sim.M);
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_get_macro(mgk_sim* s, double* eps_macro, double* sig_macro, double* time) {
+    return guard([&]() -> int {
+        return s->with([&](auto& sim) {
+            const int F = (int)(sizeof(sim.eps_macro) / sizeof(double));
+            for (int k = 0; k < F; ++k) { eps_macro[k] = sim.eps_macro.a[k]; sig_macro[k] = sim.sig_macro.a[k]; }
+            *time = sim.time;
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_set_macro(mgk_sim* s, const double* eps_macro, double time) {
+    return guard([&]() -> int {
+        return s->with([&](auto& sim) {
+            const int F = (int)(sizeof(sim.eps_macro) / sizeof(double));
+            for (int k = 0; k < F; ++k) sim.eps_macro.a[k] = eps_macro[k];
+            sim.time = time;
+            return MGK_OK;
+        });
+    });
+}
+
+long long mgk_sim_num_events(mgk_sim* s) {
+    return s->with([&](auto& sim) { return (long long)sim.events.size(); });
+}
+
+int mgk_sim_get_events(mgk_sim* s, mgk_event_record* out, long long cap) {
+    return s->with([&](auto& sim) {
+        long long n = std::min<long long>(cap, (long long)sim.events.size());
+        for (long long i = 0; i < n; ++i) {
+            const auto& e = sim.events[i];
+            std::memset(&out[i], 0, sizeof(out[i]));
+            out[i].flat_idx = e.flat_idx;
+            out[i].is_instab = e.is_instab;
+            out[i].ls_iters = e.ls_iters;
+            out[i].time = e.time;
+            out[i].total_rate = e.total_rate;
+            std::memcpy(out[i].eps_macro, e.eps_macro, sizeof(e.eps_macro));
+            std::memcpy(out[i].sig_macro, e.sig_macro, sizeof(e.sig_macro));
+        }
+        return (int)n;
+    });
+}
+
+int mgk_sim_device_ctx(mgk_sim* s, mgk_ctx** out) {
+    *out = s->with([&](auto& sim) { return sim.ctx; });
+    return MGK_OK;
+}
+
+}  // extern "C"

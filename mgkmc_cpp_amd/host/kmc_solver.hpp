@@ -1,0 +1,728 @@
+// kmc_solver.hpp -- drop-in replacement of the reference's kmc_solver.hpp:
+// KmcSimulation2D / KmcSimulation3D with the same constructors, public
+// members and methods, whose per-voxel work runs on the MI355X through the C
+// ABI (include/mgkmc_gpu.h).
+//
+// What stays on the host, as in the reference: the std::mt19937 stream and
+// every draw from it (so a seed gives the same stream of u1, u2, catalog and
+// barrier values), the time / strain bookkeeping of run_simulation, the
+// cascade ordering (std::sort of the unstable list by Q), and the mixed
+// boundary-condition update.  What runs on the device: the LS solve
+// (elastic_run), update_barriers + rates + total rate, event selection,
+// apply_flip + redraw upload, and the unstable-mode compaction.
+//
+// Field members (eps_field, sig_field, Q, ...) are host mirrors of the device
+// state: sync_from_device() refreshes them (run_simulation does so before it
+// returns), sync_to_device() pushes edits made on the host.
+//
+// Extensions (defaults reproduce the reference): ls_tol / ls_max_iter
+// (hard-coded 1e-6 / 200 in elastic_run, kmc_solver.hpp:337), max_events
+// (stop after N accepted KMC events), precision (MGK_FP64 or
+// MGK_FP32_REFINE), and a fixed shear catalog (catalog_fixed).
+#pragma once
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/mgkmc_gpu.h"
+#include "linear_elastic_solver.hpp"
+#include "mgk_tensor.hpp"
+
+std::string get_current_timestamp();
+
+// BarrierGenerator (kmc_solver.hpp:24-96)
+struct BarrierGenerator {
+    std::string type;
+    double mean = 2.0;
+    double std_val = 0.6;
+    double min_cutoff = 0.1;
+    double max_cutoff = -1.0;
+    double loc = 1.0;
+    double scale = 0.5;
+    double epsilon = 0.1;
+    double ratio = 0.8;
+
+    double modified_rayleigh(std::mt19937& rng) const {
+        const double a = std::max(0.0, mean - 6.0 * std_val);
+        const double b = mean + 6.0 * std_val;
+        const double x_max = (mean + std::sqrt(mean * mean + 4.0 * std_val * std_val)) / 2.0;
+        const double c_max = x_max * std::exp(-std::pow(x_max - mean, 2.0) / (2.0 * std_val * std_val));
+        std::uniform_real_distribution<double> dist_x(a, b);
+        std::uniform_real_distribution<double> dist_u(0.0, c_max);
+        for (;;) {
+            const double x = dist_x(rng);
+            const double u = dist_u(rng);
+            if (u < x * std::exp(-std::pow(x - mean, 2.0) / (2.0 * std_val * std_val))) return x;
+        }
+    }
+
+    double generate(std::mt19937& rng) const {
+        double val = 0.0;
+        if (type == "gaussian") {
+            std::normal_distribution<double> dist(mean, std_val);
+            val = dist(rng);
+        } else if (type == "rayleigh") {
+            std::uniform_real_distribution<double> dist(0.0, 1.0);
+            const double u = dist(rng);
+            val = loc + scale * std::sqrt(-2.0 * std::log(1.0 - u));
+        } else if (type == "modified_rayleigh") {
+            val = modified_rayleigh(rng);
+        } else if (type == "modified_rayleigh_with_exponential") {
+            std::uniform_real_distribution<double> dist_u(0.0, 1.0);
+            if (dist_u(rng) < ratio) {
+                std::uniform_real_distribution<double> dist_exp(0.0, 1.0);
+                val = -epsilon * std::log(1.0 - dist_exp(rng));
+            } else {
+                val = modified_rayleigh(rng);
+            }
+        }
+        if (min_cutoff >= 0.0 && val < min_cutoff) val = min_cutoff;
+        if (max_cutoff >= 0.0 && val > max_cutoff) val = max_cutoff;
+        return val;
+    }
+};
+
+namespace mgk_host {
+
+template <int D> using Mat = mgkt::Matrix<D, D>;
+
+// Fixed shear catalog (extension).  2D: mode m is the pure shear at
+// theta_m = m*pi/M, (g/2)[[cos 2t, sin 2t], [sin 2t, -cos 2t]].  3D: the six
+// signed simple shears (xy, xz, yz) and six signed axial shears (xx-yy,
+// yy-zz, zz-xx), each of Frobenius norm g/sqrt(2) as stz_mode_glass_3d
+// normalises its modes (kmc_solver.hpp:1062-1073).
+template <int D> inline Mat<D> fixed_shear(int m, int M, double g) {
+    Mat<D> a = Mat<D>::Zero();
+    if constexpr (D == 2) {
+        const double t = M_PI * (double)m / (double)M;
+        const double c = std::cos(2.0 * t), s = std::sin(2.0 * t);
+        a(0, 0) = 0.5 * g * c;
+        a(1, 1) = -0.5 * g * c;
+        a(0, 1) = a(1, 0) = 0.5 * g * s;
+    } else {
+        const int k = m % 12;
+        const double sg = (k % 2 == 0) ? 1.0 : -1.0;
+        const int kind = k / 2;
+        if (kind < 3) {
+            const int i = (kind == 2) ? 1 : 0, j = (kind == 0) ? 1 : 2;
+            a(i, j) = a(j, i) = sg * 0.5 * g;
+        } else {
+            const int i = kind - 3, j = (kind - 2) % 3;
+            a(i, i) = sg * 0.5 * g;
+            a(j, j) = -sg * 0.5 * g;
+        }
+    }
+    return a;
+}
+
+struct RunCounters {
+    long long kmc_events = 0, cascades = 0, cascade_flips = 0, elastic_steps = 0;
+    long long ls_solves = 0, ls_iters = 0, ls_iters_fp32 = 0;
+    bool terminated = false, event_cap_reached = false;
+};
+
+struct EventRecord {
+    int flat_idx;
+    int is_instab;
+    int ls_iters;
+    double time;
+    double total_rate;
+    double eps_macro[9];
+    double sig_macro[9];
+};
+
+template <int D>
+class KmcSimulationT {
+public:
+    int nx, ny, nz;
+    int M;
+    double gamma0;
+    double pixel;
+    double volume;
+    std::string output_dir;
+
+    std::string softening_scheme;
+    double softening_cap;
+    double jp, jt;
+    double neighbor_softening_fraction;
+    double temperature;
+    double strain_rate;
+    double stability_threshold;
+    double nu0;
+    double q_act_temp;
+    std::string plane_mode;
+
+    bool fast_patching_enabled;
+    int patch_radius;
+    int sync_interval;
+    int flips_since_sync;
+
+    std::string instability_mode;
+    std::string cascade_timing;
+    bool scale_rate_by_volume;
+    bool redraw_directions;
+    bool redraw_barriers;
+    double tau;
+    bool use_3d_barriers;
+
+    std::vector<double> E_field;
+    std::vector<double> nu_field;
+    std::vector<Mat<D>> eps_field;
+    std::vector<Mat<D>> sig_field;
+    std::vector<Mat<D>> eps_plastic;
+    std::vector<mgkt::Vector2d> soft_prop;
+    std::vector<double> last_event_time;
+    Mat<D> eps_macro;
+    Mat<D> sig_macro;
+    double time;
+
+    std::vector<double> Q;
+    std::vector<double> Q0;
+    std::vector<Mat<D>> catalog;
+    std::vector<Mat<D>> prev_strain_dir;
+
+    BarrierGenerator barrier_gen;
+    std::mt19937 rng;
+
+    std::ofstream f_summary;
+    std::ofstream f_global;
+
+    // extensions
+    double ls_tol = 1e-6;
+    int ls_max_iter = 200;
+    long long max_events = -1;
+    bool catalog_fixed = false;
+    double catalog_magnitude = 0.1;
+    RunCounters counters;
+    std::vector<EventRecord> events;
+    bool record_events = false;
+
+    mgk_ctx* ctx = nullptr;
+
+    ~KmcSimulationT() {
+        close_logs();
+        if (ctx) mgk_destroy(ctx);
+    }
+    KmcSimulationT(const KmcSimulationT&) = delete;
+    KmcSimulationT& operator=(const KmcSimulationT&) = delete;
+
+    long N() const { return (long)nx * ny * nz; }
+
+protected:
+    // Common constructor body (kmc_solver.hpp:165-257, :1142-1232).
+    KmcSimulationT(int nx_, int ny_, int nz_, int M_, double gamma0_, const std::vector<double>& E_,
+                   const std::vector<double>& nu_, double pixel_, const BarrierGenerator& bg,
+                   std::string softening_scheme_, double cap, double jp_, double jt_, double nfrac, double q_act,
+                   std::string output_dir_, double T, double rate, double thr, double nu0_, std::string plane_mode_,
+                   bool fp_enabled, int fp_radius, int fp_sync, std::string instab, std::string ctiming,
+                   bool scale_vol, bool redraw_d, bool redraw_b, bool use3d, int seed, int precision,
+                   bool fixed_catalog, double fixed_mag)
+        : nx(nx_), ny(ny_), nz(nz_), M(M_), gamma0(gamma0_), pixel(pixel_), volume(pixel_ * pixel_ * pixel_),
+          output_dir(output_dir_), softening_scheme(softening_scheme_), softening_cap(cap), jp(jp_), jt(jt_),
+          neighbor_softening_fraction(nfrac), temperature(T), strain_rate(rate), stability_threshold(thr),
+          nu0(nu0_), q_act_temp(q_act), plane_mode(plane_mode_), fast_patching_enabled(fp_enabled),
+          patch_radius(fp_radius), sync_interval(fp_sync), flips_since_sync(0), instability_mode(instab),
+          cascade_timing(ctiming), scale_rate_by_volume(scale_vol), redraw_directions(redraw_d),
+          redraw_barriers(redraw_b), use_3d_barriers(use3d), barrier_gen(bg), rng(seed),
+          catalog_fixed(fixed_catalog), catalog_magnitude(fixed_mag) {
+        if (fast_patching_enabled)
+            throw std::runtime_error("fast patching (dynamics.fast_patching) is not built on the device path");
+        E_field = E_;
+        nu_field = nu_;
+        const long n = N();
+        eps_field.assign(n, Mat<D>::Zero());
+        sig_field.assign(n, Mat<D>::Zero());
+        eps_plastic.assign(n, Mat<D>::Zero());
+        soft_prop.assign(n, mgkt::Vector2d::Zero());
+        last_event_time.assign(n, -std::numeric_limits<double>::infinity());
+        eps_macro = Mat<D>::Zero();
+        sig_macro = Mat<D>::Zero();
+        time = 0.0;
+        if (temperature > 0.0) {
+            const double kB = 8.617e-5;
+            tau = 1.0 / (nu0 * std::exp(-q_act_temp / (kB * temperature)));
+        } else {
+            tau = std::numeric_limits<double>::infinity();
+        }
+        Q.assign(n * M, 0.0);
+        Q0.resize(n * M);
+        catalog.resize(n * M);
+        prev_strain_dir.assign(n, Mat<D>::Zero());
+        std::normal_distribution<double> standard_normal(0.0, 1.0);   // :223 (2D draws share it)
+        for (long v = 0; v < n; ++v)
+            for (int m = 0; m < M; ++m) {
+                const long mi = v * M + m;
+                Q0[mi] = barrier_gen.generate(rng);
+                catalog[mi] = draw_mode(D == 2 ? &standard_normal : nullptr, m);
+            }
+        check(mgk_create(D, nx, ny, nz, pixel, plane_code(plane_mode), precision, E_field.data(), nu_field.data(),
+                         &ctx),
+              "mgk_create");
+        mgk_kmc_params kp{};
+        kp.M = M;
+        kp.softening_directional = (softening_scheme == "directional") ? 1 : 0;
+        kp.softening_cap = softening_cap;
+        kp.jp = jp;
+        kp.jt = jt;
+        kp.neighbor_softening_fraction = neighbor_softening_fraction;
+        kp.tau = tau;
+        kp.volume = volume;
+        kp.temperature = temperature;
+        kp.nu0 = nu0;
+        kp.stability_threshold = stability_threshold;
+        kp.instability_kmc = (instability_mode == "kmc") ? 1 : 0;
+        kp.scale_rate_by_volume = scale_rate_by_volume ? 1 : 0;
+        check(mgk_kmc_init(ctx, &kp), "mgk_kmc_init");
+        check(mgk_set_field(ctx, MGK_FIELD_Q0, Q0.data()), "mgk_set_field(Q0)");
+        check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "mgk_set_field(catalog)");
+    }
+
+public:
+    // --- mode draws (kmc_solver.hpp:232-237, :1042-1074) -------------------
+    Mat<D> mode_planar(std::normal_distribution<double>& g) {
+        Mat<D> a = Mat<D>::Zero();
+        const double gxx = 0.5 * gamma0 * g(rng);
+        const double gxy = 0.5 * gamma0 * g(rng);
+        a(0, 0) = gxx;
+        a(1, 1) = -gxx;
+        a(0, 1) = gxy;
+        a(1, 0) = gxy;
+        return a;
+    }
+    Mat<D> mode_glass_3d() {
+        std::normal_distribution<double> gaussian(0.0, 1.0);
+        Mat<D> m;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) m(i, j) = gaussian(rng);
+        Mat<D> s = 0.5 * (m + m.transpose());
+        Mat<D> g = s - (1.0 / 3.0) * s.trace() * Mat<D>::Identity();
+        const double nrm = g.norm();
+        if (nrm < 1e-12) return Mat<D>::Zero();
+        g = g * (1.0 / (std::sqrt(2.0) * nrm));
+        std::uniform_real_distribution<double> ud(1e-15, 1.0);
+        const double u = ud(rng);
+        return g * (gamma0 * std::sqrt(-2.0 * std::log(u)));
+    }
+    Mat<D> draw_mode(std::normal_distribution<double>* shared, int m) {
+        if (catalog_fixed) return fixed_shear<D>(m, M, catalog_magnitude);
+        if constexpr (D == 3) {
+            if (use_3d_barriers) return mode_glass_3d();
+            std::normal_distribution<double> g(0.0, 1.0);   // fresh per mode (:1043)
+            return mode_planar(g);
+        } else {
+            return mode_planar(*shared);
+        }
+    }
+
+    // --- device <-> host mirrors -----------------------------------------
+    void sync_from_device() {
+        check(mgk_get_field(ctx, MGK_FIELD_EPS, eps_field[0].data()), "get eps");
+        check(mgk_get_field(ctx, MGK_FIELD_SIG, sig_field[0].data()), "get sig");
+        check(mgk_get_field(ctx, MGK_FIELD_EPS_PLASTIC, eps_plastic[0].data()), "get eps_plastic");
+        check(mgk_get_field(ctx, MGK_FIELD_SOFT, soft_prop[0].data()), "get soft_prop");
+        check(mgk_get_field(ctx, MGK_FIELD_LAST_EVENT_TIME, last_event_time.data()), "get last_event_time");
+        check(mgk_get_field(ctx, MGK_FIELD_Q, Q.data()), "get Q");
+        check(mgk_get_field(ctx, MGK_FIELD_Q0, Q0.data()), "get Q0");
+        check(mgk_get_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "get catalog");
+        check(mgk_get_field(ctx, MGK_FIELD_PREV_DIR, prev_strain_dir[0].data()), "get prev_strain_dir");
+    }
+    void sync_to_device() {
+        check(mgk_set_field(ctx, MGK_FIELD_EPS, eps_field[0].data()), "set eps");
+        check(mgk_set_field(ctx, MGK_FIELD_SIG, sig_field[0].data()), "set sig");
+        check(mgk_set_field(ctx, MGK_FIELD_EPS_PLASTIC, eps_plastic[0].data()), "set eps_plastic");
+        check(mgk_set_field(ctx, MGK_FIELD_SOFT, soft_prop[0].data()), "set soft_prop");
+        check(mgk_set_field(ctx, MGK_FIELD_LAST_EVENT_TIME, last_event_time.data()), "set last_event_time");
+        check(mgk_set_field(ctx, MGK_FIELD_Q0, Q0.data()), "set Q0");
+        check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "set catalog");
+        check(mgk_set_field(ctx, MGK_FIELD_PREV_DIR, prev_strain_dir[0].data()), "set prev_strain_dir");
+    }
+
+    // --- the reference's methods -------------------------------------------
+    // elastic_run (kmc_solver.hpp:334-339): eps_macro in, field mean out
+    int elastic_run() {
+        int it = 0, inner = 0;
+        double diff = 0.0;
+        check(mgk_elastic_run(ctx, eps_macro.data(), sig_macro.data(), ls_max_iter, ls_tol, &it, &diff),
+              "mgk_elastic_run");
+        mgk_last_solve_stats(ctx, nullptr, &inner);
+        counters.ls_solves++;
+        counters.ls_iters += it;
+        counters.ls_iters_fp32 += inner;
+        last_iters_ = it;
+        return it;
+    }
+
+    // update_barriers (kmc_solver.hpp:341-410) fused with the rate loop
+    // (:783-806): Q on the device, the FP64 total rate here.
+    void update_barriers() {
+        check(mgk_update_rates(ctx, time, &total_rate_, &n_unstable_, &n_included_), "mgk_update_rates");
+    }
+    double total_rate() const { return total_rate_; }
+    int n_unstable() const { return n_unstable_; }
+
+    // find_unstable_indices (kmc_solver.hpp:412-421); also fills unstable_q_
+    std::vector<int> find_unstable_indices() {
+        int n = 0;
+        check(mgk_find_unstable(ctx, nullptr, nullptr, 0, &n), "mgk_find_unstable");
+        std::vector<int> idx(n);
+        unstable_q_.assign(n, 0.0);
+        if (n > 0) check(mgk_find_unstable(ctx, idx.data(), unstable_q_.data(), n, &n), "mgk_find_unstable");
+        return idx;
+    }
+
+    void apply_flip(long flat) {
+        check(mgk_apply_flip(ctx, (int)flat, time, nullptr, nullptr, 0, 0), "mgk_apply_flip");
+    }
+
+    // redraw after a flip (kmc_solver.hpp:839-864, cascade :499-523) + flip
+    void flip_and_redraw(long voxel, long flat) {
+        std::normal_distribution<double> standard_normal(0.0, 1.0);   // fresh per event (:840, :499)
+        std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
+        redraw_cat_.resize((size_t)M);
+        redraw_q0_.resize((size_t)M);
+        int first = 0, n = 0;
+        bool has_cat = false, has_q0 = false;
+        if (redraw_directions || redraw_barriers) {
+            for (int mm = 0; mm < M; ++mm) {
+                const long mi = voxel * M + mm;
+                redraw_cat_[mm] = catalog[mi];
+                redraw_q0_[mm] = Q0[mi];
+                if (redraw_directions) redraw_cat_[mm] = draw_mode(sn, mm);
+                if (redraw_barriers) redraw_q0_[mm] = barrier_gen.generate(rng);
+            }
+            first = 0;
+            n = M;
+            has_cat = redraw_directions;
+            has_q0 = redraw_barriers;
+        } else {
+            redraw_cat_[0] = draw_mode(sn, (int)(flat % M));
+            redraw_q0_[0] = barrier_gen.generate(rng);
+            first = (int)(flat % M);
+            n = 1;
+            has_cat = has_q0 = true;
+        }
+        check(mgk_apply_flip(ctx, (int)flat, time, has_cat ? redraw_cat_[0].data() : nullptr,
+                             has_q0 ? redraw_q0_.data() : nullptr, first, n),
+              "mgk_apply_flip");
+        // keep the host copies of the redrawn entries current
+        for (int j = 0; j < n; ++j) {
+            const long mi = voxel * M + first + j;
+            if (has_cat) catalog[mi] = redraw_cat_[j];
+            if (has_q0) Q0[mi] = redraw_q0_[j];
+        }
+    }
+
+    // run_cascade (kmc_solver.hpp:464-564, :1473-1590), fast patching off
+    std::pair<int, int> run_cascade(int step, std::pair<int, int> component) {
+        (void)step;
+        (void)component;
+        int total_flips = 0, local_step = 0;
+        const long limit = N();
+        while (true) {
+            update_barriers();
+            if (n_unstable_ == 0) break;
+            std::vector<int> unstable = find_unstable_indices();
+            if (unstable.empty()) break;
+            std::map<int, double> qv;
+            for (size_t k = 0; k < unstable.size(); ++k) qv[unstable[k]] = unstable_q_[k];
+            std::sort(unstable.begin(), unstable.end(), [&](int a, int b) { return qv[a] < qv[b]; });
+            std::vector<bool> flipped(N(), false);
+            int batch = 0;
+            for (int mi : unstable) {
+                const long v = mi / M;
+                if (flipped[v]) continue;
+                flipped[v] = true;
+                batch++;
+                flip_and_redraw(v, mi);
+            }
+            elastic_run();
+            total_flips += batch;
+            local_step++;
+            if (local_step > limit) break;
+        }
+        return {local_step, total_flips};
+    }
+
+    void init_logs(std::string summary_filename, bool enable_summary_log, bool enable_global_log) {
+        if (!enable_summary_log && !enable_global_log) return;
+        std::filesystem::create_directories(output_dir);
+        if (enable_summary_log) {
+            f_summary.open(output_dir + "/" + summary_filename);
+            std::string header =
+                "Timestamp Elapsed(s) Step Type Eps_xx Sig_xx(GPa) KMC Cascade Flips SimTime(s)\n";
+            f_summary << header << std::string(header.length() - 1, '-') << "\n";
+        }
+        if (enable_global_log) {
+            f_global.open(output_dir + "/global_log.txt");
+            f_global << std::left << std::setw(10) << "GlobalStep" << " " << std::setw(12) << "ElasticStep" << " "
+                     << std::setw(10) << "KMCStep" << " " << std::setw(15) << "Eps_xx" << " " << std::setw(15)
+                     << "Eps_yy" << " " << std::setw(15) << "Eps_zz" << " " << std::setw(15) << "Eps_xy" << " "
+                     << std::setw(15) << "Eps_xz" << " " << std::setw(15) << "Eps_yz" << " " << std::setw(15)
+                     << "Sig_xx(GPa)" << " " << std::setw(15) << "Sig_yy(GPa)" << " " << std::setw(15)
+                     << "Sig_zz(GPa)" << " " << std::setw(15) << "Sig_xy(GPa)" << " " << std::setw(15)
+                     << "Sig_xz(GPa)" << " " << std::setw(15) << "Sig_yz(GPa)" << " " << std::setw(14)
+                     << "CascadeSteps" << " " << std::setw(17) << "TotalCascadeFlips" << " "
+                     << "SimTime(s)\n";
+        }
+    }
+    void close_logs() {
+        if (f_summary.is_open()) f_summary.close();
+        if (f_global.is_open()) f_global.close();
+    }
+
+    // do_logging (kmc_solver.hpp:606-719): the summary and global lines; VTU
+    // export is not built on the device path.
+    void do_logging(int global_step, int elastic_steps_done, int total_kmc_steps, std::string step_type,
+                    int cascade_steps, int cascade_flips, std::pair<int, int> component, bool enable_console_log,
+                    std::chrono::high_resolution_clock::time_point start_time_total) {
+        if (step_type == "INIT") return;
+        if (!f_summary.is_open() && !f_global.is_open() && !enable_console_log) return;
+        const double elapsed =
+            std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - start_time_total).count();
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "%-22s %-12.2f %-8d %-15s %-12.6f %-15.3f %-8d %-8d %-8d %-15.6e\n",
+                      get_current_timestamp().c_str(), elapsed, global_step, step_type.c_str(),
+                      eps_macro(component.first, component.second),
+                      sig_macro(component.first, component.second) / 1e9, total_kmc_steps, cascade_steps,
+                      cascade_flips, time);
+        if (f_summary.is_open()) f_summary << buf;
+        if (enable_console_log) std::cout << buf;
+        if (f_global.is_open()) {
+            double e[6] = {eps_macro(0, 0), eps_macro(1, 1), 0.0, eps_macro(0, 1), 0.0, 0.0};
+            double s[6] = {sig_macro(0, 0), sig_macro(1, 1), 0.0, sig_macro(0, 1), 0.0, 0.0};
+            if constexpr (D == 3) {
+                e[2] = eps_macro(2, 2); e[4] = eps_macro(0, 2); e[5] = eps_macro(1, 2);
+                s[2] = sig_macro(2, 2); s[4] = sig_macro(0, 2); s[5] = sig_macro(1, 2);
+            } else if (plane_mode == "plane_strain") {
+                double nsum = 0.0;
+                for (double v : nu_field) nsum += v;
+                s[2] = (nsum / nu_field.size()) * (s[0] + s[1]);
+            }
+            char g[1024];
+            std::snprintf(g, sizeof(g),
+                          "%-10d %-12d %-10d %-15.6f %-15.6f %-15.6f %-15.6f %-15.6f %-15.6f %-15.3f %-15.3f "
+                          "%-15.3f %-15.3f %-15.3f %-15.3f %-14d %-17d %-15.6e\n",
+                          global_step, elastic_steps_done, total_kmc_steps, e[0], e[1], e[2], e[3], e[4], e[5],
+                          s[0] / 1e9, s[1] / 1e9, s[2] / 1e9, s[3] / 1e9, s[4] / 1e9, s[5] / 1e9, cascade_steps,
+                          cascade_flips, time);
+            f_global << g;
+        }
+    }
+
+    // The accepted-event body (kmc_solver.hpp:811-937).  Returns is_instab.
+    int accept_event(double t_wait, const Mat<D>& strain_unit, std::uniform_real_distribution<double>& uniform_dist) {
+        time += t_wait;
+        eps_macro += strain_unit * (strain_rate * t_wait);
+        const double u2 = uniform_dist(rng);
+        int flat = -1;
+        double q_sel = 0.0;
+        check(mgk_select(ctx, u2, &flat, &q_sel), "mgk_select");
+        const long v = flat / M;
+        const int is_instab = (q_sel <= stability_threshold) ? 1 : 0;
+        flip_and_redraw(v, flat);
+        elastic_run();
+        if (record_events) {
+            EventRecord e{};
+            e.flat_idx = flat;
+            e.is_instab = is_instab;
+            e.ls_iters = last_iters_;
+            e.time = time;
+            e.total_rate = total_rate_;
+            for (int k = 0; k < D * D; ++k) { e.eps_macro[k] = eps_macro.a[k]; e.sig_macro[k] = sig_macro.a[k]; }
+            events.push_back(e);
+        }
+        return is_instab;
+    }
+
+    // run_simulation (kmc_solver.hpp:721-1037, :1735-2062), fast patching off.
+    void run_simulation(int n_global_steps, double step_size, std::pair<int, int> component,
+                        const std::map<std::pair<int, int>, double>& stress_targets, double mixed_tol = 1e-4,
+                        int mixed_max_iter = 50, std::string checkpoint_interval = "none",
+                        std::string checkpoint_path = "checkpoint", std::string vtk_interval = "none",
+                        bool vtk_elastic_only = true, bool track_cascades = false, bool enable_console_log = true,
+                        std::string summary_filename = "summary_log.txt", bool enable_summary_log = true,
+                        bool enable_global_log = true, double max_kmc_steps_pct = 0.3) {
+        (void)checkpoint_interval; (void)checkpoint_path; (void)vtk_elastic_only; (void)track_cascades;
+        if (vtk_interval != "none") std::cerr << " [mgkmc] VTU export is not built on the device path; skipped\n";
+        init_logs(summary_filename, enable_summary_log, enable_global_log);
+        const auto start = std::chrono::high_resolution_clock::now();
+        counters = RunCounters{};
+        int elastic_steps_done = 0, total_kmc_steps = 0, step = 1, seq = 0;
+        const int max_seq = static_cast<int>(max_kmc_steps_pct * N());
+        elastic_run();
+        Mat<D> su = Mat<D>::Zero();
+        su(component.first, component.second) = 1.0;
+        const double dt_step = std::abs(step_size) / strain_rate;
+        double remaining = 0.0;
+        std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
+        auto finish = [&]() {
+            counters.kmc_events = total_kmc_steps;
+            counters.elastic_steps = elastic_steps_done;
+            sync_from_device();
+            close_logs();
+        };
+        while (elastic_steps_done < n_global_steps) {
+            remaining += dt_step;
+            while (remaining > 0.0) {
+                update_barriers();
+                if (instability_mode == "cascade" && n_unstable_ > 0) {
+                    auto cs = run_cascade(step, component);
+                    if (cs.second > 0) {
+                        counters.cascades++;
+                        counters.cascade_flips += cs.second;
+                        do_logging(step, elastic_steps_done, total_kmc_steps, "CASCADE", cs.first, cs.second,
+                                   component, enable_console_log, start);
+                        step++;
+                    }
+                    continue;
+                }
+                if (total_rate_ > 0.0) {
+                    const double r_val = uniform_dist(rng);
+                    const double t_wait = -std::log(r_val) / total_rate_;
+                    if (t_wait < remaining) {
+                        remaining -= t_wait;
+                        const int inst = accept_event(t_wait, su, uniform_dist);
+                        seq = inst ? seq + 1 : 0;
+                        if (seq > max_seq) {
+                            std::cout << "\n[TERMINATE] KMC instability sequence limit reached! " << seq
+                                      << " steps." << std::endl;
+                            counters.terminated = true;
+                            finish();
+                            return;
+                        }
+                        do_logging(step, elastic_steps_done, total_kmc_steps, inst ? "KMC_INSTAB" : "KMC", 0, 1,
+                                   component, enable_console_log, start);
+                        total_kmc_steps++;
+                        step++;
+                        if (max_events > 0 && total_kmc_steps >= max_events) {
+                            counters.event_cap_reached = true;
+                            finish();
+                            return;
+                        }
+                        continue;
+                    }
+                }
+                eps_macro += su * (strain_rate * remaining);
+                time += remaining;
+                remaining = 0.0;
+            }
+            // mixed boundary conditions (:955-987)
+            double E_sum = 0.0, nu_sum = 0.0;
+            for (double v : E_field) E_sum += v;
+            for (double v : nu_field) nu_sum += v;
+            const double E_avg = E_sum / E_field.size(), nu_avg = nu_sum / nu_field.size();
+            for (int it = 0; it < mixed_max_iter; ++it) {
+                elastic_run();
+                Mat<D> err = Mat<D>::Zero();
+                double err_max = 0.0;
+                for (const auto& t : stress_targets) {
+                    const double e = t.second - sig_macro(t.first.first, t.first.second);
+                    err(t.first.first, t.first.second) = e;
+                    err_max = std::max(err_max, std::abs(e));
+                }
+                if (err_max < mixed_tol) break;
+                const double tr_sig = err.trace();
+                const Mat<D> d_eps = (err - nu_avg * tr_sig * Mat<D>::Identity()) / E_avg;
+                for (const auto& t : stress_targets)
+                    eps_macro(t.first.first, t.first.second) += d_eps(t.first.first, t.first.second);
+            }
+            elastic_steps_done++;
+            do_logging(step, elastic_steps_done, total_kmc_steps, "ELASTIC", 0, 0, component, enable_console_log,
+                       start);
+            step++;
+        }
+        finish();
+        const double dur = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - start).count();
+        if (enable_console_log) std::cout << "\nTotal Duration: " << std::fixed << std::setprecision(2) << dur << " seconds\n";
+    }
+
+    // Forced event steps: the accepted-event body without the remaining-time
+    // test (the per-event hot path used for throughput measurement).
+    void event_steps(int count, std::pair<int, int> component) {
+        Mat<D> su = Mat<D>::Zero();
+        su(component.first, component.second) = 1.0;
+        std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
+        for (int s = 0; s < count; ++s) {
+            update_barriers();
+            if (instability_mode == "cascade" && n_unstable_ > 0) {
+                auto cs = run_cascade(0, component);
+                if (cs.second > 0) { counters.cascades++; counters.cascade_flips += cs.second; }
+                continue;
+            }
+            if (!(total_rate_ > 0.0)) break;
+            const double r_val = uniform_dist(rng);
+            const double t_wait = -std::log(r_val) / total_rate_;
+            accept_event(t_wait, su, uniform_dist);
+            counters.kmc_events++;
+        }
+    }
+
+protected:
+    double total_rate_ = 0.0;
+    int n_unstable_ = 0, n_included_ = 0;
+    int last_iters_ = 0;
+    std::vector<double> unstable_q_;
+    std::vector<Mat<D>> redraw_cat_;
+    std::vector<double> redraw_q0_;
+};
+
+}  // namespace mgk_host
+
+// KmcSimulation2D (kmc_solver.hpp:98-1038): same constructor signature.
+class KmcSimulation2D : public mgk_host::KmcSimulationT<2> {
+public:
+    KmcSimulation2D(int nx_, int ny_, int M_, double gamma0_, const std::vector<double>& E_field_,
+                    const std::vector<double>& nu_field_, double pixel_ = 1.0,
+                    const BarrierGenerator& barrier_gen_ = BarrierGenerator(),
+                    std::string softening_scheme_ = "isotropic", double softening_cap_ = 2.0, double jp_ = 10.0,
+                    double jt_ = 30.0, double neighbor_softening_fraction_ = 0.0, double q_act_temp_ = 0.37,
+                    std::string output_dir_ = "output", double temperature_ = 0.0, double strain_rate_ = 1.0,
+                    double stability_threshold_ = 0.0, double nu0_ = 1e13,
+                    std::string plane_mode_ = "plane_strain", bool fast_patching_enabled_ = false,
+                    int patch_radius_ = 5, int sync_interval_ = 100, std::string instability_mode_ = "cascade",
+                    std::string cascade_timing_ = "none", bool scale_rate_by_volume_ = true,
+                    bool redraw_directions_ = true, bool redraw_barriers_ = true, int seed_ = 42,
+                    int precision_ = MGK_FP64, bool catalog_fixed_ = false, double catalog_magnitude_ = 0.1)
+        : KmcSimulationT<2>(nx_, ny_, 1, M_, gamma0_, E_field_, nu_field_, pixel_, barrier_gen_, softening_scheme_,
+                            softening_cap_, jp_, jt_, neighbor_softening_fraction_, q_act_temp_, output_dir_,
+                            temperature_, strain_rate_, stability_threshold_, nu0_, plane_mode_,
+                            fast_patching_enabled_, patch_radius_, sync_interval_, instability_mode_,
+                            cascade_timing_, scale_rate_by_volume_, redraw_directions_, redraw_barriers_, false,
+                            seed_, precision_, catalog_fixed_, catalog_magnitude_) {}
+    void apply_flip_soa_2d(int x, int y, int m) { apply_flip(((long)x * ny + y) * M + m); }
+};
+
+// KmcSimulation3D (kmc_solver.hpp:1040-2063): same constructor signature.
+class KmcSimulation3D : public mgk_host::KmcSimulationT<3> {
+public:
+    KmcSimulation3D(int nx_, int ny_, int nz_, int M_, double gamma0_, const std::vector<double>& E_field_,
+                    const std::vector<double>& nu_field_, double pixel_ = 1.0,
+                    const BarrierGenerator& barrier_gen_ = BarrierGenerator(),
+                    std::string softening_scheme_ = "isotropic", double softening_cap_ = 2.0, double jp_ = 10.0,
+                    double jt_ = 30.0, double neighbor_softening_fraction_ = 0.0, double q_act_temp_ = 0.37,
+                    std::string output_dir_ = "output", double temperature_ = 0.0, double strain_rate_ = 1.0,
+                    double stability_threshold_ = 0.0, double nu0_ = 1e13, bool fast_patching_enabled_ = false,
+                    int patch_radius_ = 5, int sync_interval_ = 100, std::string instability_mode_ = "cascade",
+                    std::string cascade_timing_ = "none", bool scale_rate_by_volume_ = true,
+                    bool redraw_directions_ = true, bool redraw_barriers_ = true, bool use_3d_barriers_ = false,
+                    int seed_ = 42, int precision_ = MGK_FP64, bool catalog_fixed_ = false,
+                    double catalog_magnitude_ = 0.1)
+        : KmcSimulationT<3>(nx_, ny_, nz_, M_, gamma0_, E_field_, nu_field_, pixel_, barrier_gen_,
+                            softening_scheme_, softening_cap_, jp_, jt_, neighbor_softening_fraction_, q_act_temp_,
+                            output_dir_, temperature_, strain_rate_, stability_threshold_, nu0_, "plane_strain",
+                            fast_patching_enabled_, patch_radius_, sync_interval_, instability_mode_,
+                            cascade_timing_, scale_rate_by_volume_, redraw_directions_, redraw_barriers_,
+                            use_3d_barriers_, seed_, precision_, catalog_fixed_, catalog_magnitude_) {}
+    void apply_flip_soa_3d(int x, int y, int z, int m) { apply_flip((((long)x * ny + y) * nz + z) * M + m); }
+};

@@ -1,0 +1,299 @@
+"""Python mirror of the reference interface for the hot path, over the C ABI.
+
+* ``spectral_solver_2d`` / ``spectral_solver_3d`` follow
+  linear_elastic_solver.hpp:146-160 / :431-444 (tensor fields are numpy
+  arrays of shape (N, D, D); the Gamma argument is not taken: the device
+  evaluates the Green operator on the fly).
+* ``Context`` is one device context (mgk_ctx): device-resident fields and the
+  per-event KMC primitives.
+* ``Simulation`` is the native drop-in KmcSimulation2D/3D (include/mgkmc_host.h).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import DP, IP, P, check, check_host, gpu, host
+
+_PLANE = {"plane_strain": abi.MGK_PLANE_STRAIN, "plane_stress": abi.MGK_PLANE_STRESS}
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP) if a is not None else None
+
+
+def _aos(x, N, D):
+    """(N, D, D) row-major numpy -> contiguous column-major AoS doubles."""
+    x = np.asarray(x, dtype=np.float64).reshape(N, D, D)
+    return np.ascontiguousarray(np.transpose(x, (0, 2, 1))).reshape(-1)
+
+
+def _from_aos(buf, N, D):
+    return np.ascontiguousarray(np.transpose(buf.reshape(N, D, D), (0, 2, 1)))
+
+
+def _mat_aos(m, D):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(D, D).T).reshape(-1)
+
+
+def _mat_from(buf, D):
+    return np.asarray(buf, dtype=np.float64).reshape(D, D).T.copy()
+
+
+class Context:
+    """One trajectory's device state (mgk_ctx)."""
+
+    def __init__(self, shape, E, nu, pixel=1.0, plane_mode="plane_strain", precision=abi.MGK_FP64):
+        self.shape = tuple(int(s) for s in shape)
+        self.dims = len(self.shape)
+        self.D = self.dims
+        self.N = int(np.prod(self.shape))
+        E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
+        nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(-1))
+        if E.size != self.N or nu.size != self.N:
+            raise ValueError("E and nu must have one value per voxel")
+        pm = _PLANE[plane_mode] if isinstance(plane_mode, str) else int(plane_mode)
+        nx, ny = self.shape[0], self.shape[1]
+        nz = self.shape[2] if self.dims == 3 else 1
+        h = P()
+        check(gpu().mgk_create(self.dims, nx, ny, nz, float(pixel), pm, int(precision), _dp(E), _dp(nu), C.byref(h)))
+        self._h = h
+        self.M = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            gpu().mgk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, precision):
+        check(gpu().mgk_set_precision(self._h, int(precision)))
+
+    # ---- LS solve with host arrays (spectral_solver_2d/3d) ------------------
+    def ls_solve(self, eps_bar, eps=None, eps_plastic=None, max_iter=200, tol=1e-6, want_sig=True):
+        D, N = self.D, self.N
+        bar = _mat_aos(eps_bar, D)
+        warm = eps is not None
+        eps_buf = _aos(eps, N, D) if warm else np.zeros(N * D * D)
+        ep = _aos(eps_plastic, N, D) if eps_plastic is not None else None
+        sig = np.zeros(N * D * D) if want_sig else None
+        em = np.zeros(D * D)
+        sm = np.zeros(D * D)
+        it = C.c_int(0)
+        diff = C.c_double(0.0)
+        check(gpu().mgk_ls_solve(self._h, _dp(bar), _dp(eps_buf), 1 if warm else 0, _dp(ep), _dp(sig), _dp(em),
+                                 _dp(sm), int(max_iter), float(tol), C.byref(it), C.byref(diff)))
+        return dict(eps=_from_aos(eps_buf, N, D), sig=_from_aos(sig, N, D) if want_sig else None,
+                    eps_macro=_mat_from(em, D), sig_macro=_mat_from(sm, D), iters=it.value, diff=diff.value)
+
+    # ---- device-resident fields --------------------------------------------
+    def _field_shape(self, which):
+        D, N, M = self.D, self.N, self.M
+        if which in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_PREV_DIR):
+            return (N, D, D), True
+        if which in (abi.FIELD_Q, abi.FIELD_Q0):
+            return (N * M,), False
+        if which == abi.FIELD_CATALOG:
+            return (N * M, D, D), True
+        if which == abi.FIELD_SOFT:
+            return (N, 2), False
+        return (N,), False
+
+    def get_field(self, which):
+        shape, tensor = self._field_shape(which)
+        buf = np.zeros(int(np.prod(shape)))
+        check(gpu().mgk_get_field(self._h, which, _dp(buf)))
+        if tensor:
+            return _from_aos(buf, shape[0], self.D)
+        return buf.reshape(shape)
+
+    def set_field(self, which, value):
+        shape, tensor = self._field_shape(which)
+        if tensor:
+            buf = _aos(value, shape[0], self.D)
+        else:
+            buf = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1))
+        check(gpu().mgk_set_field(self._h, which, _dp(buf)))
+
+    # ---- KMC primitives ----------------------------------------------------
+    def kmc_init(self, **kw):
+        p = abi.KmcParams()
+        vals = dict(M=6, softening_directional=0, softening_cap=2.0, jp=10.0, jt=30.0,
+                    neighbor_softening_fraction=0.0, tau=float("inf"), volume=1.0, temperature=0.0, nu0=1e13,
+                    stability_threshold=0.0, instability_kmc=0, scale_rate_by_volume=1)
+        vals.update(kw)
+        for k, v in vals.items():
+            setattr(p, k, v)
+        check(gpu().mgk_kmc_init(self._h, C.byref(p)))
+        self.M = int(p.M)
+
+    def elastic_run(self, eps_macro, max_iter=200, tol=1e-6):
+        D = self.D
+        em = _mat_aos(eps_macro, D)
+        sm = np.zeros(D * D)
+        it = C.c_int(0)
+        diff = C.c_double(0.0)
+        check(gpu().mgk_elastic_run(self._h, _dp(em), _dp(sm), int(max_iter), float(tol), C.byref(it), C.byref(diff)))
+        return dict(eps_macro=_mat_from(em, D), sig_macro=_mat_from(sm, D), iters=it.value, diff=diff.value)
+
+    def update_rates(self, time):
+        tot = C.c_double(0.0)
+        nu = C.c_int(0)
+        ni = C.c_int(0)
+        check(gpu().mgk_update_rates(self._h, float(time), C.byref(tot), C.byref(nu), C.byref(ni)))
+        return tot.value, nu.value, ni.value
+
+    def select(self, u2):
+        idx = C.c_int(-1)
+        q = C.c_double(0.0)
+        check(gpu().mgk_select(self._h, float(u2), C.byref(idx), C.byref(q)))
+        return idx.value, q.value
+
+    def apply_flip(self, flat, time, new_catalog=None, new_Q0=None, first_mode=0):
+        D = self.D
+        cat = None
+        n = 0
+        if new_catalog is not None:
+            new_catalog = np.asarray(new_catalog, dtype=np.float64)
+            n = new_catalog.shape[0]
+            cat = _aos(new_catalog, n, D)
+        q0 = None
+        if new_Q0 is not None:
+            q0 = np.ascontiguousarray(np.asarray(new_Q0, dtype=np.float64).reshape(-1))
+            n = q0.size
+        check(gpu().mgk_apply_flip(self._h, int(flat), float(time), _dp(cat), _dp(q0), int(first_mode), int(n)))
+
+    def find_unstable(self):
+        n = C.c_int(0)
+        check(gpu().mgk_find_unstable(self._h, None, None, 0, C.byref(n)))
+        idx = np.zeros(max(n.value, 1), dtype=np.int32)
+        q = np.zeros(max(n.value, 1))
+        if n.value:
+            check(gpu().mgk_find_unstable(self._h, idx.ctypes.data_as(IP), _dp(q), n.value, C.byref(n)))
+        return idx[: n.value], q[: n.value]
+
+    def sync(self):
+        check(gpu().mgk_sync(self._h))
+
+    def timer_start(self):
+        check(gpu().mgk_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = C.c_double(0.0)
+        check(gpu().mgk_timer_stop(self._h, C.byref(ms)))
+        return ms.value
+
+    def last_solve_stats(self):
+        a = C.c_int(0)
+        b = C.c_int(0)
+        check(gpu().mgk_last_solve_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+def spectral_solver_2d(nx, ny, E, nu, eps_bar, plane_mode, eps=None, max_iter=200, tol=1e-6, eps_plastic=None):
+    """spectral_solver_2d (linear_elastic_solver.hpp:146-309) on the device.
+    eps: None for a cold start, else the (N, 2, 2) warm-start field."""
+    ctx = Context((nx, ny), E, nu, plane_mode=plane_mode)
+    try:
+        return ctx.ls_solve(eps_bar, eps=eps, eps_plastic=eps_plastic, max_iter=max_iter, tol=tol)
+    finally:
+        ctx.close()
+
+
+def spectral_solver_3d(nx, ny, nz, E, nu, eps_bar, eps=None, max_iter=200, tol=1e-6, eps_plastic=None):
+    """spectral_solver_3d (linear_elastic_solver.hpp:431-593) on the device."""
+    ctx = Context((nx, ny, nz), E, nu)
+    try:
+        return ctx.ls_solve(eps_bar, eps=eps, eps_plastic=eps_plastic, max_iter=max_iter, tol=tol)
+    finally:
+        ctx.close()
+
+
+class Simulation:
+    """The native drop-in KmcSimulation2D/3D (libmgkmc_host.so)."""
+
+    def __init__(self, config, E, nu):
+        self.config = config
+        self.dims = config.dims
+        self.D = config.dims
+        shape = (config.nx, config.ny) + ((config.nz,) if config.dims == 3 else ())
+        self.N = int(np.prod(shape))
+        self.M = config.M
+        E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
+        nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(-1))
+        h = P()
+        check_host(host().mgk_sim_create(C.byref(config), _dp(E), _dp(nu), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            host().mgk_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, run_config):
+        st = abi.RunStats()
+        check_host(host().mgk_sim_run(self._h, C.byref(run_config), C.byref(st)))
+        return st
+
+    def event_steps(self, n, drv=(0, 0)):
+        st = abi.RunStats()
+        check_host(host().mgk_sim_event_steps(self._h, int(n), drv[0], drv[1], C.byref(st)))
+        return st
+
+    def elastic_run(self):
+        rc = host().mgk_sim_elastic_run(self._h)
+        if rc < 0:
+            check_host(rc)
+        return rc
+
+    def _shape(self, which):
+        return Context._field_shape(self, which)
+
+    def get_field(self, which):
+        shape, tensor = self._shape(which)
+        buf = np.zeros(int(np.prod(shape)))
+        check_host(host().mgk_sim_get_field(self._h, which, _dp(buf)))
+        return _from_aos(buf, shape[0], self.D) if tensor else buf.reshape(shape)
+
+    def set_field(self, which, value):
+        shape, tensor = self._shape(which)
+        buf = _aos(value, shape[0], self.D) if tensor else np.ascontiguousarray(
+            np.asarray(value, dtype=np.float64).reshape(-1))
+        check_host(host().mgk_sim_set_field(self._h, which, _dp(buf)))
+
+    def macro(self):
+        D = self.D
+        e = np.zeros(D * D)
+        s = np.zeros(D * D)
+        t = C.c_double(0.0)
+        check_host(host().mgk_sim_get_macro(self._h, _dp(e), _dp(s), C.byref(t)))
+        return _mat_from(e, D), _mat_from(s, D), t.value
+
+    def set_macro(self, eps_macro, time):
+        check_host(host().mgk_sim_set_macro(self._h, _dp(_mat_aos(eps_macro, self.D)), float(time)))
+
+    def events(self):
+        n = host().mgk_sim_num_events(self._h)
+        arr = (abi.EventRecord * max(n, 1))()
+        host().mgk_sim_get_events(self._h, arr, n)
+        return list(arr)[:n]
+
+    def device_context_handle(self):
+        h = P()
+        host().mgk_sim_device_ctx(self._h, C.byref(h))
+        return h

@@ -1,0 +1,282 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so, or the build against
+the reference's own pocketfft in oracle/_ref/liboracle_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py -- as the checker, never as the measured path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from mgkmc_cpp_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "liboracle_ref.so")
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+P = C.c_void_p
+
+_libs = {}
+
+
+def build(ref=None):
+    """Compile the oracle (and, when the reference tree is present, its
+    pocketfft variant).  Returns the list of built libraries."""
+    import subprocess
+    out = []
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    out.append(LIB)
+    ref_dir = ref or "/root/reference"
+    if os.path.exists(os.path.join(ref_dir, "third_party", "pocketfft_hdronly.h")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "REF_DIR=" + ref_dir], check=True)
+        out.append(REF_LIB)
+    return out
+
+
+def _sig(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+
+
+def lib(ref=False):
+    path = REF_LIB if ref else LIB
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise ImportError(f"{path} not built (make -C oracle{' ref' if ref else ''})")
+    L = C.CDLL(path)
+    _sig(L, "orc_fft_c2c", None, [C.c_int, IP, C.c_int, DP, DP, C.c_double])
+    _sig(L, "orc_fftfreq", None, [C.c_int, C.c_double, DP])
+    _sig(L, "orc_lame", None, [C.c_int, C.c_double, C.c_double, C.c_int, DP, DP])
+    _sig(L, "orc_green", None, [C.c_int, IP, C.c_double, C.c_double, C.c_double, DP])
+    _sig(L, "orc_ls_solve", C.c_int, [C.c_int, IP, C.c_double, C.c_int, DP, DP, DP, DP, C.c_int, DP, DP, DP, DP,
+                                      C.c_int, C.c_double, DP])
+    _sig(L, "orc_correlated_field", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, DP])
+    _sig(L, "orc_rng_stream", None, [C.c_int, C.c_int, C.c_int, DP])
+    _sig(L, "orc_barrier_draws", None, [C.POINTER(abi.SimConfig), C.c_int, DP])
+    _sig(L, "orc_sim_create", C.c_int, [C.POINTER(abi.SimConfig), DP, DP, C.POINTER(P)])
+    _sig(L, "orc_sim_destroy", C.c_int, [P])
+    _sig(L, "orc_sim_run", C.c_int, [P, C.POINTER(abi.RunConfig), C.POINTER(abi.RunStats)])
+    _sig(L, "orc_sim_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
+    _sig(L, "orc_sim_elastic_run", C.c_int, [P])
+    _sig(L, "orc_sim_update_barriers", C.c_int, [P])
+    _sig(L, "orc_sim_rates", C.c_int, [P, DP, IP, DP, IP])
+    _sig(L, "orc_select_sequential", C.c_int, [DP, C.c_int, C.c_double])
+    _sig(L, "orc_sim_apply_flip", C.c_int, [P, C.c_int, C.c_double])
+    _sig(L, "orc_sim_get_field", C.c_int, [P, C.c_int, DP])
+    _sig(L, "orc_sim_set_field", C.c_int, [P, C.c_int, DP])
+    _sig(L, "orc_sim_get_macro", C.c_int, [P, DP, DP, DP])
+    _sig(L, "orc_sim_set_macro", C.c_int, [P, DP, C.c_double])
+    _sig(L, "orc_sim_tau", C.c_double, [P])
+    _sig(L, "orc_sim_num_events", C.c_longlong, [P])
+    _sig(L, "orc_sim_get_events", C.c_int, [P, C.POINTER(abi.EventRecord), C.c_longlong])
+    _sig(L, "orc_fixed_mode", None, [C.c_int, C.c_int, C.c_int, C.c_double, DP])
+    _libs[path] = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP) if a is not None else None
+
+
+def _aos(x, N, D):
+    x = np.asarray(x, dtype=np.float64).reshape(N, D, D)
+    return np.ascontiguousarray(np.transpose(x, (0, 2, 1))).reshape(-1)
+
+
+def _from_aos(buf, N, D):
+    return np.ascontiguousarray(np.transpose(buf.reshape(N, D, D), (0, 2, 1)))
+
+
+def _mat_aos(m, D):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(D, D).T).reshape(-1)
+
+
+def fft_c2c(x, forward=True, fct=None, ref=False):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    shape = (C.c_int * x.ndim)(*x.shape)
+    if fct is None:
+        fct = 1.0 / np.sqrt(x.size)
+    lib(ref).orc_fft_c2c(x.ndim, shape, 1 if forward else 0, x.ctypes.data_as(DP), out.ctypes.data_as(DP), fct)
+    return out
+
+
+def green(shape, pixel, lam0, mu0):
+    D = len(shape)
+    N = int(np.prod(shape))
+    out = np.zeros(N * D ** 4)
+    n = (C.c_int * D)(*shape)
+    lib().orc_green(D, n, pixel, lam0, mu0, _dp(out))
+    return out.reshape((N,) + (D,) * 4)
+
+
+def lame(dims, E, nu, plane_stress=False):
+    a = C.c_double(0)
+    b = C.c_double(0)
+    lib().orc_lame(dims, E, nu, 1 if plane_stress else 0, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def ls_solve(shape, E, nu, eps_bar, eps=None, eps_plastic=None, max_iter=200, tol=1e-6, plane_stress=False,
+             pixel=1.0, ref=False):
+    """spectral_solver_2d/3d restated on the CPU; same return dict as the
+    device Context.ls_solve."""
+    D = len(shape)
+    N = int(np.prod(shape))
+    n = (C.c_int * D)(*shape)
+    E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
+    nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(-1))
+    bar = _mat_aos(eps_bar, D)
+    warm = eps is not None
+    eb = _aos(eps, N, D) if warm else np.zeros(N * D * D)
+    ep = _aos(eps_plastic, N, D) if eps_plastic is not None else None
+    sig = np.zeros(N * D * D)
+    em = np.zeros(D * D)
+    sm = np.zeros(D * D)
+    diff = C.c_double(0)
+    it = lib(ref).orc_ls_solve(D, n, pixel, 1 if plane_stress else 0, _dp(E), _dp(nu), _dp(bar), _dp(eb),
+                               1 if warm else 0, _dp(ep), _dp(sig), _dp(em), _dp(sm), int(max_iter), float(tol),
+                               C.byref(diff))
+    return dict(eps=_from_aos(eb, N, D), sig=_from_aos(sig, N, D), eps_macro=em.reshape(D, D).T.copy(),
+                sig_macro=sm.reshape(D, D).T.copy(), iters=it, diff=diff.value)
+
+
+def correlated_field(shape, mean, std, corr, seed, clip_min=None, clip_max=None):
+    """generate_correlated_field_cpp (material_generator.hpp:130-258)."""
+    is3d = len(shape) == 3
+    nx, ny = shape[0], shape[1]
+    nz = shape[2] if is3d else 1
+    out = np.zeros(nx * ny * nz)
+    lib().orc_correlated_field(nx, ny, nz, 1 if is3d else 0, mean, std, corr, clip_min is not None,
+                               clip_min or 0.0, clip_max is not None, clip_max or 0.0, seed, _dp(out))
+    return out
+
+
+def rng_stream(seed, kind, k):
+    out = np.zeros(k)
+    lib().orc_rng_stream(seed, kind, k, _dp(out))
+    return out
+
+
+def barrier_draws(config, k):
+    out = np.zeros(k)
+    lib().orc_barrier_draws(C.byref(config), k, _dp(out))
+    return out
+
+
+def fixed_mode(dims, m, M, g):
+    out = np.zeros(dims * dims)
+    lib().orc_fixed_mode(dims, m, M, g, _dp(out))
+    return out.reshape(dims, dims).T.copy()
+
+
+class OracleSim:
+    """KmcSimulation2D/3D restated on the CPU (same surface as
+    mgkmc_cpp_amd.Simulation)."""
+
+    def __init__(self, config, E, nu, ref=False):
+        self.config = config
+        self.D = config.dims
+        shape = (config.nx, config.ny) + ((config.nz,) if config.dims == 3 else ())
+        self.N = int(np.prod(shape))
+        self.M = config.M
+        self._lib = lib(ref)
+        E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
+        nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(-1))
+        h = P()
+        self._lib.orc_sim_create(C.byref(config), _dp(E), _dp(nu), C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, run_config):
+        st = abi.RunStats()
+        self._lib.orc_sim_run(self._h, C.byref(run_config), C.byref(st))
+        return st
+
+    def event_steps(self, n, drv=(0, 0)):
+        st = abi.RunStats()
+        self._lib.orc_sim_event_steps(self._h, int(n), drv[0], drv[1], C.byref(st))
+        return st
+
+    def elastic_run(self):
+        return self._lib.orc_sim_elastic_run(self._h)
+
+    def update_barriers(self):
+        self._lib.orc_sim_update_barriers(self._h)
+
+    def rates(self):
+        nm = self.N * self.M
+        tot = C.c_double(0)
+        n = C.c_int(0)
+        r = np.zeros(nm)
+        idx = np.zeros(nm, dtype=np.int32)
+        self._lib.orc_sim_rates(self._h, C.byref(tot), C.byref(n), _dp(r), idx.ctypes.data_as(IP))
+        return tot.value, r[: n.value], idx[: n.value]
+
+    def apply_flip(self, flat, time):
+        self._lib.orc_sim_apply_flip(self._h, int(flat), float(time))
+
+    def _shape(self, which):
+        D, N, M = self.D, self.N, self.M
+        if which in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_PREV_DIR):
+            return (N, D, D), True
+        if which in (abi.FIELD_Q, abi.FIELD_Q0):
+            return (N * M,), False
+        if which == abi.FIELD_CATALOG:
+            return (N * M, D, D), True
+        if which == abi.FIELD_SOFT:
+            return (N, 2), False
+        return (N,), False
+
+    def get_field(self, which):
+        shape, tensor = self._shape(which)
+        buf = np.zeros(int(np.prod(shape)))
+        self._lib.orc_sim_get_field(self._h, which, _dp(buf))
+        return _from_aos(buf, shape[0], self.D) if tensor else buf.reshape(shape)
+
+    def set_field(self, which, value):
+        shape, tensor = self._shape(which)
+        buf = _aos(value, shape[0], self.D) if tensor else np.ascontiguousarray(
+            np.asarray(value, dtype=np.float64).reshape(-1))
+        self._lib.orc_sim_set_field(self._h, which, _dp(buf))
+
+    def macro(self):
+        D = self.D
+        e = np.zeros(D * D)
+        s = np.zeros(D * D)
+        t = C.c_double(0)
+        self._lib.orc_sim_get_macro(self._h, _dp(e), _dp(s), C.byref(t))
+        return e.reshape(D, D).T.copy(), s.reshape(D, D).T.copy(), t.value
+
+    def set_macro(self, eps_macro, time):
+        self._lib.orc_sim_set_macro(self._h, _dp(_mat_aos(eps_macro, self.D)), float(time))
+
+    def tau(self):
+        return self._lib.orc_sim_tau(self._h)
+
+    def events(self):
+        n = self._lib.orc_sim_num_events(self._h)
+        arr = (abi.EventRecord * max(n, 1))()
+        self._lib.orc_sim_get_events(self._h, arr, n)
+        return list(arr)[:n]
+
+
+def select_sequential(rates, target):
+    r = np.ascontiguousarray(rates, dtype=np.float64)
+    return lib().orc_select_sequential(_dp(r), r.size, float(target))
