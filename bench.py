@@ -1,0 +1,185 @@
+"""KMC events/s at 64^3 (3D heterogeneous LS-KMC, config c2), FP32+refine vs
+FP64, on one MI355X, with the HBM roofline of the dominant kernel and the
+CPU restatement timed on the host cores beside it.
+
+A step is one KMC event of the per-event hot path on a pre-loaded state:
+update_barriers + rates + FP64 total, u1 residence time, u2 selection, flip
++ redraw, and the warm-started Lippmann-Schwinger solve at tolerance 1e-8
+(the accepted-event branch of run_simulation, kmc_solver.hpp:1776-1965).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  Synthetic inputs: E is a Gaussian random field (mean
+85 GPa, CoV 0.15, correlation length 4 voxels, seed 42) generated on the host,
+nu = 0.35; 12 fixed deviatoric shears of magnitude 0.1; barriers N(1.0, 0.1)
+eV; T = 300 K; the state is pre-loaded at 1% uniaxial strain (eps_xx = 0.01).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "KMC events/s at 64^3, FP32+refine vs FP64; p50/p99 latency of mixed requests"
+SHAPE = (64, 64, 64)
+PRELOAD_STRAIN = 0.01
+
+
+def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
+    """Bytes per voxel one launch of a pipeline pass must move (read + write),
+    symmetric 6-component tensors, half spectrum nl/2+1 of nl bins."""
+    h = (nl // 2 + 1) / nl
+    cs = 8 if fp32 else 16           # complex bytes
+    spec = 6 * h * cs                # one half-spectrum tensor field per voxel
+    if pass_id == 0:
+        real_in = (6 * 4 + 2 * 4) if fp32 else (6 * 8 + 6 * 8 + 2 * 8)   # y, dlam, dmu | eps, eps_p, lam, mu
+        return real_in + spec
+    return 2 * spec                  # B, C, D: read and write the spectrum
+
+
+def _preload(sim, D):
+    e = np.zeros((D, D))
+    e[0, 0] = PRELOAD_STRAIN
+    sim.set_macro(e, 0.0)
+    sim.elastic_run()
+
+
+def run_gpu(precision, steps, warmup, E, nu):
+    from mgkmc_cpp_amd import Simulation, abi, configs
+    cfg = configs.sim_config(SHAPE, precision=precision, seed=42)
+    sim = Simulation(cfg, E, nu)
+    _preload(sim, 3)
+    if warmup:
+        sim.event_steps(warmup)
+    t0 = time.perf_counter()
+    st = sim.event_steps(steps)
+    dt = time.perf_counter() - t0
+    ev = st.kmc_events
+    info = dict(events=int(ev), seconds=dt, events_per_s=ev / dt if dt > 0 else 0.0,
+                ls_iters_per_event=st.ls_iters / max(st.ls_solves, 1),
+                fp32_iters_per_event=st.ls_iters_fp32 / max(st.ls_solves, 1),
+                cascades=int(st.cascades))
+    return sim, info
+
+
+def kernel_roofline(sim, info64, info32):
+    """Average launch time of each pipeline pass (HIP events on the context's
+    stream), weighted by launches per event; the heaviest is reported."""
+    nl = SHAPE[2]
+    N = int(np.prod(SHAPE))
+    rows = []
+    for fp32 in (False, True):
+        per_event = info32["fp32_iters_per_event"] if fp32 else info32["ls_iters_per_event"]
+        for pass_id, name in ((0, "k_fwd_last"), (1, "k_mid<fwd>"), (2, "k_first_gamma"), (3, "k_mid<inv>")):
+            ms = sim.time_ls_pass(pass_id, fp32=fp32, reps=20)
+            b = _algorithmic_bytes_per_voxel(pass_id, fp32, nl) * N
+            rows.append(dict(kernel=name + ("<float>" if fp32 else "<double>"), ms=ms, bytes=b,
+                             ms_per_event=ms * per_event))
+    top = max(rows, key=lambda r: r["ms_per_event"])
+    achieved = top["bytes"] / (top["ms"] * 1e-3) / 1e9
+    return top, rows, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+
+
+def cpu_baseline(E, nu, budget_s=15.0):
+    """The oracle (CPU restatement; FFTs through the reference's own pocketfft
+    when oracle/_ref is built) on one host core, same workload and state, for
+    a bounded number of events."""
+    from mgkmc_cpp_amd import abi, configs
+    from oracle import pyoracle as po
+    ref = os.path.exists(po.REF_LIB)
+    cfg = configs.sim_config(SHAPE, precision=abi.MGK_FP64, seed=42)
+    osim = po.OracleSim(cfg, E, nu, ref=ref)
+    e = np.zeros((3, 3))
+    e[0, 0] = PRELOAD_STRAIN
+    osim.set_macro(e, 0.0)
+    osim.elastic_run()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s and n < 50:
+        st = osim.event_steps(1)
+        n += int(st.kmc_events)
+        if st.kmc_events == 0:
+            break
+    dt = time.perf_counter() - t0
+    try:
+        cpu = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in cpu.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        model = ""
+    return dict(value=round(n / dt, 4), unit="events/s", cores=1, kind="port",
+                sample=f"{n} FP64 KMC events of the same 64^3 workload after the same 1% pre-load, "
+                       f"{dt:.1f} s on one core ({model}); FFTs via "
+                       f"{'the reference pocketfft (oracle/_ref)' if ref else 'the oracle FFT'}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+              "trajectories shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+    from mgkmc_cpp_amd import abi, configs
+    E, nu = configs.material(SHAPE, 4.0, 42)
+    print(f"[bench] FP64 run: {a.steps} events after {a.warmup} warm-up", file=sys.stderr, flush=True)
+    sim64, i64 = run_gpu(abi.MGK_FP64, a.steps, a.warmup, E, nu)
+    print(f"[bench] FP64 {i64}", file=sys.stderr, flush=True)
+    sim64.close()
+    print(f"[bench] FP32+refine run", file=sys.stderr, flush=True)
+    sim32, i32 = run_gpu(abi.MGK_FP32_REFINE, a.steps, a.warmup, E, nu)
+    print(f"[bench] FP32+refine {i32}", file=sys.stderr, flush=True)
+    top, rows, roof = kernel_roofline(sim32, i64, i32)
+    print(f"[bench] pass timings {json.dumps(rows)}", file=sys.stderr, flush=True)
+    roof["kernel"] = top["kernel"]
+    roof["bytes_per_launch"] = int(top["bytes"])
+    roof["avg_launch_ms"] = round(top["ms"], 5)
+    sim32.close()
+    cpu = None
+    if rank == 0 and not a.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(E, nu, a.cpu_budget)
+    value = i32["events_per_s"]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * i32["seconds"] / max(i32["events"], 1), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-refine",
+        "data": "synthetic",
+        "config": {"workload": "c2: 3D 64^3 heterogeneous E (CoV 0.15, corr 4), 12 fixed shears |0.1|, "
+                               "T=300K, LS tol 1e-8, one KMC event per step, FP32 iterations + FP64 refinement",
+                   "grid": list(SHAPE), "modes": 12, "ls_tol": 1e-8, "preload_strain_xx": PRELOAD_STRAIN},
+        "fp64_events_per_s": round(i64["events_per_s"], 3),
+        "fp32_refine_speedup_vs_fp64": round(value / i64["events_per_s"], 3) if i64["events_per_s"] else None,
+        "ls_iters_per_event": {"fp64": round(i64["ls_iters_per_event"], 2),
+                               "fp32_refine_fp64": round(i32["ls_iters_per_event"], 2),
+                               "fp32_refine_fp32": round(i32["fp32_iters_per_event"], 2)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -164,6 +164,15 @@ int mgk_sync(mgk_ctx* ctx);
 int mgk_timer_start(mgk_ctx* ctx);
 int mgk_timer_stop(mgk_ctx* ctx, double* ms);
 
+/* Measurement helper (no counterpart in the reference): average duration in
+ * ms of one pass of the LS pipeline (0: tau + r2c along the last axis,
+ * 1: forward c2c along the middle axis (3D), 2: c2c + Green operator +
+ * inverse c2c along the first axis, 3: inverse c2c along the middle axis),
+ * FP64 (fp32 == 0) or FP32, launched reps times back to back on the
+ * context's stream between two HIP events.  Leaves the spectral buffer
+ * undefined; the strain, stress and KMC fields are not touched. */
+int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms);
+
 /* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
 

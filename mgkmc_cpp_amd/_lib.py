@@ -58,6 +58,7 @@ def gpu():
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])
         _sig(lib, "mgk_last_solve_stats", C.c_int, [P, IP, IP])
+        _sig(lib, "mgk_time_ls_pass", C.c_int, [P, C.c_int, C.c_int, C.c_int, DP])
         _gpu = lib
     return _gpu
 

@@ -839,6 +839,31 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
     });
 }
 
+int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
+    return guarded([&]() -> int {
+        if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
+        if (pass < 0 || pass > 3 || (ctx->g.dims == 2 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        mgk_ctx& c = *ctx;
+        // a control block that lets every pass run (not done, shift off)
+        std::memset(c.hctl, 0, sizeof(Ctl));
+        c.hctl->iters = 1;
+        c.hctl->max_iter = 1 << 30;
+        MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
+        LsParams p = c.ls_params();
+        p.corr = nullptr;
+        const int mode = fp32 ? LS_DELTA : LS_FULL;
+        ls_enqueue_pass(p, pass, mode, c.stream);   // warm-up launch
+        MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+        for (int r = 0; r < reps; ++r) ls_enqueue_pass(p, pass, mode, c.stream);
+        MGK_HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+        MGK_HIP_CHECK(hipEventSynchronize(c.ev1));
+        float ms = 0.f;
+        MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        *avg_ms = (double)ms / reps;
+        return MGK_OK;
+    });
+}
+
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters) {
     return guarded([&]() -> int {
         if (!ctx) throw ArgError("null context");

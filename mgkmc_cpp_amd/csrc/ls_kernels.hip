@@ -694,6 +694,22 @@ void ls_enqueue_iteration(const LsParams& p, int mode, hipStream_t s) {
     MGK_HIP_CHECK(hipGetLastError());
 }
 
+void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
+    const bool full = (mode == LS_FULL);
+    if (p.g.C == 3) {
+        if (pass == 0) { if (full) launch_fwd_last<double, 3, LS_FULL>(p, s); else launch_fwd_last<float, 3, LS_DELTA>(p, s); }
+        else if (pass == 2) { if (full) launch_first_gamma<double, 3>(p, s, mode); else launch_first_gamma<float, 3>(p, s, mode); }
+        else throw ArgError("pass not present in 2D");
+    } else {
+        if (pass == 0) { if (full) launch_fwd_last<double, 6, LS_FULL>(p, s); else launch_fwd_last<float, 6, LS_DELTA>(p, s); }
+        else if (pass == 1) { if (full) launch_mid<double, -1>(p, s, mode); else launch_mid<float, -1>(p, s, mode); }
+        else if (pass == 2) { if (full) launch_first_gamma<double, 6>(p, s, mode); else launch_first_gamma<float, 6>(p, s, mode); }
+        else if (pass == 3) { if (full) launch_mid<double, +1>(p, s, mode); else launch_mid<float, +1>(p, s, mode); }
+        else throw ArgError("unknown pass");
+    }
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
 void ls_enqueue_start(const LsParams& p, bool warm, hipStream_t s) {
     const int nb = reduce_grid(p.g.N);
     if (p.g.C == 3) {

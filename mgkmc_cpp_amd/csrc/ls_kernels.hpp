@@ -35,5 +35,7 @@ int ls_partial_slots(const Geom& g);
 void ls_enqueue_start(const LsParams& p, bool warm, hipStream_t s);
 void ls_enqueue_iteration(const LsParams& p, int mode, hipStream_t s);
 void ls_enqueue_final(const LsParams& p, hipStream_t s);
+// one pass of the pipeline alone (0 A, 1 B, 2 C, 3 D), for kernel timing
+void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s);
 
 }  // namespace mgk

@@ -192,6 +192,11 @@ class Context:
         check(gpu().mgk_timer_stop(self._h, C.byref(ms)))
         return ms.value
 
+    def time_ls_pass(self, pass_id, fp32=False, reps=20):
+        ms = C.c_double(0.0)
+        check(gpu().mgk_time_ls_pass(self._h, int(pass_id), 1 if fp32 else 0, int(reps), C.byref(ms)))
+        return ms.value
+
     def last_solve_stats(self):
         a = C.c_int(0)
         b = C.c_int(0)
@@ -297,3 +302,9 @@ class Simulation:
         h = P()
         host().mgk_sim_device_ctx(self._h, C.byref(h))
         return h
+
+    def time_ls_pass(self, pass_id, fp32=False, reps=20):
+        ms = C.c_double(0.0)
+        check(gpu().mgk_time_ls_pass(self.device_context_handle(), int(pass_id), 1 if fp32 else 0, int(reps),
+                                     C.byref(ms)))
+        return ms.value

@@ -1,0 +1,169 @@
+"""KMC event primitives and trajectories on the device against the oracle.
+
+* fed the oracle's state, update_barriers / rates / total rate agree to
+  rounding and the selected event index is bit-exact for every u2 drawn;
+* adversarial targets on exact prefix boundaries (T = 0, rates exactly
+  V*nu0 or 0) go through the exact sequential fallback and still agree;
+* apply_flip agrees bit-exactly;
+* whole trajectories of the drop-in host driver reproduce the oracle's event
+  sequence, with stress within 1e-10."""
+import numpy as np
+import pytest
+
+from mgkmc_cpp_amd import Context, Simulation, abi
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def _cfg(**kw):
+    base = dict(dims=2, nx=16, ny=16, M=6, gamma0=0.14, b_mean=1.0, b_std=0.1, temperature=300.0, seed=42,
+                ls_tol=1e-8, catalog_mode=abi.CATALOG_FIXED, catalog_magnitude=0.1)
+    base.update(kw)
+    return abi.default_sim_config(**base)
+
+
+def _field(N, seed=3, cov=0.15):
+    rng = np.random.default_rng(seed)
+    return 85e9 * (1.0 + cov * rng.standard_normal(N)), np.full(N, 0.35)
+
+
+def _device_from_oracle(osim, c, E, nu):
+    shape = (c.nx, c.ny) + ((c.nz,) if c.dims == 3 else ())
+    ctx = Context(shape, E, nu, pixel=c.pixel)
+    tau = osim.tau()
+    ctx.kmc_init(M=c.M, softening_directional=c.softening_directional, softening_cap=c.softening_cap, jp=c.jp,
+                 jt=c.jt, neighbor_softening_fraction=c.neighbor_softening_fraction, tau=tau,
+                 volume=c.pixel ** 3, temperature=c.temperature, nu0=c.nu0,
+                 stability_threshold=c.stability_threshold, instability_kmc=c.instability_kmc,
+                 scale_rate_by_volume=c.scale_rate_by_volume)
+    for f in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_Q0, abi.FIELD_CATALOG,
+              abi.FIELD_SOFT, abi.FIELD_LAST_EVENT_TIME, abi.FIELD_PREV_DIR):
+        ctx.set_field(f, osim.get_field(f))
+    return ctx
+
+
+def _evolved_oracle(c, E, nu, events=25):
+    osim = po.OracleSim(c, E, nu)
+    osim.event_steps(events)
+    return osim
+
+
+@pytest.mark.parametrize("dims,directional", [(2, 0), (2, 1), (3, 1)])
+def test_rates_and_selection_fed_oracle_state(dims, directional):
+    kw = dict(dims=dims, softening_directional=directional, instability_kmc=1)
+    if dims == 3:
+        kw.update(nx=8, ny=8, nz=8, M=12)
+    c = _cfg(**kw)
+    N = c.nx * c.ny * (c.nz if dims == 3 else 1)
+    E, nu = _field(N)
+    osim = _evolved_oracle(c, E, nu)
+    ctx = _device_from_oracle(osim, c, E, nu)
+    _, _, t = osim.macro()
+    osim.update_barriers()
+    total_o, rates_o, idx_o = osim.rates()
+    total_d, n_unst, n_incl = ctx.update_rates(t)
+    assert n_incl == len(rates_o)
+    assert _rel(ctx.get_field(abi.FIELD_Q), osim.get_field(abi.FIELD_Q)) < 1e-14
+    assert abs(total_d - total_o) <= 1e-13 * total_o
+    rng = np.random.default_rng(11)
+    for u2 in rng.random(3000):
+        want = idx_o[po.select_sequential(rates_o, u2 * total_o)]
+        got, q = ctx.select(u2)
+        assert got == want, (u2, got, want)
+
+
+def test_selection_on_exact_boundaries():
+    # T = 0 and instability_mode "kmc": rates are exactly V*nu0 (Q <= 0) or 0,
+    # so prefix sums are exact and targets can sit on a boundary.
+    c = _cfg(nx=8, ny=8, M=6, temperature=0.0, instability_kmc=1, b_mean=0.0, b_std=1.0, b_min_cutoff=-10.0)
+    N = 64
+    E, nu = _field(N)
+    osim = po.OracleSim(c, E, nu)
+    ctx = _device_from_oracle(osim, c, E, nu)
+    osim.update_barriers()
+    total_o, rates_o, idx_o = osim.rates()
+    total_d, _, _ = ctx.update_rates(0.0)
+    assert total_d == total_o
+    nz = int((rates_o > 0).sum())
+    assert 0 < nz < len(rates_o)
+    us = [k / nz for k in range(nz + 1)] + [np.nextafter(k / nz, 1.0) for k in range(nz)] + [0.0]
+    for u2 in us:
+        if u2 >= 1.0:
+            continue
+        want = idx_o[po.select_sequential(rates_o, u2 * total_o)]
+        got, _ = ctx.select(u2)
+        assert got == want, (u2, got, want)
+
+
+def test_flip_and_redraw_match_oracle():
+    c = _cfg(nx=4, ny=4, M=3, neighbor_softening_fraction=0.3, catalog_mode=abi.CATALOG_RANDOM)
+    E, nu = _field(16)
+    osim = _evolved_oracle(c, E, nu, events=5)
+    ctx = _device_from_oracle(osim, c, E, nu)
+    for flat, t in ((7, 1.25), (0, 1.5), (47, 2.0)):
+        osim.apply_flip(flat, t)
+        ctx.apply_flip(flat, t)
+        for f in (abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_LAST_EVENT_TIME, abi.FIELD_PREV_DIR):
+            assert np.array_equal(ctx.get_field(f), osim.get_field(f)), f
+    # redraw upload: catalog / Q0 of the flipped voxel replaced after the flip used the old entry
+    cat = np.array([[[0.01, 0.02], [0.02, -0.01]]] * 3)
+    q0 = np.array([1.1, 1.2, 1.3])
+    ctx.apply_flip(5, 3.0, new_catalog=cat, new_Q0=q0, first_mode=0)
+    got_cat = ctx.get_field(abi.FIELD_CATALOG)[3:6]
+    assert np.array_equal(got_cat, cat) and np.array_equal(ctx.get_field(abi.FIELD_Q0)[3:6], q0)
+
+
+@pytest.mark.parametrize("kw,events", [
+    (dict(), 40),                                                          # 2D fixed catalog, kmc-free cascade mode
+    (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM), 40),        # random catalog, host-drawn redraws
+    (dict(softening_directional=1, neighbor_softening_fraction=0.2), 30),
+    (dict(dims=3, nx=8, ny=8, nz=8, M=12), 15),
+])
+def test_trajectory_matches_oracle(kw, events):
+    c = _cfg(**kw)
+    N = c.nx * c.ny * (c.nz if c.dims == 3 else 1)
+    E, nu = _field(N, seed=17)
+    osim = po.OracleSim(c, E, nu)
+    dsim = Simulation(c, E, nu)
+    osim.elastic_run()
+    dsim.elastic_run()
+    so = osim.event_steps(events)
+    sd = dsim.event_steps(events)
+    eo, ed = osim.events(), dsim.events()
+    assert len(eo) == len(ed) == so.kmc_events == sd.kmc_events
+    assert [e.flat_idx for e in ed] == [e.flat_idx for e in eo]
+    for a, b in zip(ed, eo):
+        assert a.ls_iters == b.ls_iters
+        assert abs(a.time - b.time) <= 1e-12 * abs(b.time)
+        assert _rel(np.array(a.sig_macro), np.array(b.sig_macro)) < 1e-10
+    assert _rel(dsim.get_field(abi.FIELD_SIG), osim.get_field(abi.FIELD_SIG)) < 1e-10
+    assert _rel(dsim.get_field(abi.FIELD_EPS_PLASTIC), osim.get_field(abi.FIELD_EPS_PLASTIC)) < 1e-15
+
+
+def test_run_simulation_with_mixed_bc_and_cascades():
+    # large strain steps drive avalanches: events, cascades (unstable sort,
+    # per-voxel dedupe, host RNG order) and the yy stress-free mixed BC loop.
+    # Oracle: 150 events, ~96 cascades, 5 elastic steps, < 1 s on one core.
+    c = _cfg(nx=16, ny=16, M=4, b_mean=1.3, b_std=0.1, b_min_cutoff=0.01, temperature=300.0,
+             catalog_mode=abi.CATALOG_RANDOM, gamma0=0.1, ls_tol=1e-8)
+    rng = np.random.default_rng(23)
+    E = 85e9 * (1 + 0.15 * rng.standard_normal(256))
+    nu = np.full(256, 0.35)
+    run = abi.default_run_config(n_global_steps=6, step_size=3e-3, targets={(1, 1): 0.0}, mixed_tol=1e6,
+                                 max_events=150)
+    osim = po.OracleSim(c, E, nu)
+    dsim = Simulation(c, E, nu)
+    so = osim.run(run)
+    sd = dsim.run(run)
+    assert so.cascades > 10 and so.elastic_steps > 0
+    assert (sd.kmc_events, sd.cascades, sd.cascade_flips, sd.elastic_steps) == \
+           (so.kmc_events, so.cascades, so.cascade_flips, so.elastic_steps)
+    assert [e.flat_idx for e in dsim.events()] == [e.flat_idx for e in osim.events()]
+    eo, so_, to = osim.macro()
+    ed, sd_, td = dsim.macro()
+    assert _rel(sd_, so_) < 1e-9 and abs(td - to) <= 1e-12 * abs(to)
