@@ -159,6 +159,19 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time,
  * Writes at most cap entries; n_out is the full count. */
 int mgk_find_unstable(mgk_ctx* ctx, int* idx_out, double* q_out, int cap, int* n_out);
 
+/* Selection and flip chained on the device with no host round trip: the
+ * accepted-event body of run_simulation from the u2 draw to the redraw
+ * (kmc_solver.hpp:816-864, :1828-1875), for hosts that draw the redraw
+ * values before knowing the selected voxel (the draws do not depend on it).
+ * The redraw covers modes [first_mode, first_mode + n_modes) of the
+ * selected voxel, or with first_mode = -1 and n_modes = 1 only the selected
+ * mode.  With a shared (fixed) catalog a redraw must reproduce it.  The
+ * chosen index is read back with mgk_last_selection, which syncs only if the
+ * next mgk_elastic_run has not already done so. */
+int mgk_select_flip(mgk_ctx* ctx, double u2, double time, const double* new_catalog,
+                    const double* new_Q0, int first_mode, int n_modes);
+int mgk_last_selection(mgk_ctx* ctx, int* flat_idx, double* q_sel);
+
 /* Stream synchronisation and timing of the context's stream (HIP events). */
 int mgk_sync(mgk_ctx* ctx);
 int mgk_timer_start(mgk_ctx* ctx);

@@ -54,6 +54,8 @@ def gpu():
         _sig(lib, "mgk_select", C.c_int, [P, C.c_double, IP, DP])
         _sig(lib, "mgk_apply_flip", C.c_int, [P, C.c_int, C.c_double, DP, DP, C.c_int, C.c_int])
         _sig(lib, "mgk_find_unstable", C.c_int, [P, IP, DP, C.c_int, IP])
+        _sig(lib, "mgk_select_flip", C.c_int, [P, C.c_double, C.c_double, DP, DP, C.c_int, C.c_int])
+        _sig(lib, "mgk_last_selection", C.c_int, [P, IP, DP])
         _sig(lib, "mgk_sync", C.c_int, [P])
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])

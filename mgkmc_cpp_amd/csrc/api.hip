@@ -95,7 +95,7 @@ struct mgk_ctx {
 
     // spectra and FP32 refinement buffers
     void* spec = nullptr;
-    float *y32 = nullptr, *corr = nullptr;
+    float *cg = nullptr, *corr = nullptr;   // CG vectors r, w, p, s (4 x C x N) and iterate x
     double* partials = nullptr;
     Ctl* ctl = nullptr;
     Ctl* hctl = nullptr;
@@ -109,7 +109,9 @@ struct mgk_ctx {
 
     // statistics
     int last_iters = 0, last_inner = 0, predicted = 4;
-    int inner_per_outer = 6;
+    int inner_per_outer = 12;
+    bool mean_valid = false;
+    double mean_eps[6] = {0, 0, 0, 0, 0, 0};
 
     // KMC
     bool kmc = false;
@@ -121,17 +123,19 @@ struct mgk_ctx {
     double* hkstage = nullptr;
     int* unst_idx = nullptr;
     double* unst_q = nullptr;
+    std::vector<double> shared_cat;   // [C][M] host copy of a shared catalog
+    bool kctl_pending = false;        // a device-side selection not yet copied back
 
     ~mgk_ctx() { release(); }
 
     void release() {
         auto f = [](void* p) { if (p) (void)hipFree(p); };
         auto fh = [](void* p) { if (p) (void)hipHostFree(p); };
-        f(E); f(nu); f(lam); f(mu); f(dlam32); f(dmu32); f(eps); f(sig); f(epsp); f(spec); f(y32); f(corr);
+        f(E); f(nu); f(lam); f(mu); f(dlam32); f(dmu32); f(eps); f(sig); f(epsp); f(spec); f(cg); f(corr);
         f(partials); f(ctl); f(dstage);
         for (int a = 0; a < 3; ++a) { f(tw64[a]); f(tw32[a]); }
         fh(hctl); fh(hstage);
-        f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.rates); f(K.bsum); f(K.bincl);
+        f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.bsum); f(K.bincl);
         f(K.bunst); f(K.bunst_off); f(K.binc); f(kctl); f(kstage); f(unst_idx); f(unst_q);
         fh(hkctl); fh(hkstage);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -174,8 +178,11 @@ struct mgk_ctx {
         p.mu0 = mu0;
         p.spec64 = spec;
         p.spec32 = spec;
-        p.r32 = (precision == MGK_FP32_REFINE) ? y32 : nullptr;
-        p.y32 = y32;
+        const long CN = (long)g.C * g.N;
+        p.r32 = (precision == MGK_FP32_REFINE) ? cg : nullptr;
+        p.w32 = cg + CN;
+        p.p32 = cg + 2 * CN;
+        p.s32 = cg + 3 * CN;
         p.corr = (precision == MGK_FP32_REFINE) ? corr : nullptr;
         p.dlam32 = dlam32;
         p.dmu32 = dmu32;
@@ -249,62 +256,84 @@ struct mgk_ctx {
         }
     }
 
-    // The LS solve on the device-resident fields.  eps_bar_sym / norm terms
-    // describe the imposed strain; returns iterations.
-    int solve(const double* eps_bar_sym, double norm_extra0, double diff_extra0, double norm_extra, bool warm,
-              int max_iter, double tol, double* diff_out) {
-        (void)diff_extra0;
+    // The LS solve on the device-resident fields (spectral_solver_*, warm
+    // start when `warm`).  eps_bar_sym is the imposed strain's symmetric part,
+    // norm_extra the ||eps||^2 contribution of its uniform antisymmetric part.
+    // Returns the FP64 iteration count.
+    int solve(const double* eps_bar_sym, double norm_extra, bool warm, int max_iter, double tol, double* diff_out) {
         Ctl& c = *hctl;
         std::memset(&c, 0, sizeof(Ctl));
         c.max_iter = max_iter;
         c.tol = tol;
         c.norm_extra = norm_extra;
         for (int k = 0; k < g.C; ++k) c.eps_bar[k] = eps_bar_sym[k];
-        (void)norm_extra0;
+        int start = LS_START_FILL;
+        if (warm && mean_valid) {
+            // the field is unchanged since the last solve: its mean is known
+            for (int k = 0; k < g.C; ++k) c.shift[k] = eps_bar_sym[k] - mean_eps[k];
+            start = LS_START_KEEP;
+        } else if (warm) {
+            start = LS_START_MEAN;
+        }
         MGK_HIP_CHECK(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         LsParams p = ls_params();
-        ls_enqueue_start(p, warm, stream);
-        int enqueued = 0;
+        ls_enqueue_start(p, start, stream);
+        const bool refine = (precision == MGK_FP32_REFINE);
+        const int J = inner_per_outer;
+        int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
+        bool final_done = false;
         if (max_iter > 0) {
-            int batch = std::min(max_iter, std::max(2, predicted));
+            int batch = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
             while (true) {
                 for (int b = 0; b < batch; ++b) {
-                    ls_enqueue_iteration(p, LS_FULL, stream);
-                    if (precision == MGK_FP32_REFINE)
-                        for (int j = 0; j < inner_per_outer; ++j) ls_enqueue_iteration(p, LS_DELTA, stream);
+                    const int k = enqueued + b;
+                    const int flags = ((k == 0 && warm) ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
+                    ls_enqueue_iteration(p, LS_FULL, flags, stream);
+                    if (refine)
+                        for (int j = 0; j < J; ++j)
+                            ls_enqueue_iteration(p, LS_CG, j == 0 ? LS_FLAG_CG_FIRST
+                                                          : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), stream);
                 }
                 enqueued += batch;
+                // the final pass runs with the batch: one host sync per solve
+                // when the prediction holds; it is repeated if more iterations follow
+                ls_enqueue_final(p, stream);
                 MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+                if (kctl_pending) {
+                    MGK_HIP_CHECK(hipMemcpyAsync(hkctl, kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, stream));
+                    kctl_pending = false;
+                }
                 MGK_HIP_CHECK(hipStreamSynchronize(stream));
-                if (hctl->done || enqueued >= max_iter) break;
-                batch = std::min(max_iter - enqueued, std::max(2, batch / 2));
+                if (hctl->done || enqueued >= max_iter) { final_done = true; break; }
+                batch = std::min(max_iter - enqueued, std::max(refine ? 1 : 2, batch / 2));
             }
         } else if (warm) {
             // no iteration: the reference still applies the warm-start shift (:181-191)
-            apply_shift_now();
+            ls_enqueue_start(p, LS_START_APPLY, stream);
         }
-        ls_enqueue_final(p, stream);
-        MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        if (!final_done) {
+            ls_enqueue_final(p, stream);
+            MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+            MGK_HIP_CHECK(hipStreamSynchronize(stream));
+        }
         last_iters = hctl->iters;
         last_inner = hctl->inner;
-        predicted = std::max(2, last_iters);
+        predicted = refine ? std::max(1, last_iters) : std::max(2, last_iters);
+        for (int k = 0; k < g.C; ++k) mean_eps[k] = hctl->mean_eps[k];
+        mean_valid = true;
+        if (refine) tune_inner(tol);
         if (diff_out) *diff_out = hctl->diff;
         return hctl->iters;
     }
 
-    void apply_shift_now() {
-        // eps += shift; done on the host via staging (only for max_iter == 0)
-        MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-        MGK_HIP_CHECK(hipStreamSynchronize(stream));
-        const long N = g.N;
-        double* h = stage_host((size_t)N * g.C);
-        MGK_HIP_CHECK(hipMemcpyAsync(h, eps, sizeof(double) * N * g.C, hipMemcpyDeviceToHost, stream));
-        MGK_HIP_CHECK(hipStreamSynchronize(stream));
-        for (int c = 0; c < g.C; ++c)
-            for (long v = 0; v < N; ++v) h[(size_t)c * N + v] += hctl->shift[c];
-        MGK_HIP_CHECK(hipMemcpyAsync(eps, h, sizeof(double) * N * g.C, hipMemcpyHostToDevice, stream));
-        MGK_HIP_CHECK(hipStreamSynchronize(stream));
+    // CG steps per refinement cycle for the next solve: the steps the last
+    // cycle used plus two (steps past convergence return at once).
+    void tune_inner(double tol) {
+        (void)tol;
+        const int cycles = std::max(1, hctl->iters);
+        const int per = (hctl->inner + cycles - 1) / cycles;
+        if (hctl->done && hctl->iters <= 1) inner_per_outer = std::max(4, std::min(40, per + 2));
+        else inner_per_outer = std::min(40, inner_per_outer + 4);
     }
 };
 
@@ -384,7 +413,7 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             MGK_HIP_CHECK(hipMemset(c->sig, 0, sizeof(double) * C * N));
             MGK_HIP_CHECK(hipMemset(c->epsp, 0, sizeof(double) * C * N));
             c->spec = dalloc<double>((size_t)2 * C * g.NS);
-            c->y32 = dalloc<float>((size_t)C * N);
+            c->cg = dalloc<float>((size_t)4 * C * N);
             c->corr = dalloc<float>((size_t)C * N);
             c->partials = dalloc<double>((size_t)ls_partial_slots(g) + 64);
             c->ctl = dalloc<Ctl>(1);
@@ -447,32 +476,20 @@ int mgk_ls_solve(mgk_ctx* ctx, const double* eps_bar, double* eps, int warm_star
             c.epsp_set = false;
         }
         std::vector<double> eskew;
-        double norm_extra0 = 0.0, diff_extra0 = 0.0;
+        // ||eps||^2 of the uniform antisymmetric part (each w appears at (i,j) and (j,i))
         double norm_extra = 0.0;
         for (int w = 0; w < ns; ++w) norm_extra += 2.0 * (double)N * bar_skew[w] * bar_skew[w];
         if (warm_start) {
             c.upload_tensor(eps, c.eps, &eskew);
-            // antisymmetric part after the warm-start shift: W_v + W_bar - mean(W)
-            double mw[3] = {0, 0, 0};
-            for (long v = 0; v < N; ++v)
-                for (int w = 0; w < ns; ++w) mw[w] += eskew[(size_t)v * ns + w];
-            for (int w = 0; w < ns; ++w) mw[w] /= N;
-            double extra_norm = 0.0, extra_diff = 0.0, uniform_norm = 0.0;
-            for (long v = 0; v < N; ++v)
-                for (int w = 0; w < ns; ++w) {
-                    const double sh = eskew[(size_t)v * ns + w] + bar_skew[w] - mw[w];
-                    extra_norm += 2.0 * sh * sh;
-                    const double d = bar_skew[w] - sh;
-                    extra_diff += 2.0 * d * d;
-                }
-            (void)uniform_norm;
-            norm_extra0 = extra_norm;
-            diff_extra0 = extra_diff;
-            const bool uniform = (extra_diff == 0.0);
-            if (!uniform)
-                throw ArgError("the antisymmetric part of a warm-start strain field must be uniform");
+            // The device keeps symmetric parts; the antisymmetric part after the
+            // warm-start shift is W_v + W_bar - mean(W), uniform only if W_v is.
+            for (long v = 1; v < N; ++v)
+                for (int w = 0; w < ns; ++w)
+                    if (eskew[(size_t)v * ns + w] != eskew[w])
+                        throw ArgError("the antisymmetric part of a warm-start strain field must be uniform");
         }
-        int it = c.solve(bar_sym, norm_extra0, diff_extra0, norm_extra, warm_start != 0, max_iter, tol, diff_out);
+        if (warm_start) c.mean_valid = false;   // a new field was uploaded
+        int it = c.solve(bar_sym, norm_extra, warm_start != 0, max_iter, tol, diff_out);
         if (iters_out) *iters_out = it;
         for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
         // outputs
@@ -530,6 +547,7 @@ int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
         std::vector<double> sk;
         switch (which) {
             case MGK_FIELD_EPS: {
+                c.mean_valid = false;
                 c.upload_tensor(host, c.eps, &sk);
                 for (int w = 0; w < ns; ++w) c.W[w] = sk.empty() ? 0.0 : sk[w];
                 for (long v = 0; v < N; ++v)
@@ -547,16 +565,25 @@ int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
             }
             case MGK_FIELD_PREV_DIR: c.upload_tensor(host, c.K.prev_dir, nullptr); break;
             case MGK_FIELD_CATALOG: {
-                // N*M tensors -> [C][N*M]
+                // N*M tensors -> [C][N*M]; when every voxel holds voxel 0's M
+                // strains, one shared [C][M] copy instead
                 const long nm = N * c.kp.M;
-                const int C = c.g.C, F = c.full();
-                std::vector<double> h((size_t)nm * C);
+                const int C = c.g.C, F = c.full(), M = c.kp.M;
+                bool shared = true;
+                for (long i = M; i < nm && shared; ++i)
+                    for (int k = 0; k < F; ++k)
+                        if (host[i * F + k] != host[(i % M) * F + k]) { shared = false; break; }
+                const long n = shared ? M : nm;
+                std::vector<double> h((size_t)n * C);
                 double sym[6], skw[3];
-                for (long i = 0; i < nm; ++i) {
+                for (long i = 0; i < n; ++i) {
                     c.split(host + i * F, sym, skw);
-                    for (int k = 0; k < C; ++k) h[(size_t)k * nm + i] = sym[k];
+                    for (int k = 0; k < C; ++k) h[(size_t)k * n + i] = sym[k];
                 }
                 MGK_HIP_CHECK(hipMemcpy(c.K.cat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+                c.K.cat_shared = shared ? 1 : 0;
+                if (shared) c.shared_cat = h;
+                else c.shared_cat.clear();
                 break;
             }
             case MGK_FIELD_Q: MGK_HIP_CHECK(hipMemcpy(c.K.Q, host, N * c.kp.M * sizeof(double), hipMemcpyHostToDevice)); break;
@@ -604,12 +631,14 @@ int mgk_get_field(mgk_ctx* ctx, int which, double* host) {
                 break;
             case MGK_FIELD_CATALOG: {
                 const long nm = N * c.kp.M;
-                const int C = c.g.C, F = c.full();
-                std::vector<double> h((size_t)nm * C);
+                const int C = c.g.C, F = c.full(), M = c.kp.M;
+                const long n = c.K.cat_shared ? M : nm;
+                std::vector<double> h((size_t)n * C);
                 MGK_HIP_CHECK(hipMemcpy(h.data(), c.K.cat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
                 double sym[6];
                 for (long i = 0; i < nm; ++i) {
-                    for (int k = 0; k < C; ++k) sym[k] = h[(size_t)k * nm + i];
+                    const long src = c.K.cat_shared ? i % M : i;
+                    for (int k = 0; k < C; ++k) sym[k] = h[(size_t)k * n + src];
                     c.join(sym, zero, host + i * F);
                 }
                 break;
@@ -642,7 +671,7 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         if (nm > (long)std::numeric_limits<int>::max()) throw ArgError("N*M exceeds the int index range of the reference");
         if (c.kmc) {
             auto f = [](void* p) { if (p) (void)hipFree(p); };
-            f(c.K.Q); f(c.K.Q0); f(c.K.cat); f(c.K.soft); f(c.K.last_t); f(c.K.prev_dir); f(c.K.rates); f(c.K.bsum);
+            f(c.K.Q); f(c.K.Q0); f(c.K.cat); f(c.K.soft); f(c.K.last_t); f(c.K.prev_dir); f(c.K.bsum);
             f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kctl); f(c.kstage); f(c.unst_idx); f(c.unst_q);
             if (c.hkctl) (void)hipHostFree(c.hkctl);
             if (c.hkstage) (void)hipHostFree(c.hkstage);
@@ -676,7 +705,6 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         K.soft = dalloc<double>(2 * N);
         K.last_t = dalloc<double>(N);
         K.prev_dir = dalloc<double>((size_t)C * N);
-        K.rates = dalloc<double>(nm);
         const int nb = kmc_nblocks(nm);
         K.bsum = dalloc<double>(nb);
         K.bincl = dalloc<double>(nb);
@@ -720,7 +748,7 @@ int mgk_elastic_run(mgk_ctx* ctx, double* eps_macro, double* sig_macro, int max_
         double norm_extra = 0.0;
         for (int w = 0; w < ns; ++w) norm_extra += 2.0 * (double)c.g.N * bar_skew[w] * bar_skew[w];
         // warm start always: the KMC fields are sized N (kmc_solver.hpp:197)
-        int it = c.solve(bar_sym, norm_extra, 0.0, norm_extra, true, max_iter, tol, diff_out);
+        int it = c.solve(bar_sym, norm_extra, true, max_iter, tol, diff_out);
         if (iters_out) *iters_out = it;
         for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
         double sk_sig[3];
@@ -755,7 +783,7 @@ int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
         MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
         MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         if (c.hkctl->ambiguous) {
-            kmc_enqueue_select_fallback(c.K, c.stream);
+            kmc_enqueue_select_fallback(c.K, 0, c.stream);
             MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
             MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         }
@@ -776,11 +804,32 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
         if ((new_catalog || new_Q0) && (first_mode < 0 || n_modes < 0 || first_mode + n_modes > c.kp.M))
             throw ArgError("redraw mode range out of range");
         if (!new_catalog && !new_Q0) n_modes = 0;
-        const int C = c.g.C, F = c.full();
+        const int C = c.g.C, F = c.full(), M = c.kp.M;
         double* h = c.hkstage;
         double skw[3];
         if (new_catalog)
             for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
+        if (new_catalog && c.K.cat_shared) {
+            // a redraw that reproduces the shared strains keeps the shared copy;
+            // any other redraw turns the catalog into a per-(voxel, mode) array
+            const std::vector<double>& shared = c.shared_cat;   // [C][M], kept since mgk_set_field
+            bool same = true;
+            for (int j = 0; j < n_modes && same; ++j)
+                for (int k = 0; k < C; ++k)
+                    if (h[(size_t)j * C + k] != shared[(size_t)k * M + first_mode + j]) { same = false; break; }
+            if (same) {
+                new_catalog = nullptr;
+            } else {
+                std::vector<double> full((size_t)C * nm);
+                for (int k = 0; k < C; ++k)
+                    for (long i = 0; i < nm; ++i) full[(size_t)k * nm + i] = shared[(size_t)k * M + i % M];
+                MGK_HIP_CHECK(hipMemcpyAsync(c.K.cat, full.data(), full.size() * sizeof(double),
+                                             hipMemcpyHostToDevice, c.stream));
+                MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+                c.K.cat_shared = 0;
+            }
+        }
+        if (!new_catalog && !new_Q0) n_modes = 0;
         if (new_Q0)
             for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
         if (n_modes > 0)
@@ -788,6 +837,63 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
         kmc_enqueue_flip(c.K, flat_idx, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
         // the pinned staging buffer is reused by the next call: wait for the copy
         if (n_modes > 0) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return MGK_OK;
+    });
+}
+
+int mgk_select_flip(mgk_ctx* ctx, double u2, double time, const double* new_catalog, const double* new_Q0,
+                    int first_mode, int n_modes) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const int C = c.g.C, F = c.full(), M = c.kp.M;
+        if ((new_catalog || new_Q0) &&
+            (n_modes < 0 || (first_mode >= 0 && first_mode + n_modes > M) || (first_mode < 0 && n_modes != 1)))
+            throw ArgError("redraw mode range out of range");
+        if (new_catalog && c.K.cat_shared) {
+            // the voxel is chosen on the device: only a redraw that reproduces the
+            // shared strains (the fixed catalog) keeps the shared copy
+            bool same = first_mode >= 0;
+            double sym[6], skw[3];
+            for (int j = 0; j < n_modes && same; ++j) {
+                c.split(new_catalog + (size_t)j * F, sym, skw);
+                for (int k = 0; k < C; ++k)
+                    if (sym[k] != c.shared_cat[(size_t)k * M + first_mode + j]) { same = false; break; }
+            }
+            if (!same) throw ArgError("a redraw that differs from the shared catalog needs mgk_select + mgk_apply_flip");
+            new_catalog = nullptr;
+        }
+        if (!new_catalog && !new_Q0) n_modes = 0;
+        // the pinned staging buffer may still feed the previous call's copy
+        if (c.kctl_pending) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        double* h = c.hkstage;
+        double skw[3];
+        if (new_catalog)
+            for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
+        if (new_Q0)
+            for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
+        kmc_enqueue_select(c.K, u2, c.stream);
+        kmc_enqueue_select_fallback(c.K, 1, c.stream);
+        if (n_modes > 0)
+            MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, h, sizeof(double) * n_modes * (C + 1), hipMemcpyHostToDevice, c.stream));
+        kmc_enqueue_flip(c.K, -1, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
+        c.kctl_pending = true;   // the selection is read back with the next solve's sync
+        return MGK_OK;
+    });
+}
+
+int mgk_last_selection(mgk_ctx* ctx, int* flat_idx, double* q_sel) {
+    return guarded([&]() -> int {
+        if (!ctx || !flat_idx) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (c.kctl_pending) {
+            MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+            MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+            c.kctl_pending = false;
+        }
+        *flat_idx = c.hkctl->sel;
+        if (q_sel) *q_sel = c.hkctl->q_sel;
         return MGK_OK;
     });
 }
@@ -851,7 +957,8 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
         LsParams p = c.ls_params();
         p.corr = nullptr;
-        const int mode = fp32 ? LS_DELTA : LS_FULL;
+        const int mode = fp32 ? LS_CG : LS_FULL;
+        p.r32 = fp32 ? c.cg : nullptr;   // the FP32 pass A reads the CG residual buffer
         ls_enqueue_pass(p, pass, mode, c.stream);   // warm-up launch
         MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
         for (int r = 0; r < reps; ++r) ls_enqueue_pass(p, pass, mode, c.stream);

@@ -48,6 +48,11 @@ struct Ctl {
     double mean_eps[6];  // outputs of the final pass
     double mean_sig[6];
     double diff_sq, norm_sq;
+    double diff_hist[32];  // relative increment of each iteration (FP64 and CG)
+    double eps_norm;       // ||eps|| after the last FP64 iteration
+    double cg_alpha, cg_beta, cg_gamma;
+    int cg_stop;           // CG breakdown: the rest of the cycle is skipped
+    int pad2;
 };
 
 struct Status {

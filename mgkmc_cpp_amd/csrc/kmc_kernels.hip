@@ -19,18 +19,20 @@ template <typename T> __device__ __forceinline__ T wsum(T v) {
     return v;
 }
 
-__device__ __forceinline__ double load_coh(const double* p) {
-    unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-__device__ __forceinline__ int load_coh(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Rate of one mode from its barrier (kmc_solver.hpp:795-801): included when
+// instability_mode == "kmc" or Q > 0; V_eff nu0 for Q <= 0, else
+// V_eff nu0 exp(-Q beta) (beta = +inf at T = 0 gives 0).
+__device__ __forceinline__ double mode_rate(const KmcPhys& ph, double q, bool& incl) {
+    incl = ph.instability_kmc || q > 0.0;
+    if (!incl) return 0.0;
+    return (q <= 0.0) ? ph.eff_volume * ph.nu0 : ph.eff_volume * ph.nu0 * exp(-q * ph.beta);
 }
 
 // ---------------------------------------------------------------------------
 // update_barriers + rates (kmc_solver.hpp:341-410, :783-806; 3D :1340-1411,
-// :1796-1818).  One lane per (voxel, mode), 1024 modes per block.
+// :1796-1818).  One lane per (voxel, mode), 1024 modes per block.  The
+// catalog is per (voxel, mode) or, when every voxel holds the same M
+// strains, one shared copy of M entries.
 // ---------------------------------------------------------------------------
 template <int D, int C>
 __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
         double cat[C], s[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            cat[c] = p.cat[c * nmodes + i];
+            cat[c] = p.cat_shared ? p.cat[c * ph.M + (i - v * ph.M)] : p.cat[c * nmodes + i];
             s[c] = p.sig[c * g.N + v];
         }
         double modifier = 1.0;
@@ -98,16 +100,13 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
         const double w = 0.5 * ph.volume * (w_sum / 1e9) * GPa_nm3_to_eV;
         const double q = p.Q0[i] * exp(-g_eff) - w;
         p.Q[i] = q;
-        const bool incl = ph.instability_kmc || q > 0.0;
-        double r = -1.0;
-        if (incl) r = (q <= 0.0) ? ph.eff_volume * ph.nu0 : ph.eff_volume * ph.nu0 * exp(-q * ph.beta);
-        p.rates[i] = r;
+        bool incl;
+        const double r = mode_rate(ph, q, incl);
         if (incl) { lsum += r; linc++; }
         if (q < ph.thr) lun++;
     }
     __shared__ double sd[4];
     __shared__ int su[4], si[4];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     lsum = wsum(lsum);
     lun = wsum(lun);
@@ -118,24 +117,21 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
         p.bsum[blockIdx.x] = (sd[0] + sd[1]) + (sd[2] + sd[3]);
         p.bunst[blockIdx.x] = su[0] + su[1] + su[2] + su[3];
         p.binc[blockIdx.x] = si[0] + si[1] + si[2] + si[3];
-        __threadfence();
-        unsigned t = atomicAdd(&p.kctl->ticket, 1u);
-        s_last = (t == gridDim.x - 1) ? 1 : 0;
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // last block: inclusive prefix of block sums (chunked, fixed order), and
-    // exclusive prefix of unstable counts.
-    const int nb = gridDim.x;
+}
+
+// One block: inclusive prefix of the block sums (chunked, fixed order), the
+// exclusive prefix of unstable counts and the totals.  A kernel of its own so
+// that the rate blocks need no device-scope fence.
+__global__ __launch_bounds__(256) void k_rates_scan(KmcParams p, int nb) {
     const int chunk = (nb + 255) / 256;
     const int b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
     double csum = 0.0;
     int cun = 0, cinc = 0;
     for (int b = b0; b < b1; ++b) {
-        csum += load_coh(&p.bsum[b]);
-        cun += load_coh(&p.bunst[b]);
-        cinc += load_coh(&p.binc[b]);
+        csum += p.bsum[b];
+        cun += p.bunst[b];
+        cinc += p.binc[b];
     }
     __shared__ double sc[256];
     __shared__ int scu[256], sci[256];
@@ -157,10 +153,10 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
     double run = (threadIdx.x > 0) ? sc[threadIdx.x - 1] : 0.0;
     int runu = (threadIdx.x > 0) ? scu[threadIdx.x - 1] : 0;
     for (int b = b0; b < b1; ++b) {
-        run += load_coh(&p.bsum[b]);
+        run += p.bsum[b];
         p.bincl[b] = run;
         p.bunst_off[b] = runu;
-        runu += load_coh(&p.bunst[b]);
+        runu += p.bunst[b];
     }
     __shared__ double s_total;
     // the total is the last inclusive prefix, so that target <= bincl[nb-1]
@@ -176,7 +172,6 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
         kc->sel = -1;
         kc->ambiguous = 0;
         kc->ticket = 0;
-        __threadfence();
     }
 }
 
@@ -209,13 +204,16 @@ __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
     const double prev = (b > 0) ? p.bincl[b - 1] : 0.0;
     // 4 consecutive modes per lane, local inclusive prefix
     double loc[4];
+    bool inc[4];
     double run = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const long i = base + threadIdx.x * 4 + k;
-        double r = (i < nmodes) ? p.rates[i] : -1.0;
-        run += (r >= 0.0) ? r : 0.0;
+        bool incl = false;
+        const double r = (i < nmodes) ? mode_rate(p.ph, p.Q[i], incl) : 0.0;
+        run += incl ? r : 0.0;
         loc[k] = run;
+        inc[k] = incl;
     }
     s_tot[threadIdx.x] = run;
     __syncthreads();
@@ -232,8 +230,7 @@ __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
         const long i = base + li;
         const double S = prev + (excl + loc[k]);
         s_scan[li] = S;
-        const double r = (i < nmodes) ? p.rates[i] : -1.0;
-        if (r >= 0.0 && S >= target) atomicMin(&s_first, li);
+        if (i < nmodes && inc[k] && S >= target) atomicMin(&s_first, li);
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -256,27 +253,30 @@ __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
     kc->sel = sel;
     kc->ambiguous = amb;
     kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
-    __threadfence();
 }
 
 // Exact left-to-right pass of the reference (single lane): the total in flat
-// order, target = u2 * total, then the first prefix >= target.
-__global__ void k_select_seq(KmcParams p) {
+// order, target = u2 * total, then the first prefix >= target.  With
+// only_if_ambiguous it runs only when k_select flagged its answer.
+__global__ void k_select_seq(KmcParams p, int only_if_ambiguous) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
+    if (only_if_ambiguous && !kc->ambiguous) return;
     const long nmodes = p.g.N * p.ph.M;
     double tot = 0.0;
     long last = -1;
     for (long i = 0; i < nmodes; ++i) {
-        const double r = p.rates[i];
-        if (r >= 0.0) { tot += r; last = i; }
+        bool incl;
+        const double r = mode_rate(p.ph, p.Q[i], incl);
+        if (incl) { tot += r; last = i; }
     }
     const double target = kc->u2 * tot;
     double cur = 0.0;
     long sel = last;
     for (long i = 0; i < nmodes; ++i) {
-        const double r = p.rates[i];
-        if (r < 0.0) continue;
+        bool incl;
+        const double r = mode_rate(p.ph, p.Q[i], incl);
+        if (!incl) continue;
         cur += r;
         if (cur >= target) { sel = i; break; }
     }
@@ -284,7 +284,6 @@ __global__ void k_select_seq(KmcParams p) {
     kc->sel = (int)sel;
     kc->ambiguous = 0;
     kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
-    __threadfence();
 }
 
 // ---------------------------------------------------------------------------
@@ -296,13 +295,18 @@ __global__ void k_select_seq(KmcParams p) {
 template <int D, int C>
 __global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n_modes, int has_cat, int has_q0) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // flat < 0: the mode the last selection chose; first_mode < 0: redraw
+    // only that mode (both redraw switches off, kmc_solver.hpp:856-864)
+    if (flat < 0) flat = p.kctl->sel;
+    if (flat < 0) return;
+    if (first_mode < 0) first_mode = flat % p.ph.M;
     const Geom g = p.g;
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long v = flat / ph.M;
     double c[C];
 #pragma unroll
-    for (int k = 0; k < C; ++k) c[k] = p.cat[k * nmodes + flat];
+    for (int k = 0; k < C; ++k) c[k] = p.cat_shared ? p.cat[k * ph.M + flat % ph.M] : p.cat[k * nmodes + flat];
 #pragma unroll
     for (int k = 0; k < C; ++k) p.eps_p[k * g.N + v] += c[k];
     double sum_sq;
@@ -342,7 +346,7 @@ __global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n
     for (int k = 0; k < C; ++k) p.prev_dir[k * g.N + v] = c[k];
     for (int j = 0; j < n_modes; ++j) {
         const long mi = v * ph.M + first_mode + j;
-        if (has_cat)
+        if (has_cat && !p.cat_shared)
             for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = p.staging[j * C + k];
         if (has_q0) p.Q0[mi] = p.staging[n_modes * C + j];
     }
@@ -391,6 +395,7 @@ void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s) {
     const int nb = kmc_nblocks(p.g.N * p.ph.M);
     if (p.g.dims == 2) hipLaunchKernelGGL((k_rates<2, 3>), dim3(nb), dim3(256), 0, s, p, time);
     else hipLaunchKernelGGL((k_rates<3, 6>), dim3(nb), dim3(256), 0, s, p, time);
+    hipLaunchKernelGGL(k_rates_scan, dim3(1), dim3(256), 0, s, p, nb);
     MGK_HIP_CHECK(hipGetLastError());
 }
 
@@ -399,8 +404,8 @@ void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s) {
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void kmc_enqueue_select_fallback(const KmcParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_select_seq, dim3(1), dim3(64), 0, s, p);
+void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipStream_t s) {
+    hipLaunchKernelGGL(k_select_seq, dim3(1), dim3(64), 0, s, p, only_if_ambiguous);
     MGK_HIP_CHECK(hipGetLastError());
 }
 

@@ -38,11 +38,11 @@ struct KmcParams {
     double* eps_p;           // [C][N]
     double* Q;               // [N*M]
     double* Q0;              // [N*M]
-    double* cat;             // [C][N*M]
+    double* cat;             // [C][N*M], or [C][M] when cat_shared
+    int cat_shared;          // every voxel holds the same M catalog strains
     double* soft;            // [2][N]  g_p, g_t
     double* last_t;          // [N]
     double* prev_dir;        // [C][N]
-    double* rates;           // [N*M], -1 for excluded modes
     double* bsum;            // [nblocks] block sums of rates
     double* bincl;           // [nblocks] inclusive block prefix
     int* bunst;              // [nblocks] unstable counts
@@ -59,7 +59,7 @@ constexpr int kRateBlock = 1024;   // modes per rate block (256 lanes x 4)
 int kmc_nblocks(long nmodes);
 void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s);
 void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s);
-void kmc_enqueue_select_fallback(const KmcParams& p, hipStream_t s);
+void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipStream_t s);
 void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s);
 void kmc_enqueue_unstable(const KmcParams& p, hipStream_t s);

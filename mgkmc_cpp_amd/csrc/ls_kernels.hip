@@ -15,12 +15,20 @@
 //                   sets the convergence flag in the control block
 //
 // Every kernel returns at once when the control block says the solve is done,
-// so the host can enqueue iterations in batches without a sync per iteration.
+// so the host enqueues iterations in batches without a sync per iteration.
+//
+// Spectrum layout: component-interleaved half spectrum,
+//   spec[((f*nm + m)*nlh + l)*C + c]   (complex),
+// so every pass moves contiguous rows of (bin, component) pairs.
 //
 // The same passes run in two arithmetic modes:
 //   FULL  (FP64 spectra): the reference's iteration on eps.
-//   DELTA (FP32 spectra): y -> Gamma~(y) = -Gamma0 * ((C - C0):y), the inner
-//         iteration of the FP32 iterative-refinement path.
+//   CG    (FP32 spectra): one conjugate-gradient step (Chronopoulos-Gear
+//         form, one fused reduction) on A d = r with A = I + Gamma0*(C - C0),
+//         SPD on compatible fields in the C0-energy inner product: pass A
+//         updates the FP32 vectors (p, s, x, r) and forms tau = (C - C0):r,
+//         pass E forms w = A r and the products (r,r)_C0, (w,r)_C0, ||r||_W.
+//         This is the inner solver of the FP32 iterative-refinement path.
 //
 // Half-spectrum equivalence.  The reference transforms real fields with c2c
 // and keeps the real part of the inverse (linear_elastic_solver.hpp:134-143).
@@ -42,8 +50,6 @@
 #include <algorithm>
 
 namespace mgk {
-
-using d2 = cplx<double>;
 
 extern __shared__ __align__(16) unsigned char smem_raw[];
 
@@ -78,27 +84,8 @@ template <int K> __device__ __forceinline__ void block_sum(double (&v)[K], doubl
     __syncthreads();
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-
-// After each block wrote its K partials to part[block*K + k]: returns true in
-// the block that arrives last; that block then sees every partial.
-__device__ __forceinline__ bool last_block(Ctl* ctl) {
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(&ctl->ticket, 1u);
-        s_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    return s_last != 0;
-}
-
-// Fixed-order reduction of part[b*K + k] over blocks by the last block.
+// Fixed-order reduction of part[b*K + k] over blocks (one block, plain loads:
+// the partials were written by an earlier kernel on the same stream).
 template <int K> __device__ __forceinline__ void reduce_partials(const double* part, int nblocks, double (&out)[K],
                                                                  double* scratch) {
     double v[K];
@@ -106,7 +93,7 @@ template <int K> __device__ __forceinline__ void reduce_partials(const double* p
     for (int k = 0; k < K; ++k) v[k] = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += load_coherent(&part[(long)b * K + k]);
+        for (int k = 0; k < K; ++k) v[k] += part[(long)b * K + k];
     block_sum<K>(v, scratch);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = v[k];
@@ -160,8 +147,8 @@ __device__ __forceinline__ void gamma_apply(const cplx<R>* t, cplx<R>* o, const 
 // Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW lines per block.
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
-    if (p.ctl->done) return;
+__global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW, int flags) {
+    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
     constexpr int NLP = padded<NL>();
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
@@ -172,8 +159,11 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
     const long line0 = (long)blockIdx.x * LPW;
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
 
-    const bool first = (p.ctl->iters == 0) && (MODE == LS_FULL);
-    const bool use_corr = (MODE == LS_FULL) && p.corr && p.ctl->corr_valid;
+    const bool add_shift = (MODE == LS_FULL) && (flags & LS_FLAG_SHIFT);
+    const bool add_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.ctl->corr_valid;
+    double shift[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
     SR* bufr = reinterpret_cast<SR*>(buf);
     for (int vox = threadIdx.x; vox < LPW * NL; vox += blockDim.x) {
         const int li = vox / NL, j = vox % NL;
@@ -185,9 +175,8 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
                 double e[C], ee[C];
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
-                    e[c] = p.eps[c * g.N + v];
-                    if (first) e[c] += p.ctl->shift[c];
-                    if (use_corr) e[c] += (double)p.corr[c * g.N + v];
+                    e[c] = p.eps[c * g.N + v] + shift[c];
+                    if (add_corr) e[c] += (double)p.corr[c * g.N + v];
                     ee[c] = p.epsp ? e[c] - p.epsp[c * g.N + v] : e[c];
                 }
                 const double mu = p.mu[v], lam = p.lam[v];
@@ -202,9 +191,32 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
                     tau[c] = (SR)(s - s0);
                 }
             } else {
+                // CG vector updates (Chronopoulos-Gear, lines 7-10), then tau = (C - C0):r
                 float y[C];
+                const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
+                if (upd) {
+                    const float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
+                    const bool first = flags & LS_FLAG_CG_UPDATE1;
 #pragma unroll
-                for (int c = 0; c < C; ++c) y[c] = p.y32[c * g.N + v];
+                    for (int c = 0; c < C; ++c) {
+                        const long o = c * g.N + v;
+                        const float r = p.r32[o], w = p.w32[o];
+                        const float pp = first ? r : r + be * p.p32[o];
+                        const float ss = first ? w : w + be * p.s32[o];
+                        // x is the correction of eps_1 = eps_0 + r0, the field the FP64
+                        // step left: x_j = d_j - r0 where d_1 = alpha_0 r0 solves A d = r0
+                        const float xx = first ? (al - 1.0f) * r : p.corr[o] + al * pp;
+                        const float rn = r - al * ss;
+                        p.p32[o] = pp;
+                        p.s32[o] = ss;
+                        p.corr[o] = xx;
+                        p.r32[o] = rn;
+                        y[c] = rn;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) y[c] = p.r32[c * g.N + v];
+                }
                 float tr = 0.f;
 #pragma unroll
                 for (int c = 0; c < D; ++c) tr += y[c];
@@ -225,8 +237,8 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
     // split the packed spectrum into the two half spectra
     cx* spec = reinterpret_cast<cx*>(MODE == LS_FULL ? p.spec64 : p.spec32);
     const int nlh = g.nlh;
-    for (int idx = threadIdx.x; idx < C * LPW * nlh; idx += blockDim.x) {
-        const int k = idx % nlh, li = (idx / nlh) % LPW, c = idx / (nlh * LPW);
+    for (int idx = threadIdx.x; idx < LPW * nlh * C; idx += blockDim.x) {
+        const int c = idx % C, k = (idx / C) % nlh, li = idx / (C * nlh);
         const long line = line0 + li;
         if (line >= g.nlines()) continue;
         const int pr = li >> 1;
@@ -235,36 +247,38 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW) {
         cx out;
         if ((li & 1) == 0) out = {(SR)0.5 * (z.x + zm.x), (SR)0.5 * (z.y - zm.y)};
         else out = {(SR)0.5 * (z.y + zm.y), (SR)-0.5 * (z.x - zm.x)};
-        spec[(long)c * g.NS + line * nlh + k] = out;
+        spec[(line * nlh + k) * C + c] = out;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Passes B/D: c2c along M for one (component, f) plane, a tile of L-bins.
+// Passes B/D: c2c along M for one f plane, a tile of (l-bin, component)
+// columns.
 // ---------------------------------------------------------------------------
 template <typename SR, int NM, int DIR>
 __global__ __launch_bounds__(256) void k_mid(LsParams p, int TK, int ntiles, int mode) {
-    if (p.ctl->done) return;
+    if (p.ctl->done || (mode == LS_CG && p.ctl->cg_stop)) return;
     constexpr int NMP = padded<NM>();
     using cx = cplx<SR>;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NM;
     const Geom g = p.g;
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_m64 : p.tw_m32), NM);
+    const int ncol = g.nlh * g.C;
     const int tile = blockIdx.x % ntiles;
-    const long plane = blockIdx.x / ntiles;   // c*nf + f
+    const long f = blockIdx.x / ntiles;
     const int k0 = tile * TK;
-    const int tk = min(TK, g.nlh - k0);
-    cx* base = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32) + plane * (long)NM * g.nlh + k0;
+    const int tk = min(TK, ncol - k0);
+    cx* base = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32) + f * (long)NM * ncol + k0;
     for (int idx = threadIdx.x; idx < NM * TK; idx += blockDim.x) {
         const int kk = idx % TK, m = idx / TK;
-        if (kk < tk) buf[kk * NMP + m] = base[(long)m * g.nlh + kk];
+        if (kk < tk) buf[kk * NMP + m] = base[(long)m * ncol + kk];
     }
     __syncthreads();
     lds_fft<NM, DIR, SR>(buf, tk, NMP, 1, tw);
     for (int idx = threadIdx.x; idx < NM * TK; idx += blockDim.x) {
         const int kk = idx % TK, m = idx / TK;
-        if (kk < tk) base[(long)m * g.nlh + kk] = buf[kk * NMP + m];
+        if (kk < tk) base[(long)m * ncol + kk] = buf[kk * NMP + m];
     }
 }
 
@@ -274,7 +288,7 @@ __global__ __launch_bounds__(256) void k_mid(LsParams p, int TK, int ntiles, int
 // ---------------------------------------------------------------------------
 template <typename SR, int NF, int C>
 __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mode) {
-    if (p.ctl->done) return;
+    if (p.ctl->done || (mode == LS_CG && p.ctl->cg_stop)) return;
     constexpr int NFP = padded<NF>();
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
@@ -285,11 +299,11 @@ __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mod
     const long ncols = (long)g.nm * g.nlh;
     const long col0 = (long)blockIdx.x * TC;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
-    for (int idx = threadIdx.x; idx < C * NF * TC; idx += blockDim.x) {
-        const int cc = idx % TC, f = (idx / TC) % NF, c = idx / (TC * NF);
+    for (int idx = threadIdx.x; idx < NF * TC * C; idx += blockDim.x) {
+        const int c = idx % C, cc = (idx / C) % TC, f = idx / (C * TC);
         const long col = col0 + cc;
         cx v = {(SR)0, (SR)0};
-        if (col < ncols) v = spec[(long)c * g.NS + (long)f * ncols + col];
+        if (col < ncols) v = spec[((long)f * ncols + col) * C + c];
         buf[(c * TC + cc) * NFP + f] = v;
     }
     __syncthreads();
@@ -349,37 +363,38 @@ __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mod
     }
     __syncthreads();
     lds_fft<NF, +1, SR>(buf, C * TC, NFP, 1, tw);
-    for (int idx = threadIdx.x; idx < C * NF * TC; idx += blockDim.x) {
-        const int cc = idx % TC, f = (idx / TC) % NF, c = idx / (TC * NF);
+    for (int idx = threadIdx.x; idx < NF * TC * C; idx += blockDim.x) {
+        const int c = idx % C, cc = (idx / C) % TC, f = idx / (C * TC);
         const long col = col0 + cc;
-        if (col < ncols) spec[(long)c * g.NS + (long)f * ncols + col] = buf[(c * TC + cc) * NFP + f];
+        if (col < ncols) spec[((long)f * ncols + col) * C + c] = buf[(c * TC + cc) * NFP + f];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Pass E: c2r along L + update + norms + last-block convergence test.
+// Pass E: c2r along L + update.  FULL: norms and the last-block convergence
+// test.  DELTA: y <- Gamma~(y), corr (+)= y; no reduction.
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_inv_last(LsParams p, int LPW) {
-    if (p.ctl->done) return;
+__global__ __launch_bounds__(256) void k_inv_last(LsParams p, int LPW, int flags) {
+    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
     constexpr int NLP = padded<NL>();
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
-    __shared__ double red[8 * 4];
+    __shared__ double red[8 * 4 * 2];
     const int P = LPW / 2;
     const Geom g = p.g;
     const long line0 = (long)blockIdx.x * LPW;
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
     const cx* spec = reinterpret_cast<const cx*>(MODE == LS_FULL ? p.spec64 : p.spec32);
     const int nlh = g.nlh;
-    for (int idx = threadIdx.x; idx < C * P * nlh; idx += blockDim.x) {
-        const int k = idx % nlh, pr = (idx / nlh) % P, c = idx / (nlh * P);
+    for (int idx = threadIdx.x; idx < P * nlh * C; idx += blockDim.x) {
+        const int c = idx % C, k = (idx / C) % nlh, pr = idx / (C * nlh);
         const long la = line0 + 2 * pr, lb = la + 1;
         cx A = {(SR)0, (SR)0}, Bv = {(SR)0, (SR)0};
-        if (la < g.nlines()) A = spec[(long)c * g.NS + la * nlh + k];
-        if (lb < g.nlines()) Bv = spec[(long)c * g.NS + lb * nlh + k];
+        if (la < g.nlines()) A = spec[(la * nlh + k) * C + c];
+        if (lb < g.nlines()) Bv = spec[(lb * nlh + k) * C + c];
         cx* line = buf + (c * P + pr) * NLP;
         if (k == 0 || 2 * k == NL) {
             line[k] = {A.x, Bv.x};   // self-conjugate bins: real parts only (c2r)
@@ -391,47 +406,75 @@ __global__ __launch_bounds__(256) void k_inv_last(LsParams p, int LPW) {
     __syncthreads();
     lds_fft<NL, +1, SR>(buf, C * P, NLP, 1, tw);
 
-    const bool first = (p.ctl->iters == 0) && (MODE == LS_FULL);
-    const bool use_corr = (MODE == LS_FULL) && p.corr && p.ctl->corr_valid;
-    const bool first_inner = (MODE == LS_DELTA) && (p.ctl->corr_valid == 0);
+    const bool add_shift = (MODE == LS_FULL) && (flags & LS_FLAG_SHIFT);
+    const bool add_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.ctl->corr_valid;
+    double shift[C], bar[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
+        bar[c] = (MODE == LS_FULL) ? p.ctl->eps_bar[c] : 0.0;
+    }
+    const double mu2 = 2.0 * p.mu0, lam0 = p.lam0;
     const SR* bufr = reinterpret_cast<const SR*>(buf);
-    double acc[2] = {0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
     for (int vox = threadIdx.x; vox < LPW * NL; vox += blockDim.x) {
         const int li = vox / NL, j = vox % NL;
         const long line = line0 + li;
         if (line >= g.nlines()) continue;
         const long v = line * NL + j;
         const int pr = li >> 1, part = li & 1;
+        if constexpr (MODE == LS_FULL) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const SR et = bufr[2 * ((c * P + pr) * NLP + j) + part];
-            const double w = (c < D) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
-            if constexpr (MODE == LS_FULL) {
-                double old = p.eps[c * g.N + v];
-                if (first) old += p.ctl->shift[c];
-                if (use_corr) old += (double)p.corr[c * g.N + v];
-                const double nw = p.ctl->eps_bar[c] + (double)et;
+            for (int c = 0; c < C; ++c) {
+                const SR et = bufr[2 * ((c * P + pr) * NLP + j) + part];
+                const double w = (c < D) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
+                double old = p.eps[c * g.N + v] + shift[c];
+                if (add_corr) old += (double)p.corr[c * g.N + v];
+                const double nw = bar[c] + (double)et;
                 const double d = nw - old;
                 acc[0] += w * d * d;
                 acc[1] += w * old * old;
+                acc[2] += w * nw * nw;
                 p.eps[c * g.N + v] = nw;
                 if (p.r32) p.r32[c * g.N + v] = (float)d;
-            } else {
-                const float y = (float)et;
-                p.y32[c * g.N + v] = y;
-                p.corr[c * g.N + v] = first_inner ? y : p.corr[c * g.N + v] + y;
-                acc[0] += w * (double)y * (double)y;
             }
+        } else {
+            // w = A r = r - Gamma~(r); C0-energy products (r, r), (w, r) and ||r||_W^2
+            double rr = 0.0, wr = 0.0, nr = 0.0, trr = 0.0, trw = 0.0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float et = (float)bufr[2 * ((c * P + pr) * NLP + j) + part];
+                const float r = p.r32[c * g.N + v];
+                const float w = r - et;
+                p.w32[c * g.N + v] = w;
+                const double wt = (c < D) ? 1.0 : 2.0;
+                rr += wt * (double)r * (double)r;
+                wr += wt * (double)w * (double)r;
+                if (c < D) { trr += r; trw += w; }
+            }
+            nr = rr;
+            acc[0] += mu2 * rr + lam0 * trr * trr;
+            acc[1] += mu2 * wr + lam0 * trw * trr;
+            acc[2] += nr;
         }
     }
-    block_sum<2>(acc, red);
+    block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
-        p.partials[2 * blockIdx.x] = acc[0];
-        p.partials[2 * blockIdx.x + 1] = acc[1];
+        p.partials[3 * blockIdx.x] = acc[0];
+        p.partials[3 * blockIdx.x + 1] = acc[1];
+        p.partials[3 * blockIdx.x + 2] = acc[2];
     }
-    if (!last_block(p.ctl)) return;
-    double tot[2];
-    reduce_partials<2>(p.partials, gridDim.x, tot, red);
+}
+
+// Reduction of pass E's block partials and the control-block update.  A kernel
+// of its own: the kernel boundary makes every block's partials visible, with
+// no per-block device-scope fence (which writes back the XCD's L2).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ls_reduce(LsParams p, int nblocks, int flags) {
+    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
+    __shared__ double red[8 * 4 * 2];
+    double tot[3];
+    reduce_partials<3>(p.partials, nblocks, tot, red);
     if (threadIdx.x == 0) {
         Ctl* ctl = p.ctl;
         if constexpr (MODE == LS_FULL) {
@@ -439,17 +482,42 @@ __global__ __launch_bounds__(256) void k_inv_last(LsParams p, int LPW) {
             ctl->diff = diff;
             ctl->diff_sq = tot[0];
             ctl->norm_sq = tot[1];
+            ctl->eps_norm = sqrt(tot[2] + ctl->norm_extra);
+            if (ctl->iters + ctl->inner < 32) ctl->diff_hist[ctl->iters + ctl->inner] = diff;
             ctl->iters += 1;
-            ctl->corr_valid = 0;
-            if (diff < ctl->tol || ctl->iters >= ctl->max_iter) ctl->done = 1;
+            ctl->corr_valid = 0;   // a pending correction was absorbed into eps
+            ctl->cg_stop = 0;
+            if (diff < ctl->tol || ctl->iters + ctl->inner >= ctl->max_iter) ctl->done = 1;
         } else {
+            // Chronopoulos-Gear scalars (alpha, beta) for the next update, and the
+            // stopping test on ||r||_W / ||eps||, the relative increment the
+            // reference's basic scheme would measure at this iterate.
+            const double gam = tot[0], del = tot[1];
+            const bool first = flags & LS_FLAG_CG_FIRST;
             ctl->inner += 1;
-            ctl->corr_valid = 1;
+            if (!first) {
+                ctl->corr_valid = 1;
+                const double diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
+                ctl->diff = diff;
+                if (ctl->iters + ctl->inner < 32) ctl->diff_hist[ctl->iters + ctl->inner] = diff;
+                if (diff < ctl->tol || ctl->iters + ctl->inner >= ctl->max_iter) ctl->done = 1;
+            }
+            double al, be;
+            if (first) {
+                be = 0.0;
+                al = gam / del;
+            } else {
+                be = gam / ctl->cg_gamma;
+                al = gam / (del - be * gam / ctl->cg_alpha);
+            }
+            if (!(al > 0.0) || !isfinite(al) || !isfinite(be)) ctl->cg_stop = 1;   // breakdown: back to FP64
+            ctl->cg_alpha = al;
+            ctl->cg_beta = be;
+            ctl->cg_gamma = gam;
         }
-        ctl->ticket = 0;
-        __threadfence();
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // Field mean -> warm-start shift (linear_elastic_solver.hpp:181-191); or fill
@@ -469,14 +537,16 @@ __global__ __launch_bounds__(256) void k_mean_shift(LsParams p) {
     if (threadIdx.x == 0)
 #pragma unroll
         for (int c = 0; c < C; ++c) p.partials[C * blockIdx.x + c] = acc[c];
-    if (!last_block(p.ctl)) return;
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_mean_reduce(LsParams p, int nblocks) {
+    __shared__ double red[8 * 6];
     double tot[C];
-    reduce_partials<C>(p.partials, gridDim.x, tot, red);
+    reduce_partials<C>(p.partials, nblocks, tot, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) p.ctl->shift[c] = p.ctl->eps_bar[c] - tot[c] / (double)g.N;
-        p.ctl->ticket = 0;
-        __threadfence();
+        for (int c = 0; c < C; ++c) p.ctl->shift[c] = p.ctl->eps_bar[c] - tot[c] / (double)p.g.N;
     }
 }
 
@@ -486,6 +556,15 @@ __global__ __launch_bounds__(256) void k_fill_bar(LsParams p) {
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x)
 #pragma unroll
         for (int c = 0; c < C; ++c) p.eps[c * g.N + v] = p.ctl->eps_bar[c];
+}
+
+// eps += shift (a warm start with no iteration still shifts, :181-191)
+template <int C>
+__global__ __launch_bounds__(256) void k_apply_shift(LsParams p) {
+    const Geom g = p.g;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < C; ++c) p.eps[c * g.N + v] += p.ctl->shift[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -499,11 +578,16 @@ __global__ __launch_bounds__(256) void k_final_stress(LsParams p) {
     double acc[2 * C];
 #pragma unroll
     for (int c = 0; c < 2 * C; ++c) acc[c] = 0.0;
+    const bool absorb = p.corr && p.ctl->corr_valid;   // the solve ended inside an FP32 CG cycle
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x) {
         double e[C], ee[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             e[c] = p.eps[c * g.N + v];
+            if (absorb) {
+                e[c] += (double)p.corr[c * g.N + v];
+                p.eps[c * g.N + v] = e[c];
+            }
             ee[c] = p.epsp ? e[c] - p.epsp[c * g.N + v] : e[c];
         }
         double tre = 0.0;
@@ -522,29 +606,21 @@ __global__ __launch_bounds__(256) void k_final_stress(LsParams p) {
     if (threadIdx.x == 0)
 #pragma unroll
         for (int c = 0; c < 2 * C; ++c) p.partials[2 * C * blockIdx.x + c] = acc[c];
-    if (!last_block(p.ctl)) return;
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_final_reduce(LsParams p, int nblocks) {
+    __shared__ double red[8 * 12];
     double tot[2 * C];
-    reduce_partials<2 * C>(p.partials, gridDim.x, tot, red);
+    reduce_partials<2 * C>(p.partials, nblocks, tot, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            p.ctl->mean_eps[c] = tot[c] / (double)g.N;
-            p.ctl->mean_sig[c] = tot[C + c] / (double)g.N;
+            p.ctl->mean_eps[c] = tot[c] / (double)p.g.N;
+            p.ctl->mean_sig[c] = tot[C + c] / (double)p.g.N;
         }
-        p.ctl->ticket = 0;
-        __threadfence();
+        p.ctl->corr_valid = 0;
     }
-}
-
-// eps += corr (FP32 correction) -- used when the refinement path stops with a
-// pending correction (never the case after a converged FP64 step).
-template <int C>
-__global__ __launch_bounds__(256) void k_apply_corr(LsParams p) {
-    if (!p.ctl->corr_valid) return;
-    const Geom g = p.g;
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x)
-#pragma unroll
-        for (int c = 0; c < C; ++c) p.eps[c * g.N + v] += (double)p.corr[c * g.N + v];
 }
 
 // ---------------------------------------------------------------------------
@@ -596,28 +672,32 @@ template <typename SR> static size_t smem_bytes(int n, int lines, int nlp) {
 }
 
 template <typename SR, int C, int MODE>
-static void launch_fwd_last(const LsParams& p, hipStream_t s) {
+static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
     const int lpw = lines_per_block<SR>(nl, C);
     const int nb = blocks_for(p.g.nlines(), lpw);
     const int nlp = nl + ((nl % 16 == 0) ? 1 : 0);
     const size_t sm = smem_bytes<SR>(nl, C * lpw / 2, nlp);
 #define MGK_CASE(NN) \
-    if (nl == NN) { hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw); return; }
+    if (nl == NN) { hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw, flags); return; }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("FFT length not supported on the last axis");
 }
 
 template <typename SR, int C, int MODE>
-static int launch_inv_last(const LsParams& p, hipStream_t s) {
+static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
     const int lpw = lines_per_block<SR>(nl, C);
     const int nb = blocks_for(p.g.nlines(), lpw);
     const int nlp = nl + ((nl % 16 == 0) ? 1 : 0);
     const size_t sm = smem_bytes<SR>(nl, C * lpw / 2, nlp);
 #define MGK_CASE(NN) \
-    if (nl == NN) { hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw); return nb; }
+    if (nl == NN) { \
+        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw, flags); \
+        hipLaunchKernelGGL((k_ls_reduce<MODE>), dim3(1), dim3(256), 0, s, p, nb, flags); \
+        return; \
+    }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("FFT length not supported on the last axis");
@@ -627,11 +707,12 @@ template <typename SR, int DIR>
 static void launch_mid(const LsParams& p, hipStream_t s, int mode) {
     const int nm = p.g.nm;
     const int nmp = nm + ((nm % 16 == 0) ? 1 : 0);
+    const int ncol = p.g.nlh * p.g.C;
     int tkmax = (kLdsBudget - nm * (int)sizeof(cplx<SR>)) / (nmp * (int)sizeof(cplx<SR>));
-    tkmax = std::max(1, std::min(tkmax, 32));
-    const int ntiles = (p.g.nlh + tkmax - 1) / tkmax;
-    const int tk = (p.g.nlh + ntiles - 1) / ntiles;
-    const int nb = p.g.C * p.g.nf * ntiles;
+    tkmax = std::max(1, std::min(tkmax, std::max(1, 256 / fft_threads(nm))));
+    const int ntiles = (ncol + tkmax - 1) / tkmax;
+    const int tk = (ncol + ntiles - 1) / ntiles;
+    const int nb = p.g.nf * ntiles;
     const size_t sm = smem_bytes<SR>(nm, tk, nmp);
 #define MGK_CASE(NN) \
     if (nm == NN) { hipLaunchKernelGGL((k_mid<SR, NN, DIR>), dim3(nb), dim3(256), sm, s, p, tk, ntiles, mode); return; }
@@ -658,26 +739,24 @@ static void launch_first_gamma(const LsParams& p, hipStream_t s, int mode) {
 int ls_partial_slots(const Geom& g) {
     // largest grid of any reducing kernel times its partial width
     long lines = g.nlines();
-    int lpw_min = 2;
-    long nb = (lines + lpw_min - 1) / lpw_min;
-    long nb2 = 1024;
-    return (int)std::max(nb * 2, nb2 * 12);
+    long nb = (lines + 1) / 2;
+    return (int)std::max(nb * 3, 1024L * 12);
 }
 
-template <int C> static void iteration_full(const LsParams& p, hipStream_t s) {
-    launch_fwd_last<double, C, LS_FULL>(p, s);
+template <int C> static void iteration_full(const LsParams& p, int flags, hipStream_t s) {
+    launch_fwd_last<double, C, LS_FULL>(p, flags, s);
     if (C == 6) launch_mid<double, -1>(p, s, LS_FULL);
     launch_first_gamma<double, C>(p, s, LS_FULL);
     if (C == 6) launch_mid<double, +1>(p, s, LS_FULL);
-    launch_inv_last<double, C, LS_FULL>(p, s);
+    launch_inv_last<double, C, LS_FULL>(p, flags, s);
 }
 
-template <int C> static void iteration_delta(const LsParams& p, hipStream_t s) {
-    launch_fwd_last<float, C, LS_DELTA>(p, s);
-    if (C == 6) launch_mid<float, -1>(p, s, LS_DELTA);
-    launch_first_gamma<float, C>(p, s, LS_DELTA);
-    if (C == 6) launch_mid<float, +1>(p, s, LS_DELTA);
-    launch_inv_last<float, C, LS_DELTA>(p, s);
+template <int C> static void iteration_delta(const LsParams& p, int flags, hipStream_t s) {
+    launch_fwd_last<float, C, LS_CG>(p, flags, s);
+    if (C == 6) launch_mid<float, -1>(p, s, LS_CG);
+    launch_first_gamma<float, C>(p, s, LS_CG);
+    if (C == 6) launch_mid<float, +1>(p, s, LS_CG);
+    launch_inv_last<float, C, LS_CG>(p, flags, s);
 }
 
 static int reduce_grid(long n) {
@@ -685,11 +764,11 @@ static int reduce_grid(long n) {
     return (int)std::min<long>(std::max<long>(nb, 1), 1024);
 }
 
-void ls_enqueue_iteration(const LsParams& p, int mode, hipStream_t s) {
+void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s) {
     if (p.g.C == 3) {
-        if (mode == LS_FULL) iteration_full<3>(p, s); else iteration_delta<3>(p, s);
+        if (mode == LS_FULL) iteration_full<3>(p, flags, s); else iteration_delta<3>(p, flags, s);
     } else {
-        if (mode == LS_FULL) iteration_full<6>(p, s); else iteration_delta<6>(p, s);
+        if (mode == LS_FULL) iteration_full<6>(p, flags, s); else iteration_delta<6>(p, flags, s);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
@@ -697,11 +776,11 @@ void ls_enqueue_iteration(const LsParams& p, int mode, hipStream_t s) {
 void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
     const bool full = (mode == LS_FULL);
     if (p.g.C == 3) {
-        if (pass == 0) { if (full) launch_fwd_last<double, 3, LS_FULL>(p, s); else launch_fwd_last<float, 3, LS_DELTA>(p, s); }
+        if (pass == 0) { if (full) launch_fwd_last<double, 3, LS_FULL>(p, 0, s); else launch_fwd_last<float, 3, LS_CG>(p, 0, s); }
         else if (pass == 2) { if (full) launch_first_gamma<double, 3>(p, s, mode); else launch_first_gamma<float, 3>(p, s, mode); }
         else throw ArgError("pass not present in 2D");
     } else {
-        if (pass == 0) { if (full) launch_fwd_last<double, 6, LS_FULL>(p, s); else launch_fwd_last<float, 6, LS_DELTA>(p, s); }
+        if (pass == 0) { if (full) launch_fwd_last<double, 6, LS_FULL>(p, 0, s); else launch_fwd_last<float, 6, LS_CG>(p, 0, s); }
         else if (pass == 1) { if (full) launch_mid<double, -1>(p, s, mode); else launch_mid<float, -1>(p, s, mode); }
         else if (pass == 2) { if (full) launch_first_gamma<double, 6>(p, s, mode); else launch_first_gamma<float, 6>(p, s, mode); }
         else if (pass == 3) { if (full) launch_mid<double, +1>(p, s, mode); else launch_mid<float, +1>(p, s, mode); }
@@ -710,14 +789,23 @@ void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void ls_enqueue_start(const LsParams& p, bool warm, hipStream_t s) {
+void ls_enqueue_start(const LsParams& p, int start, hipStream_t s) {
     const int nb = reduce_grid(p.g.N);
+    if (start == LS_START_KEEP) return;
     if (p.g.C == 3) {
-        if (warm) hipLaunchKernelGGL((k_mean_shift<3>), dim3(nb), dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_fill_bar<3>), dim3(nb), dim3(256), 0, s, p);
+        if (start == LS_START_MEAN) {
+            hipLaunchKernelGGL((k_mean_shift<3>), dim3(nb), dim3(256), 0, s, p);
+            hipLaunchKernelGGL((k_mean_reduce<3>), dim3(1), dim3(256), 0, s, p, nb);
+        }
+        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<3>), dim3(nb), dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_apply_shift<3>), dim3(nb), dim3(256), 0, s, p);
     } else {
-        if (warm) hipLaunchKernelGGL((k_mean_shift<6>), dim3(nb), dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_fill_bar<6>), dim3(nb), dim3(256), 0, s, p);
+        if (start == LS_START_MEAN) {
+            hipLaunchKernelGGL((k_mean_shift<6>), dim3(nb), dim3(256), 0, s, p);
+            hipLaunchKernelGGL((k_mean_reduce<6>), dim3(1), dim3(256), 0, s, p, nb);
+        }
+        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<6>), dim3(nb), dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_apply_shift<6>), dim3(nb), dim3(256), 0, s, p);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
@@ -725,11 +813,11 @@ void ls_enqueue_start(const LsParams& p, bool warm, hipStream_t s) {
 void ls_enqueue_final(const LsParams& p, hipStream_t s) {
     const int nb = reduce_grid(p.g.N);
     if (p.g.C == 3) {
-        if (p.corr) hipLaunchKernelGGL((k_apply_corr<3>), dim3(nb), dim3(256), 0, s, p);
         hipLaunchKernelGGL((k_final_stress<3>), dim3(nb), dim3(256), 0, s, p);
+        hipLaunchKernelGGL((k_final_reduce<3>), dim3(1), dim3(256), 0, s, p, nb);
     } else {
-        if (p.corr) hipLaunchKernelGGL((k_apply_corr<6>), dim3(nb), dim3(256), 0, s, p);
         hipLaunchKernelGGL((k_final_stress<6>), dim3(nb), dim3(256), 0, s, p);
+        hipLaunchKernelGGL((k_final_reduce<6>), dim3(1), dim3(256), 0, s, p, nb);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }

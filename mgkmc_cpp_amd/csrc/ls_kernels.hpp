@@ -4,7 +4,24 @@
 
 namespace mgk {
 
-enum { LS_FULL = 0, LS_DELTA = 1 };
+enum { LS_FULL = 0, LS_CG = 1 };
+
+// per-launch flags of passes A and E
+enum {
+    LS_FLAG_SHIFT = 1,        // FULL: add the warm-start shift (iteration 0 of a warm solve)
+    LS_FLAG_CORR = 2,         // FULL: add the pending FP32 correction (refinement path)
+    LS_FLAG_CG_FIRST = 4,     // CG step 0 of a cycle: operator on r0 only
+    LS_FLAG_CG_UPDATE1 = 8,   // CG step 1: p = r, s = w, x = alpha p (no history)
+    LS_FLAG_CG_UPDATE = 16    // CG step >= 2: full recurrence
+};
+
+// how a solve starts
+enum {
+    LS_START_KEEP = 0,        // warm start, shift already in the control block
+    LS_START_MEAN = 1,        // warm start, compute the field mean and the shift
+    LS_START_FILL = 2,        // cold start: eps = eps_bar everywhere
+    LS_START_APPLY = 3        // apply the shift to eps now (warm start with no iteration)
+};
 
 // Device pointers and scalars of one solve.  Real fields are SoA
 // [component][voxel] over the symmetric components (common.hpp order);
@@ -19,9 +36,11 @@ struct LsParams {
     double lam0, mu0;       // reference medium (mean E, mean nu)
     void* spec64;           // FP64 half spectrum, C*NS complex
     void* spec32;           // FP32 half spectrum (may alias spec64's memory)
-    float* r32;             // FP32 copy of the last FP64 increment, or null
-    float* y32;             // FP32 inner iterate
-    float* corr;            // FP32 accumulated correction, or null
+    float* r32;             // CG residual r (the FP64 increment starts it), or null
+    float* w32;             // CG w = A r
+    float* p32;             // CG direction p
+    float* s32;             // CG s = A p
+    float* corr;            // CG iterate x: FP32 correction of eps, or null
     const float* dlam32;    // lam - lam0, mu - mu0 in FP32
     const float* dmu32;
     const void* tw_f64; const void* tw_m64; const void* tw_l64;   // twiddle tables per axis
@@ -32,8 +51,8 @@ struct LsParams {
 
 bool ls_length_supported(int n);
 int ls_partial_slots(const Geom& g);
-void ls_enqueue_start(const LsParams& p, bool warm, hipStream_t s);
-void ls_enqueue_iteration(const LsParams& p, int mode, hipStream_t s);
+void ls_enqueue_start(const LsParams& p, int start, hipStream_t s);
+void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s);
 void ls_enqueue_final(const LsParams& p, hipStream_t s);
 // one pass of the pipeline alone (0 A, 1 B, 2 C, 3 D), for kernel timing
 void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s);

@@ -525,13 +525,46 @@ public:
         time += t_wait;
         eps_macro += strain_unit * (strain_rate * t_wait);
         const double u2 = uniform_dist(rng);
+        // The redraw values come next in the stream (kmc_solver.hpp:839-864)
+        // and do not depend on which voxel was selected, so they are drawn
+        // before the selection is known and the device chains select -> flip.
+        std::normal_distribution<double> standard_normal(0.0, 1.0);   // fresh per event (:840)
+        std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
+        redraw_cat_.resize((size_t)M);
+        redraw_q0_.resize((size_t)M);
+        int first = 0, n = 0;
+        bool has_cat = false, has_q0 = false;
+        if (redraw_directions || redraw_barriers) {
+            for (int mm = 0; mm < M; ++mm) {
+                if (redraw_directions) redraw_cat_[mm] = draw_mode(sn, mm);
+                if (redraw_barriers) redraw_q0_[mm] = barrier_gen.generate(rng);
+            }
+            n = M;
+            has_cat = redraw_directions && !catalog_fixed;   // a fixed catalog redraws to itself
+            has_q0 = redraw_barriers;
+        } else {
+            redraw_cat_[0] = catalog_fixed ? Mat<D>::Zero() : draw_mode(sn, 0);
+            redraw_q0_[0] = barrier_gen.generate(rng);
+            first = -1;   // the selected mode only
+            n = 1;
+            has_cat = !catalog_fixed;
+            has_q0 = true;
+        }
+        check(mgk_select_flip(ctx, u2, time, has_cat ? redraw_cat_[0].data() : nullptr,
+                              has_q0 ? redraw_q0_.data() : nullptr, first, n),
+              "mgk_select_flip");
+        elastic_run();
         int flat = -1;
         double q_sel = 0.0;
-        check(mgk_select(ctx, u2, &flat, &q_sel), "mgk_select");
+        check(mgk_last_selection(ctx, &flat, &q_sel), "mgk_last_selection");
         const long v = flat / M;
         const int is_instab = (q_sel <= stability_threshold) ? 1 : 0;
-        flip_and_redraw(v, flat);
-        elastic_run();
+        // host copies of the redrawn entries
+        for (int j = 0; j < n; ++j) {
+            const long mi = (first < 0) ? flat : v * M + first + j;
+            if (has_cat) catalog[mi] = redraw_cat_[j];
+            if (has_q0) Q0[mi] = redraw_q0_[j];
+        }
         if (record_events) {
             EventRecord e{};
             e.flat_idx = flat;

@@ -172,6 +172,27 @@ class Context:
             n = q0.size
         check(gpu().mgk_apply_flip(self._h, int(flat), float(time), _dp(cat), _dp(q0), int(first_mode), int(n)))
 
+    def select_flip(self, u2, time, new_catalog=None, new_Q0=None, first_mode=0):
+        """Selection and flip chained on the device (no host round trip)."""
+        D = self.D
+        cat = None
+        n = 0
+        if new_catalog is not None:
+            new_catalog = np.asarray(new_catalog, dtype=np.float64)
+            n = new_catalog.shape[0]
+            cat = _aos(new_catalog, n, D)
+        q0 = None
+        if new_Q0 is not None:
+            q0 = np.ascontiguousarray(np.asarray(new_Q0, dtype=np.float64).reshape(-1))
+            n = q0.size
+        check(gpu().mgk_select_flip(self._h, float(u2), float(time), _dp(cat), _dp(q0), int(first_mode), int(n)))
+
+    def last_selection(self):
+        idx = C.c_int(-1)
+        q = C.c_double(0.0)
+        check(gpu().mgk_last_selection(self._h, C.byref(idx), C.byref(q)))
+        return idx.value, q.value
+
     def find_unstable(self):
         n = C.c_int(0)
         check(gpu().mgk_find_unstable(self._h, None, None, 0, C.byref(n)))

@@ -1,0 +1,37 @@
+"""The C-ABI libraries load and export every function include/*.h declares
+(no device call is made: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "mgkmc_cpp_amd", "lib")
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+[\s\*]+(mgk_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.mark.parametrize("header,lib", [("mgkmc_gpu.h", "libmgkmc_gpu.so"), ("mgkmc_host.h", "libmgkmc_host.so")])
+def test_every_declared_symbol_is_exported(header, lib):
+    path = os.path.join(LIB, lib)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (python -m mgkmc_cpp_amd.build)")
+    names = _declared(header)
+    assert len(names) > 10
+    so = ctypes.CDLL(path)
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_abi_version_and_error_path_without_device():
+    from mgkmc_cpp_amd import _lib
+    g = _lib.gpu()
+    assert g.mgk_abi_version() == 1
+    # argument validation happens before any device call
+    rc = g.mgk_create(4, 8, 8, 1, 1.0, 0, 0, None, None, ctypes.byref(ctypes.c_void_p()))
+    assert rc == 1 and b"null" in g.mgk_last_error()

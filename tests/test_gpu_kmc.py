@@ -118,6 +118,38 @@ def test_flip_and_redraw_match_oracle():
     assert np.array_equal(got_cat, cat) and np.array_equal(ctx.get_field(abi.FIELD_Q0)[3:6], q0)
 
 
+def test_select_flip_chain_equals_select_then_flip():
+    c = _cfg(nx=8, ny=8, M=4, instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM,
+             neighbor_softening_fraction=0.2)
+    E, nu = _field(64)
+    osim = _evolved_oracle(c, E, nu, events=6)
+    a = _device_from_oracle(osim, c, E, nu)
+    b = _device_from_oracle(osim, c, E, nu)
+    rng = np.random.default_rng(5)
+    for t, u2 in ((1.0, 0.3), (1.5, 0.91), (2.0, 0.0003)):
+        cat = rng.standard_normal((4, 2, 2)) * 0.05
+        cat = 0.5 * (cat + np.transpose(cat, (0, 2, 1)))
+        q0 = rng.uniform(0.8, 1.2, 4)
+        a.update_rates(t)
+        b.update_rates(t)
+        flat, _ = a.select(u2)
+        a.apply_flip(flat, t, new_catalog=cat, new_Q0=q0, first_mode=0)
+        b.select_flip(u2, t, new_catalog=cat, new_Q0=q0, first_mode=0)
+        assert b.last_selection()[0] == flat
+        for f in (abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_LAST_EVENT_TIME, abi.FIELD_PREV_DIR,
+                  abi.FIELD_CATALOG, abi.FIELD_Q0):
+            assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    # redraw of the selected mode only (both redraw switches off)
+    a.update_rates(3.0)
+    b.update_rates(3.0)
+    flat, _ = a.select(0.5)
+    a.apply_flip(flat, 3.0, new_catalog=cat[:1], new_Q0=q0[:1], first_mode=flat % 4)
+    b.select_flip(0.5, 3.0, new_catalog=cat[:1], new_Q0=q0[:1], first_mode=-1)
+    assert b.last_selection()[0] == flat
+    for f in (abi.FIELD_CATALOG, abi.FIELD_Q0, abi.FIELD_EPS_PLASTIC):
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+
+
 @pytest.mark.parametrize("kw,events", [
     (dict(), 40),                                                          # 2D fixed catalog, kmc-free cascade mode
     (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM), 40),        # random catalog, host-drawn redraws
