@@ -13,9 +13,9 @@
 // direct DFT for other prime factors) with the reference's "ortho" scaling
 // 1/sqrt(N) on both directions (pocketfft c2c with fct = 1/sqrt(N),
 // pocketfft_hdronly.h:3383-3394, :3122-3161).  Built with
-// -DMGK_ORACLE_POCKETFFT and -I<reference>/third_party (oracle/build_ref.sh)
-// the same calls go to the reference's vendored pocketfft instead; that build
-// lives in oracle/_ref/ and pins this FFT in tests.
+// -DMGK_ORACLE_POCKETFFT and -I<reference>/third_party (the `ref` target of
+// oracle/Makefile) the same calls go to the reference's vendored pocketfft
+// instead; that build lives in oracle/_ref/ and pins this FFT in tests.
 //
 // RNG: the same libstdc++ std::mt19937 / std::normal_distribution /
 // std::uniform_real_distribution objects, created and drawn in the same order
