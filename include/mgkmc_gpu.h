@@ -172,6 +172,54 @@ int mgk_select_flip(mgk_ctx* ctx, double u2, double time, const double* new_cata
                     const double* new_Q0, int first_mode, int n_modes);
 int mgk_last_selection(mgk_ctx* ctx, int* flat_idx, double* q_sel);
 
+/* One attempt of run_simulation's inner loop (kmc_solver.hpp:763-952,
+ * 3D :1776-1980) with one host round trip: update_barriers and the FP64 total
+ * rate R, the residence time t_wait = neg_log_u1 / R (the reference's
+ * -log(u1) / total_rate), and, when the attempt is an event, time and
+ * eps_macro(drv_i, drv_j) advanced by t_wait and strain_rate * t_wait,
+ * selection with u2, flip + redraw upload (as mgk_select_flip) and the LS
+ * solve (elastic_run).  The caller draws u1, u2 and the redraw values before
+ * the outcome is known, in the reference's order, and rewinds its generator
+ * when the outcome says the reference would not have drawn them:
+ *   MGK_EVENT_DONE      event applied; eps_macro/sig_macro hold the
+ *                       post-solve macros (eps_macro aliased to the mean)
+ *   MGK_EVENT_UNSTABLE  cascade_check and unstable modes exist (the
+ *                       reference runs run_cascade first; nothing changed)
+ *   MGK_EVENT_NO_RATE   R is not > 0 (nothing drawn in the reference)
+ *   MGK_EVENT_REJECTED  t_wait >= remaining (only u1 drawn in the reference)
+ * remaining = +inf accepts every finite t_wait. */
+enum mgk_event_outcome {
+    MGK_EVENT_DONE = 0,
+    MGK_EVENT_UNSTABLE = 1,
+    MGK_EVENT_NO_RATE = 2,
+    MGK_EVENT_REJECTED = 3
+};
+typedef struct mgk_event_in {
+    double time;                 /* simulation time before the attempt */
+    double neg_log_u1;           /* -log(u1) */
+    double remaining;            /* time left in the strain step */
+    double strain_rate;
+    int drv_i, drv_j;            /* driven component of eps_macro */
+    double u2;
+    const double* new_catalog;   /* redraw, as mgk_select_flip (may be NULL) */
+    const double* new_Q0;
+    int first_mode, n_modes;
+    int max_iter;                /* LS solve (>= 1) */
+    double tol;
+    int cascade_check;           /* instability_mode == "cascade" */
+} mgk_event_in;
+typedef struct mgk_event_out {
+    int outcome;                 /* mgk_event_outcome */
+    int n_unstable, n_included;
+    int flat_idx;                /* selected flat mode, -1 without an event */
+    double total_rate, t_wait, q_sel;
+    int iters;                   /* LS iterations of the solve */
+    int pad;
+    double diff;
+} mgk_event_out;
+int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, double* sig_macro,
+                  mgk_event_out* out);
+
 /* Stream synchronisation and timing of the context's stream (HIP events). */
 int mgk_sync(mgk_ctx* ctx);
 int mgk_timer_start(mgk_ctx* ctx);
@@ -180,10 +228,12 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms);
 /* Measurement helper (no counterpart in the reference): average duration in
  * ms of one pass of the LS pipeline (0: tau + r2c along the last axis,
  * 1: forward c2c along the middle axis (3D), 2: c2c + Green operator +
- * inverse c2c along the first axis, 3: inverse c2c along the middle axis),
- * FP64 (fp32 == 0) or FP32, launched reps times back to back on the
- * context's stream between two HIP events.  Leaves the spectral buffer
- * undefined; the strain, stress and KMC fields are not touched. */
+ * inverse c2c along the first axis, 3: inverse c2c along the middle axis,
+ * 4: c2r along the last axis + update, 5: pass 0 with the FP32 CG vector
+ * update, FP32+refine contexts only), FP64 (fp32 == 0) or FP32, launched
+ * reps times back to back on the context's stream between two HIP events.
+ * Leaves the spectral buffer, the FP32 CG vectors and (pass 4 in FP64) the
+ * strain field undefined: a measurement context, not a simulation one. */
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms);
 
 /* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */

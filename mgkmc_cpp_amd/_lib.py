@@ -56,6 +56,8 @@ def gpu():
         _sig(lib, "mgk_find_unstable", C.c_int, [P, IP, DP, C.c_int, IP])
         _sig(lib, "mgk_select_flip", C.c_int, [P, C.c_double, C.c_double, DP, DP, C.c_int, C.c_int])
         _sig(lib, "mgk_last_selection", C.c_int, [P, IP, DP])
+        _sig(lib, "mgk_kmc_event", C.c_int,
+             [P, C.POINTER(abi.EventIn), DP, DP, C.POINTER(abi.EventOut)])
         _sig(lib, "mgk_sync", C.c_int, [P])
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])

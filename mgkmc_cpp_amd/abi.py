@@ -35,6 +35,39 @@ BARRIER_NONE = 4
 CATALOG_RANDOM = 0
 CATALOG_FIXED = 1
 
+EVENT_DONE = 0
+EVENT_UNSTABLE = 1
+EVENT_NO_RATE = 2
+EVENT_REJECTED = 3
+
+
+class EventIn(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("neg_log_u1", C.c_double),
+        ("remaining", C.c_double),
+        ("strain_rate", C.c_double),
+        ("drv_i", C.c_int), ("drv_j", C.c_int),
+        ("u2", C.c_double),
+        ("new_catalog", C.POINTER(C.c_double)),
+        ("new_Q0", C.POINTER(C.c_double)),
+        ("first_mode", C.c_int), ("n_modes", C.c_int),
+        ("max_iter", C.c_int),
+        ("tol", C.c_double),
+        ("cascade_check", C.c_int),
+    ]
+
+
+class EventOut(C.Structure):
+    _fields_ = [
+        ("outcome", C.c_int),
+        ("n_unstable", C.c_int), ("n_included", C.c_int),
+        ("flat_idx", C.c_int),
+        ("total_rate", C.c_double), ("t_wait", C.c_double), ("q_sel", C.c_double),
+        ("iters", C.c_int), ("pad", C.c_int),
+        ("diff", C.c_double),
+    ]
+
 
 class KmcParams(C.Structure):
     _fields_ = [

@@ -110,6 +110,8 @@ struct mgk_ctx {
     // statistics
     int last_iters = 0, last_inner = 0, predicted = 4;
     int inner_per_outer = 12;
+    int inner_hist[4] = {0, 0, 0, 0};
+    unsigned inner_pos = 0;
     bool mean_valid = false;
     double mean_eps[6] = {0, 0, 0, 0, 0, 0};
 
@@ -262,22 +264,35 @@ struct mgk_ctx {
     // Returns the FP64 iteration count.
     int solve(const double* eps_bar_sym, double norm_extra, bool warm, int max_iter, double tol, double* diff_out) {
         Ctl& c = *hctl;
-        std::memset(&c, 0, sizeof(Ctl));
-        c.max_iter = max_iter;
-        c.tol = tol;
+        init_ctl(max_iter, tol);
         c.norm_extra = norm_extra;
         for (int k = 0; k < g.C; ++k) c.eps_bar[k] = eps_bar_sym[k];
-        int start = LS_START_FILL;
-        if (warm && mean_valid) {
-            // the field is unchanged since the last solve: its mean is known
+        const int start = start_mode(warm);
+        if (start == LS_START_KEEP)   // the field is unchanged since the last solve: its mean is known
             for (int k = 0; k < g.C; ++k) c.shift[k] = eps_bar_sym[k] - mean_eps[k];
-            start = LS_START_KEEP;
-        } else if (warm) {
-            start = LS_START_MEAN;
-        }
         MGK_HIP_CHECK(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        ls_enqueue_start(ls_params(), start, stream);
+        return run_batches(warm, max_iter, tol, diff_out, false);
+    }
+
+    void init_ctl(int max_iter, double tol) {
+        std::memset(hctl, 0, sizeof(Ctl));
+        hctl->max_iter = max_iter;
+        hctl->tol = tol;
+        for (int k = 0; k < g.C; ++k) hctl->mean_eps[k] = mean_eps[k];
+    }
+    int start_mode(bool warm) const {
+        return (warm && mean_valid) ? LS_START_KEEP : (warm ? LS_START_MEAN : LS_START_FILL);
+    }
+
+    // Iteration batches until the control block says done, the final pass
+    // and the read-back: one host sync per solve when the CG-step prediction
+    // holds.  `event`: the solve sits behind a device-decided KMC event
+    // attempt whose control block comes back with it; an attempt without an
+    // event left the solve marked done before its first iteration, and the
+    // solver statistics are then left alone.
+    int run_batches(bool warm, int max_iter, double tol, double* diff_out, bool event) {
         LsParams p = ls_params();
-        ls_enqueue_start(p, start, stream);
         const bool refine = (precision == MGK_FP32_REFINE);
         const int J = inner_per_outer;
         int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
@@ -316,6 +331,7 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
             MGK_HIP_CHECK(hipStreamSynchronize(stream));
         }
+        if (event && !hkctl->go) return 0;
         last_iters = hctl->iters;
         last_inner = hctl->inner;
         predicted = refine ? std::max(1, last_iters) : std::max(2, last_iters);
@@ -326,14 +342,21 @@ struct mgk_ctx {
         return hctl->iters;
     }
 
-    // CG steps per refinement cycle for the next solve: the steps the last
-    // cycle used plus two (steps past convergence return at once).
+    // CG steps per refinement cycle for the next solve: one more than the
+    // most any of the last few single-cycle solves used (steps past
+    // convergence return at once, but each still costs a launch).
     void tune_inner(double tol) {
         (void)tol;
         const int cycles = std::max(1, hctl->iters);
         const int per = (hctl->inner + cycles - 1) / cycles;
-        if (hctl->done && hctl->iters <= 1) inner_per_outer = std::max(4, std::min(40, per + 2));
-        else inner_per_outer = std::min(40, inner_per_outer + 4);
+        if (hctl->done && hctl->iters <= 1) {
+            inner_hist[inner_pos++ % 4] = per;
+            int mx = 0;
+            for (int h : inner_hist) mx = std::max(mx, h);
+            inner_per_outer = std::max(4, std::min(40, mx + 1));
+        } else {
+            inner_per_outer = std::min(40, inner_per_outer + 4);
+        }
     }
 };
 
@@ -841,44 +864,123 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
     });
 }
 
+// Redraw values of a selected voxel into the pinned staging buffer, checked
+// against the mode range and the shared-catalog mode.  Returns the number of
+// modes staged; *has_cat / *has_q0 say which halves the flip applies.
+static int stage_redraw(mgk_ctx& c, const double* new_catalog, const double* new_Q0, int first_mode, int n_modes,
+                        int* has_cat, int* has_q0) {
+    const int C = c.g.C, F = c.full(), M = c.kp.M;
+    if ((new_catalog || new_Q0) &&
+        (n_modes < 0 || (first_mode >= 0 && first_mode + n_modes > M) || (first_mode < 0 && n_modes != 1)))
+        throw ArgError("redraw mode range out of range");
+    if (new_catalog && c.K.cat_shared) {
+        // the voxel is chosen on the device: only a redraw that reproduces the
+        // shared strains (the fixed catalog) keeps the shared copy
+        bool same = first_mode >= 0;
+        double sym[6], skw[3];
+        for (int j = 0; j < n_modes && same; ++j) {
+            c.split(new_catalog + (size_t)j * F, sym, skw);
+            for (int k = 0; k < C; ++k)
+                if (sym[k] != c.shared_cat[(size_t)k * M + first_mode + j]) { same = false; break; }
+        }
+        if (!same) throw ArgError("a redraw that differs from the shared catalog needs mgk_select + mgk_apply_flip");
+        new_catalog = nullptr;
+    }
+    if (!new_catalog && !new_Q0) n_modes = 0;
+    // the pinned staging buffer may still feed the previous call's copy
+    if (c.kctl_pending) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+    double* h = c.hkstage;
+    double skw[3];
+    if (new_catalog)
+        for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
+    if (new_Q0)
+        for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
+    *has_cat = new_catalog ? 1 : 0;
+    *has_q0 = new_Q0 ? 1 : 0;
+    return n_modes;
+}
+
 int mgk_select_flip(mgk_ctx* ctx, double u2, double time, const double* new_catalog, const double* new_Q0,
                     int first_mode, int n_modes) {
     return guarded([&]() -> int {
         if (!ctx) throw ArgError("null context");
         mgk_ctx& c = *ctx;
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
-        const int C = c.g.C, F = c.full(), M = c.kp.M;
-        if ((new_catalog || new_Q0) &&
-            (n_modes < 0 || (first_mode >= 0 && first_mode + n_modes > M) || (first_mode < 0 && n_modes != 1)))
-            throw ArgError("redraw mode range out of range");
-        if (new_catalog && c.K.cat_shared) {
-            // the voxel is chosen on the device: only a redraw that reproduces the
-            // shared strains (the fixed catalog) keeps the shared copy
-            bool same = first_mode >= 0;
-            double sym[6], skw[3];
-            for (int j = 0; j < n_modes && same; ++j) {
-                c.split(new_catalog + (size_t)j * F, sym, skw);
-                for (int k = 0; k < C; ++k)
-                    if (sym[k] != c.shared_cat[(size_t)k * M + first_mode + j]) { same = false; break; }
-            }
-            if (!same) throw ArgError("a redraw that differs from the shared catalog needs mgk_select + mgk_apply_flip");
-            new_catalog = nullptr;
-        }
-        if (!new_catalog && !new_Q0) n_modes = 0;
-        // the pinned staging buffer may still feed the previous call's copy
-        if (c.kctl_pending) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
-        double* h = c.hkstage;
-        double skw[3];
-        if (new_catalog)
-            for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
-        if (new_Q0)
-            for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
+        int has_cat = 0, has_q0 = 0;
+        n_modes = stage_redraw(c, new_catalog, new_Q0, first_mode, n_modes, &has_cat, &has_q0);
         kmc_enqueue_select(c.K, u2, c.stream);
         kmc_enqueue_select_fallback(c.K, 1, c.stream);
         if (n_modes > 0)
-            MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, h, sizeof(double) * n_modes * (C + 1), hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(c.K, -1, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
+            MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
+                                         hipMemcpyHostToDevice, c.stream));
+        kmc_enqueue_flip(c.K, -1, time, first_mode, n_modes, has_cat, has_q0, c.stream);
         c.kctl_pending = true;   // the selection is read back with the next solve's sync
+        return MGK_OK;
+    });
+}
+
+int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, double* sig_macro, mgk_event_out* out) {
+    return guarded([&]() -> int {
+        if (!ctx || !in || !eps_macro || !sig_macro || !out) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const int Dd = c.D();
+        if (in->drv_i < 0 || in->drv_i >= Dd || in->drv_j < 0 || in->drv_j >= Dd) throw ArgError("driven component out of range");
+        if (in->max_iter < 1) throw ArgError("max_iter must be >= 1");
+        int has_cat = 0, has_q0 = 0;
+        const int n_modes = stage_redraw(c, in->new_catalog, in->new_Q0, in->first_mode, in->n_modes, &has_cat, &has_q0);
+        // LS control block: the device fills the imposed strain and the shift
+        c.init_ctl(in->max_iter, in->tol);
+        MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
+        const int start = c.start_mode(true);
+        EventArgs ea{};
+        ea.time = in->time;
+        ea.neg_log_u1 = in->neg_log_u1;
+        ea.remaining = in->remaining;
+        ea.strain_rate = in->strain_rate;
+        for (int k = 0; k < Dd * Dd; ++k) ea.a[k] = eps_macro[k];
+        ea.n_vox = (double)c.g.N;
+        ea.drv = in->drv_i + Dd * in->drv_j;
+        ea.cascade_check = in->cascade_check ? 1 : 0;
+        ea.keep_shift = (start == LS_START_KEEP) ? 1 : 0;
+        kmc_enqueue_rates(c.K, in->time, c.stream);
+        kmc_enqueue_event_prep(c.K, c.ctl, ea, c.stream);
+        kmc_enqueue_select(c.K, in->u2, c.stream);
+        kmc_enqueue_select_fallback(c.K, 1, c.stream);
+        if (n_modes > 0)
+            MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
+                                         hipMemcpyHostToDevice, c.stream));
+        kmc_enqueue_flip(c.K, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1);
+        ls_enqueue_start(c.ls_params(), start, c.stream);
+        c.kctl_pending = true;
+        double diff = 0.0;
+        const int it = c.run_batches(true, in->max_iter, in->tol, &diff, true);
+        const KmcCtl& k = *c.hkctl;
+        std::memset(out, 0, sizeof(*out));
+        out->outcome = k.outcome;
+        out->total_rate = k.total_rate;
+        out->n_unstable = k.n_unstable;
+        out->n_included = k.n_included;
+        out->t_wait = k.t_wait;
+        out->flat_idx = k.go ? k.sel : -1;
+        out->q_sel = k.go ? k.q_sel : 0.0;
+        if (k.outcome != MGK_EVENT_DONE) return MGK_OK;
+        out->iters = it;
+        out->diff = diff;
+        // eps_macro as the device updated it, then replaced by the field mean
+        // (elastic_run's alias, kmc_solver.hpp:334-339)
+        const double x = in->strain_rate * k.t_wait;
+        const int drv = in->drv_i + Dd * in->drv_j;
+        double a[9];
+        for (int q = 0; q < Dd * Dd; ++q) a[q] = eps_macro[q] + ((q == drv) ? 1.0 : 0.0) * x;
+        const int ns = c.nskew();
+        double bar_sym[6], bar_skew[3] = {0, 0, 0};
+        c.split(a, bar_sym, bar_skew);
+        for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
+        double sk_sig[3];
+        for (int w = 0; w < ns; ++w) sk_sig[w] = 2.0 * c.mean_mu * bar_skew[w];
+        c.join(c.hctl->mean_eps, bar_skew, eps_macro);
+        c.join(c.hctl->mean_sig, sk_sig, sig_macro);
         return MGK_OK;
     });
 }
@@ -948,7 +1050,8 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
     return guarded([&]() -> int {
         if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
-        if (pass < 0 || pass > 3 || (ctx->g.dims == 2 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        if (pass < 0 || pass > 5 || (ctx->g.dims == 2 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        if (pass == 5 && (!fp32 || !ctx->corr)) throw ArgError("pass 5 needs an FP32+refine context");
         mgk_ctx& c = *ctx;
         // a control block that lets every pass run (not done, shift off)
         std::memset(c.hctl, 0, sizeof(Ctl));
@@ -956,9 +1059,9 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         c.hctl->max_iter = 1 << 30;
         MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
         LsParams p = c.ls_params();
-        p.corr = nullptr;
+        p.corr = (pass == 5) ? c.corr : nullptr;
         const int mode = fp32 ? LS_CG : LS_FULL;
-        p.r32 = fp32 ? c.cg : nullptr;   // the FP32 pass A reads the CG residual buffer
+        p.r32 = fp32 ? c.cg : nullptr;   // the FP32 passes read and write the CG vectors
         ls_enqueue_pass(p, pass, mode, c.stream);   // warm-up launch
         MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
         for (int r = 0; r < reps; ++r) ls_enqueue_pass(p, pass, mode, c.stream);

@@ -145,41 +145,64 @@ template <int DIR, typename T> struct Bfly<5, DIR, T> {
 };
 
 // ---------------------------------------------------------------------------
+// Line accessors.  A stage reads element j of its line through in(j) and
+// writes element k through out(k, v): LDS lines, or global columns with a
+// stride (the first and last stages of a register-resident transform).
+// ---------------------------------------------------------------------------
+template <typename R> struct LineRef {
+    cplx<R>* p;
+    long es;
+    __device__ __forceinline__ cplx<R> operator()(int j) const { return p[j * es]; }
+    __device__ __forceinline__ void operator()(int j, cplx<R> v) const { p[j * es] = v; }
+};
+
+// ---------------------------------------------------------------------------
 // One Stockham stage for one line: input element j + r*N/R, output
 // (j/Ns)*Ns*R + j%Ns + r*Ns, twiddle W_N^{r*(j%Ns)*N/(Ns*R)}.
 // ---------------------------------------------------------------------------
-template <int N, int R, int NS, int DIR, int T, typename Real>
-__device__ __forceinline__ void stage_load(const cplx<Real>* line, int es, int t, const cplx<Real>* tw,
-                                           cplx<Real> (&v)[N / T]) {
+template <int N, int R, int NS, int DIR, int T, typename Real, class In>
+__device__ __forceinline__ void stage_gather(const In& in, int t, cplx<Real> (&v)[N / T]) {
     static_assert(N % (R * T) == 0, "threads per line must divide N/R at every stage");
     constexpr int B = N / (R * T);   // butterflies per lane
 #pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        const int kk = j % NS;
+    for (int b = 0; b < B; ++b)
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            cplx<Real> x = line[(j + r * (N / R)) * es];
-            if (NS > 1 && r > 0) {
+        for (int r = 0; r < R; ++r) v[b * R + r] = in(t + b * T + r * (N / R));
+}
+
+template <int N, int R, int NS, int DIR, int T, typename Real>
+__device__ __forceinline__ void stage_compute(int t, const cplx<Real>* tw, cplx<Real> (&v)[N / T]) {
+    constexpr int B = N / (R * T);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int kk = (t + b * T) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            if (NS > 1) {
                 cplx<Real> w = tw[(r * kk * (N / (NS * R))) % N];
                 if (DIR > 0) w.y = -w.y;
-                x = cmul(x, w);
+                v[b * R + r] = cmul(v[b * R + r], w);
             }
-            v[b * R + r] = x;
         }
         Bfly<R, DIR, Real>::run(&v[b * R]);
     }
 }
 
-template <int N, int R, int NS, int T, typename Real>
-__device__ __forceinline__ void stage_store(cplx<Real>* line, int es, int t, const cplx<Real> (&v)[N / T]) {
+template <int N, int R, int NS, int DIR, int T, typename Real, class In>
+__device__ __forceinline__ void stage_load(const In& in, int t, const cplx<Real>* tw, cplx<Real> (&v)[N / T]) {
+    stage_gather<N, R, NS, DIR, T, Real>(in, t, v);
+    stage_compute<N, R, NS, DIR, T, Real>(t, tw, v);
+}
+
+template <int N, int R, int NS, int T, typename Real, class Out>
+__device__ __forceinline__ void stage_store(const Out& out, int t, const cplx<Real> (&v)[N / T]) {
     constexpr int B = N / (R * T);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = t + b * T;
         const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-        for (int r = 0; r < R; ++r) line[(base + r * NS) * es] = v[b * R + r];
+        for (int r = 0; r < R; ++r) out(base + r * NS, v[b * R + r]);
     }
 }
 
@@ -189,10 +212,11 @@ __device__ __forceinline__ void run_stages(cplx<Real>* line, int es, int t, bool
     if constexpr (S < P::stages) {
         constexpr int R = P::radix[S];
         constexpr int T = P::threads;
+        const LineRef<Real> ln{line, es};
         cplx<Real> v[N / T];
-        if (active) stage_load<N, R, NS, DIR, T, Real>(line, es, t, tw, v);
+        if (active) stage_load<N, R, NS, DIR, T, Real>(ln, t, tw, v);
         __syncthreads();
-        if (active) stage_store<N, R, NS, T, Real>(line, es, t, v);
+        if (active) stage_store<N, R, NS, T, Real>(ln, t, v);
         __syncthreads();
         run_stages<N, S + 1, NS * R, DIR, Real>(line, es, t, active, tw);
     }
@@ -212,6 +236,63 @@ __device__ __forceinline__ void lds_fft(cplx<Real>* base, int nlines, int ls, in
         const bool active = ((int)threadIdx.x < lines_per_pass * T) && (l < nlines);
         cplx<Real>* line = base + (active ? l : 0) * ls;
         run_stages<N, 0, 1, DIR, Real>(line, es, t, active, tw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident line transform with arbitrary endpoints.  Lane t (of T)
+// of one line first gathers its stage-0 inputs (line_gather: plain loads,
+// nothing waits on them), then line_finish runs the stages: stage 0 in
+// registers, the middle stages through the line's LDS scratch `work`, and the
+// last stage straight to `dst` (global or LDS).  With lanes of consecutive
+// lines adjacent in a wave, the gather and the final store are coalesced
+// across lines.  Barriers: every lane of the block must call line_finish.
+//   SRC_LDS: the gathered source lives in LDS that `work` may overlap, so a
+//            barrier separates all stage-0 reads from the first LDS write;
+//   DST_LDS: dst is LDS (a barrier precedes the last-stage writes, which
+//            then need one more barrier before anyone reads them).
+// Stage 0 has no twiddles, so `tw` is first read after a barrier.
+// ---------------------------------------------------------------------------
+template <int N, typename Real, class Src>
+__device__ __forceinline__ void line_gather(const Src& src, int t, bool active, cplx<Real> (&v)[N / FftPlan<N>::threads]) {
+    using P = FftPlan<N>;
+    if (active) stage_gather<N, P::radix[0], 1, -1, P::threads, Real>(src, t, v);
+}
+
+template <int N, int S, int NS, int DIR, typename Real, bool DST_LDS, class Dst>
+__device__ __forceinline__ void line_rest(int t, bool active, const Dst& dst, const LineRef<Real>& work,
+                                          const cplx<Real>* tw) {
+    using P = FftPlan<N>;
+    constexpr int R = P::radix[S];
+    constexpr int T = P::threads;
+    cplx<Real> v[N / T];
+    if (active) stage_load<N, R, NS, DIR, T, Real>(work, t, tw, v);
+    if constexpr (S + 1 == P::stages) {
+        if constexpr (DST_LDS) __syncthreads();
+        if (active) stage_store<N, R, NS, T, Real>(dst, t, v);
+    } else {
+        __syncthreads();
+        if (active) stage_store<N, R, NS, T, Real>(work, t, v);
+        __syncthreads();
+        line_rest<N, S + 1, NS * R, DIR, Real, DST_LDS>(t, active, dst, work, tw);
+    }
+}
+
+template <int N, int DIR, typename Real, bool SRC_LDS, bool DST_LDS, class Dst>
+__device__ __forceinline__ void line_finish(cplx<Real> (&v)[N / FftPlan<N>::threads], int t, bool active,
+                                            const Dst& dst, const LineRef<Real>& work, const cplx<Real>* tw) {
+    using P = FftPlan<N>;
+    constexpr int R = P::radix[0];
+    constexpr int T = P::threads;
+    if (active) stage_compute<N, R, 1, DIR, T, Real>(t, tw, v);
+    if constexpr (P::stages == 1) {
+        if constexpr (SRC_LDS && DST_LDS) __syncthreads();
+        if (active) stage_store<N, R, 1, T, Real>(dst, t, v);
+    } else {
+        if constexpr (SRC_LDS) __syncthreads();
+        if (active) stage_store<N, R, 1, T, Real>(work, t, v);
+        __syncthreads();
+        line_rest<N, 1, R, DIR, Real, DST_LDS>(t, active, dst, work, tw);
     }
 }
 

@@ -30,29 +30,56 @@ __device__ __forceinline__ double mode_rate(const KmcPhys& ph, double q, bool& i
 
 // ---------------------------------------------------------------------------
 // update_barriers + rates (kmc_solver.hpp:341-410, :783-806; 3D :1340-1411,
-// :1796-1818).  One lane per (voxel, mode), 1024 modes per block.  The
-// catalog is per (voxel, mode) or, when every voxel holds the same M
-// strains, one shared copy of M entries.
+// :1796-1818).  One lane per (voxel, mode), kRateBlock modes per block of
+// kRateThreads lanes: every input of a lane's modes is loaded before any is
+// used.  The catalog is per (voxel, mode) or, when every voxel holds the same
+// M strains, one shared copy of M entries.
 // ---------------------------------------------------------------------------
 template <int D, int C>
-__global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
+__global__ __launch_bounds__(kRateThreads) void k_rates(KmcParams p, double time) {
+    constexpr int K = kRateBlock / kRateThreads;
     const Geom g = p.g;
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long base = (long)blockIdx.x * kRateBlock;
+    const double GPa_nm3_to_eV = 6.241509;
+
+    long mi[K], vi[K];
+    bool ok[K];
+    double gp[K], gt[K], tl[K], q0[K], s[K][C], cat[K][C], pd[K][C];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const long i = base + k * kRateThreads + threadIdx.x;
+        ok[k] = i < nmodes;
+        mi[k] = ok[k] ? i : 0;
+        vi[k] = mi[k] / ph.M;
+        gp[k] = p.soft[vi[k]];
+        gt[k] = p.soft[g.N + vi[k]];
+        tl[k] = p.last_t[vi[k]];
+        q0[k] = p.Q0[mi[k]];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            s[k][c] = p.sig[c * g.N + vi[k]];
+            cat[k][c] = p.cat_shared ? p.cat[c * ph.M + (mi[k] - vi[k] * ph.M)] : p.cat[c * nmodes + mi[k]];
+        }
+    }
+    if (ph.directional) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < C; ++c) pd[k][c] = p.prev_dir[c * g.N + vi[k]];
+    }
+
     double lsum = 0.0;
     int lun = 0, linc = 0;
-    const double GPa_nm3_to_eV = 6.241509;
 #pragma unroll
-    for (int k = 0; k < kRateBlock / 256; ++k) {
-        const long i = base + k * 256 + threadIdx.x;
-        if (i >= nmodes) break;
-        const long v = i / ph.M;
+    for (int k = 0; k < K; ++k) {
+        if (!ok[k]) continue;
         // softening state of the voxel
-        const double g_t = p.soft[g.N + v];
+        const double g_t = gt[k];
         double g_t_curr = 0.0;
         if (g_t > 0.0) {
-            const double t_last = p.last_t[v];
+            const double t_last = tl[k];
             if (ph.tau_inf) g_t_curr = g_t;
             else if (t_last == -INFINITY) g_t_curr = 0.0;
             else {
@@ -61,14 +88,8 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
                 g_t_curr = g_t * exp(-dt / ph.tau);
             }
         }
-        double g_base = p.soft[v] + g_t_curr;
+        double g_base = gp[k] + g_t_curr;
         if (ph.cap > 0.0 && g_base > ph.cap) g_base = ph.cap;
-        double cat[C], s[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            cat[c] = p.cat_shared ? p.cat[c * ph.M + (i - v * ph.M)] : p.cat[c * nmodes + i];
-            s[c] = p.sig[c * g.N + v];
-        }
         double modifier = 1.0;
         if (ph.directional) {
             double dot = 0.0, nmq = 0.0, npq = 0.0;
@@ -77,7 +98,7 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
 #pragma unroll
                 for (int b = 0; b < D; ++b) {
                     const int c = comp_of(D, a, b);
-                    const double vm = cat[c], vp = p.prev_dir[c * g.N + v];
+                    const double vm = cat[k][c], vp = pd[k][c];
                     dot += vm * vp;
                     nmq += vm * vm;
                     npq += vp * vp;
@@ -95,18 +116,19 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
 #pragma unroll
             for (int b = 0; b < D; ++b) {
                 const int c = comp_of(D, a, b);
-                w_sum += s[c] * cat[c];
+                w_sum += s[k][c] * cat[k][c];
             }
         const double w = 0.5 * ph.volume * (w_sum / 1e9) * GPa_nm3_to_eV;
-        const double q = p.Q0[i] * exp(-g_eff) - w;
-        p.Q[i] = q;
+        const double q = q0[k] * exp(-g_eff) - w;
+        p.Q[mi[k]] = q;
         bool incl;
         const double r = mode_rate(ph, q, incl);
         if (incl) { lsum += r; linc++; }
         if (q < ph.thr) lun++;
     }
-    __shared__ double sd[4];
-    __shared__ int su[4], si[4];
+    constexpr int NW = kRateThreads / 64;
+    __shared__ double sd[NW];
+    __shared__ int su[NW], si[NW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     lsum = wsum(lsum);
     lun = wsum(lun);
@@ -114,89 +136,154 @@ __global__ __launch_bounds__(256) void k_rates(KmcParams p, double time) {
     if (lane == 0) { sd[wid] = lsum; su[wid] = lun; si[wid] = linc; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        p.bsum[blockIdx.x] = (sd[0] + sd[1]) + (sd[2] + sd[3]);
-        p.bunst[blockIdx.x] = su[0] + su[1] + su[2] + su[3];
-        p.binc[blockIdx.x] = si[0] + si[1] + si[2] + si[3];
+        double t = 0.0;
+        int tu = 0, ti = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) { t += sd[w]; tu += su[w]; ti += si[w]; }
+        p.bsum[blockIdx.x] = t;
+        p.bunst[blockIdx.x] = tu;
+        p.binc[blockIdx.x] = ti;
+    }
+}
+
+// Inclusive wave scan (lane order) of a double and two ints.
+__device__ __forceinline__ void wave_scan(double& v, int& a, int& b) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double vv = __shfl_up(v, off, 64);
+        const int aa = __shfl_up(a, off, 64), bb = __shfl_up(b, off, 64);
+        if (lane >= off) { v += vv; a += aa; b += bb; }
     }
 }
 
 // One block: inclusive prefix of the block sums (chunked, fixed order), the
 // exclusive prefix of unstable counts and the totals.  A kernel of its own so
 // that the rate blocks need no device-scope fence.
-__global__ __launch_bounds__(256) void k_rates_scan(KmcParams p, int nb) {
-    const int chunk = (nb + 255) / 256;
-    const int b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
+__global__ __launch_bounds__(1024) void k_rates_scan(KmcParams p, int nb) {
+    constexpr int G = 8;   // block sums per lane loaded together
+    const int chunk = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * chunk;
     double csum = 0.0;
     int cun = 0, cinc = 0;
-    for (int b = b0; b < b1; ++b) {
-        csum += p.bsum[b];
-        cun += p.bunst[b];
-        cinc += p.binc[b];
+    for (int g0 = 0; g0 < chunk; g0 += G) {
+        double vs[G];
+        int vu[G], vi[G];
+        bool ok[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int b = b0 + g0 + k;
+            ok[k] = (g0 + k < chunk) && (b < nb);
+            vs[k] = p.bsum[ok[k] ? b : 0];
+            vu[k] = p.bunst[ok[k] ? b : 0];
+            vi[k] = p.binc[ok[k] ? b : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (ok[k]) { csum += vs[k]; cun += vu[k]; cinc += vi[k]; }
     }
-    __shared__ double sc[256];
-    __shared__ int scu[256], sci[256];
-    sc[threadIdx.x] = csum;
-    scu[threadIdx.x] = cun;
-    sci[threadIdx.x] = cinc;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over the 256 chunk totals
-    for (int off = 1; off < 256; off <<= 1) {
-        double a = 0.0;
-        int au = 0, ai = 0;
-        if ((int)threadIdx.x >= off) { a = sc[threadIdx.x - off]; au = scu[threadIdx.x - off]; ai = sci[threadIdx.x - off]; }
-        __syncthreads();
-        sc[threadIdx.x] += a;
-        scu[threadIdx.x] += au;
-        sci[threadIdx.x] += ai;
-        __syncthreads();
-    }
-    double run = (threadIdx.x > 0) ? sc[threadIdx.x - 1] : 0.0;
-    int runu = (threadIdx.x > 0) ? scu[threadIdx.x - 1] : 0;
-    for (int b = b0; b < b1; ++b) {
-        run += p.bsum[b];
-        p.bincl[b] = run;
-        p.bunst_off[b] = runu;
-        runu += p.bunst[b];
-    }
-    __shared__ double s_total;
-    // the total is the last inclusive prefix, so that target <= bincl[nb-1]
-    if (b0 < nb && b1 == nb) s_total = run;
+    // exclusive prefix of the lane totals: wave scans, then the wave totals
+    __shared__ double ws[16];
+    __shared__ int wu[16], wi[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double incl_s = csum;
+    int incl_u = cun, incl_i = cinc;
+    wave_scan(incl_s, incl_u, incl_i);
+    if (lane == 63) { ws[wid] = incl_s; wu[wid] = incl_u; wi[wid] = incl_i; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double tot = s_total;
+        double r = 0.0;
+        int ru = 0, ri = 0;
+        for (int w = 0; w < 16; ++w) {
+            const double t = ws[w];
+            const int tu = wu[w], ti = wi[w];
+            ws[w] = r; wu[w] = ru; wi[w] = ri;   // exclusive wave offsets
+            r += t; ru += tu; ri += ti;
+        }
+    }
+    __syncthreads();
+    double ex_s = __shfl_up(incl_s, 1, 64);
+    int ex_u = __shfl_up(incl_u, 1, 64);
+    if (lane == 0) { ex_s = 0.0; ex_u = 0; }
+    double run = ws[wid] + ex_s;
+    int runu = wu[wid] + ex_u;
+    for (int g0 = 0; g0 < chunk; g0 += G) {
+        double vs[G];
+        int vu[G];
+        bool ok[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int b = b0 + g0 + k;
+            ok[k] = (g0 + k < chunk) && (b < nb);
+            vs[k] = p.bsum[ok[k] ? b : 0];
+            vu[k] = p.bunst[ok[k] ? b : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if (!ok[k]) continue;
+            const int b = b0 + g0 + k;
+            run += vs[k];
+            p.bincl[b] = run;
+            p.bunst_off[b] = runu;
+            runu += vu[k];
+        }
+    }
+    __shared__ double s_total;
+    __shared__ int s_unst, s_inc;
+    // the total is the last inclusive prefix, so that target <= bincl[nb-1]
+    if (b0 < nb && b0 + chunk >= nb) s_total = run;
+    if (threadIdx.x == 1023) { s_unst = wu[15] + incl_u; s_inc = wi[15] + incl_i; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         KmcCtl* kc = p.kctl;
-        kc->total_rate = tot;
-        kc->n_unstable = scu[255];
-        kc->n_included = sci[255];
+        kc->total_rate = s_total;
+        kc->n_unstable = s_unst;
+        kc->n_included = s_inc;
         kc->nblocks = nb;
         kc->sel = -1;
         kc->ambiguous = 0;
         kc->ticket = 0;
+        kc->go = 1;
+        kc->outcome = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Selection: block by binary search over inclusive block prefixes, mode by an
-// in-block scan; a rounding-bound guard flags targets too close to call.
+// Selection: block by a two-level search over the inclusive block prefixes
+// (non-decreasing), mode by an in-block scan; a rounding-bound guard flags
+// targets too close to call.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
     const long nmodes = p.g.N * p.ph.M;
     KmcCtl* kc = p.kctl;
+    if (!kc->go) return;
     const int nb = kc->nblocks;
     const double total = kc->total_rate;
     const double target = u2 * total;
-    __shared__ int s_b;
+    __shared__ int s_b, s_chunk;
     __shared__ double s_scan[kRateBlock];
     __shared__ double s_tot[256];
     __shared__ int s_first;
     if (threadIdx.x == 0) {
-        int lo = 0, hi = nb - 1;
-        while (lo < hi) {   // first b with bincl[b] >= target
-            int mid = (lo + hi) / 2;
-            if (p.bincl[mid] >= target) hi = mid; else lo = mid + 1;
-        }
-        s_b = lo;
+        s_b = nb - 1;   // no prefix reaches the target: the last block
+        s_chunk = 0x7fffffff;
         s_first = kRateBlock;
+    }
+    __syncthreads();
+    // first b with bincl[b] >= target: the first lane chunk whose last prefix
+    // reaches it, then the first entry inside that chunk
+    const int chunk = (nb + 255) / 256;
+    {
+        const int lo = threadIdx.x * chunk;
+        const int last = min(nb, lo + chunk) - 1;
+        if (lo < nb && p.bincl[last] >= target) atomicMin(&s_chunk, (int)threadIdx.x);
+    }
+    __syncthreads();
+    if (s_chunk != 0x7fffffff) {
+        for (int k = threadIdx.x; k < chunk; k += 256) {
+            const int bb = s_chunk * chunk + k;
+            if (bb < nb && p.bincl[bb] >= target) atomicMin(&s_b, bb);
+        }
     }
     __syncthreads();
     const int b = s_b;
@@ -261,7 +348,7 @@ __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
 __global__ void k_select_seq(KmcParams p, int only_if_ambiguous) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
-    if (only_if_ambiguous && !kc->ambiguous) return;
+    if (!kc->go || (only_if_ambiguous && !kc->ambiguous)) return;
     const long nmodes = p.g.N * p.ph.M;
     double tot = 0.0;
     long last = -1;
@@ -293,12 +380,18 @@ __global__ void k_select_seq(KmcParams p, int only_if_ambiguous) {
 // voxel several times and each hit accumulates.
 // ---------------------------------------------------------------------------
 template <int D, int C>
-__global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n_modes, int has_cat, int has_q0) {
+__global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n_modes, int has_cat, int has_q0,
+                       int time_from_ctl) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    // flat < 0: the mode the last selection chose; first_mode < 0: redraw
-    // only that mode (both redraw switches off, kmc_solver.hpp:856-864)
-    if (flat < 0) flat = p.kctl->sel;
+    // flat < 0: the mode the last selection chose, if that attempt produced
+    // an event; first_mode < 0: redraw only that mode (both redraw switches
+    // off, kmc_solver.hpp:856-864)
+    if (flat < 0) {
+        if (!p.kctl->go) return;
+        flat = p.kctl->sel;
+    }
     if (flat < 0) return;
+    if (time_from_ctl) time = p.kctl->time_new;
     if (first_mode < 0) first_mode = flat % p.ph.M;
     const Geom g = p.g;
     const KmcPhys ph = p.ph;
@@ -387,15 +480,80 @@ __global__ __launch_bounds__(256) void k_unstable(KmcParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// One attempt of the reference's inner loop decided on the device
+// (kmc_solver.hpp:763-815, :942-952; 3D :1776-1826, :1967-1977), after the
+// rate pass:
+//   instability_mode "cascade" with unstable modes  -> MGK_EVENT_UNSTABLE
+//   total rate not > 0                              -> MGK_EVENT_NO_RATE
+//   t_wait = -log(u1) / R not < remaining           -> MGK_EVENT_REJECTED
+//   else                                            -> MGK_EVENT_DONE:
+//     time_new = time + t_wait, eps_macro(drv) += strain_rate * t_wait, and
+//     the LS control block gets the imposed strain (symmetric part, norm of
+//     the antisymmetric part) and the warm-start shift.
+// Every operation is the host's own, in its order (no contraction in this
+// file), so time_new and the imposed strain equal the host's bit for bit.
+// ---------------------------------------------------------------------------
+template <int D, int C>
+__global__ void k_event_prep(KmcParams p, Ctl* ctl, EventArgs ea) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    KmcCtl* kc = p.kctl;
+    const double R = kc->total_rate;
+    int outcome;
+    double tw = 0.0;
+    if (ea.cascade_check && kc->n_unstable > 0) outcome = 1;          // MGK_EVENT_UNSTABLE
+    else if (!(R > 0.0)) outcome = 2;                                 // MGK_EVENT_NO_RATE
+    else {
+        tw = ea.neg_log_u1 / R;
+        outcome = (tw < ea.remaining) ? 0 : 3;                        // MGK_EVENT_DONE / _REJECTED
+    }
+    kc->t_wait = tw;
+    kc->outcome = outcome;
+    kc->go = (outcome == 0) ? 1 : 0;
+    if (outcome != 0) {
+        ctl->done = 1;
+        return;
+    }
+    kc->time_new = ea.time + tw;
+    // eps_macro += strain_unit * (strain_rate * t_wait), entry by entry
+    const double x = ea.strain_rate * tw;
+    double a[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) a[k] = ea.a[k] + ((k == ea.drv) ? 1.0 : 0.0) * x;
+    double sym[C], skew[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = i; j < D; ++j) {
+            const int c = comp_of(D, i, j);
+            if (i == j) sym[c] = a[i + D * i];
+            else {
+                sym[c] = 0.5 * (a[i + D * j] + a[j + D * i]);
+                const int w = (D == 2) ? 0 : (c - 3);
+                skew[w] = 0.5 * (a[i + D * j] - a[j + D * i]);
+            }
+        }
+    double norm_extra = 0.0;
+    constexpr int NS = (D == 2) ? 1 : 3;
+#pragma unroll
+    for (int w = 0; w < NS; ++w) norm_extra += 2.0 * ea.n_vox * skew[w] * skew[w];
+    ctl->norm_extra = norm_extra;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        ctl->eps_bar[c] = sym[c];
+        if (ea.keep_shift) ctl->shift[c] = sym[c] - ctl->mean_eps[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlock); }
 
 void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s) {
     const int nb = kmc_nblocks(p.g.N * p.ph.M);
-    if (p.g.dims == 2) hipLaunchKernelGGL((k_rates<2, 3>), dim3(nb), dim3(256), 0, s, p, time);
-    else hipLaunchKernelGGL((k_rates<3, 6>), dim3(nb), dim3(256), 0, s, p, time);
-    hipLaunchKernelGGL(k_rates_scan, dim3(1), dim3(256), 0, s, p, nb);
+    if (p.g.dims == 2) hipLaunchKernelGGL((k_rates<2, 3>), dim3(nb), dim3(kRateThreads), 0, s, p, time);
+    else hipLaunchKernelGGL((k_rates<3, 6>), dim3(nb), dim3(kRateThreads), 0, s, p, time);
+    hipLaunchKernelGGL(k_rates_scan, dim3(1), dim3(1024), 0, s, p, nb);
     MGK_HIP_CHECK(hipGetLastError());
 }
 
@@ -410,11 +568,19 @@ void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipS
 }
 
 void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
-                      int has_q0, hipStream_t s) {
+                      int has_q0, hipStream_t s, int time_from_ctl) {
     if (p.g.dims == 2)
-        hipLaunchKernelGGL((k_flip<2, 3>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0);
+        hipLaunchKernelGGL((k_flip<2, 3>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0,
+                           time_from_ctl);
     else
-        hipLaunchKernelGGL((k_flip<3, 6>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0);
+        hipLaunchKernelGGL((k_flip<3, 6>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0,
+                           time_from_ctl);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_event_prep(const KmcParams& p, Ctl* ctl, const EventArgs& ea, hipStream_t s) {
+    if (p.g.dims == 2) hipLaunchKernelGGL((k_event_prep<2, 3>), dim3(1), dim3(64), 0, s, p, ctl, ea);
+    else hipLaunchKernelGGL((k_event_prep<3, 6>), dim3(1), dim3(64), 0, s, p, ctl, ea);
     MGK_HIP_CHECK(hipGetLastError());
 }
 

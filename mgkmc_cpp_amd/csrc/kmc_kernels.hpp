@@ -10,13 +10,31 @@ struct KmcCtl {
     double u2;             // selection uniform
     double target;         // u2 * total_rate used by the selection
     double q_sel;
+    double t_wait;         // residence time of a device-decided event
+    double time_new;       // time + t_wait
     int n_unstable;        // Q < stability_threshold
     int n_included;        // rated modes
     int sel;               // selected flat mode index, -1 if none
     int ambiguous;         // 1: the parallel prefix cannot decide bit-exactly
     unsigned ticket;
     int nblocks;
-    int pad0, pad1;
+    int go;                // 1: selection and flip proceed (0: no event this attempt)
+    int outcome;           // MGK_EVENT_* of a device-decided event
+};
+
+// Arguments of the device-side event decision (one attempt of the
+// reference's inner loop, kmc_solver.hpp:763-952 / :1776-1980).
+struct EventArgs {
+    double time;           // simulation time before the attempt
+    double neg_log_u1;     // -log(u1) of the residence-time draw
+    double remaining;      // time left in the strain step
+    double strain_rate;    // eps_macro(drv) += strain_rate * t_wait
+    double a[9];           // eps_macro before the attempt, D x D column-major
+    double n_vox;          // N, for the antisymmetric norm term
+    int drv;               // driven entry of a (row + D * col)
+    int cascade_check;     // instability_mode == "cascade": unstable modes pre-empt the event
+    int keep_shift;        // the field mean is known: shift = eps_bar - mean (else a mean pass follows)
+    int pad;
 };
 
 // Per-trajectory KMC physics scalars (device copy of mgk_kmc_params plus
@@ -54,14 +72,19 @@ struct KmcParams {
     double* unst_q;
 };
 
-constexpr int kRateBlock = 1024;   // modes per rate block (256 lanes x 4)
+constexpr int kRateBlock = 1024;   // modes per rate block
+constexpr int kRateThreads = 512;  // lanes per rate block (2 modes each)
 
 int kmc_nblocks(long nmodes);
 void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s);
 void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s);
 void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipStream_t s);
+// time_from_ctl: the flip time is the device-decided event's time_new
 void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
-                      int has_q0, hipStream_t s);
+                      int has_q0, hipStream_t s, int time_from_ctl = 0);
 void kmc_enqueue_unstable(const KmcParams& p, hipStream_t s);
+// decision of one event attempt after kmc_enqueue_rates; on "no event" it
+// marks the LS control block done so a solve enqueued behind it returns at once
+void kmc_enqueue_event_prep(const KmcParams& p, Ctl* ctl, const EventArgs& ea, hipStream_t s);
 
 }  // namespace mgk

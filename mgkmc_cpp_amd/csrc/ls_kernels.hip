@@ -48,6 +48,7 @@
 #include "fft_lds.hpp"
 #include "ls_kernels.hpp"
 #include <algorithm>
+#include <type_traits>
 
 namespace mgk {
 
@@ -144,178 +145,298 @@ __device__ __forceinline__ void gamma_apply(const cplx<R>* t, cplx<R>* o, const 
 }
 
 // ---------------------------------------------------------------------------
-// Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW lines per block.
+// Block shapes (compile time, so every per-thread loop unrolls and a thread
+// issues all of its global loads before it waits on any of them).
+// ---------------------------------------------------------------------------
+static constexpr int kLdsBudget = 48 * 1024;     // passes A and E
+static constexpr int kColLdsBudget = 64 * 1024;  // passes B, C and D
+
+// Upper limits of the block shapes (line pairs of passes A/E, columns of
+// B/D, frequency points of C); build-time tuning knobs.
+#ifndef MGK_LAST_PAIRS_CAP
+#define MGK_LAST_PAIRS_CAP 8
+#endif
+#ifndef MGK_MID_COLS_CAP
+#define MGK_MID_COLS_CAP 256
+#endif
+#ifndef MGK_GAMMA_POINTS_CAP
+#define MGK_GAMMA_POINTS_CAP 256
+#endif
+
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int pow2floor(int x) { int p = 1; while (2 * p <= x) p *= 2; return p; }
+
+// Line pairs per block of the L passes (two real lines share one complex FFT
+// line): a power of two whose FFT lanes fit 256 and whose LDS fits the budget.
+template <typename SR, int N, int C> constexpr int last_pairs() {
+    constexpr int T = FftPlan<N>::threads;
+    constexpr int by_thr = 256 / (C * T);
+    constexpr int by_lds = (int)((kLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
+    return pow2floor(cmax(1, cmin(cmin(by_thr, by_lds), MGK_LAST_PAIRS_CAP)));
+}
+
+// Columns per block of the M passes.
+template <typename SR, int N> constexpr int mid_cols() {
+    constexpr int T = FftPlan<N>::threads;
+    constexpr int by_lds = (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (padded<N>() * (int)sizeof(cplx<SR>)));
+    return pow2floor(cmax(1, cmin(cmin(256 / T, by_lds), MGK_MID_COLS_CAP)));
+}
+
+// Frequency points (all C components each) per block of the F pass.
+template <typename SR, int N, int C> constexpr int gamma_points() {
+    constexpr int T = FftPlan<N>::threads;
+    constexpr int by_lds =
+        (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
+    return cmax(1, cmin(cmin(256 / (T * C), by_lds), MGK_GAMMA_POINTS_CAP));
+}
+
+// ---------------------------------------------------------------------------
+// Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW = 2*PAIRS lines per
+// block, VPT voxels per lane.
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int LPW, int flags) {
-    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
+__global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int flags) {
+    constexpr int PAIRS = last_pairs<SR, NL, C>();
+    constexpr int LPW = 2 * PAIRS;
+    constexpr int VPT = (LPW * NL + 255) / 256;
     constexpr int NLP = padded<NL>();
+    constexpr int NLH = NL / 2 + 1;
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
+    const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
-    const int P = LPW / 2;
     const Geom g = p.g;
+    const long nlines = g.nlines();
     const long line0 = (long)blockIdx.x * LPW;
-    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
 
-    const bool add_shift = (MODE == LS_FULL) && (flags & LS_FLAG_SHIFT);
-    const bool add_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.ctl->corr_valid;
-    double shift[C];
+    // gather every per-voxel input of this lane first
+    long vid[VPT];
+    bool ok[VPT];
 #pragma unroll
-    for (int c = 0; c < C; ++c) shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
-    SR* bufr = reinterpret_cast<SR*>(buf);
-    for (int vox = threadIdx.x; vox < LPW * NL; vox += blockDim.x) {
-        const int li = vox / NL, j = vox % NL;
-        const long line = line0 + li;
-        SR tau[C];
-        if (line < g.nlines()) {
-            const long v = line * NL + j;
-            if constexpr (MODE == LS_FULL) {
-                double e[C], ee[C];
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + 256 * k;
+        const long line = line0 + vox / NL;
+        ok[k] = (vox < LPW * NL) && (line < nlines);
+        vid[k] = ok[k] ? line * NL + vox % NL : 0;
+    }
+    const bool want_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.corr;
+    const bool has_epsp = p.epsp != nullptr;
+    const bool upd = (MODE == LS_CG) && (flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE));
+    const bool upd1 = (MODE == LS_CG) && (flags & LS_FLAG_CG_UPDATE1);
+    // Loads are unconditional (lanes past the end read voxel 0; absent
+    // optional fields read a valid stand-in) and nothing consumes them before
+    // the last one is issued.
+    // FULL: e = eps, ep = eps_p, cr = pending FP32 correction.
+    // CG:   e = r; w, p, s, x (x in cr).
+    using VR = typename std::conditional<MODE == LS_FULL, double, float>::type;
+    VR e[VPT][C], ep[VPT][C];
+    float cr[VPT][C], w4[VPT][C], p4[VPT][C], s4[VPT][C];
+    float mu_[VPT], lam_[VPT];
+    double mu64[VPT], lam64[VPT];
+    if constexpr (MODE == LS_FULL) {
+        const double* src_ep = has_epsp ? p.epsp : p.eps;
+        const float* src_cr = want_corr ? p.corr : reinterpret_cast<const float*>(p.eps);
 #pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    e[c] = p.eps[c * g.N + v] + shift[c];
-                    if (add_corr) e[c] += (double)p.corr[c * g.N + v];
-                    ee[c] = p.epsp ? e[c] - p.epsp[c * g.N + v] : e[c];
-                }
-                const double mu = p.mu[v], lam = p.lam[v];
-                double tre = 0.0, tr0 = 0.0;
+        for (int k = 0; k < VPT; ++k) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) { tre += ee[c]; tr0 += e[c]; }
-                const double tm = 2.0 * mu, lt = lam * tre, tm0 = 2.0 * p.mu0, lt0 = p.lam0 * tr0;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const double s = tm * ee[c] + (c < D ? lt : 0.0);
-                    const double s0 = tm0 * e[c] + (c < D ? lt0 : 0.0);
-                    tau[c] = (SR)(s - s0);
-                }
-            } else {
-                // CG vector updates (Chronopoulos-Gear, lines 7-10), then tau = (C - C0):r
-                float y[C];
-                const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
-                if (upd) {
-                    const float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
-                    const bool first = flags & LS_FLAG_CG_UPDATE1;
-#pragma unroll
-                    for (int c = 0; c < C; ++c) {
-                        const long o = c * g.N + v;
-                        const float r = p.r32[o], w = p.w32[o];
-                        const float pp = first ? r : r + be * p.p32[o];
-                        const float ss = first ? w : w + be * p.s32[o];
-                        // x is the correction of eps_1 = eps_0 + r0, the field the FP64
-                        // step left: x_j = d_j - r0 where d_1 = alpha_0 r0 solves A d = r0
-                        const float xx = first ? (al - 1.0f) * r : p.corr[o] + al * pp;
-                        const float rn = r - al * ss;
-                        p.p32[o] = pp;
-                        p.s32[o] = ss;
-                        p.corr[o] = xx;
-                        p.r32[o] = rn;
-                        y[c] = rn;
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < C; ++c) y[c] = p.r32[c * g.N + v];
-                }
-                float tr = 0.f;
-#pragma unroll
-                for (int c = 0; c < D; ++c) tr += y[c];
-                const float dm2 = 2.0f * p.dmu32[v], dl = p.dlam32[v] * tr;
-#pragma unroll
-                for (int c = 0; c < C; ++c) tau[c] = (SR)(dm2 * y[c] + (c < D ? dl : 0.0f));
+            for (int c = 0; c < C; ++c) {
+                e[k][c] = p.eps[c * g.N + vid[k]];
+                ep[k][c] = src_ep[c * g.N + vid[k]];
+                cr[k][c] = src_cr[c * g.N + vid[k]];
             }
-        } else {
-#pragma unroll
-            for (int c = 0; c < C; ++c) tau[c] = (SR)0;
+            mu64[k] = p.mu[vid[k]];
+            lam64[k] = p.lam[vid[k]];
         }
-        const int pr = li >> 1, part = li & 1;
+    } else {
 #pragma unroll
-        for (int c = 0; c < C; ++c) bufr[2 * ((c * P + pr) * NLP + j) + part] = tau[c];
+        for (int k = 0; k < VPT; ++k) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) e[k][c] = p.r32[c * g.N + vid[k]];
+            mu_[k] = p.dmu32[vid[k]];
+            lam_[k] = p.dlam32[vid[k]];
+        }
+        if (upd) {
+#pragma unroll
+            for (int k = 0; k < VPT; ++k)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const long o = c * g.N + vid[k];
+                    w4[k][c] = p.w32[o];
+                    cr[k][c] = p.corr[o];
+                    p4[k][c] = p.p32[o];
+                    s4[k][c] = p.s32[o];
+                }
+        }
+    }
+    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
+    if (stop) return;
+
+    SR* bufr = reinterpret_cast<SR*>(buf);
+    if constexpr (MODE == LS_FULL) {
+        const bool add_shift = flags & LS_FLAG_SHIFT;
+        const bool add_corr = want_corr && p.ctl->corr_valid;
+        double shift[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
+        const double tm0 = 2.0 * p.mu0;
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const int vox = threadIdx.x + 256 * k;
+            if (vox >= LPW * NL) continue;
+            const int li = vox / NL, j = vox % NL;
+            double ee[C], ek[C];
+            double tre = 0.0, tr0 = 0.0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                ek[c] = e[k][c] + shift[c];
+                if (add_corr) ek[c] += (double)cr[k][c];
+                ee[c] = has_epsp ? ek[c] - ep[k][c] : ek[c];
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) { tre += ee[c]; tr0 += ek[c]; }
+            const double tm = 2.0 * mu64[k], lt = lam64[k] * tre, lt0 = p.lam0 * tr0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const double s = tm * ee[c] + (c < D ? lt : 0.0);
+                const double s0 = tm0 * ek[c] + (c < D ? lt0 : 0.0);
+                bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)] = ok[k] ? (SR)(s - s0) : (SR)0;
+            }
+        }
+    } else {
+        // CG vector updates (Chronopoulos-Gear), then tau = (C - C0):r
+        const float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const int vox = threadIdx.x + 256 * k;
+            if (vox >= LPW * NL) continue;
+            const int li = vox / NL, j = vox % NL;
+            float y[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (upd && ok[k]) {
+                    const long o = c * g.N + vid[k];
+                    const float r = e[k][c], w = w4[k][c];
+                    const float pp = upd1 ? r : r + be * p4[k][c];
+                    const float ss = upd1 ? w : w + be * s4[k][c];
+                    // x is the correction of eps_1 = eps_0 + r0, the field the FP64
+                    // step left: x_j = d_j - r0 where d_1 = alpha_0 r0 solves A d = r0
+                    const float xx = upd1 ? (al - 1.0f) * r : cr[k][c] + al * pp;
+                    const float rn = r - al * ss;
+                    p.p32[o] = pp;
+                    p.s32[o] = ss;
+                    p.corr[o] = xx;
+                    p.r32[o] = rn;
+                    y[c] = rn;
+                } else {
+                    y[c] = e[k][c];
+                }
+            }
+            float tr = 0.f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) tr += y[c];
+            const float dm2 = 2.0f * mu_[k], dl = lam_[k] * tr;
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)] =
+                    ok[k] ? (SR)(dm2 * y[c] + (c < D ? dl : 0.0f)) : (SR)0;
+        }
     }
     __syncthreads();
-    lds_fft<NL, -1, SR>(buf, C * P, NLP, 1, tw);
+    lds_fft<NL, -1, SR>(buf, C * PAIRS, NLP, 1, tw);
     // split the packed spectrum into the two half spectra
     cx* spec = reinterpret_cast<cx*>(MODE == LS_FULL ? p.spec64 : p.spec32);
-    const int nlh = g.nlh;
-    for (int idx = threadIdx.x; idx < LPW * nlh * C; idx += blockDim.x) {
-        const int c = idx % C, k = (idx / C) % nlh, li = idx / (C * nlh);
+    constexpr int SPT = (LPW * NLH * C + 255) / 256;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        if (idx >= LPW * NLH * C) continue;
+        const int c = idx % C, kb = (idx / C) % NLH, li = idx / (C * NLH);
         const long line = line0 + li;
-        if (line >= g.nlines()) continue;
+        if (line >= nlines) continue;
         const int pr = li >> 1;
-        const cx z = buf[(c * P + pr) * NLP + k];
-        const cx zm = buf[(c * P + pr) * NLP + (NL - k) % NL];
+        const cx z = buf[(c * PAIRS + pr) * NLP + kb];
+        const cx zm = buf[(c * PAIRS + pr) * NLP + (NL - kb) % NL];
         cx out;
         if ((li & 1) == 0) out = {(SR)0.5 * (z.x + zm.x), (SR)0.5 * (z.y - zm.y)};
         else out = {(SR)0.5 * (z.y + zm.y), (SR)-0.5 * (z.x - zm.x)};
-        spec[(line * nlh + k) * C + c] = out;
+        spec[(line * NLH + kb) * C + c] = out;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Passes B/D: c2c along M for one f plane, a tile of (l-bin, component)
-// columns.
+// Passes B/D: c2c along M.  Columns are the (f, l-bin, component) triples,
+// contiguous in (l-bin, component) for one f, stride nlh*C along m.  A block
+// owns L consecutive columns; lane (l, t) = (tid % L, tid / L), so a wave's
+// gather and store touch consecutive columns of one row.
 // ---------------------------------------------------------------------------
 template <typename SR, int NM, int DIR>
-__global__ __launch_bounds__(256) void k_mid(LsParams p, int TK, int ntiles, int mode) {
-    if (p.ctl->done || (mode == LS_CG && p.ctl->cg_stop)) return;
+__global__ __launch_bounds__(256) void k_mid(LsParams p, int mode) {
+    constexpr int T = FftPlan<NM>::threads;
+    constexpr int L = mid_cols<SR, NM>();
     constexpr int NMP = padded<NM>();
     using cx = cplx<SR>;
+    const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
-    cx* buf = tw + NM;
+    cx* work = tw + NM;
     const Geom g = p.g;
-    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_m64 : p.tw_m32), NM);
     const int ncol = g.nlh * g.C;
-    const int tile = blockIdx.x % ntiles;
-    const long f = blockIdx.x / ntiles;
-    const int k0 = tile * TK;
-    const int tk = min(TK, ncol - k0);
-    cx* base = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32) + f * (long)NM * ncol + k0;
-    for (int idx = threadIdx.x; idx < NM * TK; idx += blockDim.x) {
-        const int kk = idx % TK, m = idx / TK;
-        if (kk < tk) buf[kk * NMP + m] = base[(long)m * ncol + kk];
-    }
-    __syncthreads();
-    lds_fft<NM, DIR, SR>(buf, tk, NMP, 1, tw);
-    for (int idx = threadIdx.x; idx < NM * TK; idx += blockDim.x) {
-        const int kk = idx % TK, m = idx / TK;
-        if (kk < tk) base[(long)m * ncol + kk] = buf[kk * NMP + m];
-    }
+    const long total = (long)g.nf * ncol;
+    const int l = threadIdx.x % L, t = threadIdx.x / L;
+    const long F = (long)blockIdx.x * L + l;
+    const bool active = F < total;
+    const long f = active ? F / ncol : 0;
+    const long col = active ? F % ncol : 0;
+    cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
+    const LineRef<SR> column{spec + f * (long)NM * ncol + col, ncol};
+    cx v[NM / T];
+    line_gather<NM, SR>(column, t, active, v);
+    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_m64 : p.tw_m32), NM);
+    if (stop) return;
+    line_finish<NM, DIR, SR, false, false>(v, t, active, column, LineRef<SR>{work + l * NMP, 1}, tw);
 }
 
 // ---------------------------------------------------------------------------
-// Pass C: c2c along F, Green operator, inverse c2c along F.  A block owns TC
-// consecutive columns of the flattened (m, l-bin) index, all C components.
+// Pass C: c2c along F, Green operator, inverse c2c along F.  Columns are the
+// (m, l-bin, component) triples, contiguous for one f (stride nm*nlh*C along
+// f).  A block owns G frequency points (m, l-bin) with all C components, i.e.
+// L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
 template <typename SR, int NF, int C>
-__global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mode) {
-    if (p.ctl->done || (mode == LS_CG && p.ctl->cg_stop)) return;
-    constexpr int NFP = padded<NF>();
+__global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int mode) {
     constexpr int D = (C == 3) ? 2 : 3;
+    constexpr int T = FftPlan<NF>::threads;
+    constexpr int G = gamma_points<SR, NF, C>();
+    constexpr int L = G * C;
+    constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
+    const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
-    cx* buf = tw + NF;
+    cx* work = tw + NF;
     const Geom g = p.g;
-    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
-    const long ncols = (long)g.nm * g.nlh;
-    const long col0 = (long)blockIdx.x * TC;
+    const long nxi = (long)g.nm * g.nlh;
+    const int l = threadIdx.x % L, t = threadIdx.x / L;
+    const long xi0 = (long)blockIdx.x * G;
+    const bool active = xi0 + l / C < nxi;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
-    for (int idx = threadIdx.x; idx < NF * TC * C; idx += blockDim.x) {
-        const int c = idx % C, cc = (idx / C) % TC, f = idx / (C * TC);
-        const long col = col0 + cc;
-        cx v = {(SR)0, (SR)0};
-        if (col < ncols) v = spec[((long)f * ncols + col) * C + c];
-        buf[(c * TC + cc) * NFP + f] = v;
-    }
+    const LineRef<SR> column{spec + (active ? xi0 * C + l : 0), nxi * C};
+    const LineRef<SR> mine{work + l * NFP, 1};
+    cx v[NF / T];
+    line_gather<NF, SR>(column, t, active, v);
+    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
+    if (stop) return;
+    line_finish<NF, -1, SR, false, true>(v, t, active, mine, mine, tw);
     __syncthreads();
-    lds_fft<NF, -1, SR>(buf, C * TC, NFP, 1, tw);
+
     const SR a = (SR)(0.5 / p.mu0);
     const SR B = (SR)((p.lam0 + p.mu0) / (p.mu0 * (p.lam0 + 2.0 * p.mu0)));
     const SR scale = (SR)(1.0 / (double)g.N);
-    for (int idx = threadIdx.x; idx < NF * TC; idx += blockDim.x) {
-        const int cc = idx % TC, f = idx / TC;
-        const long col = col0 + cc;
-        if (col >= ncols) continue;
-        const int km = (int)(col / g.nlh), kl = (int)(col % g.nlh);
+    for (int idx = threadIdx.x; idx < NF * G; idx += blockDim.x) {
+        const int gg = idx % G, f = idx / G;
+        const long xi = xi0 + gg;
+        if (xi >= nxi) continue;
+        const int km = (int)(xi / g.nlh), kl = (int)(xi % g.nlh);
         // signed frequencies k/n as fftfreq (linear_elastic_solver.hpp:22-33)
         auto sfreq = [](int k, int n, bool& nyq) -> SR {
             nyq = (n % 2 == 0) && (k == n / 2) && (n > 1);
@@ -332,9 +453,9 @@ __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mod
             nm_ = false;
             qa[1] = sfreq(kl, g.nl, nl_);
         }
-        cx t[C], o[C];
+        cx tt[C], o[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) t[c] = buf[(c * TC + cc) * NFP + f];
+        for (int c = 0; c < C; ++c) tt[c] = work[(gg * C + c) * NFP + f];
         SR k2 = (SR)0;
 #pragma unroll
         for (int d = 0; d < D; ++d) k2 += qa[d] * qa[d];
@@ -346,116 +467,165 @@ __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int TC, int mod
             SR q[3];
 #pragma unroll
             for (int d = 0; d < D; ++d) q[d] = qa[d] * inv;
-            gamma_apply<D, SR>(t, o, q, a, B);
+            gamma_apply<D, SR>(tt, o, q, a, B);
             const bool fl[3] = {nf_, D == 3 ? nm_ : nl_, nl_};
             if (fl[0] || fl[1] || (D == 3 && fl[2])) {
                 SR q2[3];
 #pragma unroll
                 for (int d = 0; d < D; ++d) q2[d] = fl[d] ? -q[d] : q[d];
                 cx o2[C];
-                gamma_apply<D, SR>(t, o2, q2, a, B);
+                gamma_apply<D, SR>(tt, o2, q2, a, B);
 #pragma unroll
                 for (int c = 0; c < C; ++c) o[c] = {(SR)0.5 * (o[c].x + o2[c].x), (SR)0.5 * (o[c].y + o2[c].y)};
             }
         }
 #pragma unroll
-        for (int c = 0; c < C; ++c) buf[(c * TC + cc) * NFP + f] = {-o[c].x * scale, -o[c].y * scale};
+        for (int c = 0; c < C; ++c) work[(gg * C + c) * NFP + f] = {-o[c].x * scale, -o[c].y * scale};
     }
     __syncthreads();
-    lds_fft<NF, +1, SR>(buf, C * TC, NFP, 1, tw);
-    for (int idx = threadIdx.x; idx < NF * TC * C; idx += blockDim.x) {
-        const int c = idx % C, cc = (idx / C) % TC, f = idx / (C * TC);
-        const long col = col0 + cc;
-        if (col < ncols) spec[((long)f * ncols + col) * C + c] = buf[(c * TC + cc) * NFP + f];
-    }
+    line_gather<NF, SR>(mine, t, active, v);
+    line_finish<NF, +1, SR, true, false>(v, t, active, column, mine, tw);
 }
 
 // ---------------------------------------------------------------------------
-// Pass E: c2r along L + update.  FULL: norms and the last-block convergence
-// test.  DELTA: y <- Gamma~(y), corr (+)= y; no reduction.
+// Pass E: c2r along L + update.  FULL: eps_new = eps_bar + eps_tilde and the
+// norms of the relative-increment test.  CG: w = A r and its C0 products.
+// Block partials go to p.partials; k_ls_reduce finishes.
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_inv_last(LsParams p, int LPW, int flags) {
-    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
+__global__ __launch_bounds__(256) void k_inv_last(LsParams p, int flags) {
+    constexpr int PAIRS = last_pairs<SR, NL, C>();
+    constexpr int LPW = 2 * PAIRS;
+    constexpr int VPT = (LPW * NL + 255) / 256;
     constexpr int NLP = padded<NL>();
+    constexpr int NLH = NL / 2 + 1;
+    constexpr int SPT = (PAIRS * NLH * C + 255) / 256;
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
+    const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
     __shared__ double red[8 * 4 * 2];
-    const int P = LPW / 2;
     const Geom g = p.g;
+    const long nlines = g.nlines();
     const long line0 = (long)blockIdx.x * LPW;
-    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
+
+    // gather: the two half spectra of each line pair, and the voxel inputs
     const cx* spec = reinterpret_cast<const cx*>(MODE == LS_FULL ? p.spec64 : p.spec32);
-    const int nlh = g.nlh;
-    for (int idx = threadIdx.x; idx < P * nlh * C; idx += blockDim.x) {
-        const int c = idx % C, k = (idx / C) % nlh, pr = idx / (C * nlh);
+    cx A[SPT], Bv[SPT];
+    bool okA[SPT], okB[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        const int c = idx % C, kb = (idx / C) % NLH, pr = idx / (C * NLH);
         const long la = line0 + 2 * pr, lb = la + 1;
-        cx A = {(SR)0, (SR)0}, Bv = {(SR)0, (SR)0};
-        if (la < g.nlines()) A = spec[(la * nlh + k) * C + c];
-        if (lb < g.nlines()) Bv = spec[(lb * nlh + k) * C + c];
-        cx* line = buf + (c * P + pr) * NLP;
-        if (k == 0 || 2 * k == NL) {
-            line[k] = {A.x, Bv.x};   // self-conjugate bins: real parts only (c2r)
+        const bool in = idx < PAIRS * NLH * C;
+        okA[k] = in && la < nlines;
+        okB[k] = in && lb < nlines;
+        A[k] = spec[okA[k] ? (la * NLH + kb) * C + c : 0];
+        Bv[k] = spec[okB[k] ? (lb * NLH + kb) * C + c : 0];
+    }
+    long vid[VPT];
+    bool ok[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + 256 * k;
+        const long line = line0 + vox / NL;
+        ok[k] = (vox < LPW * NL) && (line < nlines);
+        vid[k] = ok[k] ? line * NL + vox % NL : 0;
+    }
+    const bool want_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.corr;
+    using VR = typename std::conditional<MODE == LS_FULL, double, float>::type;
+    VR old[VPT][C];
+    float cr[VPT][C];
+    const float* src_cr = want_corr ? p.corr : reinterpret_cast<const float*>(p.eps);
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if constexpr (MODE == LS_FULL) {
+                old[k][c] = p.eps[c * g.N + vid[k]];
+                cr[k][c] = src_cr[c * g.N + vid[k]];
+            } else {
+                old[k][c] = p.r32[c * g.N + vid[k]];
+            }
+        }
+    }
+    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(MODE == LS_FULL ? p.tw_l64 : p.tw_l32), NL);
+    if (stop) return;
+
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        if (idx >= PAIRS * NLH * C) continue;
+        const int c = idx % C, kb = (idx / C) % NLH, pr = idx / (C * NLH);
+        cx* line = buf + (c * PAIRS + pr) * NLP;
+        if (!okA[k]) A[k] = {(SR)0, (SR)0};
+        if (!okB[k]) Bv[k] = {(SR)0, (SR)0};
+        if (kb == 0 || 2 * kb == NL) {
+            line[kb] = {A[k].x, Bv[k].x};   // self-conjugate bins: real parts only (c2r)
         } else {
-            line[k] = {A.x - Bv.y, A.y + Bv.x};
-            line[NL - k] = {A.x + Bv.y, Bv.x - A.y};
+            line[kb] = {A[k].x - Bv[k].y, A[k].y + Bv[k].x};
+            line[NL - kb] = {A[k].x + Bv[k].y, Bv[k].x - A[k].y};
         }
     }
     __syncthreads();
-    lds_fft<NL, +1, SR>(buf, C * P, NLP, 1, tw);
+    lds_fft<NL, +1, SR>(buf, C * PAIRS, NLP, 1, tw);
 
-    const bool add_shift = (MODE == LS_FULL) && (flags & LS_FLAG_SHIFT);
-    const bool add_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.ctl->corr_valid;
-    double shift[C], bar[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
-        bar[c] = (MODE == LS_FULL) ? p.ctl->eps_bar[c] : 0.0;
-    }
     const double mu2 = 2.0 * p.mu0, lam0 = p.lam0;
     const SR* bufr = reinterpret_cast<const SR*>(buf);
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int vox = threadIdx.x; vox < LPW * NL; vox += blockDim.x) {
-        const int li = vox / NL, j = vox % NL;
-        const long line = line0 + li;
-        if (line >= g.nlines()) continue;
-        const long v = line * NL + j;
-        const int pr = li >> 1, part = li & 1;
-        if constexpr (MODE == LS_FULL) {
+    if constexpr (MODE == LS_FULL) {
+        const bool add_shift = flags & LS_FLAG_SHIFT;
+        const bool add_corr = want_corr && p.ctl->corr_valid;
+        double shift[C], bar[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            shift[c] = add_shift ? p.ctl->shift[c] : 0.0;
+            bar[c] = p.ctl->eps_bar[c];
+        }
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            if (!ok[k]) continue;
+            const int vox = threadIdx.x + 256 * k;
+            const int li = vox / NL, j = vox % NL;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const SR et = bufr[2 * ((c * P + pr) * NLP + j) + part];
+                const SR et = bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)];
                 const double w = (c < D) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
-                double old = p.eps[c * g.N + v] + shift[c];
-                if (add_corr) old += (double)p.corr[c * g.N + v];
+                double o = old[k][c] + shift[c];
+                if (add_corr) o += (double)cr[k][c];
                 const double nw = bar[c] + (double)et;
-                const double d = nw - old;
+                const double d = nw - o;
                 acc[0] += w * d * d;
-                acc[1] += w * old * old;
+                acc[1] += w * o * o;
                 acc[2] += w * nw * nw;
-                p.eps[c * g.N + v] = nw;
-                if (p.r32) p.r32[c * g.N + v] = (float)d;
+                p.eps[c * g.N + vid[k]] = nw;
+                if (p.r32) p.r32[c * g.N + vid[k]] = (float)d;
             }
-        } else {
-            // w = A r = r - Gamma~(r); C0-energy products (r, r), (w, r) and ||r||_W^2
-            double rr = 0.0, wr = 0.0, nr = 0.0, trr = 0.0, trw = 0.0;
+        }
+    } else {
+        // w = A r = r - Gamma~(r); C0-energy products (r, r), (w, r) and ||r||_W^2
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            if (!ok[k]) continue;
+            const int vox = threadIdx.x + 256 * k;
+            const int li = vox / NL, j = vox % NL;
+            double rr = 0.0, wr = 0.0, trr = 0.0, trw = 0.0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const float et = (float)bufr[2 * ((c * P + pr) * NLP + j) + part];
-                const float r = p.r32[c * g.N + v];
+                const float et = (float)bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)];
+                const float r = old[k][c];
                 const float w = r - et;
-                p.w32[c * g.N + v] = w;
+                p.w32[c * g.N + vid[k]] = w;
                 const double wt = (c < D) ? 1.0 : 2.0;
                 rr += wt * (double)r * (double)r;
                 wr += wt * (double)w * (double)r;
                 if (c < D) { trr += r; trw += w; }
             }
-            nr = rr;
             acc[0] += mu2 * rr + lam0 * trr * trr;
             acc[1] += mu2 * wr + lam0 * trw * trr;
-            acc[2] += nr;
+            acc[2] += rr;
         }
     }
     block_sum<3>(acc, red);
@@ -638,48 +808,17 @@ bool ls_length_supported(int n) {
 
 static int blocks_for(long n, int per) { return (int)((n + per - 1) / per); }
 
-// LDS budget per block (bytes); keeps several blocks resident per CU.
-static constexpr int kLdsBudget = 48 * 1024;
-
-static int fft_threads(int n) {
-#define MGK_CASE(NN) if (n == NN) return FftPlan<NN>::threads;
-    MGK_LEN_CASES(MGK_CASE)
-#undef MGK_CASE
-    return 1;
-}
-
-// Lines per block of the L passes: an even count whose FFT lanes fit one
-// 256-lane pass and whose LDS stays within the budget.
-template <typename SR> static int lines_per_block(int nl, int C) {
-    int nlp = nl + ((nl % 16 == 0) ? 1 : 0);
-    int bytes_per_pair = C * nlp * (int)sizeof(cplx<SR>);
-    int pairs_lds = (kLdsBudget - nl * (int)sizeof(cplx<SR>)) / bytes_per_pair;
-    int pairs_thr = 256 / (C * fft_threads(nl));
-    int pairs = std::max(1, std::min(std::min(pairs_lds, pairs_thr), 8));
-    return 2 * pairs;
-}
-
-// Columns per block of the F pass, same two limits.
-template <typename SR> static int cols_per_block(int nf, int C) {
-    int nfp = nf + ((nf % 16 == 0) ? 1 : 0);
-    int tc_lds = (kLdsBudget - nf * (int)sizeof(cplx<SR>)) / (C * nfp * (int)sizeof(cplx<SR>));
-    int tc_thr = 256 / (C * fft_threads(nf));
-    return std::max(1, std::min(std::min(tc_lds, tc_thr), 16));
-}
-
-template <typename SR> static size_t smem_bytes(int n, int lines, int nlp) {
-    return (size_t)(n + (size_t)lines * nlp) * sizeof(cplx<SR>);
-}
-
 template <typename SR, int C, int MODE>
 static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
-    const int lpw = lines_per_block<SR>(nl, C);
-    const int nb = blocks_for(p.g.nlines(), lpw);
-    const int nlp = nl + ((nl % 16 == 0) ? 1 : 0);
-    const size_t sm = smem_bytes<SR>(nl, C * lpw / 2, nlp);
-#define MGK_CASE(NN) \
-    if (nl == NN) { hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw, flags); return; }
+#define MGK_CASE(NN)                                                                                          \
+    if (nl == NN) {                                                                                           \
+        constexpr int pairs = last_pairs<SR, NN, C>();                                                        \
+        const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                        \
+        hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), dim3(256), \
+                           sm, s, p, flags);                                                                  \
+        return;                                                                                               \
+    }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("FFT length not supported on the last axis");
@@ -688,15 +827,14 @@ static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
 template <typename SR, int C, int MODE>
 static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
-    const int lpw = lines_per_block<SR>(nl, C);
-    const int nb = blocks_for(p.g.nlines(), lpw);
-    const int nlp = nl + ((nl % 16 == 0) ? 1 : 0);
-    const size_t sm = smem_bytes<SR>(nl, C * lpw / 2, nlp);
-#define MGK_CASE(NN) \
-    if (nl == NN) { \
-        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, lpw, flags); \
-        hipLaunchKernelGGL((k_ls_reduce<MODE>), dim3(1), dim3(256), 0, s, p, nb, flags); \
-        return; \
+#define MGK_CASE(NN)                                                                                  \
+    if (nl == NN) {                                                                                   \
+        constexpr int pairs = last_pairs<SR, NN, C>();                                                \
+        const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                \
+        const int nb = blocks_for(p.g.nlines(), 2 * pairs);                                           \
+        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, flags);     \
+        hipLaunchKernelGGL((k_ls_reduce<MODE>), dim3(1), dim3(256), 0, s, p, nb, flags);              \
+        return;                                                                                       \
     }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
@@ -706,16 +844,15 @@ static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
 template <typename SR, int DIR>
 static void launch_mid(const LsParams& p, hipStream_t s, int mode) {
     const int nm = p.g.nm;
-    const int nmp = nm + ((nm % 16 == 0) ? 1 : 0);
-    const int ncol = p.g.nlh * p.g.C;
-    int tkmax = (kLdsBudget - nm * (int)sizeof(cplx<SR>)) / (nmp * (int)sizeof(cplx<SR>));
-    tkmax = std::max(1, std::min(tkmax, std::max(1, 256 / fft_threads(nm))));
-    const int ntiles = (ncol + tkmax - 1) / tkmax;
-    const int tk = (ncol + ntiles - 1) / ntiles;
-    const int nb = p.g.nf * ntiles;
-    const size_t sm = smem_bytes<SR>(nm, tk, nmp);
-#define MGK_CASE(NN) \
-    if (nm == NN) { hipLaunchKernelGGL((k_mid<SR, NN, DIR>), dim3(nb), dim3(256), sm, s, p, tk, ntiles, mode); return; }
+    const long total = (long)p.g.nf * p.g.nlh * p.g.C;
+#define MGK_CASE(NN)                                                                                       \
+    if (nm == NN) {                                                                                        \
+        constexpr int L = mid_cols<SR, NN>();                                                              \
+        const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);                              \
+        hipLaunchKernelGGL((k_mid<SR, NN, DIR>), dim3(blocks_for(total, L)), dim3(L * FftPlan<NN>::threads), \
+                           sm, s, p, mode);                                                                \
+        return;                                                                                            \
+    }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("FFT length not supported on the middle axis");
@@ -724,20 +861,23 @@ static void launch_mid(const LsParams& p, hipStream_t s, int mode) {
 template <typename SR, int C>
 static void launch_first_gamma(const LsParams& p, hipStream_t s, int mode) {
     const int nf = p.g.nf;
-    const int tc = cols_per_block<SR>(nf, C);
-    const long ncols = (long)p.g.nm * p.g.nlh;
-    const int nb = blocks_for(ncols, tc);
-    const int nfp = nf + ((nf % 16 == 0) ? 1 : 0);
-    const size_t sm = smem_bytes<SR>(nf, C * tc, nfp);
-#define MGK_CASE(NN) \
-    if (nf == NN) { hipLaunchKernelGGL((k_first_gamma<SR, NN, C>), dim3(nb), dim3(256), sm, s, p, tc, mode); return; }
+    const long nxi = (long)p.g.nm * p.g.nlh;
+#define MGK_CASE(NN)                                                                                 \
+    if (nf == NN) {                                                                                  \
+        constexpr int G = gamma_points<SR, NN, C>();                                                 \
+        const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
+        hipLaunchKernelGGL((k_first_gamma<SR, NN, C>), dim3(blocks_for(nxi, G)),                     \
+                           dim3(G * C * FftPlan<NN>::threads), sm, s, p, mode);                      \
+        return;                                                                                      \
+    }
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("FFT length not supported on the first axis");
 }
 
 int ls_partial_slots(const Geom& g) {
-    // largest grid of any reducing kernel times its partial width
+    // largest grid of any reducing kernel times its partial width: pass E
+    // runs at least two lines per block, the final pass at most 1024 blocks
     long lines = g.nlines();
     long nb = (lines + 1) / 2;
     return (int)std::max(nb * 3, 1024L * 12);
@@ -773,18 +913,44 @@ void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s)
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
+template <typename SR, int C, int MODE>
+static void launch_inv_last_only(const LsParams& p, int flags, hipStream_t s) {
+    const int nl = p.g.nl;
+#define MGK_CASE(NN)                                                                              \
+    if (nl == NN) {                                                                               \
+        constexpr int pairs = last_pairs<SR, NN, C>();                                            \
+        const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);            \
+        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), \
+                           dim3(256), sm, s, p, flags);                                           \
+        return;                                                                                   \
+    }
+    MGK_LEN_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("FFT length not supported on the last axis");
+}
+
+template <int C> static void enqueue_pass_c(const LsParams& p, int pass, int mode, hipStream_t s) {
     const bool full = (mode == LS_FULL);
+    switch (pass) {
+        case 0: if (full) launch_fwd_last<double, C, LS_FULL>(p, 0, s); else launch_fwd_last<float, C, LS_CG>(p, 0, s); break;
+        case 1: if (full) launch_mid<double, -1>(p, s, mode); else launch_mid<float, -1>(p, s, mode); break;
+        case 2: if (full) launch_first_gamma<double, C>(p, s, mode); else launch_first_gamma<float, C>(p, s, mode); break;
+        case 3: if (full) launch_mid<double, +1>(p, s, mode); else launch_mid<float, +1>(p, s, mode); break;
+        case 4: if (full) launch_inv_last_only<double, C, LS_FULL>(p, 0, s); else launch_inv_last_only<float, C, LS_CG>(p, 0, s); break;
+        case 5:
+            if (full) throw ArgError("the CG update pass runs in FP32 only");
+            launch_fwd_last<float, C, LS_CG>(p, LS_FLAG_CG_UPDATE, s);
+            break;
+        default: throw ArgError("unknown pass");
+    }
+}
+
+void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
     if (p.g.C == 3) {
-        if (pass == 0) { if (full) launch_fwd_last<double, 3, LS_FULL>(p, 0, s); else launch_fwd_last<float, 3, LS_CG>(p, 0, s); }
-        else if (pass == 2) { if (full) launch_first_gamma<double, 3>(p, s, mode); else launch_first_gamma<float, 3>(p, s, mode); }
-        else throw ArgError("pass not present in 2D");
+        if (pass == 1 || pass == 3) throw ArgError("pass not present in 2D");
+        enqueue_pass_c<3>(p, pass, mode, s);
     } else {
-        if (pass == 0) { if (full) launch_fwd_last<double, 6, LS_FULL>(p, 0, s); else launch_fwd_last<float, 6, LS_CG>(p, 0, s); }
-        else if (pass == 1) { if (full) launch_mid<double, -1>(p, s, mode); else launch_mid<float, -1>(p, s, mode); }
-        else if (pass == 2) { if (full) launch_first_gamma<double, 6>(p, s, mode); else launch_first_gamma<float, 6>(p, s, mode); }
-        else if (pass == 3) { if (full) launch_mid<double, +1>(p, s, mode); else launch_mid<float, +1>(p, s, mode); }
-        else throw ArgError("unknown pass");
+        enqueue_pass_c<6>(p, pass, mode, s);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }

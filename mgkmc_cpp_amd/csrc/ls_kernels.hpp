@@ -54,7 +54,8 @@ int ls_partial_slots(const Geom& g);
 void ls_enqueue_start(const LsParams& p, int start, hipStream_t s);
 void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s);
 void ls_enqueue_final(const LsParams& p, hipStream_t s);
-// one pass of the pipeline alone (0 A, 1 B, 2 C, 3 D), for kernel timing
+// one pass of the pipeline alone, for kernel timing: 0 A, 1 B, 2 C, 3 D,
+// 4 E (without its reduction), 5 A with the CG vector update (FP32)
 void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s);
 
 }  // namespace mgk

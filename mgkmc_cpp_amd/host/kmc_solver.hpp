@@ -520,14 +520,21 @@ public:
         }
     }
 
-    // The accepted-event body (kmc_solver.hpp:811-937).  Returns is_instab.
-    int accept_event(double t_wait, const Mat<D>& strain_unit, std::uniform_real_distribution<double>& uniform_dist) {
-        time += t_wait;
-        eps_macro += strain_unit * (strain_rate * t_wait);
+    // One attempt of run_simulation's inner loop (kmc_solver.hpp:763-952,
+    // 3D :1776-1980) on the device with one round trip (mgk_kmc_event).  u1,
+    // u2 and the redraw values are drawn in the reference's order before the
+    // outcome is known; the generator is rewound to where the reference would
+    // stand when the attempt has no event (no rate: before u1; t_wait too
+    // long: after u1; cascade pending: before u1).  On an event, the
+    // accepted-event body (:811-937) follows: time, host mirrors, records.
+    int try_event(double remaining, std::pair<int, int> component,
+                  std::uniform_real_distribution<double>& uniform_dist, int* is_instab, double* t_wait_out) {
+        const std::mt19937 rng_before = rng;
+        const double r_val = uniform_dist(rng);
+        const std::mt19937 rng_after_u1 = rng;
         const double u2 = uniform_dist(rng);
-        // The redraw values come next in the stream (kmc_solver.hpp:839-864)
-        // and do not depend on which voxel was selected, so they are drawn
-        // before the selection is known and the device chains select -> flip.
+        // the redraw values come next in the stream (:839-864) and do not
+        // depend on which voxel the device selects
         std::normal_distribution<double> standard_normal(0.0, 1.0);   // fresh per event (:840)
         std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
         redraw_cat_.resize((size_t)M);
@@ -550,15 +557,45 @@ public:
             has_cat = !catalog_fixed;
             has_q0 = true;
         }
-        check(mgk_select_flip(ctx, u2, time, has_cat ? redraw_cat_[0].data() : nullptr,
-                              has_q0 ? redraw_q0_.data() : nullptr, first, n),
-              "mgk_select_flip");
-        elastic_run();
-        int flat = -1;
-        double q_sel = 0.0;
-        check(mgk_last_selection(ctx, &flat, &q_sel), "mgk_last_selection");
+        mgk_event_in in{};
+        in.time = time;
+        in.neg_log_u1 = -std::log(r_val);
+        in.remaining = remaining;
+        in.strain_rate = strain_rate;
+        in.drv_i = component.first;
+        in.drv_j = component.second;
+        in.u2 = u2;
+        in.new_catalog = has_cat ? redraw_cat_[0].data() : nullptr;
+        in.new_Q0 = has_q0 ? redraw_q0_.data() : nullptr;
+        in.first_mode = first;
+        in.n_modes = n;
+        in.max_iter = ls_max_iter;
+        in.tol = ls_tol;
+        in.cascade_check = (instability_mode == "cascade") ? 1 : 0;
+        mgk_event_out out{};
+        check(mgk_kmc_event(ctx, &in, eps_macro.data(), sig_macro.data(), &out), "mgk_kmc_event");
+        total_rate_ = out.total_rate;
+        n_unstable_ = out.n_unstable;
+        n_included_ = out.n_included;
+        if (out.outcome == MGK_EVENT_UNSTABLE || out.outcome == MGK_EVENT_NO_RATE) {
+            rng = rng_before;
+            return out.outcome;
+        }
+        if (out.outcome == MGK_EVENT_REJECTED) {
+            rng = rng_after_u1;
+            return out.outcome;
+        }
+        const double t_wait = -std::log(r_val) / total_rate_;   // the reference's expression (= out.t_wait)
+        time += t_wait;
+        int inner = 0;
+        mgk_last_solve_stats(ctx, nullptr, &inner);
+        counters.ls_solves++;
+        counters.ls_iters += out.iters;
+        counters.ls_iters_fp32 += inner;
+        last_iters_ = out.iters;
+        const long flat = out.flat_idx;
         const long v = flat / M;
-        const int is_instab = (q_sel <= stability_threshold) ? 1 : 0;
+        const int inst = (out.q_sel <= stability_threshold) ? 1 : 0;
         // host copies of the redrawn entries
         for (int j = 0; j < n; ++j) {
             const long mi = (first < 0) ? flat : v * M + first + j;
@@ -567,15 +604,17 @@ public:
         }
         if (record_events) {
             EventRecord e{};
-            e.flat_idx = flat;
-            e.is_instab = is_instab;
+            e.flat_idx = (int)flat;
+            e.is_instab = inst;
             e.ls_iters = last_iters_;
             e.time = time;
             e.total_rate = total_rate_;
             for (int k = 0; k < D * D; ++k) { e.eps_macro[k] = eps_macro.a[k]; e.sig_macro[k] = sig_macro.a[k]; }
             events.push_back(e);
         }
-        return is_instab;
+        if (is_instab) *is_instab = inst;
+        if (t_wait_out) *t_wait_out = t_wait;
+        return MGK_EVENT_DONE;
     }
 
     // run_simulation (kmc_solver.hpp:721-1037, :1735-2062), fast patching off.
@@ -608,8 +647,10 @@ public:
         while (elastic_steps_done < n_global_steps) {
             remaining += dt_step;
             while (remaining > 0.0) {
-                update_barriers();
-                if (instability_mode == "cascade" && n_unstable_ > 0) {
+                int inst = 0;
+                double t_wait = 0.0;
+                const int outcome = try_event(remaining, component, uniform_dist, &inst, &t_wait);
+                if (outcome == MGK_EVENT_UNSTABLE) {
                     auto cs = run_cascade(step, component);
                     if (cs.second > 0) {
                         counters.cascades++;
@@ -620,32 +661,28 @@ public:
                     }
                     continue;
                 }
-                if (total_rate_ > 0.0) {
-                    const double r_val = uniform_dist(rng);
-                    const double t_wait = -std::log(r_val) / total_rate_;
-                    if (t_wait < remaining) {
-                        remaining -= t_wait;
-                        const int inst = accept_event(t_wait, su, uniform_dist);
-                        seq = inst ? seq + 1 : 0;
-                        if (seq > max_seq) {
-                            std::cout << "\n[TERMINATE] KMC instability sequence limit reached! " << seq
-                                      << " steps." << std::endl;
-                            counters.terminated = true;
-                            finish();
-                            return;
-                        }
-                        do_logging(step, elastic_steps_done, total_kmc_steps, inst ? "KMC_INSTAB" : "KMC", 0, 1,
-                                   component, enable_console_log, start);
-                        total_kmc_steps++;
-                        step++;
-                        if (max_events > 0 && total_kmc_steps >= max_events) {
-                            counters.event_cap_reached = true;
-                            finish();
-                            return;
-                        }
-                        continue;
+                if (outcome == MGK_EVENT_DONE) {
+                    remaining -= t_wait;
+                    seq = inst ? seq + 1 : 0;
+                    if (seq > max_seq) {
+                        std::cout << "\n[TERMINATE] KMC instability sequence limit reached! " << seq
+                                  << " steps." << std::endl;
+                        counters.terminated = true;
+                        finish();
+                        return;
                     }
+                    do_logging(step, elastic_steps_done, total_kmc_steps, inst ? "KMC_INSTAB" : "KMC", 0, 1,
+                               component, enable_console_log, start);
+                    total_kmc_steps++;
+                    step++;
+                    if (max_events > 0 && total_kmc_steps >= max_events) {
+                        counters.event_cap_reached = true;
+                        finish();
+                        return;
+                    }
+                    continue;
                 }
+                // no event (no rate, or t_wait >= remaining): to the end of the strain step
                 eps_macro += su * (strain_rate * remaining);
                 time += remaining;
                 remaining = 0.0;
@@ -683,20 +720,16 @@ public:
     // Forced event steps: the accepted-event body without the remaining-time
     // test (the per-event hot path used for throughput measurement).
     void event_steps(int count, std::pair<int, int> component) {
-        Mat<D> su = Mat<D>::Zero();
-        su(component.first, component.second) = 1.0;
         std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
+        const double forever = std::numeric_limits<double>::infinity();
         for (int s = 0; s < count; ++s) {
-            update_barriers();
-            if (instability_mode == "cascade" && n_unstable_ > 0) {
+            const int outcome = try_event(forever, component, uniform_dist, nullptr, nullptr);
+            if (outcome == MGK_EVENT_UNSTABLE) {
                 auto cs = run_cascade(0, component);
                 if (cs.second > 0) { counters.cascades++; counters.cascade_flips += cs.second; }
                 continue;
             }
-            if (!(total_rate_ > 0.0)) break;
-            const double r_val = uniform_dist(rng);
-            const double t_wait = -std::log(r_val) / total_rate_;
-            accept_event(t_wait, su, uniform_dist);
+            if (outcome != MGK_EVENT_DONE) break;
             counters.kmc_events++;
         }
     }

@@ -187,6 +187,34 @@ class Context:
             n = q0.size
         check(gpu().mgk_select_flip(self._h, float(u2), float(time), _dp(cat), _dp(q0), int(first_mode), int(n)))
 
+    def kmc_event(self, eps_macro, time, neg_log_u1, u2, remaining=float("inf"), strain_rate=1.0, drv=(0, 0),
+                  new_catalog=None, new_Q0=None, first_mode=0, max_iter=200, tol=1e-6, cascade_check=False):
+        """One inner-loop attempt decided and applied on the device
+        (mgk_kmc_event).  Returns (EventOut, eps_macro, sig_macro)."""
+        D = self.D
+        ev = abi.EventIn()
+        ev.time, ev.neg_log_u1, ev.remaining, ev.strain_rate = float(time), float(neg_log_u1), float(remaining), \
+            float(strain_rate)
+        ev.drv_i, ev.drv_j = int(drv[0]), int(drv[1])
+        ev.u2 = float(u2)
+        n = 0
+        cat = q0 = None
+        if new_catalog is not None:
+            new_catalog = np.asarray(new_catalog, dtype=np.float64)
+            n = new_catalog.shape[0]
+            cat = _aos(new_catalog, n, D)
+        if new_Q0 is not None:
+            q0 = np.ascontiguousarray(np.asarray(new_Q0, dtype=np.float64).reshape(-1))
+            n = q0.size
+        ev.new_catalog, ev.new_Q0 = _dp(cat), _dp(q0)
+        ev.first_mode, ev.n_modes = int(first_mode), int(n)
+        ev.max_iter, ev.tol, ev.cascade_check = int(max_iter), float(tol), 1 if cascade_check else 0
+        em = _mat_aos(eps_macro, D)
+        sm = np.zeros(D * D)
+        out = abi.EventOut()
+        check(gpu().mgk_kmc_event(self._h, C.byref(ev), _dp(em), _dp(sm), C.byref(out)))
+        return out, _mat_from(em, D), _mat_from(sm, D)
+
     def last_selection(self):
         idx = C.c_int(-1)
         q = C.c_double(0.0)

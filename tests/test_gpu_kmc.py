@@ -199,3 +199,89 @@ def test_run_simulation_with_mixed_bc_and_cascades():
     eo, so_, to = osim.macro()
     ed, sd_, td = dsim.macro()
     assert _rel(sd_, so_) < 1e-9 and abs(td - to) <= 1e-12 * abs(to)
+
+
+_STATE_FIELDS = (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_LAST_EVENT_TIME,
+                 abi.FIELD_PREV_DIR, abi.FIELD_CATALOG, abi.FIELD_Q0)
+
+
+def test_device_event_equals_host_chain():
+    # mgk_kmc_event (rates, t_wait, selection, flip and solve in one round
+    # trip) against the host chain update_rates -> t_wait on the host ->
+    # select_flip -> elastic_run: bit for bit.
+    c = _cfg(nx=16, ny=16, M=4, instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM,
+             neighbor_softening_fraction=0.2)
+    E, nu = _field(256)
+    osim = _evolved_oracle(c, E, nu, events=6)
+    em, _, t = osim.macro()
+    a = _device_from_oracle(osim, c, E, nu)
+    b = _device_from_oracle(osim, c, E, nu)
+    a.elastic_run(em, tol=1e-8)
+    b.elastic_run(em, tol=1e-8)
+    em_a = em_b = em
+    rng = np.random.default_rng(9)
+    for _ in range(4):
+        u1, u2 = rng.random(2)
+        cat = rng.standard_normal((4, 2, 2)) * 0.05
+        cat = 0.5 * (cat + np.transpose(cat, (0, 2, 1)))
+        q0 = rng.uniform(0.8, 1.2, 4)
+        L = -np.log(u1)
+        R, _, _ = b.update_rates(t)
+        tw = L / R
+        eb = em_b.copy()
+        eb[0, 1] += 1.0 * (2.0 * tw)
+        b.select_flip(u2, t + tw, new_catalog=cat, new_Q0=q0, first_mode=0)
+        rb = b.elastic_run(eb, tol=1e-8)
+        out, em_a2, sm_a2 = a.kmc_event(em_a, t, L, u2, strain_rate=2.0, drv=(0, 1), new_catalog=cat, new_Q0=q0,
+                                        first_mode=0, tol=1e-8)
+        assert out.outcome == abi.EVENT_DONE
+        assert out.total_rate == R and out.t_wait == tw
+        assert out.flat_idx == b.last_selection()[0] and out.iters == rb["iters"]
+        assert np.array_equal(em_a2, rb["eps_macro"]) and np.array_equal(sm_a2, rb["sig_macro"])
+        for f in _STATE_FIELDS:
+            assert np.array_equal(a.get_field(f), b.get_field(f)), f
+        t = t + tw
+        em_a, em_b = em_a2, rb["eps_macro"]
+    # t_wait >= remaining: nothing changes
+    before = {f: a.get_field(f) for f in _STATE_FIELDS}
+    out, em2, _ = a.kmc_event(em_a, t, 50.0, 0.5, remaining=1e-300, tol=1e-8)
+    assert out.outcome == abi.EVENT_REJECTED and out.flat_idx == -1 and out.total_rate > 0
+    assert np.array_equal(em2, em_a)
+    for f in _STATE_FIELDS:
+        assert np.array_equal(a.get_field(f), before[f]), f
+    # the next event still matches the host chain after a rejected attempt
+    R, _, _ = b.update_rates(t)
+    out, em_a2, _ = a.kmc_event(em_a, t, 0.7, 0.25, tol=1e-8)
+    b.select_flip(0.25, t + 0.7 / R)
+    eb = em_b.copy()
+    eb[0, 0] += 0.7 / R
+    rb = b.elastic_run(eb, tol=1e-8)
+    assert out.outcome == abi.EVENT_DONE and np.array_equal(em_a2, rb["eps_macro"])
+    for f in _STATE_FIELDS:
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f
+
+
+def test_device_event_without_rate_or_with_unstable_modes():
+    # T = 0 in cascade mode: every rated mode has rate 0 -> no event
+    c = _cfg(nx=8, ny=8, M=3, temperature=0.0)
+    E, nu = _field(64)
+    osim = po.OracleSim(c, E, nu)
+    a = _device_from_oracle(osim, c, E, nu)
+    em = np.zeros((2, 2))
+    em[0, 0] = 1e-3
+    a.elastic_run(em, tol=1e-8)
+    before = {f: a.get_field(f) for f in _STATE_FIELDS}
+    out, em2, _ = a.kmc_event(em, 0.0, 1.0, 0.5, tol=1e-8)
+    assert out.outcome == abi.EVENT_NO_RATE and out.total_rate == 0.0
+    assert np.array_equal(em2, em)
+    # a stability threshold above every barrier makes all modes unstable
+    c2 = _cfg(nx=8, ny=8, M=3, temperature=300.0, stability_threshold=1e3)
+    osim2 = po.OracleSim(c2, E, nu)
+    b = _device_from_oracle(osim2, c2, E, nu)
+    b.elastic_run(em, tol=1e-8)
+    out, _, _ = b.kmc_event(em, 0.0, 1.0, 0.5, tol=1e-8, cascade_check=True)
+    assert out.outcome == abi.EVENT_UNSTABLE and out.n_unstable == 64 * 3
+    out, _, _ = b.kmc_event(em, 0.0, 1.0, 0.5, tol=1e-8, cascade_check=False)
+    assert out.outcome == abi.EVENT_DONE
+    for f in _STATE_FIELDS:
+        assert np.array_equal(a.get_field(f), before[f]), f
