@@ -32,16 +32,36 @@ SHAPE = (64, 64, 64)
 PRELOAD_STRAIN = 0.01
 
 
+PASS_NAMES = {0: "k_fwd_last", 1: "k_mid<fwd>", 2: "k_first_gamma", 3: "k_mid<inv>", 4: "k_inv_last",
+              5: "k_fwd_last+cg_update"}
+
+
 def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
-    """Bytes per voxel one launch of a pipeline pass must move (read + write),
+    """Bytes per voxel one launch of a pipeline pass must move (read + write):
     symmetric 6-component tensors, half spectrum nl/2+1 of nl bins."""
     h = (nl // 2 + 1) / nl
     cs = 8 if fp32 else 16           # complex bytes
     spec = 6 * h * cs                # one half-spectrum tensor field per voxel
-    if pass_id == 0:
-        real_in = (6 * 4 + 2 * 4) if fp32 else (6 * 8 + 6 * 8 + 2 * 8)   # y, dlam, dmu | eps, eps_p, lam, mu
-        return real_in + spec
+    if pass_id == 0:                 # y, dlam, dmu | eps, eps_p, lam, mu  -> spectrum
+        return ((6 * 4 + 2 * 4) if fp32 else (6 * 8 + 6 * 8 + 2 * 8)) + spec
+    if pass_id == 5:                 # r w p s x read, p s x r written, dlam dmu -> spectrum
+        return 5 * 6 * 4 + 4 * 6 * 4 + 2 * 4 + spec
+    if pass_id == 4:                 # spectrum -> w (reads r) | eps_new (reads eps, writes the FP32 increment)
+        return spec + (2 * 6 * 4 if fp32 else 2 * 6 * 8 + 6 * 4)
     return 2 * spec                  # B, C, D: read and write the spectrum
+
+
+def _launches_per_event(pass_id, fp32, info):
+    """Launches of a pass per event in the FP32+refine run."""
+    cycles = info["ls_iters_per_event"]          # FP64 iterations = refinement cycles
+    steps = info["fp32_iters_per_event"]         # CG steps, the first of each cycle without update
+    if not fp32:
+        return cycles
+    if pass_id == 0:
+        return cycles
+    if pass_id == 5:
+        return max(steps - cycles, 0.0)
+    return steps
 
 
 def _preload(sim, D):
@@ -69,23 +89,61 @@ def run_gpu(precision, steps, warmup, E, nu):
     return sim, info
 
 
-def kernel_roofline(sim, info64, info32):
-    """Average launch time of each pipeline pass (HIP events on the context's
-    stream), weighted by launches per event; the heaviest is reported."""
+def kernel_roofline(sim, info32):
+    """Average launch time of every pipeline pass (HIP events on the
+    context's stream), weighted by its launches per event in the FP32+refine
+    run; the pass with the largest time per event is reported."""
     nl = SHAPE[2]
     N = int(np.prod(SHAPE))
     rows = []
     for fp32 in (False, True):
-        per_event = info32["fp32_iters_per_event"] if fp32 else info32["ls_iters_per_event"]
-        for pass_id, name in ((0, "k_fwd_last"), (1, "k_mid<fwd>"), (2, "k_first_gamma"), (3, "k_mid<inv>")):
+        for pass_id in range(6):
+            if pass_id == 5 and not fp32:
+                continue
             ms = sim.time_ls_pass(pass_id, fp32=fp32, reps=20)
             b = _algorithmic_bytes_per_voxel(pass_id, fp32, nl) * N
-            rows.append(dict(kernel=name + ("<float>" if fp32 else "<double>"), ms=ms, bytes=b,
-                             ms_per_event=ms * per_event))
+            n = _launches_per_event(pass_id, fp32, info32)
+            rows.append(dict(kernel=PASS_NAMES[pass_id] + ("<float>" if fp32 else "<double>"), pass_id=pass_id,
+                             fp32=fp32, ms=round(ms, 5), bytes=int(b), launches_per_event=round(n, 2),
+                             ms_per_event=round(ms * n, 5), gbs=round(b / (ms * 1e-3) / 1e9, 1)))
     top = max(rows, key=lambda r: r["ms_per_event"])
     achieved = top["bytes"] / (top["ms"] * 1e-3) / 1e9
     return top, rows, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+                           frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_measured_traffic(top["kernel"]))
+
+
+def _measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_64cube.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def probe():
+    """Launches for the counter runs: one 256 MiB device copy (calibration)
+    then 5 launches of every pass at 64^3 in both precisions."""
+    import torch
+    from mgkmc_cpp_amd import Context, abi
+    from mgkmc_cpp_amd import configs
+    x = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("[probe] device copy of 268435456 bytes", flush=True)
+    E, nu = configs.material(SHAPE, 4.0, 42)
+    ctx = Context(SHAPE, E, nu, precision=abi.MGK_FP32_REFINE)
+    for fp32 in (False, True):
+        for pass_id in range(6):
+            if pass_id == 5 and not fp32:
+                continue
+            ctx.time_ls_pass(pass_id, fp32=fp32, reps=4)
+            print(f"[probe] pass {pass_id} {'fp32' if fp32 else 'fp64'}: 5 launches", flush=True)
+    ctx.close()
 
 
 def cpu_baseline(E, nu, budget_s=15.0):
@@ -127,7 +185,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--probe", action="store_true", help="only the launches of the counter runs")
     a = ap.parse_args()
+    if a.probe:
+        probe()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
               "trajectories shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
@@ -142,7 +204,7 @@ def main():
     print(f"[bench] FP32+refine run", file=sys.stderr, flush=True)
     sim32, i32 = run_gpu(abi.MGK_FP32_REFINE, a.steps, a.warmup, E, nu)
     print(f"[bench] FP32+refine {i32}", file=sys.stderr, flush=True)
-    top, rows, roof = kernel_roofline(sim32, i64, i32)
+    top, rows, roof = kernel_roofline(sim32, i32)
     print(f"[bench] pass timings {json.dumps(rows)}", file=sys.stderr, flush=True)
     roof["kernel"] = top["kernel"]
     roof["bytes_per_launch"] = int(top["bytes"])

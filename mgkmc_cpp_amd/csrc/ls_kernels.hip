@@ -151,10 +151,21 @@ __device__ __forceinline__ void gamma_apply(const cplx<R>* t, cplx<R>* o, const 
 static constexpr int kLdsBudget = 48 * 1024;     // passes A and E
 static constexpr int kColLdsBudget = 64 * 1024;  // passes B, C and D
 
-// Upper limits of the block shapes (line pairs of passes A/E, columns of
-// B/D, frequency points of C); build-time tuning knobs.
-#ifndef MGK_LAST_PAIRS_CAP
-#define MGK_LAST_PAIRS_CAP 8
+// Block shapes, measured at 64^3 and 128^3 (3D) and 128^2 (2D):
+//  * passes A/E: two line pairs per block for pass A everywhere and for pass
+//    E in 2D, as many as 256 lanes allow for pass E in 3D;
+//  * passes B/D: 512 lanes in FP32 (64 columns of 64), 256 in FP64;
+//  * pass C in 3D: point tiles spanning whole 128-byte lines (8 points in
+//    FP32 on 512 lanes, 4 in FP64), so neighbouring blocks share no line.
+// The caps below are build-time tuning knobs over these rules.
+#ifndef MGK_FWD_PAIRS_CAP
+#define MGK_FWD_PAIRS_CAP 2
+#endif
+#ifndef MGK_INV_PAIRS_CAP_2D
+#define MGK_INV_PAIRS_CAP_2D 2
+#endif
+#ifndef MGK_INV_PAIRS_CAP_3D
+#define MGK_INV_PAIRS_CAP_3D 8
 #endif
 #ifndef MGK_MID_COLS_CAP
 #define MGK_MID_COLS_CAP 256
@@ -165,22 +176,31 @@ static constexpr int kColLdsBudget = 64 * 1024;  // passes B, C and D
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 constexpr int pow2floor(int x) { int p = 1; while (2 * p <= x) p *= 2; return p; }
+
+// Lanes per block of the column passes B, C and D.
+template <typename SR, int C> constexpr int col_threads() { return (sizeof(SR) == 4 && C == 6) ? 512 : 256; }
+constexpr int kColThreadsMax = 512;
 
 // Line pairs per block of the L passes (two real lines share one complex FFT
 // line): a power of two whose FFT lanes fit 256 and whose LDS fits the budget.
-template <typename SR, int N, int C> constexpr int last_pairs() {
+template <typename SR, int N, int C, int CAP> constexpr int last_pairs() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_thr = 256 / (C * T);
     constexpr int by_lds = (int)((kLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
-    return pow2floor(cmax(1, cmin(cmin(by_thr, by_lds), MGK_LAST_PAIRS_CAP)));
+    return pow2floor(cmax(1, cmin(cmin(by_thr, by_lds), CAP)));
+}
+template <typename SR, int N, int C> constexpr int fwd_pairs() { return last_pairs<SR, N, C, MGK_FWD_PAIRS_CAP>(); }
+template <typename SR, int N, int C> constexpr int inv_pairs() {
+    return last_pairs<SR, N, C, (C == 6) ? MGK_INV_PAIRS_CAP_3D : MGK_INV_PAIRS_CAP_2D>();
 }
 
-// Columns per block of the M passes.
+// Columns per block of the M passes (3D only, C = 6).
 template <typename SR, int N> constexpr int mid_cols() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_lds = (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (padded<N>() * (int)sizeof(cplx<SR>)));
-    return pow2floor(cmax(1, cmin(cmin(256 / T, by_lds), MGK_MID_COLS_CAP)));
+    return pow2floor(cmax(1, cmin(cmin(col_threads<SR, 6>() / T, by_lds), MGK_MID_COLS_CAP)));
 }
 
 // Frequency points (all C components each) per block of the F pass.
@@ -188,7 +208,11 @@ template <typename SR, int N, int C> constexpr int gamma_points() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_lds =
         (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
-    return cmax(1, cmin(cmin(256 / (T * C), by_lds), MGK_GAMMA_POINTS_CAP));
+    constexpr int by_thr = col_threads<SR, C>() / (T * C);
+    constexpr int most = cmax(1, cmin(cmin(by_thr, by_lds), MGK_GAMMA_POINTS_CAP));
+    constexpr int point_bytes = C * (int)sizeof(cplx<SR>);
+    constexpr int g_align = 128 / cgcd(128, point_bytes);   // points per whole number of lines
+    return (C == 6 && most >= g_align) ? (most / g_align) * g_align : most;
 }
 
 // ---------------------------------------------------------------------------
@@ -197,7 +221,7 @@ template <typename SR, int N, int C> constexpr int gamma_points() {
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
 __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int flags) {
-    constexpr int PAIRS = last_pairs<SR, NL, C>();
+    constexpr int PAIRS = fwd_pairs<SR, NL, C>();
     constexpr int LPW = 2 * PAIRS;
     constexpr int VPT = (LPW * NL + 255) / 256;
     constexpr int NLP = padded<NL>();
@@ -372,7 +396,7 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int flags) {
 // gather and store touch consecutive columns of one row.
 // ---------------------------------------------------------------------------
 template <typename SR, int NM, int DIR>
-__global__ __launch_bounds__(256) void k_mid(LsParams p, int mode) {
+__global__ __launch_bounds__(kColThreadsMax) void k_mid(LsParams p, int mode) {
     constexpr int T = FftPlan<NM>::threads;
     constexpr int L = mid_cols<SR, NM>();
     constexpr int NMP = padded<NM>();
@@ -404,7 +428,7 @@ __global__ __launch_bounds__(256) void k_mid(LsParams p, int mode) {
 // L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
 template <typename SR, int NF, int C>
-__global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int mode) {
+__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(LsParams p, int mode) {
     constexpr int D = (C == 3) ? 2 : 3;
     constexpr int T = FftPlan<NF>::threads;
     constexpr int G = gamma_points<SR, NF, C>();
@@ -494,7 +518,7 @@ __global__ __launch_bounds__(256) void k_first_gamma(LsParams p, int mode) {
 // ---------------------------------------------------------------------------
 template <typename SR, int NL, int C, int MODE>
 __global__ __launch_bounds__(256) void k_inv_last(LsParams p, int flags) {
-    constexpr int PAIRS = last_pairs<SR, NL, C>();
+    constexpr int PAIRS = inv_pairs<SR, NL, C>();
     constexpr int LPW = 2 * PAIRS;
     constexpr int VPT = (LPW * NL + 255) / 256;
     constexpr int NLP = padded<NL>();
@@ -813,7 +837,7 @@ static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                                          \
     if (nl == NN) {                                                                                           \
-        constexpr int pairs = last_pairs<SR, NN, C>();                                                        \
+        constexpr int pairs = fwd_pairs<SR, NN, C>();                                                         \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                        \
         hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), dim3(256), \
                            sm, s, p, flags);                                                                  \
@@ -829,7 +853,7 @@ static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                                  \
     if (nl == NN) {                                                                                   \
-        constexpr int pairs = last_pairs<SR, NN, C>();                                                \
+        constexpr int pairs = inv_pairs<SR, NN, C>();                                                \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                \
         const int nb = blocks_for(p.g.nlines(), 2 * pairs);                                           \
         hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, flags);     \
@@ -918,7 +942,7 @@ static void launch_inv_last_only(const LsParams& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                              \
     if (nl == NN) {                                                                               \
-        constexpr int pairs = last_pairs<SR, NN, C>();                                            \
+        constexpr int pairs = inv_pairs<SR, NN, C>();                                            \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);            \
         hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), \
                            dim3(256), sm, s, p, flags);                                           \
