@@ -128,6 +128,18 @@ struct mgk_ctx {
     std::vector<double> shared_cat;   // [C][M] host copy of a shared catalog
     bool kctl_pending = false;        // a device-side selection not yet copied back
 
+    // parameter blocks of this context's launches in device memory (LS slot
+    // 0: solves, slot 1: pass timing), uploaded from pinned copies when they
+    // change; one slot of event arguments
+    LsParams* d_lsp = nullptr;
+    LsParams* h_lsp = nullptr;
+    bool lsp_up[2] = {false, false};
+    KmcParams* d_kp = nullptr;
+    KmcParams* h_kp = nullptr;
+    bool kp_up = false;
+    EventArgs* d_ea = nullptr;
+    EventArgs* h_ea = nullptr;
+
     ~mgk_ctx() { release(); }
 
     void release() {
@@ -140,6 +152,8 @@ struct mgk_ctx {
         f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.bsum); f(K.bincl);
         f(K.bunst); f(K.bunst_off); f(K.binc); f(kctl); f(kstage); f(unst_idx); f(unst_q);
         fh(hkctl); fh(hkstage);
+        f(d_lsp); f(d_kp); f(d_ea);
+        fh(h_lsp); fh(h_kp); fh(h_ea);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -169,7 +183,8 @@ struct mgk_ctx {
     }
 
     LsParams ls_params() const {
-        LsParams p{};
+        LsParams p;
+        std::memset(&p, 0, sizeof(p));   // compared byte-wise against the uploaded copy
         p.g = g;
         p.eps = eps;
         p.epsp = epsp_set ? epsp : nullptr;
@@ -195,6 +210,28 @@ struct mgk_ctx {
         p.ctl = ctl;
         p.partials = partials;
         return p;
+    }
+
+    // This context's LS parameter block in device memory (slot 0), or a
+    // variant of it (slot 1), re-uploaded only when it changed.
+    LsLaunch lsl(int slot = 0, const LsParams* custom = nullptr) {
+        const LsParams p = custom ? *custom : ls_params();
+        if (!lsp_up[slot] || std::memcmp(&h_lsp[slot], &p, sizeof(LsParams)) != 0) {
+            MGK_HIP_CHECK(hipStreamSynchronize(stream));   // the pinned copy may still feed an earlier upload
+            h_lsp[slot] = p;
+            MGK_HIP_CHECK(hipMemcpyAsync(d_lsp + slot, h_lsp + slot, sizeof(LsParams), hipMemcpyHostToDevice, stream));
+            lsp_up[slot] = true;
+        }
+        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot]};
+    }
+    KmcLaunch kml() {
+        if (!kp_up || std::memcmp(h_kp, &K, sizeof(KmcParams)) != 0) {
+            MGK_HIP_CHECK(hipStreamSynchronize(stream));
+            *h_kp = K;
+            MGK_HIP_CHECK(hipMemcpyAsync(d_kp, h_kp, sizeof(KmcParams), hipMemcpyHostToDevice, stream));
+            kp_up = true;
+        }
+        return KmcLaunch{g, kp.M, d_kp, 1, K};
     }
 
     // Symmetric / antisymmetric split of a column-major D x D matrix.
@@ -268,10 +305,13 @@ struct mgk_ctx {
         c.norm_extra = norm_extra;
         for (int k = 0; k < g.C; ++k) c.eps_bar[k] = eps_bar_sym[k];
         const int start = start_mode(warm);
-        if (start == LS_START_KEEP)   // the field is unchanged since the last solve: its mean is known
+        if (start == LS_START_KEEP) {   // the field is unchanged since the last solve: its mean is known
             for (int k = 0; k < g.C; ++k) c.shift[k] = eps_bar_sym[k] - mean_eps[k];
+            c.keep_shift = 1;
+        }
+        const LsLaunch L = lsl();
         MGK_HIP_CHECK(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, stream));
-        ls_enqueue_start(ls_params(), start, stream);
+        ls_enqueue_start(L, start, stream);
         return run_batches(warm, max_iter, tol, diff_out, false);
     }
 
@@ -292,7 +332,7 @@ struct mgk_ctx {
     // event left the solve marked done before its first iteration, and the
     // solver statistics are then left alone.
     int run_batches(bool warm, int max_iter, double tol, double* diff_out, bool event) {
-        LsParams p = ls_params();
+        const LsLaunch p = lsl();
         const bool refine = (precision == MGK_FP32_REFINE);
         const int J = inner_per_outer;
         int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
@@ -441,6 +481,12 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             c->partials = dalloc<double>((size_t)ls_partial_slots(g) + 64);
             c->ctl = dalloc<Ctl>(1);
             c->hctl = halloc<Ctl>(1);
+            c->d_lsp = dalloc<LsParams>(2);
+            c->h_lsp = halloc<LsParams>(2);
+            c->d_kp = dalloc<KmcParams>(1);
+            c->h_kp = halloc<KmcParams>(1);
+            c->d_ea = dalloc<EventArgs>(1);
+            c->h_ea = halloc<EventArgs>(1);
             MGK_HIP_CHECK(hipMemset(c->ctl, 0, sizeof(Ctl)));
             const int lens[3] = {g.nf, g.nm, g.nl};
             for (int a = 0; a < 3; ++a) {
@@ -787,7 +833,7 @@ int mgk_update_rates(mgk_ctx* ctx, double time, double* total_rate, int* n_unsta
         if (!ctx) throw ArgError("null context");
         mgk_ctx& c = *ctx;
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
-        kmc_enqueue_rates(c.K, time, c.stream);
+        kmc_enqueue_rates(c.kml(), time, nullptr, c.stream);
         MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
         MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         if (total_rate) *total_rate = c.hkctl->total_rate;
@@ -802,11 +848,11 @@ int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
         if (!ctx || !flat_idx) throw ArgError("null argument");
         mgk_ctx& c = *ctx;
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
-        kmc_enqueue_select(c.K, u2, c.stream);
+        kmc_enqueue_select(c.kml(), u2, nullptr, c.stream);
         MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
         MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         if (c.hkctl->ambiguous) {
-            kmc_enqueue_select_fallback(c.K, 0, c.stream);
+            kmc_enqueue_select_fallback(c.kml(), 0, c.stream);
             MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
             MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         }
@@ -857,7 +903,7 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
             for (int j = 0; j < n_modes; ++j) h[(size_t)n_modes * C + j] = new_Q0[j];
         if (n_modes > 0)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, h, sizeof(double) * n_modes * (C + 1), hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(c.K, flat_idx, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
+        kmc_enqueue_flip(c.kml(), flat_idx, time, first_mode, n_modes, new_catalog ? 1 : 0, new_Q0 ? 1 : 0, c.stream);
         // the pinned staging buffer is reused by the next call: wait for the copy
         if (n_modes > 0) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         return MGK_OK;
@@ -908,12 +954,12 @@ int mgk_select_flip(mgk_ctx* ctx, double u2, double time, const double* new_cata
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
         int has_cat = 0, has_q0 = 0;
         n_modes = stage_redraw(c, new_catalog, new_Q0, first_mode, n_modes, &has_cat, &has_q0);
-        kmc_enqueue_select(c.K, u2, c.stream);
-        kmc_enqueue_select_fallback(c.K, 1, c.stream);
+        kmc_enqueue_select(c.kml(), u2, nullptr, c.stream);
+        kmc_enqueue_select_fallback(c.kml(), 1, c.stream);
         if (n_modes > 0)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(c.K, -1, time, first_mode, n_modes, has_cat, has_q0, c.stream);
+        kmc_enqueue_flip(c.kml(), -1, time, first_mode, n_modes, has_cat, has_q0, c.stream);
         c.kctl_pending = true;   // the selection is read back with the next solve's sync
         return MGK_OK;
     });
@@ -931,7 +977,6 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         const int n_modes = stage_redraw(c, in->new_catalog, in->new_Q0, in->first_mode, in->n_modes, &has_cat, &has_q0);
         // LS control block: the device fills the imposed strain and the shift
         c.init_ctl(in->max_iter, in->tol);
-        MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
         const int start = c.start_mode(true);
         EventArgs ea{};
         ea.time = in->time;
@@ -943,15 +988,21 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         ea.drv = in->drv_i + Dd * in->drv_j;
         ea.cascade_check = in->cascade_check ? 1 : 0;
         ea.keep_shift = (start == LS_START_KEEP) ? 1 : 0;
-        kmc_enqueue_rates(c.K, in->time, c.stream);
-        kmc_enqueue_event_prep(c.K, c.ctl, ea, c.stream);
-        kmc_enqueue_select(c.K, in->u2, c.stream);
-        kmc_enqueue_select_fallback(c.K, 1, c.stream);
+        ea.u2 = in->u2;
+        ea.active = 1;
+        c.hctl->keep_shift = ea.keep_shift;
+        MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
+        const KmcLaunch KL = c.kml();
+        const LsLaunch L = c.lsl();
+        kmc_enqueue_rates(KL, in->time, nullptr, c.stream);
+        kmc_enqueue_event_prep(KL, c.ctl, nullptr, ea, nullptr, c.stream);
+        kmc_enqueue_select(KL, in->u2, nullptr, c.stream);
+        kmc_enqueue_select_fallback(KL, 1, c.stream);
         if (n_modes > 0)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(c.K, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1);
-        ls_enqueue_start(c.ls_params(), start, c.stream);
+        kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1);
+        ls_enqueue_start(L, start, c.stream);
         c.kctl_pending = true;
         double diff = 0.0;
         const int it = c.run_batches(true, in->max_iter, in->tol, &diff, true);
@@ -1009,7 +1060,7 @@ int mgk_find_unstable(mgk_ctx* ctx, int* idx_out, double* q_out, int cap, int* n
         const int n = c.hkctl->n_unstable;   // from the last mgk_update_rates
         *n_out = n;
         if (n > 0) {
-            kmc_enqueue_unstable(c.K, c.stream);
+            kmc_enqueue_unstable(c.kml(), c.stream);
             const int m = std::min(n, cap);
             if (m > 0 && idx_out) MGK_HIP_CHECK(hipMemcpyAsync(idx_out, c.unst_idx, m * sizeof(int), hipMemcpyDeviceToHost, c.stream));
             if (m > 0 && q_out) MGK_HIP_CHECK(hipMemcpyAsync(q_out, c.unst_q, m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1058,10 +1109,11 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         c.hctl->iters = 1;
         c.hctl->max_iter = 1 << 30;
         MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
-        LsParams p = c.ls_params();
-        p.corr = (pass == 5) ? c.corr : nullptr;
+        LsParams q = c.ls_params();
+        q.corr = (pass == 5) ? c.corr : nullptr;
         const int mode = fp32 ? LS_CG : LS_FULL;
-        p.r32 = fp32 ? c.cg : nullptr;   // the FP32 passes read and write the CG vectors
+        q.r32 = fp32 ? c.cg : nullptr;   // the FP32 passes read and write the CG vectors
+        const LsLaunch p = c.lsl(1, &q);
         ls_enqueue_pass(p, pass, mode, c.stream);   // warm-up launch
         MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
         for (int r = 0; r < reps; ++r) ls_enqueue_pass(p, pass, mode, c.stream);

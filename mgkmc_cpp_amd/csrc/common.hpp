@@ -39,7 +39,8 @@ struct Ctl {
     int corr_valid;      // the FP32 correction field holds a pending update
     unsigned ticket;     // last-block ticket of the reduction
     int max_iter;
-    int pad0, pad1;
+    int keep_shift;      // the warm-start shift is already set: a mean pass leaves it alone
+    int pad1;
     double tol;
     double diff;         // last relative increment
     double norm_extra;   // sum over the uniform antisymmetric part of eps (host-known)

@@ -10,8 +10,16 @@
 // "ambiguous" and the host runs the exact sequential pass instead.
 #include "common.hpp"
 #include "kmc_kernels.hpp"
+#include "ls_kernels.hpp"
 
 namespace mgk {
+
+// The parameter block of this block row: the kernel argument for a single
+// trajectory (kernel arguments are preloaded), P[blockIdx.y] for a batch.
+template <bool BATCH, typename T> __device__ __forceinline__ const T& param_sel(const T& a, const T* P) {
+    if constexpr (BATCH) return P[blockIdx.y];
+    else return a;
+}
 
 template <typename T> __device__ __forceinline__ T wsum(T v) {
 #pragma unroll
@@ -35,8 +43,11 @@ __device__ __forceinline__ double mode_rate(const KmcPhys& ph, double q, bool& i
 // used.  The catalog is per (voxel, mode) or, when every voxel holds the same
 // M strains, one shared copy of M entries.
 // ---------------------------------------------------------------------------
-template <int D, int C>
-__global__ __launch_bounds__(kRateThreads) void k_rates(KmcParams p, double time) {
+template <int D, int C, bool BATCH>
+__global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, const KmcParams* __restrict__ PK,
+                                                        double time, const double* __restrict__ times) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    if (BATCH) time = times[blockIdx.y];
     constexpr int K = kRateBlock / kRateThreads;
     const Geom g = p.g;
     const KmcPhys ph = p.ph;
@@ -160,7 +171,9 @@ __device__ __forceinline__ void wave_scan(double& v, int& a, int& b) {
 // One block: inclusive prefix of the block sums (chunked, fixed order), the
 // exclusive prefix of unstable counts and the totals.  A kernel of its own so
 // that the rate blocks need no device-scope fence.
-__global__ __launch_bounds__(1024) void k_rates_scan(KmcParams p, int nb) {
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_rates_scan(const KmcParams p_arg, const KmcParams* __restrict__ PK, int nb) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     constexpr int G = 8;   // block sums per lane loaded together
     const int chunk = (nb + 1023) / 1024;
     const int b0 = threadIdx.x * chunk;
@@ -253,7 +266,11 @@ __global__ __launch_bounds__(1024) void k_rates_scan(KmcParams p, int nb) {
 // (non-decreasing), mode by an in-block scan; a rounding-bound guard flags
 // targets too close to call.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const KmcParams* __restrict__ PK, double u2,
+                                                const EventArgs* __restrict__ EA) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    if (BATCH) u2 = EA[blockIdx.y].u2;
     const long nmodes = p.g.N * p.ph.M;
     KmcCtl* kc = p.kctl;
     if (!kc->go) return;
@@ -345,7 +362,9 @@ __global__ __launch_bounds__(256) void k_select(KmcParams p, double u2) {
 // Exact left-to-right pass of the reference (single lane): the total in flat
 // order, target = u2 * total, then the first prefix >= target.  With
 // only_if_ambiguous it runs only when k_select flagged its answer.
-__global__ void k_select_seq(KmcParams p, int only_if_ambiguous) {
+template <bool BATCH>
+__global__ void k_select_seq(const KmcParams p_arg, const KmcParams* __restrict__ PK, int only_if_ambiguous) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
     if (!kc->go || (only_if_ambiguous && !kc->ambiguous)) return;
@@ -379,9 +398,10 @@ __global__ void k_select_seq(KmcParams p, int only_if_ambiguous) {
 // reference's order on one lane: periodic wraps on small grids hit the same
 // voxel several times and each hit accumulates.
 // ---------------------------------------------------------------------------
-template <int D, int C>
-__global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n_modes, int has_cat, int has_q0,
-                       int time_from_ctl) {
+template <int D, int C, bool BATCH>
+__global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, int flat, double time, int first_mode,
+                       int n_modes, int has_cat, int has_q0, int time_from_ctl) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     // flat < 0: the mode the last selection chose, if that attempt produced
     // an event; first_mode < 0: redraw only that mode (both redraw switches
@@ -449,7 +469,9 @@ __global__ void k_flip(KmcParams p, int flat, double time, int first_mode, int n
 // find_unstable_indices (kmc_solver.hpp:412-421): ascending flat indices with
 // Q < threshold, compacted with the block offsets of the rate pass.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_unstable(KmcParams p) {
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_unstable(const KmcParams p_arg, const KmcParams* __restrict__ PK) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     const long nmodes = p.g.N * p.ph.M;
     const long base = (long)blockIdx.x * kRateBlock;
     __shared__ int s_cnt[256];
@@ -493,14 +515,19 @@ __global__ __launch_bounds__(256) void k_unstable(KmcParams p) {
 // Every operation is the host's own, in its order (no contraction in this
 // file), so time_new and the imposed strain equal the host's bit for bit.
 // ---------------------------------------------------------------------------
-template <int D, int C>
-__global__ void k_event_prep(KmcParams p, Ctl* ctl, EventArgs ea) {
+template <int D, int C, bool BATCH>
+__global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict__ PK, Ctl* ctl_arg,
+                             const LsParams* __restrict__ P, const EventArgs ea_arg, const EventArgs* __restrict__ EA) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    const EventArgs& ea = param_sel<BATCH>(ea_arg, EA);
+    Ctl* ctl = BATCH ? P[blockIdx.y].ctl : ctl_arg;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
     const double R = kc->total_rate;
     int outcome;
     double tw = 0.0;
-    if (ea.cascade_check && kc->n_unstable > 0) outcome = 1;          // MGK_EVENT_UNSTABLE
+    if (!ea.active) outcome = 4;                                      // MGK_EVENT_SKIPPED
+    else if (ea.cascade_check && kc->n_unstable > 0) outcome = 1;     // MGK_EVENT_UNSTABLE
     else if (!(R > 0.0)) outcome = 2;                                 // MGK_EVENT_NO_RATE
     else {
         tw = ea.neg_log_u1 / R;
@@ -549,44 +576,81 @@ __global__ void k_event_prep(KmcParams p, Ctl* ctl, EventArgs ea) {
 // ---------------------------------------------------------------------------
 int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlock); }
 
-void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s) {
-    const int nb = kmc_nblocks(p.g.N * p.ph.M);
-    if (p.g.dims == 2) hipLaunchKernelGGL((k_rates<2, 3>), dim3(nb), dim3(kRateThreads), 0, s, p, time);
-    else hipLaunchKernelGGL((k_rates<3, 6>), dim3(nb), dim3(kRateThreads), 0, s, p, time);
-    hipLaunchKernelGGL(k_rates_scan, dim3(1), dim3(1024), 0, s, p, nb);
-    MGK_HIP_CHECK(hipGetLastError());
-}
-
-void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s) {
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, s, p, u2);
-    MGK_HIP_CHECK(hipGetLastError());
-}
-
-void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipStream_t s) {
-    hipLaunchKernelGGL(k_select_seq, dim3(1), dim3(64), 0, s, p, only_if_ambiguous);
-    MGK_HIP_CHECK(hipGetLastError());
-}
-
-void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
-                      int has_q0, hipStream_t s, int time_from_ctl) {
-    if (p.g.dims == 2)
-        hipLaunchKernelGGL((k_flip<2, 3>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0,
-                           time_from_ctl);
+template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
+    const int nb = kmc_nblocks(l.g.N * l.M);
+    if (l.g.dims == 2)
+        hipLaunchKernelGGL((k_rates<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
     else
-        hipLaunchKernelGGL((k_flip<3, 6>), dim3(1), dim3(64), 0, s, p, flat, time, first_mode, n_modes, has_cat, has_q0,
-                           time_from_ctl);
+        hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+    hipLaunchKernelGGL(k_rates_scan<BATCH>, dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, nb);
+}
+
+void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
+    if (l.B > 1) {
+        if (!times) throw ArgError("a batch needs per-trajectory times");
+        enqueue_rates_t<true>(l, time, times, s);
+    } else {
+        enqueue_rates_t<false>(l, time, times, s);
+    }
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void kmc_enqueue_event_prep(const KmcParams& p, Ctl* ctl, const EventArgs& ea, hipStream_t s) {
-    if (p.g.dims == 2) hipLaunchKernelGGL((k_event_prep<2, 3>), dim3(1), dim3(64), 0, s, p, ctl, ea);
-    else hipLaunchKernelGGL((k_event_prep<3, 6>), dim3(1), dim3(64), 0, s, p, ctl, ea);
+void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipStream_t s) {
+    if (l.B > 1) {
+        if (!ea) throw ArgError("a batch needs per-trajectory event arguments");
+        hipLaunchKernelGGL(k_select<true>, dim3(1, l.B), dim3(256), 0, s, l.h, l.P, u2, ea);
+    } else {
+        hipLaunchKernelGGL(k_select<false>, dim3(1, 1), dim3(256), 0, s, l.h, l.P, u2, ea);
+    }
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void kmc_enqueue_unstable(const KmcParams& p, hipStream_t s) {
-    const int nb = kmc_nblocks(p.g.N * p.ph.M);
-    hipLaunchKernelGGL(k_unstable, dim3(nb), dim3(256), 0, s, p);
+void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s) {
+    if (l.B > 1) hipLaunchKernelGGL(k_select_seq<true>, dim3(1, l.B), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
+    else hipLaunchKernelGGL(k_select_seq<false>, dim3(1, 1), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+template <int D, int C, bool BATCH>
+static void enqueue_flip_t(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
+                           int has_q0, hipStream_t s, int time_from_ctl) {
+    hipLaunchKernelGGL((k_flip<D, C, BATCH>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, flat, time, first_mode, n_modes,
+                       has_cat, has_q0, time_from_ctl);
+}
+
+void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
+                      int has_q0, hipStream_t s, int time_from_ctl) {
+    if (l.B > 1 && (flat >= 0 || !time_from_ctl)) throw ArgError("a batch flips its device-selected modes");
+    if (l.g.dims == 2) {
+        if (l.B > 1) enqueue_flip_t<2, 3, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        else enqueue_flip_t<2, 3, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+    } else {
+        if (l.B > 1) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        else enqueue_flip_t<3, 6, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+    }
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
+                            const EventArgs* ea, hipStream_t s) {
+    if (l.B > 1) {
+        if (l.g.dims == 2)
+            hipLaunchKernelGGL((k_event_prep<2, 3, true>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
+        else
+            hipLaunchKernelGGL((k_event_prep<3, 6, true>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
+    } else {
+        if (l.g.dims == 2)
+            hipLaunchKernelGGL((k_event_prep<2, 3, false>), dim3(1, 1), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
+        else
+            hipLaunchKernelGGL((k_event_prep<3, 6, false>), dim3(1, 1), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
+    }
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s) {
+    const int nb = kmc_nblocks(l.g.N * l.M);
+    if (l.B > 1) hipLaunchKernelGGL(k_unstable<true>, dim3(nb, l.B), dim3(256), 0, s, l.h, l.P);
+    else hipLaunchKernelGGL(k_unstable<false>, dim3(nb, 1), dim3(256), 0, s, l.h, l.P);
     MGK_HIP_CHECK(hipGetLastError());
 }
 

@@ -31,10 +31,11 @@ struct EventArgs {
     double strain_rate;    // eps_macro(drv) += strain_rate * t_wait
     double a[9];           // eps_macro before the attempt, D x D column-major
     double n_vox;          // N, for the antisymmetric norm term
+    double u2;             // selection uniform
     int drv;               // driven entry of a (row + D * col)
     int cascade_check;     // instability_mode == "cascade": unstable modes pre-empt the event
     int keep_shift;        // the field mean is known: shift = eps_bar - mean (else a mean pass follows)
-    int pad;
+    int active;            // 0: no attempt for this trajectory (MGK_EVENT_SKIPPED)
 };
 
 // Per-trajectory KMC physics scalars (device copy of mgk_kmc_params plus
@@ -75,16 +76,33 @@ struct KmcParams {
 constexpr int kRateBlock = 1024;   // modes per rate block
 constexpr int kRateThreads = 512;  // lanes per rate block (2 modes each)
 
+// One launch over B trajectories of one shape and mode count: row
+// blockIdx.y of every grid works on the trajectory whose parameter block is
+// P[blockIdx.y] (device memory).  A single trajectory is B = 1.
+struct KmcLaunch {
+    Geom g;
+    int M;
+    const KmcParams* P;
+    int B;
+    KmcParams h;   // host copy of the single parameter block (B = 1, passed by value)
+};
+
 int kmc_nblocks(long nmodes);
-void kmc_enqueue_rates(const KmcParams& p, double time, hipStream_t s);
-void kmc_enqueue_select(const KmcParams& p, double u2, hipStream_t s);
-void kmc_enqueue_select_fallback(const KmcParams& p, int only_if_ambiguous, hipStream_t s);
+// times: per-trajectory times (device, B entries) or null for `time`
+void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s);
+// ea: per-trajectory event arguments (device) whose u2 replaces `u2`, or null
+void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipStream_t s);
+void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s);
 // time_from_ctl: the flip time is the device-decided event's time_new
-void kmc_enqueue_flip(const KmcParams& p, int flat, double time, int first_mode, int n_modes, int has_cat,
+void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s, int time_from_ctl = 0);
-void kmc_enqueue_unstable(const KmcParams& p, hipStream_t s);
-// decision of one event attempt after kmc_enqueue_rates; on "no event" it
-// marks the LS control block done so a solve enqueued behind it returns at once
-void kmc_enqueue_event_prep(const KmcParams& p, Ctl* ctl, const EventArgs& ea, hipStream_t s);
+void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s);
+// decision of one event attempt per trajectory after kmc_enqueue_rates; on
+// "no event" it marks that trajectory's LS control block done so a solve
+// enqueued behind it returns at once.  B = 1: ctl and ea_one; a batch: the
+// control blocks P[y].ctl and the arguments ea[y] (device).
+struct LsParams;
+void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
+                            const EventArgs* ea, hipStream_t s);
 
 }  // namespace mgk

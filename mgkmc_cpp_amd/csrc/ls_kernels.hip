@@ -54,6 +54,13 @@ namespace mgk {
 
 extern __shared__ __align__(16) unsigned char smem_raw[];
 
+// The parameter block of this block row: the kernel argument for a single
+// solve (B = 1; kernel arguments are preloaded), P[blockIdx.y] for a batch.
+template <bool BATCH, typename T> __device__ __forceinline__ const T& param_sel(const T& a, const T* P) {
+    if constexpr (BATCH) return P[blockIdx.y];
+    else return a;
+}
+
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
@@ -89,12 +96,25 @@ template <int K> __device__ __forceinline__ void block_sum(double (&v)[K], doubl
 // the partials were written by an earlier kernel on the same stream).
 template <int K> __device__ __forceinline__ void reduce_partials(const double* part, int nblocks, double (&out)[K],
                                                                  double* scratch) {
+    constexpr int U = 4;   // partial rows per lane loaded together
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += U * blockDim.x) {
+        double t[U][K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += part[(long)b * K + k];
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * (int)blockDim.x;
+            const long bb = (b < nblocks) ? b : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) t[u][k] = part[bb * K + k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + u * (int)blockDim.x < nblocks)
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[k] += t[u][k];
+    }
     block_sum<K>(v, scratch);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = v[k];
@@ -219,8 +239,9 @@ template <typename SR, int N, int C> constexpr int gamma_points() {
 // Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW = 2*PAIRS lines per
 // block, VPT voxels per lane.
 // ---------------------------------------------------------------------------
-template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int flags) {
+template <typename SR, int NL, int C, int MODE, bool BATCH>
+__global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int PAIRS = fwd_pairs<SR, NL, C>();
     constexpr int LPW = 2 * PAIRS;
     constexpr int VPT = (LPW * NL + 255) / 256;
@@ -395,8 +416,9 @@ __global__ __launch_bounds__(256) void k_fwd_last(LsParams p, int flags) {
 // owns L consecutive columns; lane (l, t) = (tid % L, tid / L), so a wave's
 // gather and store touch consecutive columns of one row.
 // ---------------------------------------------------------------------------
-template <typename SR, int NM, int DIR>
-__global__ __launch_bounds__(kColThreadsMax) void k_mid(LsParams p, int mode) {
+template <typename SR, int NM, int DIR, bool BATCH>
+__global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, const LsParams* __restrict__ P, int mode) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int T = FftPlan<NM>::threads;
     constexpr int L = mid_cols<SR, NM>();
     constexpr int NMP = padded<NM>();
@@ -427,8 +449,9 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(LsParams p, int mode) {
 // f).  A block owns G frequency points (m, l-bin) with all C components, i.e.
 // L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
-template <typename SR, int NF, int C>
-__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(LsParams p, int mode) {
+template <typename SR, int NF, int C, bool BATCH>
+__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int D = (C == 3) ? 2 : 3;
     constexpr int T = FftPlan<NF>::threads;
     constexpr int G = gamma_points<SR, NF, C>();
@@ -516,8 +539,9 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(LsParams p, int 
 // norms of the relative-increment test.  CG: w = A r and its C0 products.
 // Block partials go to p.partials; k_ls_reduce finishes.
 // ---------------------------------------------------------------------------
-template <typename SR, int NL, int C, int MODE>
-__global__ __launch_bounds__(256) void k_inv_last(LsParams p, int flags) {
+template <typename SR, int NL, int C, int MODE, bool BATCH>
+__global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int PAIRS = inv_pairs<SR, NL, C>();
     constexpr int LPW = 2 * PAIRS;
     constexpr int VPT = (LPW * NL + 255) / 256;
@@ -663,8 +687,9 @@ __global__ __launch_bounds__(256) void k_inv_last(LsParams p, int flags) {
 // Reduction of pass E's block partials and the control-block update.  A kernel
 // of its own: the kernel boundary makes every block's partials visible, with
 // no per-block device-scope fence (which writes back the XCD's L2).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ls_reduce(LsParams p, int nblocks, int flags) {
+template <int MODE, bool BATCH>
+__global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
     __shared__ double red[8 * 4 * 2];
     double tot[3];
@@ -717,8 +742,9 @@ __global__ __launch_bounds__(256) void k_ls_reduce(LsParams p, int nblocks, int 
 // Field mean -> warm-start shift (linear_elastic_solver.hpp:181-191); or fill
 // with eps_bar for a cold start (:193).
 // ---------------------------------------------------------------------------
-template <int C>
-__global__ __launch_bounds__(256) void k_mean_shift(LsParams p) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_mean_shift(const LsParams p_arg, const LsParams* __restrict__ P) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     __shared__ double red[8 * 6];
     const Geom g = p.g;
     double acc[C];
@@ -733,8 +759,10 @@ __global__ __launch_bounds__(256) void k_mean_shift(LsParams p) {
         for (int c = 0; c < C; ++c) p.partials[C * blockIdx.x + c] = acc[c];
 }
 
-template <int C>
-__global__ __launch_bounds__(256) void k_mean_reduce(LsParams p, int nblocks) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_mean_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    if (p.ctl->keep_shift) return;
     __shared__ double red[8 * 6];
     double tot[C];
     reduce_partials<C>(p.partials, nblocks, tot, red);
@@ -744,8 +772,9 @@ __global__ __launch_bounds__(256) void k_mean_reduce(LsParams p, int nblocks) {
     }
 }
 
-template <int C>
-__global__ __launch_bounds__(256) void k_fill_bar(LsParams p) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_fill_bar(const LsParams p_arg, const LsParams* __restrict__ P) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     const Geom g = p.g;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x)
 #pragma unroll
@@ -753,8 +782,9 @@ __global__ __launch_bounds__(256) void k_fill_bar(LsParams p) {
 }
 
 // eps += shift (a warm start with no iteration still shifts, :181-191)
-template <int C>
-__global__ __launch_bounds__(256) void k_apply_shift(LsParams p) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_apply_shift(const LsParams p_arg, const LsParams* __restrict__ P) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     const Geom g = p.g;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x)
 #pragma unroll
@@ -764,8 +794,9 @@ __global__ __launch_bounds__(256) void k_apply_shift(LsParams p) {
 // ---------------------------------------------------------------------------
 // Final stress and macros (linear_elastic_solver.hpp:299-308, :583-592).
 // ---------------------------------------------------------------------------
-template <int C>
-__global__ __launch_bounds__(256) void k_final_stress(LsParams p) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, const LsParams* __restrict__ P) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int D = (C == 3) ? 2 : 3;
     __shared__ double red[8 * 12];
     const Geom g = p.g;
@@ -802,8 +833,9 @@ __global__ __launch_bounds__(256) void k_final_stress(LsParams p) {
         for (int c = 0; c < 2 * C; ++c) p.partials[2 * C * blockIdx.x + c] = acc[c];
 }
 
-template <int C>
-__global__ __launch_bounds__(256) void k_final_reduce(LsParams p, int nblocks) {
+template <int C, bool BATCH>
+__global__ __launch_bounds__(256) void k_final_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
     __shared__ double red[8 * 12];
     double tot[2 * C];
     reduce_partials<2 * C>(p.partials, nblocks, tot, red);
@@ -832,15 +864,15 @@ bool ls_length_supported(int n) {
 
 static int blocks_for(long n, int per) { return (int)((n + per - 1) / per); }
 
-template <typename SR, int C, int MODE>
-static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
+template <typename SR, int C, int MODE, bool BATCH>
+static void launch_fwd_last(const LsLaunch& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                                          \
     if (nl == NN) {                                                                                           \
         constexpr int pairs = fwd_pairs<SR, NN, C>();                                                         \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                        \
-        hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), dim3(256), \
-                           sm, s, p, flags);                                                                  \
+        hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE, BATCH>), dim3(blocks_for(p.g.nlines(), 2 * pairs), p.B), dim3(256), \
+                           sm, s, p.h, p.P, flags);                                                                  \
         return;                                                                                               \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -848,16 +880,16 @@ static void launch_fwd_last(const LsParams& p, int flags, hipStream_t s) {
     throw UnsupportedError("FFT length not supported on the last axis");
 }
 
-template <typename SR, int C, int MODE>
-static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
+template <typename SR, int C, int MODE, bool BATCH>
+static void launch_inv_last(const LsLaunch& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                                  \
     if (nl == NN) {                                                                                   \
         constexpr int pairs = inv_pairs<SR, NN, C>();                                                \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                \
         const int nb = blocks_for(p.g.nlines(), 2 * pairs);                                           \
-        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(nb), dim3(256), sm, s, p, flags);     \
-        hipLaunchKernelGGL((k_ls_reduce<MODE>), dim3(1), dim3(256), 0, s, p, nb, flags);              \
+        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE, BATCH>), dim3(nb, p.B), dim3(256), sm, s, p.h, p.P, flags);     \
+        hipLaunchKernelGGL((k_ls_reduce<MODE, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb, flags);              \
         return;                                                                                       \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -865,16 +897,16 @@ static void launch_inv_last(const LsParams& p, int flags, hipStream_t s) {
     throw UnsupportedError("FFT length not supported on the last axis");
 }
 
-template <typename SR, int DIR>
-static void launch_mid(const LsParams& p, hipStream_t s, int mode) {
+template <typename SR, int DIR, bool BATCH>
+static void launch_mid(const LsLaunch& p, hipStream_t s, int mode) {
     const int nm = p.g.nm;
     const long total = (long)p.g.nf * p.g.nlh * p.g.C;
 #define MGK_CASE(NN)                                                                                       \
     if (nm == NN) {                                                                                        \
         constexpr int L = mid_cols<SR, NN>();                                                              \
         const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);                              \
-        hipLaunchKernelGGL((k_mid<SR, NN, DIR>), dim3(blocks_for(total, L)), dim3(L * FftPlan<NN>::threads), \
-                           sm, s, p, mode);                                                                \
+        hipLaunchKernelGGL((k_mid<SR, NN, DIR, BATCH>), dim3(blocks_for(total, L), p.B), dim3(L * FftPlan<NN>::threads), \
+                           sm, s, p.h, p.P, mode);                                                                \
         return;                                                                                            \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -882,16 +914,16 @@ static void launch_mid(const LsParams& p, hipStream_t s, int mode) {
     throw UnsupportedError("FFT length not supported on the middle axis");
 }
 
-template <typename SR, int C>
-static void launch_first_gamma(const LsParams& p, hipStream_t s, int mode) {
+template <typename SR, int C, bool BATCH>
+static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode) {
     const int nf = p.g.nf;
     const long nxi = (long)p.g.nm * p.g.nlh;
 #define MGK_CASE(NN)                                                                                 \
     if (nf == NN) {                                                                                  \
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
-        hipLaunchKernelGGL((k_first_gamma<SR, NN, C>), dim3(blocks_for(nxi, G)),                     \
-                           dim3(G * C * FftPlan<NN>::threads), sm, s, p, mode);                      \
+        hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH>), dim3(blocks_for(nxi, G), p.B),                     \
+                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode);                      \
         return;                                                                                      \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -907,20 +939,20 @@ int ls_partial_slots(const Geom& g) {
     return (int)std::max(nb * 3, 1024L * 12);
 }
 
-template <int C> static void iteration_full(const LsParams& p, int flags, hipStream_t s) {
-    launch_fwd_last<double, C, LS_FULL>(p, flags, s);
-    if (C == 6) launch_mid<double, -1>(p, s, LS_FULL);
-    launch_first_gamma<double, C>(p, s, LS_FULL);
-    if (C == 6) launch_mid<double, +1>(p, s, LS_FULL);
-    launch_inv_last<double, C, LS_FULL>(p, flags, s);
+template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int flags, hipStream_t s) {
+    launch_fwd_last<double, C, LS_FULL, BATCH>(p, flags, s);
+    if (C == 6) launch_mid<double, -1, BATCH>(p, s, LS_FULL);
+    launch_first_gamma<double, C, BATCH>(p, s, LS_FULL);
+    if (C == 6) launch_mid<double, +1, BATCH>(p, s, LS_FULL);
+    launch_inv_last<double, C, LS_FULL, BATCH>(p, flags, s);
 }
 
-template <int C> static void iteration_delta(const LsParams& p, int flags, hipStream_t s) {
-    launch_fwd_last<float, C, LS_CG>(p, flags, s);
-    if (C == 6) launch_mid<float, -1>(p, s, LS_CG);
-    launch_first_gamma<float, C>(p, s, LS_CG);
-    if (C == 6) launch_mid<float, +1>(p, s, LS_CG);
-    launch_inv_last<float, C, LS_CG>(p, flags, s);
+template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
+    launch_fwd_last<float, C, LS_CG, BATCH>(p, flags, s);
+    if (C == 6) launch_mid<float, -1, BATCH>(p, s, LS_CG);
+    launch_first_gamma<float, C, BATCH>(p, s, LS_CG);
+    if (C == 6) launch_mid<float, +1, BATCH>(p, s, LS_CG);
+    launch_inv_last<float, C, LS_CG, BATCH>(p, flags, s);
 }
 
 static int reduce_grid(long n) {
@@ -928,24 +960,28 @@ static int reduce_grid(long n) {
     return (int)std::min<long>(std::max<long>(nb, 1), 1024);
 }
 
-void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s) {
+template <bool BATCH> static void enqueue_iteration_t(const LsLaunch& p, int mode, int flags, hipStream_t s) {
     if (p.g.C == 3) {
-        if (mode == LS_FULL) iteration_full<3>(p, flags, s); else iteration_delta<3>(p, flags, s);
+        if (mode == LS_FULL) iteration_full<3, BATCH>(p, flags, s); else iteration_delta<3, BATCH>(p, flags, s);
     } else {
-        if (mode == LS_FULL) iteration_full<6>(p, flags, s); else iteration_delta<6>(p, flags, s);
+        if (mode == LS_FULL) iteration_full<6, BATCH>(p, flags, s); else iteration_delta<6, BATCH>(p, flags, s);
     }
+}
+
+void ls_enqueue_iteration(const LsLaunch& p, int mode, int flags, hipStream_t s) {
+    if (p.B > 1) enqueue_iteration_t<true>(p, mode, flags, s); else enqueue_iteration_t<false>(p, mode, flags, s);
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-template <typename SR, int C, int MODE>
-static void launch_inv_last_only(const LsParams& p, int flags, hipStream_t s) {
+template <typename SR, int C, int MODE, bool BATCH>
+static void launch_inv_last_only(const LsLaunch& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
 #define MGK_CASE(NN)                                                                              \
     if (nl == NN) {                                                                               \
         constexpr int pairs = inv_pairs<SR, NN, C>();                                            \
         const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);            \
-        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE>), dim3(blocks_for(p.g.nlines(), 2 * pairs)), \
-                           dim3(256), sm, s, p, flags);                                           \
+        hipLaunchKernelGGL((k_inv_last<SR, NN, C, MODE, BATCH>), dim3(blocks_for(p.g.nlines(), 2 * pairs), p.B), \
+                           dim3(256), sm, s, p.h, p.P, flags);                                           \
         return;                                                                                   \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -953,63 +989,76 @@ static void launch_inv_last_only(const LsParams& p, int flags, hipStream_t s) {
     throw UnsupportedError("FFT length not supported on the last axis");
 }
 
-template <int C> static void enqueue_pass_c(const LsParams& p, int pass, int mode, hipStream_t s) {
+template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int pass, int mode, hipStream_t s) {
     const bool full = (mode == LS_FULL);
     switch (pass) {
-        case 0: if (full) launch_fwd_last<double, C, LS_FULL>(p, 0, s); else launch_fwd_last<float, C, LS_CG>(p, 0, s); break;
-        case 1: if (full) launch_mid<double, -1>(p, s, mode); else launch_mid<float, -1>(p, s, mode); break;
-        case 2: if (full) launch_first_gamma<double, C>(p, s, mode); else launch_first_gamma<float, C>(p, s, mode); break;
-        case 3: if (full) launch_mid<double, +1>(p, s, mode); else launch_mid<float, +1>(p, s, mode); break;
-        case 4: if (full) launch_inv_last_only<double, C, LS_FULL>(p, 0, s); else launch_inv_last_only<float, C, LS_CG>(p, 0, s); break;
+        case 0:
+            if (full) launch_fwd_last<double, C, LS_FULL, BATCH>(p, 0, s); else launch_fwd_last<float, C, LS_CG, BATCH>(p, 0, s);
+            break;
+        case 1: if (full) launch_mid<double, -1, BATCH>(p, s, mode); else launch_mid<float, -1, BATCH>(p, s, mode); break;
+        case 2: if (full) launch_first_gamma<double, C, BATCH>(p, s, mode); else launch_first_gamma<float, C, BATCH>(p, s, mode); break;
+        case 3: if (full) launch_mid<double, +1, BATCH>(p, s, mode); else launch_mid<float, +1, BATCH>(p, s, mode); break;
+        case 4:
+            if (full) launch_inv_last_only<double, C, LS_FULL, BATCH>(p, 0, s);
+            else launch_inv_last_only<float, C, LS_CG, BATCH>(p, 0, s);
+            break;
         case 5:
             if (full) throw ArgError("the CG update pass runs in FP32 only");
-            launch_fwd_last<float, C, LS_CG>(p, LS_FLAG_CG_UPDATE, s);
+            launch_fwd_last<float, C, LS_CG, BATCH>(p, LS_FLAG_CG_UPDATE, s);
             break;
         default: throw ArgError("unknown pass");
     }
 }
 
-void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s) {
+void ls_enqueue_pass(const LsLaunch& p, int pass, int mode, hipStream_t s) {
     if (p.g.C == 3) {
         if (pass == 1 || pass == 3) throw ArgError("pass not present in 2D");
-        enqueue_pass_c<3>(p, pass, mode, s);
+        if (p.B > 1) enqueue_pass_c<3, true>(p, pass, mode, s); else enqueue_pass_c<3, false>(p, pass, mode, s);
     } else {
-        enqueue_pass_c<6>(p, pass, mode, s);
+        if (p.B > 1) enqueue_pass_c<6, true>(p, pass, mode, s); else enqueue_pass_c<6, false>(p, pass, mode, s);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void ls_enqueue_start(const LsParams& p, int start, hipStream_t s) {
+template <bool BATCH> static void enqueue_start_t(const LsLaunch& p, int start, hipStream_t s) {
     const int nb = reduce_grid(p.g.N);
     if (start == LS_START_KEEP) return;
     if (p.g.C == 3) {
         if (start == LS_START_MEAN) {
-            hipLaunchKernelGGL((k_mean_shift<3>), dim3(nb), dim3(256), 0, s, p);
-            hipLaunchKernelGGL((k_mean_reduce<3>), dim3(1), dim3(256), 0, s, p, nb);
+            hipLaunchKernelGGL((k_mean_shift<3, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+            hipLaunchKernelGGL((k_mean_reduce<3, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb);
         }
-        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<3>), dim3(nb), dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_apply_shift<3>), dim3(nb), dim3(256), 0, s, p);
+        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<3, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+        else hipLaunchKernelGGL((k_apply_shift<3, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
     } else {
         if (start == LS_START_MEAN) {
-            hipLaunchKernelGGL((k_mean_shift<6>), dim3(nb), dim3(256), 0, s, p);
-            hipLaunchKernelGGL((k_mean_reduce<6>), dim3(1), dim3(256), 0, s, p, nb);
+            hipLaunchKernelGGL((k_mean_shift<6, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+            hipLaunchKernelGGL((k_mean_reduce<6, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb);
         }
-        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<6>), dim3(nb), dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_apply_shift<6>), dim3(nb), dim3(256), 0, s, p);
+        else if (start == LS_START_FILL) hipLaunchKernelGGL((k_fill_bar<6, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+        else hipLaunchKernelGGL((k_apply_shift<6, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void ls_enqueue_final(const LsParams& p, hipStream_t s) {
+template <bool BATCH> static void enqueue_final_t(const LsLaunch& p, hipStream_t s) {
     const int nb = reduce_grid(p.g.N);
     if (p.g.C == 3) {
-        hipLaunchKernelGGL((k_final_stress<3>), dim3(nb), dim3(256), 0, s, p);
-        hipLaunchKernelGGL((k_final_reduce<3>), dim3(1), dim3(256), 0, s, p, nb);
+        hipLaunchKernelGGL((k_final_stress<3, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+        hipLaunchKernelGGL((k_final_reduce<3, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb);
     } else {
-        hipLaunchKernelGGL((k_final_stress<6>), dim3(nb), dim3(256), 0, s, p);
-        hipLaunchKernelGGL((k_final_reduce<6>), dim3(1), dim3(256), 0, s, p, nb);
+        hipLaunchKernelGGL((k_final_stress<6, BATCH>), dim3(nb, p.B), dim3(256), 0, s, p.h, p.P);
+        hipLaunchKernelGGL((k_final_reduce<6, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb);
     }
     MGK_HIP_CHECK(hipGetLastError());
+}
+
+void ls_enqueue_start(const LsLaunch& p, int start, hipStream_t s) {
+    if (p.B > 1) enqueue_start_t<true>(p, start, s); else enqueue_start_t<false>(p, start, s);
+}
+
+void ls_enqueue_final(const LsLaunch& p, hipStream_t s) {
+    if (p.B > 1) enqueue_final_t<true>(p, s); else enqueue_final_t<false>(p, s);
 }
 
 }  // namespace mgk

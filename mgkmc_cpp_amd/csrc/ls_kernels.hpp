@@ -49,13 +49,24 @@ struct LsParams {
     double* partials;
 };
 
+// One launch of a pipeline step over B trajectories of one shape: row
+// blockIdx.y of every grid works on the trajectory whose parameter block is
+// P[blockIdx.y] (device memory).  A single solve is B = 1 and passes its
+// block by value instead (kernel arguments are preloaded).
+struct LsLaunch {
+    Geom g;
+    const LsParams* P;
+    int B;
+    LsParams h;   // host copy of the single parameter block (B = 1)
+};
+
 bool ls_length_supported(int n);
 int ls_partial_slots(const Geom& g);
-void ls_enqueue_start(const LsParams& p, int start, hipStream_t s);
-void ls_enqueue_iteration(const LsParams& p, int mode, int flags, hipStream_t s);
-void ls_enqueue_final(const LsParams& p, hipStream_t s);
+void ls_enqueue_start(const LsLaunch& l, int start, hipStream_t s);
+void ls_enqueue_iteration(const LsLaunch& l, int mode, int flags, hipStream_t s);
+void ls_enqueue_final(const LsLaunch& l, hipStream_t s);
 // one pass of the pipeline alone, for kernel timing: 0 A, 1 B, 2 C, 3 D,
 // 4 E (without its reduction), 5 A with the CG vector update (FP32)
-void ls_enqueue_pass(const LsParams& p, int pass, int mode, hipStream_t s);
+void ls_enqueue_pass(const LsLaunch& l, int pass, int mode, hipStream_t s);
 
 }  // namespace mgk
