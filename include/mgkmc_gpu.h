@@ -192,7 +192,8 @@ enum mgk_event_outcome {
     MGK_EVENT_DONE = 0,
     MGK_EVENT_UNSTABLE = 1,
     MGK_EVENT_NO_RATE = 2,
-    MGK_EVENT_REJECTED = 3
+    MGK_EVENT_REJECTED = 3,
+    MGK_EVENT_SKIPPED = 4        /* batch only: the trajectory was not active */
 };
 typedef struct mgk_event_in {
     double time;                 /* simulation time before the attempt */
@@ -219,6 +220,24 @@ typedef struct mgk_event_out {
 } mgk_event_out;
 int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, double* sig_macro,
                   mgk_event_out* out);
+
+/* Batches (no counterpart in the reference, which runs one trajectory per
+ * process): B contexts of one shape, precision and mode count whose event
+ * attempts run together, each launch covering all B trajectories.  The
+ * contexts keep their state and their own entry points (cascades, field
+ * reads) between batch calls; a context belongs to at most one batch at a
+ * time and must outlive it.  Per trajectory y, in[y], eps_macro + y*D*D,
+ * sig_macro + y*D*D and out[y] mean what they mean for mgk_kmc_event, and
+ * the results equal B separate mgk_kmc_event calls (FP64 bit for bit; in
+ * FP32+refine the batch uses the largest inner-step count of its members).
+ * in[y].max_iter, tol and the redraw mode range must agree across y;
+ * active (may be NULL) switches trajectories off (MGK_EVENT_SKIPPED). */
+typedef struct mgk_batch mgk_batch;
+int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out);
+int mgk_batch_destroy(mgk_batch* b);
+int mgk_batch_size(const mgk_batch* b);
+int mgk_batch_kmc_event(mgk_batch* b, const mgk_event_in* in, const int* active, double* eps_macro,
+                        double* sig_macro, mgk_event_out* out);
 
 /* Stream synchronisation and timing of the context's stream (HIP events). */
 int mgk_sync(mgk_ctx* ctx);

@@ -7,6 +7,7 @@ Native pieces (built in-tree by ``python -m mgkmc_cpp_amd.build``):
 """
 from . import abi  # noqa: F401
 from ._lib import MgkError, gpu, host  # noqa: F401
-from .solver import Context, Simulation, spectral_solver_2d, spectral_solver_3d  # noqa: F401
+from .solver import Context, Simulation, SimulationBatch, spectral_solver_2d, spectral_solver_3d  # noqa: F401
 
-__all__ = ["abi", "Context", "Simulation", "spectral_solver_2d", "spectral_solver_3d", "MgkError", "gpu", "host"]
+__all__ = ["abi", "Context", "Simulation", "SimulationBatch", "spectral_solver_2d", "spectral_solver_3d", "MgkError",
+           "gpu", "host"]

@@ -58,6 +58,11 @@ def gpu():
         _sig(lib, "mgk_last_selection", C.c_int, [P, IP, DP])
         _sig(lib, "mgk_kmc_event", C.c_int,
              [P, C.POINTER(abi.EventIn), DP, DP, C.POINTER(abi.EventOut)])
+        _sig(lib, "mgk_batch_create", C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)])
+        _sig(lib, "mgk_batch_destroy", C.c_int, [P])
+        _sig(lib, "mgk_batch_size", C.c_int, [P])
+        _sig(lib, "mgk_batch_kmc_event", C.c_int,
+             [P, C.POINTER(abi.EventIn), IP, DP, DP, C.POINTER(abi.EventOut)])
         _sig(lib, "mgk_sync", C.c_int, [P])
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])
@@ -90,6 +95,9 @@ def host():
         _sig(lib, "mgk_sim_num_events", C.c_longlong, [P])
         _sig(lib, "mgk_sim_get_events", C.c_int, [P, C.POINTER(abi.EventRecord), C.c_longlong])
         _sig(lib, "mgk_sim_device_ctx", C.c_int, [P, C.POINTER(P)])
+        _sig(lib, "mgk_sim_batch_create", C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)])
+        _sig(lib, "mgk_sim_batch_destroy", C.c_int, [P])
+        _sig(lib, "mgk_sim_batch_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
         _host = lib
     return _host
 

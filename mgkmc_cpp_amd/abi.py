@@ -39,6 +39,7 @@ EVENT_DONE = 0
 EVENT_UNSTABLE = 1
 EVENT_NO_RATE = 2
 EVENT_REJECTED = 3
+EVENT_SKIPPED = 4
 
 
 class EventIn(C.Structure):

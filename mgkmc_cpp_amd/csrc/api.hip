@@ -372,14 +372,40 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipStreamSynchronize(stream));
         }
         if (event && !hkctl->go) return 0;
+        absorb_solve(tol);
+        if (diff_out) *diff_out = hctl->diff;
+        return hctl->iters;
+    }
+
+    // Solver statistics and the field mean after a solve whose control block
+    // is in hctl.
+    void absorb_solve(double tol) {
+        const bool refine = (precision == MGK_FP32_REFINE);
         last_iters = hctl->iters;
         last_inner = hctl->inner;
         predicted = refine ? std::max(1, last_iters) : std::max(2, last_iters);
         for (int k = 0; k < g.C; ++k) mean_eps[k] = hctl->mean_eps[k];
         mean_valid = true;
         if (refine) tune_inner(tol);
-        if (diff_out) *diff_out = hctl->diff;
-        return hctl->iters;
+    }
+
+    // eps_macro / sig_macro after an event attempt that produced an event:
+    // eps_macro advanced by strain_rate * t_wait, then replaced by the field
+    // mean (elastic_run's alias, kmc_solver.hpp:334-339).
+    void event_macros(const mgk_event_in& in, double t_wait, double* eps_macro, double* sig_macro) {
+        const int Dd = D();
+        const double x = in.strain_rate * t_wait;
+        const int drv = in.drv_i + Dd * in.drv_j;
+        double a[9];
+        for (int q = 0; q < Dd * Dd; ++q) a[q] = eps_macro[q] + ((q == drv) ? 1.0 : 0.0) * x;
+        const int ns = nskew();
+        double bar_sym[6], bar_skew[3] = {0, 0, 0};
+        split(a, bar_sym, bar_skew);
+        for (int w = 0; w < ns; ++w) W[w] = bar_skew[w];
+        double sk_sig[3];
+        for (int w = 0; w < ns; ++w) sk_sig[w] = 2.0 * mean_mu * bar_skew[w];
+        join(hctl->mean_eps, bar_skew, eps_macro);
+        join(hctl->mean_sig, sk_sig, sig_macro);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -914,7 +940,7 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
 // against the mode range and the shared-catalog mode.  Returns the number of
 // modes staged; *has_cat / *has_q0 say which halves the flip applies.
 static int stage_redraw(mgk_ctx& c, const double* new_catalog, const double* new_Q0, int first_mode, int n_modes,
-                        int* has_cat, int* has_q0) {
+                        int* has_cat, int* has_q0, double* dest = nullptr) {
     const int C = c.g.C, F = c.full(), M = c.kp.M;
     if ((new_catalog || new_Q0) &&
         (n_modes < 0 || (first_mode >= 0 && first_mode + n_modes > M) || (first_mode < 0 && n_modes != 1)))
@@ -934,8 +960,8 @@ static int stage_redraw(mgk_ctx& c, const double* new_catalog, const double* new
     }
     if (!new_catalog && !new_Q0) n_modes = 0;
     // the pinned staging buffer may still feed the previous call's copy
-    if (c.kctl_pending) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
-    double* h = c.hkstage;
+    if (!dest && c.kctl_pending) MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+    double* h = dest ? dest : c.hkstage;
     double skw[3];
     if (new_catalog)
         for (int j = 0; j < n_modes; ++j) c.split(new_catalog + (size_t)j * F, h + (size_t)j * C, skw);
@@ -1018,20 +1044,7 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         if (k.outcome != MGK_EVENT_DONE) return MGK_OK;
         out->iters = it;
         out->diff = diff;
-        // eps_macro as the device updated it, then replaced by the field mean
-        // (elastic_run's alias, kmc_solver.hpp:334-339)
-        const double x = in->strain_rate * k.t_wait;
-        const int drv = in->drv_i + Dd * in->drv_j;
-        double a[9];
-        for (int q = 0; q < Dd * Dd; ++q) a[q] = eps_macro[q] + ((q == drv) ? 1.0 : 0.0) * x;
-        const int ns = c.nskew();
-        double bar_sym[6], bar_skew[3] = {0, 0, 0};
-        c.split(a, bar_sym, bar_skew);
-        for (int w = 0; w < ns; ++w) c.W[w] = bar_skew[w];
-        double sk_sig[3];
-        for (int w = 0; w < ns; ++w) sk_sig[w] = 2.0 * c.mean_mu * bar_skew[w];
-        c.join(c.hctl->mean_eps, bar_skew, eps_macro);
-        c.join(c.hctl->mean_sig, sk_sig, sig_macro);
+        c.event_macros(*in, k.t_wait, eps_macro, sig_macro);
         return MGK_OK;
     });
 }
@@ -1131,6 +1144,235 @@ int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters) {
         if (!ctx) throw ArgError("null context");
         if (fp64_iters) *fp64_iters = ctx->last_iters;
         if (fp32_iters) *fp32_iters = ctx->last_inner;
+        return MGK_OK;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// A batch of trajectories of one shape that advance together: every launch
+// of a batch event covers all of them (block row y of each grid works on
+// trajectory y, through its own parameter block).  The contexts keep their
+// own fields and host state, and stay usable on their own between batch
+// calls (cascades, field reads).
+// ---------------------------------------------------------------------------
+struct mgk_batch {
+    std::vector<mgk_ctx*> c;
+    int B = 0;
+    hipStream_t stream = nullptr;
+    LsParams *d_lsp = nullptr, *h_lsp = nullptr;
+    KmcParams *d_kp = nullptr, *h_kp = nullptr;
+    EventArgs *d_ea = nullptr, *h_ea = nullptr;
+    double *d_times = nullptr, *h_times = nullptr;
+    Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+    KmcCtl *d_kctl = nullptr, *h_kctl = nullptr;
+    double *d_stage = nullptr, *h_stage = nullptr;
+    size_t stage_per = 0;   // M * (C + 1) doubles per trajectory
+
+    ~mgk_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        auto f = [](void* p) { if (p) (void)hipFree(p); };
+        auto fh = [](void* p) { if (p) (void)hipHostFree(p); };
+        f(d_lsp); f(d_kp); f(d_ea); f(d_times); f(d_ctl); f(d_kctl); f(d_stage);
+        fh(h_lsp); fh(h_kp); fh(h_ea); fh(h_times); fh(h_ctl); fh(h_kctl); fh(h_stage);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// One event attempt for every active trajectory of the batch, the same
+// sequence of launches as mgk_kmc_event with grids B rows tall.
+void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double* eps_macro, double* sig_macro,
+                 mgk_event_out* out) {
+    const int B = b.B;
+    mgk_ctx& c0 = *b.c[0];
+    const Geom g = c0.g;
+    const int Dd = c0.D(), F = Dd * Dd, C = g.C;
+    const bool refine = (c0.precision == MGK_FP32_REFINE);
+    int max_iter = 0, J = 1, predicted = 1;
+    double tol = 0.0;
+    int first_mode = 0, n_modes = -1, has_cat = 0, has_q0 = 0;
+    bool all_keep = true;
+    int n_active = 0;
+    for (int y = 0; y < B; ++y) {
+        mgk_ctx& c = *b.c[y];
+        const mgk_event_in& e = in[y];
+        const bool on = !active || active[y];
+        // the contexts' own streams hold no pending work behind a batch call
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        c.kctl_pending = false;
+        if (on) {
+            if (e.drv_i < 0 || e.drv_i >= Dd || e.drv_j < 0 || e.drv_j >= Dd) throw ArgError("driven component out of range");
+            if (e.max_iter < 1) throw ArgError("max_iter must be >= 1");
+            int hc = 0, hq = 0;
+            const int n = stage_redraw(c, e.new_catalog, e.new_Q0, e.first_mode, e.n_modes, &hc, &hq,
+                                       b.h_stage + (size_t)y * b.stage_per);
+            if (n_active == 0) {
+                max_iter = e.max_iter; tol = e.tol; first_mode = e.first_mode;
+                n_modes = n; has_cat = hc; has_q0 = hq;
+            } else if (e.max_iter != max_iter || e.tol != tol || e.first_mode != first_mode || n != n_modes ||
+                       hc != has_cat || hq != has_q0) {
+                throw ArgError("the trajectories of a batch share max_iter, tol and the redrawn modes");
+            }
+            ++n_active;
+        }
+        // LS control block, parameter blocks and event arguments
+        c.init_ctl(on ? e.max_iter : 1, on ? e.tol : 0.0);
+        Ctl& h = b.h_ctl[y];
+        h = *c.hctl;
+        const int keep = c.mean_valid ? 1 : 0;
+        if (on) all_keep = all_keep && keep;
+        h.keep_shift = keep;
+        LsParams lp = c.ls_params();
+        lp.ctl = b.d_ctl + y;
+        b.h_lsp[y] = lp;
+        KmcParams kp = c.K;
+        kp.kctl = b.d_kctl + y;
+        kp.staging = b.d_stage + (size_t)y * b.stage_per;
+        b.h_kp[y] = kp;
+        EventArgs& ea = b.h_ea[y];
+        std::memset(&ea, 0, sizeof(ea));
+        if (!on) {
+            ea.n_vox = (double)g.N;
+            ea.active = 0;
+            b.h_times[y] = 0.0;
+            continue;
+        }
+        ea.time = e.time;
+        ea.neg_log_u1 = e.neg_log_u1;
+        ea.remaining = e.remaining;
+        ea.strain_rate = e.strain_rate;
+        for (int k = 0; k < F; ++k) ea.a[k] = eps_macro[(size_t)y * F + k];
+        ea.n_vox = (double)g.N;
+        ea.u2 = e.u2;
+        ea.drv = e.drv_i + Dd * e.drv_j;
+        ea.cascade_check = e.cascade_check ? 1 : 0;
+        ea.keep_shift = keep;
+        ea.active = on ? 1 : 0;
+        b.h_times[y] = on ? e.time : 0.0;
+        J = std::max(J, c.inner_per_outer);
+        predicted = std::max(predicted, c.predicted);
+    }
+    if (n_active == 0) throw ArgError("no active trajectory in the batch call");
+    hipStream_t s = b.stream;
+    MGK_HIP_CHECK(hipMemcpyAsync(b.d_lsp, b.h_lsp, sizeof(LsParams) * B, hipMemcpyHostToDevice, s));
+    MGK_HIP_CHECK(hipMemcpyAsync(b.d_kp, b.h_kp, sizeof(KmcParams) * B, hipMemcpyHostToDevice, s));
+    MGK_HIP_CHECK(hipMemcpyAsync(b.d_ea, b.h_ea, sizeof(EventArgs) * B, hipMemcpyHostToDevice, s));
+    MGK_HIP_CHECK(hipMemcpyAsync(b.d_times, b.h_times, sizeof(double) * B, hipMemcpyHostToDevice, s));
+    MGK_HIP_CHECK(hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(Ctl) * B, hipMemcpyHostToDevice, s));
+    if (n_modes > 0)
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_stage, b.h_stage, sizeof(double) * b.stage_per * B, hipMemcpyHostToDevice, s));
+    const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0]};
+    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0]};
+    kmc_enqueue_rates(KL, 0.0, b.d_times, s);
+    kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, s);
+    kmc_enqueue_select(KL, 0.0, b.d_ea, s);
+    kmc_enqueue_select_fallback(KL, 1, s);
+    kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, s, 1);
+    if (!all_keep) ls_enqueue_start(L, LS_START_MEAN, s);   // trajectories with a known mean keep their shift
+    // iterations until every trajectory's control block says done
+    int enqueued = 0;
+    int nb = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
+    while (true) {
+        for (int q = 0; q < nb; ++q) {
+            const int k = enqueued + q;
+            const int flags = (k == 0 ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
+            ls_enqueue_iteration(L, LS_FULL, flags, s);
+            if (refine)
+                for (int j = 0; j < J; ++j)
+                    ls_enqueue_iteration(L, LS_CG, j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), s);
+        }
+        enqueued += nb;
+        ls_enqueue_final(L, s);
+        MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, s));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.h_kctl, b.d_kctl, sizeof(KmcCtl) * B, hipMemcpyDeviceToHost, s));
+        MGK_HIP_CHECK(hipStreamSynchronize(s));
+        bool all_done = true;
+        for (int y = 0; y < B; ++y) all_done = all_done && b.h_ctl[y].done;
+        if (all_done || enqueued >= max_iter) break;
+        nb = std::min(max_iter - enqueued, std::max(refine ? 1 : 2, nb / 2));
+    }
+    for (int y = 0; y < B; ++y) {
+        mgk_ctx& c = *b.c[y];
+        const KmcCtl& k = b.h_kctl[y];
+        *c.hctl = b.h_ctl[y];
+        *c.hkctl = k;
+        mgk_event_out& o = out[y];
+        std::memset(&o, 0, sizeof(o));
+        o.outcome = k.outcome;
+        o.total_rate = k.total_rate;
+        o.n_unstable = k.n_unstable;
+        o.n_included = k.n_included;
+        o.t_wait = k.t_wait;
+        o.flat_idx = k.go ? k.sel : -1;
+        o.q_sel = k.go ? k.q_sel : 0.0;
+        if (k.outcome != MGK_EVENT_DONE) continue;
+        c.absorb_solve(tol);
+        o.iters = c.hctl->iters;
+        o.diff = c.hctl->diff;
+        c.event_macros(in[y], k.t_wait, eps_macro + (size_t)y * F, sig_macro + (size_t)y * F);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out) {
+    return guarded([&]() -> int {
+        if (!ctxs || !out || B < 1) throw ArgError("bad argument");
+        *out = nullptr;
+        const mgk_ctx* a = ctxs[0];
+        if (!a) throw ArgError("null context");
+        for (int y = 0; y < B; ++y) {
+            const mgk_ctx* c = ctxs[y];
+            if (!c) throw ArgError("null context");
+            if (!c->kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+            if (c->g.dims != a->g.dims || c->g.nf != a->g.nf || c->g.nm != a->g.nm || c->g.nl != a->g.nl ||
+                c->precision != a->precision || c->kp.M != a->kp.M)
+                throw ArgError("the trajectories of a batch share shape, precision and mode count");
+            for (int z = 0; z < y; ++z)
+                if (ctxs[z] == c) throw ArgError("a context appears twice in a batch");
+        }
+        mgk_batch* b = new mgk_batch();
+        try {
+            b->c.assign(ctxs, ctxs + B);
+            b->B = B;
+            b->stage_per = (size_t)a->kp.M * (a->g.C + 1);
+            MGK_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+            b->d_lsp = dalloc<LsParams>(B); b->h_lsp = halloc<LsParams>(B);
+            b->d_kp = dalloc<KmcParams>(B); b->h_kp = halloc<KmcParams>(B);
+            b->d_ea = dalloc<EventArgs>(B); b->h_ea = halloc<EventArgs>(B);
+            b->d_times = dalloc<double>(B); b->h_times = halloc<double>(B);
+            b->d_ctl = dalloc<Ctl>(B); b->h_ctl = halloc<Ctl>(B);
+            b->d_kctl = dalloc<KmcCtl>(B); b->h_kctl = halloc<KmcCtl>(B);
+            b->d_stage = dalloc<double>(b->stage_per * B); b->h_stage = halloc<double>(b->stage_per * B);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+        return MGK_OK;
+    });
+}
+
+int mgk_batch_destroy(mgk_batch* b) {
+    if (!b) return MGK_OK;
+    return guarded([&]() -> int {
+        delete b;
+        return MGK_OK;
+    });
+}
+
+int mgk_batch_size(const mgk_batch* b) { return b ? b->B : 0; }
+
+int mgk_batch_kmc_event(mgk_batch* b, const mgk_event_in* in, const int* active, double* eps_macro, double* sig_macro,
+                        mgk_event_out* out) {
+    return guarded([&]() -> int {
+        if (!b || !in || !eps_macro || !sig_macro || !out) throw ArgError("null argument");
+        batch_event(*b, in, active, eps_macro, sig_macro, out);
         return MGK_OK;
     });
 }

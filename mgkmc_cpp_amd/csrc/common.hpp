@@ -40,7 +40,7 @@ struct Ctl {
     unsigned ticket;     // last-block ticket of the reduction
     int max_iter;
     int keep_shift;      // the warm-start shift is already set: a mean pass leaves it alone
-    int pad1;
+    int skip_final;      // an event attempt without an event: the fields did not change
     double tol;
     double diff;         // last relative increment
     double norm_extra;   // sum over the uniform antisymmetric part of eps (host-known)

@@ -538,6 +538,7 @@ __global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict_
     kc->go = (outcome == 0) ? 1 : 0;
     if (outcome != 0) {
         ctl->done = 1;
+        ctl->skip_final = 1;
         return;
     }
     kc->time_new = ea.time + tw;

@@ -797,6 +797,7 @@ __global__ __launch_bounds__(256) void k_apply_shift(const LsParams p_arg, const
 template <int C, bool BATCH>
 __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, const LsParams* __restrict__ P) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
+    if (p.ctl->skip_final) return;
     constexpr int D = (C == 3) ? 2 : 3;
     __shared__ double red[8 * 12];
     const Geom g = p.g;
@@ -836,6 +837,7 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
 template <int C, bool BATCH>
 __global__ __launch_bounds__(256) void k_final_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
+    if (p.ctl->skip_final) return;
     __shared__ double red[8 * 12];
     double tot[2 * C];
     reduce_partials<2 * C>(p.partials, nblocks, tot, red);

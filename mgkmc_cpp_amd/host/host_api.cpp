@@ -7,6 +7,8 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "../../include/mgkmc_gpu.h"
 #include "../../include/mgkmc_host.h"
@@ -271,6 +273,63 @@ int mgk_sim_get_events(mgk_sim* s, mgk_event_record* out, long long cap) {
 int mgk_sim_device_ctx(mgk_sim* s, mgk_ctx** out) {
     *out = s->with([&](auto& sim) { return sim.ctx; });
     return MGK_OK;
+}
+
+}  // extern "C"
+
+struct mgk_sim_batch {
+    int dims = 2;
+    std::vector<mgk_sim*> sims;
+    mgk_batch* dev = nullptr;
+    ~mgk_sim_batch() {
+        if (dev) mgk_batch_destroy(dev);
+    }
+};
+
+extern "C" {
+
+int mgk_sim_batch_create(mgk_sim* const* sims, int B, mgk_sim_batch** out) {
+    return guard([&]() -> int {
+        if (!sims || !out || B < 1) throw std::runtime_error("bad argument");
+        *out = nullptr;
+        auto b = std::make_unique<mgk_sim_batch>();
+        b->dims = sims[0]->dims;
+        std::vector<mgk_ctx*> ctxs;
+        for (int i = 0; i < B; ++i) {
+            if (!sims[i] || sims[i]->dims != b->dims) throw std::runtime_error("the simulations of a batch share dims");
+            b->sims.push_back(sims[i]);
+            ctxs.push_back(sims[i]->with([&](auto& sim) { return sim.ctx; }));
+        }
+        if (mgk_batch_create(ctxs.data(), B, &b->dev) != MGK_OK) throw std::runtime_error(mgk_last_error());
+        *out = b.release();
+        return MGK_OK;
+    });
+}
+
+int mgk_sim_batch_destroy(mgk_sim_batch* b) {
+    delete b;
+    return MGK_OK;
+}
+
+int mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, mgk_run_stats* stats) {
+    return guard([&]() -> int {
+        if (!b) throw std::runtime_error("null batch");
+        auto run = [&](auto tag) {
+            constexpr int D = decltype(tag)::value;
+            std::vector<mgk_host::KmcSimulationT<D>*> v;
+            for (mgk_sim* s : b->sims) {
+                mgk_host::KmcSimulationT<D>* p;
+                if constexpr (D == 2) p = s->s2.get(); else p = s->s3.get();
+                p->counters = mgk_host::RunCounters{};
+                v.push_back(p);
+            }
+            mgk_host::batch_event_steps<D>(b->dev, v, n, {drv_i, drv_j});
+            for (size_t i = 0; i < v.size(); ++i) fill_stats(v[i]->counters, v[i]->time, stats ? stats + i : nullptr);
+        };
+        if (b->dims == 2) run(std::integral_constant<int, 2>{});
+        else run(std::integral_constant<int, 3>{});
+        return MGK_OK;
+    });
 }
 
 }  // extern "C"

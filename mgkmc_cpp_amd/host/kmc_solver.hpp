@@ -28,6 +28,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <random>
@@ -520,18 +521,23 @@ public:
         }
     }
 
-    // One attempt of run_simulation's inner loop (kmc_solver.hpp:763-952,
-    // 3D :1776-1980) on the device with one round trip (mgk_kmc_event).  u1,
-    // u2 and the redraw values are drawn in the reference's order before the
-    // outcome is known; the generator is rewound to where the reference would
-    // stand when the attempt has no event (no rate: before u1; t_wait too
-    // long: after u1; cascade pending: before u1).  On an event, the
-    // accepted-event body (:811-937) follows: time, host mirrors, records.
-    int try_event(double remaining, std::pair<int, int> component,
-                  std::uniform_real_distribution<double>& uniform_dist, int* is_instab, double* t_wait_out) {
-        const std::mt19937 rng_before = rng;
-        const double r_val = uniform_dist(rng);
-        const std::mt19937 rng_after_u1 = rng;
+    // Random draws of one attempt of run_simulation's inner loop, taken
+    // before its outcome is known, and the generator states to rewind to.
+    struct EventDraw {
+        std::mt19937 rng_before, rng_after_u1;
+        double r_val = 0.0;
+        int first = 0, n = 0;
+        bool has_cat = false, has_q0 = false;
+    };
+
+    // First half of an attempt: u1, u2 and the redraw values in the
+    // reference's order (kmc_solver.hpp:787-864), and the device arguments.
+    // The redraw values stay in this object until complete_event.
+    void prepare_event(double remaining, std::pair<int, int> component,
+                       std::uniform_real_distribution<double>& uniform_dist, EventDraw& d, mgk_event_in& in) {
+        d.rng_before = rng;
+        d.r_val = uniform_dist(rng);
+        d.rng_after_u1 = rng;
         const double u2 = uniform_dist(rng);
         // the redraw values come next in the stream (:839-864) and do not
         // depend on which voxel the device selects
@@ -539,53 +545,62 @@ public:
         std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
         redraw_cat_.resize((size_t)M);
         redraw_q0_.resize((size_t)M);
-        int first = 0, n = 0;
-        bool has_cat = false, has_q0 = false;
         if (redraw_directions || redraw_barriers) {
             for (int mm = 0; mm < M; ++mm) {
                 if (redraw_directions) redraw_cat_[mm] = draw_mode(sn, mm);
                 if (redraw_barriers) redraw_q0_[mm] = barrier_gen.generate(rng);
             }
-            n = M;
-            has_cat = redraw_directions && !catalog_fixed;   // a fixed catalog redraws to itself
-            has_q0 = redraw_barriers;
+            d.first = 0;
+            d.n = M;
+            d.has_cat = redraw_directions && !catalog_fixed;   // a fixed catalog redraws to itself
+            d.has_q0 = redraw_barriers;
         } else {
             redraw_cat_[0] = catalog_fixed ? Mat<D>::Zero() : draw_mode(sn, 0);
             redraw_q0_[0] = barrier_gen.generate(rng);
-            first = -1;   // the selected mode only
-            n = 1;
-            has_cat = !catalog_fixed;
-            has_q0 = true;
+            d.first = -1;   // the selected mode only
+            d.n = 1;
+            d.has_cat = !catalog_fixed;
+            d.has_q0 = true;
         }
-        mgk_event_in in{};
+        std::memset(&in, 0, sizeof(in));
         in.time = time;
-        in.neg_log_u1 = -std::log(r_val);
+        in.neg_log_u1 = -std::log(d.r_val);
         in.remaining = remaining;
         in.strain_rate = strain_rate;
         in.drv_i = component.first;
         in.drv_j = component.second;
         in.u2 = u2;
-        in.new_catalog = has_cat ? redraw_cat_[0].data() : nullptr;
-        in.new_Q0 = has_q0 ? redraw_q0_.data() : nullptr;
-        in.first_mode = first;
-        in.n_modes = n;
+        in.new_catalog = d.has_cat ? redraw_cat_[0].data() : nullptr;
+        in.new_Q0 = d.has_q0 ? redraw_q0_.data() : nullptr;
+        in.first_mode = d.first;
+        in.n_modes = d.n;
         in.max_iter = ls_max_iter;
         in.tol = ls_tol;
         in.cascade_check = (instability_mode == "cascade") ? 1 : 0;
-        mgk_event_out out{};
-        check(mgk_kmc_event(ctx, &in, eps_macro.data(), sig_macro.data(), &out), "mgk_kmc_event");
+    }
+
+    // Second half: the generator is rewound to where the reference would
+    // stand when the attempt has no event (no rate or cascade pending: before
+    // u1; t_wait too long: after u1); on an event the accepted-event body
+    // (:811-937) follows: time, statistics, host mirrors, records.  eps_macro
+    // and sig_macro already hold what the device call returned.
+    int complete_event(const EventDraw& d, const mgk_event_out& out, int* is_instab, double* t_wait_out) {
         total_rate_ = out.total_rate;
         n_unstable_ = out.n_unstable;
         n_included_ = out.n_included;
         if (out.outcome == MGK_EVENT_UNSTABLE || out.outcome == MGK_EVENT_NO_RATE) {
-            rng = rng_before;
+            rng = d.rng_before;
             return out.outcome;
         }
         if (out.outcome == MGK_EVENT_REJECTED) {
-            rng = rng_after_u1;
+            rng = d.rng_after_u1;
             return out.outcome;
         }
-        const double t_wait = -std::log(r_val) / total_rate_;   // the reference's expression (= out.t_wait)
+        if (out.outcome != MGK_EVENT_DONE) {
+            rng = d.rng_before;
+            return out.outcome;
+        }
+        const double t_wait = -std::log(d.r_val) / total_rate_;   // the reference's expression (= out.t_wait)
         time += t_wait;
         int inner = 0;
         mgk_last_solve_stats(ctx, nullptr, &inner);
@@ -597,10 +612,10 @@ public:
         const long v = flat / M;
         const int inst = (out.q_sel <= stability_threshold) ? 1 : 0;
         // host copies of the redrawn entries
-        for (int j = 0; j < n; ++j) {
-            const long mi = (first < 0) ? flat : v * M + first + j;
-            if (has_cat) catalog[mi] = redraw_cat_[j];
-            if (has_q0) Q0[mi] = redraw_q0_[j];
+        for (int j = 0; j < d.n; ++j) {
+            const long mi = (d.first < 0) ? flat : v * M + d.first + j;
+            if (d.has_cat) catalog[mi] = redraw_cat_[j];
+            if (d.has_q0) Q0[mi] = redraw_q0_[j];
         }
         if (record_events) {
             EventRecord e{};
@@ -615,6 +630,18 @@ public:
         if (is_instab) *is_instab = inst;
         if (t_wait_out) *t_wait_out = t_wait;
         return MGK_EVENT_DONE;
+    }
+
+    // One attempt of run_simulation's inner loop (kmc_solver.hpp:763-952,
+    // 3D :1776-1980) on the device with one round trip (mgk_kmc_event).
+    int try_event(double remaining, std::pair<int, int> component,
+                  std::uniform_real_distribution<double>& uniform_dist, int* is_instab, double* t_wait_out) {
+        EventDraw d;
+        mgk_event_in in;
+        prepare_event(remaining, component, uniform_dist, d, in);
+        mgk_event_out out{};
+        check(mgk_kmc_event(ctx, &in, eps_macro.data(), sig_macro.data(), &out), "mgk_kmc_event");
+        return complete_event(d, out, is_instab, t_wait_out);
     }
 
     // run_simulation (kmc_solver.hpp:721-1037, :1735-2062), fast patching off.
@@ -742,6 +769,55 @@ protected:
     std::vector<Mat<D>> redraw_cat_;
     std::vector<double> redraw_q0_;
 };
+
+// Forced event steps (as KmcSimulationT::event_steps) of several
+// trajectories advanced together through a device batch built from their
+// contexts (mgk_batch_kmc_event): one launch sequence per step covers every
+// trajectory still running.  A trajectory whose attempt finds unstable modes
+// runs its cascade alone and goes on; one without rate stops.
+template <int D>
+void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& sims, int count,
+                       std::pair<int, int> component) {
+    const int B = (int)sims.size();
+    if (B != mgk_batch_size(batch)) throw std::runtime_error("batch size differs from the trajectory count");
+    std::vector<typename KmcSimulationT<D>::EventDraw> draws(B);
+    std::vector<mgk_event_in> in(B);
+    std::vector<mgk_event_out> out(B);
+    std::vector<int> active(B, 1);
+    std::vector<double> em((size_t)B * D * D), sm((size_t)B * D * D);
+    std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
+    const double forever = std::numeric_limits<double>::infinity();
+    for (int step = 0; step < count; ++step) {
+        bool any = false;
+        for (int b = 0; b < B; ++b) {
+            if (!active[b]) continue;
+            sims[b]->prepare_event(forever, component, uniform_dist, draws[b], in[b]);
+            for (int k = 0; k < D * D; ++k) em[(size_t)b * D * D + k] = sims[b]->eps_macro.a[k];
+            any = true;
+        }
+        if (!any) break;
+        check(mgk_batch_kmc_event(batch, in.data(), active.data(), em.data(), sm.data(), out.data()),
+              "mgk_batch_kmc_event");
+        for (int b = 0; b < B; ++b) {
+            if (!active[b]) continue;
+            KmcSimulationT<D>& sim = *sims[b];
+            if (out[b].outcome == MGK_EVENT_DONE)
+                for (int k = 0; k < D * D; ++k) {
+                    sim.eps_macro.a[k] = em[(size_t)b * D * D + k];
+                    sim.sig_macro.a[k] = sm[(size_t)b * D * D + k];
+                }
+            const int outcome = sim.complete_event(draws[b], out[b], nullptr, nullptr);
+            if (outcome == MGK_EVENT_UNSTABLE) {
+                auto cs = sim.run_cascade(0, component);
+                if (cs.second > 0) { sim.counters.cascades++; sim.counters.cascade_flips += cs.second; }
+            } else if (outcome == MGK_EVENT_DONE) {
+                sim.counters.kmc_events++;
+            } else {
+                active[b] = 0;
+            }
+        }
+    }
+}
 
 }  // namespace mgk_host
 

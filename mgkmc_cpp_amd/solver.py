@@ -357,3 +357,35 @@ class Simulation:
         check(gpu().mgk_time_ls_pass(self.device_context_handle(), int(pass_id), 1 if fp32 else 0, int(reps),
                                      C.byref(ms)))
         return ms.value
+
+
+class SimulationBatch:
+    """Simulations of one shape, precision and mode count whose forced event
+    steps run together on the device (mgk_sim_batch_event_steps): one launch
+    sequence per step for all of them.  The simulations stay usable on their
+    own and must outlive the batch."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (P * len(self.sims))(*[s._h for s in self.sims])
+        h = P()
+        check_host(host().mgk_sim_batch_create(arr, len(self.sims), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            host().mgk_sim_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def event_steps(self, n, drv=(0, 0)):
+        """n attempts for every simulation; returns one RunStats per simulation."""
+        st = (abi.RunStats * len(self.sims))()
+        check_host(host().mgk_sim_batch_event_steps(self._h, int(n), drv[0], drv[1], st))
+        return list(st)
+

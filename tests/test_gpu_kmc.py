@@ -285,3 +285,41 @@ def test_device_event_without_rate_or_with_unstable_modes():
     assert out.outcome == abi.EVENT_DONE
     for f in _STATE_FIELDS:
         assert np.array_equal(a.get_field(f), before[f]), f
+
+
+@pytest.mark.parametrize("kw,events", [
+    (dict(), 25),                                                          # 2D fixed catalog, cascade mode
+    (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM, neighbor_softening_fraction=0.2), 25),
+    (dict(dims=3, nx=8, ny=8, nz=8, M=12), 10),
+])
+def test_batch_equals_separate_trajectories(kw, events):
+    # trajectories advanced together (one launch sequence per step) equal the
+    # same trajectories run one by one, bit for bit in FP64, and the oracle
+    from mgkmc_cpp_amd import SimulationBatch
+    seeds = (42, 7, 1234)
+    cfgs = [_cfg(seed=sd, **kw) for sd in seeds]
+    N = cfgs[0].nx * cfgs[0].ny * (cfgs[0].nz if cfgs[0].dims == 3 else 1)
+    mats = [_field(N, seed=17 + i) for i in range(len(seeds))]
+    single = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
+    batched = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
+    for s in single + batched:
+        s.elastic_run()
+    st_single = [s.event_steps(events) for s in single]
+    batch = SimulationBatch(batched)
+    st_batch = batch.event_steps(events)
+    for a, b, sa, sb in zip(single, batched, st_single, st_batch):
+        assert (sa.kmc_events, sa.cascades, sa.cascade_flips) == (sb.kmc_events, sb.cascades, sb.cascade_flips)
+        ea, eb = a.events(), b.events()
+        assert [e.flat_idx for e in ea] == [e.flat_idx for e in eb]
+        assert [e.time for e in ea] == [e.time for e in eb]
+        assert [list(e.sig_macro) for e in ea] == [list(e.sig_macro) for e in eb]
+        for f in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_Q0):
+            assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    # and the first trajectory against the oracle
+    osim = po.OracleSim(cfgs[0], *mats[0])
+    osim.elastic_run()
+    so = osim.event_steps(events)
+    assert so.kmc_events == st_batch[0].kmc_events
+    assert [e.flat_idx for e in osim.events()] == [e.flat_idx for e in batched[0].events()]
+    assert _rel(batched[0].get_field(abi.FIELD_SIG), osim.get_field(abi.FIELD_SIG)) < 1e-10
+    batch.close()
