@@ -124,6 +124,48 @@ def _measured_traffic(kernel):
         return None
 
 
+def fp64_rates(shapes, events=8):
+    """Plain FP64 single-trajectory events/s per grid shape (the sequential
+    baseline of the serving and batch workloads), from a short run of each."""
+    from mgkmc_cpp_amd import abi
+    from mgkmc_cpp_amd.serving import Request, request_simulation
+    rates = {}
+    for shape in shapes:
+        sim = request_simulation(Request(rid=-1, shape=shape, events=events, seed=99, precision=abi.MGK_FP64))
+        sim.elastic_run()
+        sim.event_steps(2)
+        t0 = time.perf_counter()
+        st = sim.event_steps(events)
+        dt = time.perf_counter() - t0
+        rates[shape] = st.kmc_events / dt if dt > 0 else 0.0
+        sim.close()
+    return rates
+
+
+def serve(requests, budget_gb, label):
+    """Runs `requests` through the serving loop; latency summary plus the
+    sequential plain FP64 time of the same events."""
+    from mgkmc_cpp_amd import serving
+    print(f"[bench] {label}: {len(requests)} requests, {sum(r.events for r in requests)} events", file=sys.stderr,
+          flush=True)
+    srv = serving.Server(int(budget_gb * 2 ** 30),
+                         progress=lambda t: print(f"[bench] {label}: {t}", file=sys.stderr, flush=True))
+    t0 = time.perf_counter()
+    srv.run(requests)
+    wall = time.perf_counter() - t0
+    summ = serving.summary(requests)
+    rates = fp64_rates(sorted({r.shape for r in requests}, key=lambda x: (len(x), x)))
+    seq = sum(r.events_done / rates[r.shape] for r in requests if rates[r.shape] > 0)
+    summ.update(wall_s=round(wall, 3), sequential_fp64_s=round(seq, 3),
+                speedup_vs_sequential_fp64=round(seq / summ["span_s"], 2) if summ["span_s"] else None,
+                fp64_single_events_per_s={"x".join(map(str, k)): round(v, 1) for k, v in rates.items()})
+    for k in ("p50_latency_s", "p99_latency_s", "events_per_s", "span_s"):
+        if summ.get(k) is not None:
+            summ[k] = round(summ[k], 4)
+    print(f"[bench] {label}: {summ}", file=sys.stderr, flush=True)
+    return summ
+
+
 def probe():
     """Launches for the counter runs: one 256 MiB device copy (calibration)
     then 5 launches of every pass at 64^3 in both precisions."""
@@ -186,6 +228,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--probe", action="store_true", help="only the launches of the counter runs")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
+                    help="c2: the headline line (with a bounded c4 sample); c3 / c4: the full batch / serving runs")
+    ap.add_argument("--serving-requests", type=int, default=48, help="requests of the bounded c4 sample")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="HBM admission budget of the serving loop")
     a = ap.parse_args()
     if a.probe:
         probe()
@@ -196,6 +243,8 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     from mgkmc_cpp_amd import abi, configs
+    if a.workload in ("c3", "c4"):
+        return main_many(a)
     E, nu = configs.material(SHAPE, 4.0, 42)
     print(f"[bench] FP64 run: {a.steps} events after {a.warmup} warm-up", file=sys.stderr, flush=True)
     sim64, i64 = run_gpu(abi.MGK_FP64, a.steps, a.warmup, E, nu)
@@ -214,6 +263,15 @@ def main():
     if rank == 0 and not a.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(E, nu, a.cpu_budget)
+    serving_sample = None
+    if not a.no_serving:
+        # bounded c4 sample: Poisson arrivals at 50/s, grids up to 64^3, events
+        # per request one tenth of c4's 500-5000
+        reqs = configs.c4_requests(a.serving_requests, rate=50.0, seed=11, events=(50, 500),
+                                   grids=configs.C4_GRIDS[:-1])
+        serving_sample = serve(reqs, a.budget_gb, "c4 sample")
+        serving_sample["workload"] = (f"c4 sample: {a.serving_requests} requests, Poisson arrivals 50/s, grids "
+                                      "32^2..64^3, 50-500 events each, FP32+refine")
     value = i32["events_per_s"]
     out = {
         "metric": METRIC,
@@ -238,9 +296,32 @@ def main():
                                "fp32_refine_fp32": round(i32["fp32_iters_per_event"], 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving_sample,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)
+
+
+def main_many(a):
+    """The batch (c3) and serving (c4) workloads in full: one JSON line."""
+    from mgkmc_cpp_amd import configs
+    if a.workload == "c3":
+        reqs = configs.c3_requests(256, events=2000)
+        summ = serve(reqs, a.budget_gb, "c3")
+        metric, workload = ("aggregate KMC events/s, 256 independent trajectories",
+                            "c3: 256 trajectories on grids drawn from 32^2, 64^2, 128^2, 32^3, 48^3 (seed 7), "
+                            "2000 events each, FP32+refine, batched by grid")
+    else:
+        reqs = configs.c4_requests(256, rate=50.0, seed=11)
+        summ = serve(reqs, a.budget_gb, "c4")
+        metric, workload = ("p50/p99 latency of mixed requests and aggregate KMC events/s",
+                            "c4: 256 requests, Poisson arrivals 50/s, grids 32^2..128^3, 500-5000 events each, "
+                            "FP32+refine, admitted by HBM footprint")
+    out = {"metric": metric, "value": summ["events_per_s"], "unit": "events/s", "n_gpus": 1, "steps": summ["events"],
+           "warmup": 0, "ms_per_step": round(1e3 * summ["span_s"] / max(summ["events"], 1), 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64-refine",
+           "data": "synthetic", "config": {"workload": workload}, "serving": summ}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

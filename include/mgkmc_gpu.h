@@ -258,6 +258,14 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
 /* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
 
+/* Device memory (no counterpart in the reference): the bytes a context
+ * holds, and the bytes a context of this shape and mode count will hold
+ * after mgk_create + mgk_kmc_init (M = 0: no KMC state) and a catalog upload
+ * (per_voxel_catalog = 0: every voxel holds the same M strains).  Used to
+ * admit work by HBM footprint. */
+long long mgk_device_bytes(const mgk_ctx* ctx);
+long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_voxel_catalog);
+
 #ifdef __cplusplus
 }
 #endif

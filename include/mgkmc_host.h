@@ -130,13 +130,15 @@ int  mgk_sim_device_ctx(mgk_sim* s, struct mgk_ctx** out);
  * event steps (as mgk_sim_event_steps) of B simulations of one shape,
  * precision and mode count, one device launch sequence per step for all of
  * them (mgk_batch in mgkmc_gpu.h).  A simulation whose attempt finds
- * unstable modes runs its cascade alone; one without rate stops.  Each
- * trajectory equals its own mgk_sim_event_steps run (FP64 bit for bit).
- * The simulations must outlive the batch; stats holds B entries. */
+ * unstable modes runs its cascade alone; one without rate stops, and so
+ * does one that has made caps[b] events in the call (caps may be NULL).
+ * Each trajectory equals its own mgk_sim_event_steps run (FP64 bit for
+ * bit).  The simulations must outlive the batch; stats holds B entries. */
 typedef struct mgk_sim_batch mgk_sim_batch;
 int  mgk_sim_batch_create(mgk_sim* const* sims, int B, mgk_sim_batch** out);
 int  mgk_sim_batch_destroy(mgk_sim_batch* b);
-int  mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, mgk_run_stats* stats);
+int  mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, const long long* caps,
+                               mgk_run_stats* stats);
 
 #ifdef __cplusplus
 }

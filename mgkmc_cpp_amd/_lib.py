@@ -67,6 +67,8 @@ def gpu():
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])
         _sig(lib, "mgk_last_solve_stats", C.c_int, [P, IP, IP])
+        _sig(lib, "mgk_device_bytes", C.c_longlong, [P])
+        _sig(lib, "mgk_estimate_bytes", C.c_longlong, [C.c_int] * 6)
         _sig(lib, "mgk_time_ls_pass", C.c_int, [P, C.c_int, C.c_int, C.c_int, DP])
         _gpu = lib
     return _gpu
@@ -97,7 +99,8 @@ def host():
         _sig(lib, "mgk_sim_device_ctx", C.c_int, [P, C.POINTER(P)])
         _sig(lib, "mgk_sim_batch_create", C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)])
         _sig(lib, "mgk_sim_batch_destroy", C.c_int, [P])
-        _sig(lib, "mgk_sim_batch_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
+        _sig(lib, "mgk_sim_batch_event_steps", C.c_int,
+             [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(abi.RunStats)])
         _host = lib
     return _host
 

@@ -65,11 +65,36 @@ def c2(seed=42, precision=abi.MGK_FP32_REFINE):
 
 
 C3_GRIDS = [(32, 32), (64, 64), (128, 128), (32, 32, 32), (48, 48, 48)]
+C4_GRIDS = [(32, 32), (64, 64), (128, 128), (32, 32, 32), (48, 48, 48), (64, 64, 64), (128, 128, 128)]
 
 
 def c3_grids(n=256, seed=7):
     rng = np.random.default_rng(seed)
     return [C3_GRIDS[k] for k in rng.integers(0, len(C3_GRIDS), n)]
+
+
+def c3_requests(n=256, events=2000, seed=7, base_seed=1000, precision=abi.MGK_FP32_REFINE):
+    """c3: n trajectories present from the start, grid k drawn with `seed`,
+    trajectory i seeded base_seed + i.  The batch and serving workloads run
+    instability_mode "kmc" (unstable modes are rated events), so the work of
+    a request is its event count; "cascade" mode can flip for up to N rounds
+    between two events."""
+    from .serving import Request
+    return [Request(rid=i, shape=g, events=events, seed=base_seed + i, arrival=0.0, precision=precision,
+                    instability_kmc=1) for i, g in enumerate(c3_grids(n, seed))]
+
+
+def c4_requests(n, rate=50.0, seed=11, events=(500, 5000), grids=C4_GRIDS, precision=abi.MGK_FP32_REFINE):
+    """c4: n requests with Poisson arrivals at `rate` per second, a grid drawn
+    from `grids` and an event count uniform in [events[0], events[1]], all
+    fixed by `seed`; request i is seeded 100000 + seed * 1000 + i."""
+    from .serving import Request
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    g = rng.integers(0, len(grids), n)
+    ev = rng.integers(events[0], events[1] + 1, n)
+    return [Request(rid=i, shape=tuple(grids[g[i]]), events=int(ev[i]), seed=100000 + seed * 1000 + i,
+                    arrival=float(t[i]), precision=precision, instability_kmc=1) for i in range(n)]
 
 
 def uniaxial_run(events, step_size=1e-4):

@@ -140,6 +140,31 @@ struct mgk_ctx {
     EventArgs* d_ea = nullptr;
     EventArgs* h_ea = nullptr;
 
+    // device bytes this context holds (every allocation goes through dal)
+    size_t dev_bytes = 0;
+    size_t kmc_bytes = 0;   // the part held by the KMC state
+    size_t cat_cap = 0;     // doubles of catalog storage
+    template <typename T> T* dal(size_t n, bool kmc_state = false) {
+        T* p = dalloc<T>(n);
+        const size_t b = std::max<size_t>(n, 1) * sizeof(T);
+        dev_bytes += b;
+        if (kmc_state) kmc_bytes += b;
+        return p;
+    }
+    // catalog storage of at least n doubles: M*C for a catalog every voxel
+    // shares, C*N*M for a per-voxel one (allocated when first needed)
+    void ensure_cat(size_t n) {
+        if (n <= cat_cap) return;
+        double* p = dal<double>(n, true);
+        if (K.cat) {
+            MGK_HIP_CHECK(hipFree(K.cat));
+            dev_bytes -= std::max<size_t>(cat_cap, 1) * sizeof(double);
+            kmc_bytes -= std::max<size_t>(cat_cap, 1) * sizeof(double);
+        }
+        K.cat = p;
+        cat_cap = n;
+    }
+
     ~mgk_ctx() { release(); }
 
     void release() {
@@ -167,8 +192,11 @@ struct mgk_ctx {
 
     double* stage_dev(size_t n) {
         if (n > dstage_n) {
-            if (dstage) MGK_HIP_CHECK(hipFree(dstage));
-            dstage = dalloc<double>(n);
+            if (dstage) {
+                MGK_HIP_CHECK(hipFree(dstage));
+                dev_bytes -= std::max<size_t>(dstage_n, 1) * sizeof(double);
+            }
+            dstage = dal<double>(n);
             dstage_n = n;
         }
         return dstage;
@@ -222,7 +250,7 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(d_lsp + slot, h_lsp + slot, sizeof(LsParams), hipMemcpyHostToDevice, stream));
             lsp_up[slot] = true;
         }
-        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot]};
+        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false};
     }
     KmcLaunch kml() {
         if (!kp_up || std::memcmp(h_kp, &K, sizeof(KmcParams)) != 0) {
@@ -231,7 +259,7 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(d_kp, h_kp, sizeof(KmcParams), hipMemcpyHostToDevice, stream));
             kp_up = true;
         }
-        return KmcLaunch{g, kp.M, d_kp, 1, K};
+        return KmcLaunch{g, kp.M, d_kp, 1, K, false};
     }
 
     // Symmetric / antisymmetric split of a column-major D x D matrix.
@@ -482,12 +510,12 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             }
             c->mean_mu = msum / N;
             c->h_mu = hm;
-            c->E = dalloc<double>(N);
-            c->nu = dalloc<double>(N);
-            c->lam = dalloc<double>(N);
-            c->mu = dalloc<double>(N);
-            c->dlam32 = dalloc<float>(N);
-            c->dmu32 = dalloc<float>(N);
+            c->E = c->dal<double>(N);
+            c->nu = c->dal<double>(N);
+            c->lam = c->dal<double>(N);
+            c->mu = c->dal<double>(N);
+            c->dlam32 = c->dal<float>(N);
+            c->dmu32 = c->dal<float>(N);
             MGK_HIP_CHECK(hipMemcpy(c->E, E, N * sizeof(double), hipMemcpyHostToDevice));
             MGK_HIP_CHECK(hipMemcpy(c->nu, nu, N * sizeof(double), hipMemcpyHostToDevice));
             MGK_HIP_CHECK(hipMemcpy(c->lam, hl.data(), N * sizeof(double), hipMemcpyHostToDevice));
@@ -495,31 +523,31 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             MGK_HIP_CHECK(hipMemcpy(c->dlam32, hdl.data(), N * sizeof(float), hipMemcpyHostToDevice));
             MGK_HIP_CHECK(hipMemcpy(c->dmu32, hdm.data(), N * sizeof(float), hipMemcpyHostToDevice));
             const int C = g.C;
-            c->eps = dalloc<double>((size_t)C * N);
-            c->sig = dalloc<double>((size_t)C * N);
-            c->epsp = dalloc<double>((size_t)C * N);
+            c->eps = c->dal<double>((size_t)C * N);
+            c->sig = c->dal<double>((size_t)C * N);
+            c->epsp = c->dal<double>((size_t)C * N);
             MGK_HIP_CHECK(hipMemset(c->eps, 0, sizeof(double) * C * N));
             MGK_HIP_CHECK(hipMemset(c->sig, 0, sizeof(double) * C * N));
             MGK_HIP_CHECK(hipMemset(c->epsp, 0, sizeof(double) * C * N));
-            c->spec = dalloc<double>((size_t)2 * C * g.NS);
-            c->cg = dalloc<float>((size_t)4 * C * N);
-            c->corr = dalloc<float>((size_t)C * N);
-            c->partials = dalloc<double>((size_t)ls_partial_slots(g) + 64);
-            c->ctl = dalloc<Ctl>(1);
+            c->spec = c->dal<double>((size_t)2 * C * g.NS);
+            c->cg = c->dal<float>((size_t)4 * C * N);
+            c->corr = c->dal<float>((size_t)C * N);
+            c->partials = c->dal<double>((size_t)ls_partial_slots(g) + 64);
+            c->ctl = c->dal<Ctl>(1);
             c->hctl = halloc<Ctl>(1);
-            c->d_lsp = dalloc<LsParams>(2);
+            c->d_lsp = c->dal<LsParams>(2);
             c->h_lsp = halloc<LsParams>(2);
-            c->d_kp = dalloc<KmcParams>(1);
+            c->d_kp = c->dal<KmcParams>(1);
             c->h_kp = halloc<KmcParams>(1);
-            c->d_ea = dalloc<EventArgs>(1);
+            c->d_ea = c->dal<EventArgs>(1);
             c->h_ea = halloc<EventArgs>(1);
             MGK_HIP_CHECK(hipMemset(c->ctl, 0, sizeof(Ctl)));
             const int lens[3] = {g.nf, g.nm, g.nl};
             for (int a = 0; a < 3; ++a) {
                 auto t64 = twiddle_table<double>(lens[a]);
                 auto t32 = twiddle_table<float>(lens[a]);
-                c->tw64[a] = dalloc<double>(t64.size());
-                c->tw32[a] = dalloc<float>(t32.size());
+                c->tw64[a] = c->dal<double>(t64.size());
+                c->tw32[a] = c->dal<float>(t32.size());
                 MGK_HIP_CHECK(hipMemcpy(c->tw64[a], t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
                 MGK_HIP_CHECK(hipMemcpy(c->tw32[a], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
             }
@@ -675,6 +703,7 @@ int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
                     c.split(host + i * F, sym, skw);
                     for (int k = 0; k < C; ++k) h[(size_t)k * n + i] = sym[k];
                 }
+                c.ensure_cat(h.size());
                 MGK_HIP_CHECK(hipMemcpy(c.K.cat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
                 c.K.cat_shared = shared ? 1 : 0;
                 if (shared) c.shared_cat = h;
@@ -770,6 +799,9 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
             f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kctl); f(c.kstage); f(c.unst_idx); f(c.unst_q);
             if (c.hkctl) (void)hipHostFree(c.hkctl);
             if (c.hkstage) (void)hipHostFree(c.hkstage);
+            c.dev_bytes -= c.kmc_bytes;
+            c.kmc_bytes = 0;
+            c.cat_cap = 0;
         }
         c.kp = *kp;
         const int C = c.g.C;
@@ -794,31 +826,34 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         ph.thr = kp->stability_threshold;
         K.sig = c.sig;
         K.eps_p = c.epsp;
-        K.Q = dalloc<double>(nm);
-        K.Q0 = dalloc<double>(nm);
-        K.cat = dalloc<double>((size_t)C * nm);
-        K.soft = dalloc<double>(2 * N);
-        K.last_t = dalloc<double>(N);
-        K.prev_dir = dalloc<double>((size_t)C * N);
+        K.Q = c.dal<double>(nm, true);
+        K.Q0 = c.dal<double>(nm, true);
+        K.soft = c.dal<double>(2 * N, true);
+        K.last_t = c.dal<double>(N, true);
+        K.prev_dir = c.dal<double>((size_t)C * N, true);
         const int nb = kmc_nblocks(nm);
-        K.bsum = dalloc<double>(nb);
-        K.bincl = dalloc<double>(nb);
-        K.bunst = dalloc<int>(nb);
-        K.bunst_off = dalloc<int>(nb);
-        K.binc = dalloc<int>(nb);
-        c.kctl = dalloc<KmcCtl>(1);
+        K.bsum = c.dal<double>(nb, true);
+        K.bincl = c.dal<double>(nb, true);
+        K.bunst = c.dal<int>(nb, true);
+        K.bunst_off = c.dal<int>(nb, true);
+        K.binc = c.dal<int>(nb, true);
+        c.kctl = c.dal<KmcCtl>(1, true);
         c.hkctl = halloc<KmcCtl>(1);
         K.kctl = c.kctl;
-        c.kstage = dalloc<double>((size_t)kp->M * (C + 1));
+        c.kstage = c.dal<double>((size_t)kp->M * (C + 1), true);
         c.hkstage = halloc<double>((size_t)kp->M * (C + 1));
         K.staging = c.kstage;
-        c.unst_idx = dalloc<int>(nm);
-        c.unst_q = dalloc<double>(nm);
+        c.unst_idx = c.dal<int>(nm, true);
+        c.unst_q = c.dal<double>(nm, true);
         K.unst_idx = c.unst_idx;
         K.unst_q = c.unst_q;
         MGK_HIP_CHECK(hipMemset(K.Q, 0, nm * sizeof(double)));
         MGK_HIP_CHECK(hipMemset(K.Q0, 0, nm * sizeof(double)));
-        MGK_HIP_CHECK(hipMemset(K.cat, 0, (size_t)C * nm * sizeof(double)));
+        // the zero catalog every voxel starts with is one shared copy
+        c.ensure_cat((size_t)C * kp->M);
+        MGK_HIP_CHECK(hipMemset(K.cat, 0, (size_t)C * kp->M * sizeof(double)));
+        K.cat_shared = 1;
+        c.shared_cat.assign((size_t)C * kp->M, 0.0);
         MGK_HIP_CHECK(hipMemset(K.soft, 0, 2 * N * sizeof(double)));
         MGK_HIP_CHECK(hipMemset(K.prev_dir, 0, (size_t)C * N * sizeof(double)));
         MGK_HIP_CHECK(hipMemset(c.kctl, 0, sizeof(KmcCtl)));
@@ -918,6 +953,8 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
                 std::vector<double> full((size_t)C * nm);
                 for (int k = 0; k < C; ++k)
                     for (long i = 0; i < nm; ++i) full[(size_t)k * nm + i] = shared[(size_t)k * M + i % M];
+                MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+                c.ensure_cat(full.size());
                 MGK_HIP_CHECK(hipMemcpyAsync(c.K.cat, full.data(), full.size() * sizeof(double),
                                              hipMemcpyHostToDevice, c.stream));
                 MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -1139,6 +1176,42 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
     });
 }
 
+long long mgk_device_bytes(const mgk_ctx* ctx) { return ctx ? (long long)ctx->dev_bytes : -1; }
+
+long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_voxel_catalog) {
+    if ((dims != 2 && dims != 3) || nx < 1 || ny < 1 || (dims == 3 && nz < 1) || M < 0) return -1;
+    Geom g{};
+    g.dims = dims;
+    g.nf = nx;
+    g.nm = (dims == 2) ? 1 : ny;
+    g.nl = (dims == 2) ? ny : nz;
+    g.nlh = g.nl / 2 + 1;
+    g.N = (long)g.nf * g.nm * g.nl;
+    g.NS = (long)g.nf * g.nm * g.nlh;
+    g.C = (dims == 2) ? 3 : 6;
+    const long long N = g.N, C = g.C, NS = g.NS;
+    auto r = [](long long n, long long sz) { return std::max<long long>(n, 1) * sz; };
+    long long b = 0;
+    b += 4 * r(N, 8) + 2 * r(N, 4);                      // E, nu, lam, mu; dlam, dmu
+    b += 3 * r(C * N, 8);                                 // eps, sig, eps_p
+    b += r(2 * C * NS, 8) + r(4 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
+    b += r((long long)ls_partial_slots(g) + 64, 8);
+    b += (long long)sizeof(Ctl) + 2 * (long long)sizeof(LsParams) + (long long)sizeof(KmcParams) +
+         (long long)sizeof(EventArgs);
+    const int lens[3] = {g.nf, g.nm, g.nl};
+    for (int a = 0; a < 3; ++a) b += r(2LL * lens[a], 8) + r(2LL * lens[a], 4);   // twiddles
+    if (M > 0) {
+        const long long nm = N * M, nb = kmc_nblocks(nm);
+        b += 2 * r(nm, 8);                                // Q, Q0
+        b += per_voxel_catalog ? r(C * nm, 8) : r(C * M, 8);
+        b += r(2 * N, 8) + r(N, 8) + r(C * N, 8);         // softening, last event time, previous direction
+        b += 2 * r(nb, 8) + 3 * r(nb, 4);                 // block sums and counts
+        b += (long long)sizeof(KmcCtl) + r((long long)M * (C + 1), 8);
+        b += r(nm, 4) + r(nm, 8);                         // unstable list
+    }
+    return b;
+}
+
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters) {
     return guarded([&]() -> int {
         if (!ctx) throw ArgError("null context");
@@ -1264,8 +1337,8 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     MGK_HIP_CHECK(hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(Ctl) * B, hipMemcpyHostToDevice, s));
     if (n_modes > 0)
         MGK_HIP_CHECK(hipMemcpyAsync(b.d_stage, b.h_stage, sizeof(double) * b.stage_per * B, hipMemcpyHostToDevice, s));
-    const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0]};
-    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0]};
+    const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0], true};
+    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true};
     kmc_enqueue_rates(KL, 0.0, b.d_times, s);
     kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, s);
     kmc_enqueue_select(KL, 0.0, b.d_ea, s);

@@ -587,7 +587,7 @@ template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double tim
 }
 
 void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
-    if (l.B > 1) {
+    if (l.batch) {
         if (!times) throw ArgError("a batch needs per-trajectory times");
         enqueue_rates_t<true>(l, time, times, s);
     } else {
@@ -597,7 +597,7 @@ void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hip
 }
 
 void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipStream_t s) {
-    if (l.B > 1) {
+    if (l.batch) {
         if (!ea) throw ArgError("a batch needs per-trajectory event arguments");
         hipLaunchKernelGGL(k_select<true>, dim3(1, l.B), dim3(256), 0, s, l.h, l.P, u2, ea);
     } else {
@@ -607,7 +607,7 @@ void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipS
 }
 
 void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s) {
-    if (l.B > 1) hipLaunchKernelGGL(k_select_seq<true>, dim3(1, l.B), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
+    if (l.batch) hipLaunchKernelGGL(k_select_seq<true>, dim3(1, l.B), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
     else hipLaunchKernelGGL(k_select_seq<false>, dim3(1, 1), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
     MGK_HIP_CHECK(hipGetLastError());
 }
@@ -621,12 +621,12 @@ static void enqueue_flip_t(const KmcLaunch& l, int flat, double time, int first_
 
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s, int time_from_ctl) {
-    if (l.B > 1 && (flat >= 0 || !time_from_ctl)) throw ArgError("a batch flips its device-selected modes");
+    if (l.batch && (flat >= 0 || !time_from_ctl)) throw ArgError("a batch flips its device-selected modes");
     if (l.g.dims == 2) {
-        if (l.B > 1) enqueue_flip_t<2, 3, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        if (l.batch) enqueue_flip_t<2, 3, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
         else enqueue_flip_t<2, 3, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
     } else {
-        if (l.B > 1) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        if (l.batch) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
         else enqueue_flip_t<3, 6, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
     }
     MGK_HIP_CHECK(hipGetLastError());
@@ -634,7 +634,8 @@ void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode,
 
 void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
                             const EventArgs* ea, hipStream_t s) {
-    if (l.B > 1) {
+    if (l.batch ? (!P || !ea) : (!ctl || l.B != 1)) throw ArgError("event decision: missing control blocks");
+    if (l.batch) {
         if (l.g.dims == 2)
             hipLaunchKernelGGL((k_event_prep<2, 3, true>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
         else
@@ -650,7 +651,7 @@ void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, con
 
 void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s) {
     const int nb = kmc_nblocks(l.g.N * l.M);
-    if (l.B > 1) hipLaunchKernelGGL(k_unstable<true>, dim3(nb, l.B), dim3(256), 0, s, l.h, l.P);
+    if (l.batch) hipLaunchKernelGGL(k_unstable<true>, dim3(nb, l.B), dim3(256), 0, s, l.h, l.P);
     else hipLaunchKernelGGL(k_unstable<false>, dim3(nb, 1), dim3(256), 0, s, l.h, l.P);
     MGK_HIP_CHECK(hipGetLastError());
 }

@@ -78,13 +78,15 @@ constexpr int kRateThreads = 512;  // lanes per rate block (2 modes each)
 
 // One launch over B trajectories of one shape and mode count: row
 // blockIdx.y of every grid works on the trajectory whose parameter block is
-// P[blockIdx.y] (device memory).  A single trajectory is B = 1.
+// P[blockIdx.y] (device memory).  A single context (batch = false, B = 1)
+// passes its block by value instead.
 struct KmcLaunch {
     Geom g;
     int M;
     const KmcParams* P;
     int B;
-    KmcParams h;   // host copy of the single parameter block (B = 1, passed by value)
+    KmcParams h;   // host copy of a single context's parameter block (passed by value)
+    bool batch;    // a batch (any B, also 1): parameters, times and event arguments from device arrays
 };
 
 int kmc_nblocks(long nmodes);

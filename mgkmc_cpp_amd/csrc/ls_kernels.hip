@@ -971,7 +971,7 @@ template <bool BATCH> static void enqueue_iteration_t(const LsLaunch& p, int mod
 }
 
 void ls_enqueue_iteration(const LsLaunch& p, int mode, int flags, hipStream_t s) {
-    if (p.B > 1) enqueue_iteration_t<true>(p, mode, flags, s); else enqueue_iteration_t<false>(p, mode, flags, s);
+    if (p.batch) enqueue_iteration_t<true>(p, mode, flags, s); else enqueue_iteration_t<false>(p, mode, flags, s);
     MGK_HIP_CHECK(hipGetLastError());
 }
 
@@ -1015,9 +1015,9 @@ template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int p
 void ls_enqueue_pass(const LsLaunch& p, int pass, int mode, hipStream_t s) {
     if (p.g.C == 3) {
         if (pass == 1 || pass == 3) throw ArgError("pass not present in 2D");
-        if (p.B > 1) enqueue_pass_c<3, true>(p, pass, mode, s); else enqueue_pass_c<3, false>(p, pass, mode, s);
+        if (p.batch) enqueue_pass_c<3, true>(p, pass, mode, s); else enqueue_pass_c<3, false>(p, pass, mode, s);
     } else {
-        if (p.B > 1) enqueue_pass_c<6, true>(p, pass, mode, s); else enqueue_pass_c<6, false>(p, pass, mode, s);
+        if (p.batch) enqueue_pass_c<6, true>(p, pass, mode, s); else enqueue_pass_c<6, false>(p, pass, mode, s);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
@@ -1056,11 +1056,11 @@ template <bool BATCH> static void enqueue_final_t(const LsLaunch& p, hipStream_t
 }
 
 void ls_enqueue_start(const LsLaunch& p, int start, hipStream_t s) {
-    if (p.B > 1) enqueue_start_t<true>(p, start, s); else enqueue_start_t<false>(p, start, s);
+    if (p.batch) enqueue_start_t<true>(p, start, s); else enqueue_start_t<false>(p, start, s);
 }
 
 void ls_enqueue_final(const LsLaunch& p, hipStream_t s) {
-    if (p.B > 1) enqueue_final_t<true>(p, s); else enqueue_final_t<false>(p, s);
+    if (p.batch) enqueue_final_t<true>(p, s); else enqueue_final_t<false>(p, s);
 }
 
 }  // namespace mgk

@@ -51,13 +51,14 @@ struct LsParams {
 
 // One launch of a pipeline step over B trajectories of one shape: row
 // blockIdx.y of every grid works on the trajectory whose parameter block is
-// P[blockIdx.y] (device memory).  A single solve is B = 1 and passes its
-// block by value instead (kernel arguments are preloaded).
+// P[blockIdx.y] (device memory).  A single context (batch = false, B = 1)
+// passes its block by value instead (kernel arguments are preloaded).
 struct LsLaunch {
     Geom g;
     const LsParams* P;
     int B;
-    LsParams h;   // host copy of the single parameter block (B = 1)
+    LsParams h;   // host copy of a single context's parameter block (passed by value)
+    bool batch;   // a batch (any B, also 1): parameter blocks from P
 };
 
 bool ls_length_supported(int n);

@@ -18,7 +18,19 @@ def mt19937_words(seed):
 
 
 def box_muller_normals(seed, n):
-    """n draws of box_muller_normal (material_generator.hpp:121-128)."""
+    """n draws of box_muller_normal (material_generator.hpp:121-128).  u1 is
+    redrawn only for a zero word (word / 2^32 <= 1e-15 means word == 0), so
+    unless a zero word comes up the draws are the word pairs in order."""
+    bg = mt19937_words(seed)
+    w = bg.random_raw(2 * n).astype(np.float64)
+    u1 = w[0::2] / 4294967296.0
+    if not (u1 <= 1e-15).any():
+        u2 = w[1::2] / 4294967296.0
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    return _box_muller_sequential(seed, n)
+
+
+def _box_muller_sequential(seed, n):
     bg = mt19937_words(seed)
     out = np.empty(n)
     buf = bg.random_raw(2 * n + 64).astype(np.float64)

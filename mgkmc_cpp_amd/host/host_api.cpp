@@ -311,7 +311,8 @@ int mgk_sim_batch_destroy(mgk_sim_batch* b) {
     return MGK_OK;
 }
 
-int mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, mgk_run_stats* stats) {
+int mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, const long long* caps,
+                              mgk_run_stats* stats) {
     return guard([&]() -> int {
         if (!b) throw std::runtime_error("null batch");
         auto run = [&](auto tag) {
@@ -323,7 +324,7 @@ int mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, mgk
                 p->counters = mgk_host::RunCounters{};
                 v.push_back(p);
             }
-            mgk_host::batch_event_steps<D>(b->dev, v, n, {drv_i, drv_j});
+            mgk_host::batch_event_steps<D>(b->dev, v, n, {drv_i, drv_j}, caps);
             for (size_t i = 0; i < v.size(); ++i) fill_stats(v[i]->counters, v[i]->time, stats ? stats + i : nullptr);
         };
         if (b->dims == 2) run(std::integral_constant<int, 2>{});

@@ -774,10 +774,11 @@ protected:
 // trajectories advanced together through a device batch built from their
 // contexts (mgk_batch_kmc_event): one launch sequence per step covers every
 // trajectory still running.  A trajectory whose attempt finds unstable modes
-// runs its cascade alone and goes on; one without rate stops.
+// runs its cascade alone and goes on; one without rate stops, and so does
+// one that has made caps[b] events in this call (caps may be null).
 template <int D>
 void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& sims, int count,
-                       std::pair<int, int> component) {
+                       std::pair<int, int> component, const long long* caps = nullptr) {
     const int B = (int)sims.size();
     if (B != mgk_batch_size(batch)) throw std::runtime_error("batch size differs from the trajectory count");
     std::vector<typename KmcSimulationT<D>::EventDraw> draws(B);
@@ -787,6 +788,9 @@ void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& 
     std::vector<double> em((size_t)B * D * D), sm((size_t)B * D * D);
     std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
     const double forever = std::numeric_limits<double>::infinity();
+    std::vector<long long> made(B, 0);
+    for (int b = 0; b < B; ++b)
+        if (caps && caps[b] <= 0) active[b] = 0;
     for (int step = 0; step < count; ++step) {
         bool any = false;
         for (int b = 0; b < B; ++b) {
@@ -812,6 +816,7 @@ void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& 
                 if (cs.second > 0) { sim.counters.cascades++; sim.counters.cascade_flips += cs.second; }
             } else if (outcome == MGK_EVENT_DONE) {
                 sim.counters.kmc_events++;
+                if (caps && ++made[b] >= caps[b]) active[b] = 0;
             } else {
                 active[b] = 0;
             }

@@ -1,0 +1,245 @@
+"""Many independent KMC trajectories on one GPU: batched execution and a
+request-serving loop.
+
+The reference runs one trajectory per process (main.cpp:380-412).  Here
+independent trajectories of one grid shape advance together through a device
+batch (SimulationBatch: one launch sequence per event step for all of them),
+and a server admits arriving requests by device-memory footprint:
+
+* requests wait in arrival order until their estimated footprint
+  (mgk_estimate_bytes) fits the HBM budget;
+* an admitted request joins the worker of its grid shape; each worker owns
+  its trajectories and their batch, rebuilt when the membership changes, and
+  advances them a few events per step of the loop;
+* workers of different shapes run in their own threads (the native calls
+  release the interpreter lock), so their launches overlap on the GPU;
+* a request completes when its trajectory has made its events (or has no
+  rate left); its latency is completion time minus arrival time.
+"""
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import abi, configs
+from .solver import Simulation, SimulationBatch, estimate_bytes
+
+
+@dataclass
+class Request:
+    rid: int
+    shape: Tuple[int, ...]
+    events: int
+    seed: int
+    arrival: float = 0.0               # seconds after the start of the run
+    precision: int = abi.MGK_FP32_REFINE
+    instability_kmc: int = 0           # instability_mode "kmc" (1) or "cascade" (0)
+    # filled by the server
+    admitted: Optional[float] = None
+    started: Optional[float] = None
+    finished: Optional[float] = None
+    events_done: int = 0
+    footprint: int = 0
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def request_simulation(req):
+    """The trajectory of a request: the benchmark workloads' material and
+    catalog (correlation length 3 voxels in 2D, 4 in 3D) with its own seed."""
+    corr = 3.0 if len(req.shape) == 2 else 4.0
+    cfg = configs.sim_config(req.shape, precision=req.precision, seed=req.seed, instability_kmc=req.instability_kmc)
+    E, nu = configs.material(req.shape, corr, req.seed)
+    return Simulation(cfg, E, nu)
+
+
+def request_footprint(req):
+    M = 6 if len(req.shape) == 2 else 12
+    return estimate_bytes(req.shape, M, per_voxel_catalog=False)
+
+
+class _Worker(threading.Thread):
+    """The trajectories of one grid shape and precision."""
+
+    def __init__(self, server, key):
+        super().__init__(daemon=True)
+        self.server = server
+        self.key = key
+        self.inbox: List[Request] = []
+        self.members: List[Tuple[Request, Simulation]] = []
+        self.batch: Optional[SimulationBatch] = None
+        self.cv = threading.Condition()
+        self.error: Optional[BaseException] = None
+
+    def submit(self, req):
+        with self.cv:
+            self.inbox.append(req)
+            self.cv.notify()
+
+    def run(self):
+        try:
+            self._loop()
+        except BaseException as e:   # reported by Server.run
+            self.error = e
+            self.server._wake()
+
+    def _loop(self):
+        srv = self.server
+        while True:
+            with self.cv:
+                while not self.inbox and not self.members and not srv._closing:
+                    self.cv.wait(0.05)
+                new, self.inbox = self.inbox, []
+                if not new and not self.members and srv._closing:
+                    return
+            for req in new:
+                sim = request_simulation(req)
+                sim.elastic_run()
+                req.started = srv.now()
+                self.members.append((req, sim))
+            if new or self.batch is None:
+                if self.batch is not None:
+                    self.batch.close()
+                self.batch = SimulationBatch([s for _, s in self.members]) if self.members else None
+            if not self.members:
+                continue
+            caps = [r.events - r.events_done for r, _ in self.members]
+            stats = self.batch.event_steps(srv.step_events, caps=caps)
+            done = []
+            for (req, sim), st in zip(self.members, stats):
+                made = int(st.kmc_events)
+                req.events_done += made
+                # every attempt makes an event, runs a cascade or finds no
+                # rate: a step with neither events nor cascades has stopped
+                stopped = made == 0 and int(st.cascades) == 0
+                if req.events_done >= req.events or stopped:
+                    done.append((req, sim))
+            if done:
+                t = srv.now()
+                self.batch.close()
+                self.batch = None
+                for req, sim in done:
+                    req.finished = t
+                    if srv.on_complete is not None:
+                        srv.on_complete(req, sim)
+                    sim.close()
+                    srv._release(req)
+                ids = {id(r) for r, _ in done}
+                self.members = [(r, s) for r, s in self.members if id(r) not in ids]
+
+
+class Server:
+    """Admits requests by device-memory footprint and runs them batched by
+    grid shape (see the module documentation)."""
+
+    def __init__(self, hbm_budget_bytes, step_events=8, on_complete=None, progress=None, progress_s=5.0):
+        """on_complete(request, simulation): called by the worker thread when
+        a request finishes, before its simulation is released.
+        progress(text): called every progress_s seconds while serving."""
+        self.progress = progress
+        self.progress_s = float(progress_s)
+        self._last_progress = 0.0
+        self._requests = []
+        self.budget = int(hbm_budget_bytes)
+        self.step_events = int(step_events)
+        self.on_complete = on_complete
+        self.peak_used = 0
+        self.used = 0
+        self._lock = threading.Lock()
+        self._wake_cv = threading.Condition(self._lock)
+        self._closing = False
+        self._t0 = None
+        self.workers = {}
+
+    def now(self):
+        return time.perf_counter() - self._t0
+
+    def _wake(self):
+        with self._wake_cv:
+            self._wake_cv.notify_all()
+
+    def _release(self, req):
+        with self._wake_cv:
+            self.used -= req.footprint
+            self._wake_cv.notify_all()
+
+    def _worker(self, req):
+        key = (tuple(req.shape), req.precision)
+        w = self.workers.get(key)
+        if w is None:
+            w = _Worker(self, key)
+            self.workers[key] = w
+            w.start()
+        return w
+
+    def run(self, requests: List[Request]):
+        """Serves `requests` (arrival times relative to the start) and
+        returns them with admission, start and completion times filled."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        self._requests = list(requests)
+        for r in pending:
+            r.footprint = request_footprint(r)
+            if r.footprint > self.budget:
+                raise ValueError(f"request {r.rid} ({r.shape}) needs {r.footprint} bytes, over the budget")
+        self._t0 = time.perf_counter()
+        self._closing = False
+        i = 0
+        while i < len(pending):
+            r = pending[i]
+            wait = r.arrival - self.now()
+            if wait > 0:
+                time.sleep(min(wait, 0.01))
+                self._check()
+                continue
+            with self._wake_cv:
+                while self.used + r.footprint > self.budget:
+                    self._wake_cv.wait(0.01)
+                    self._check()
+                self.used += r.footprint
+                self.peak_used = max(self.peak_used, self.used)
+            r.admitted = self.now()
+            self._worker(r).submit(r)
+            i += 1
+        self._closing = True
+        for w in self.workers.values():
+            with w.cv:
+                w.cv.notify()
+        for w in self.workers.values():
+            while w.is_alive():
+                w.join(timeout=0.2)
+                self._check()
+        self._check()
+        return requests
+
+    def _report(self):
+        if self.progress is None or self.now() - self._last_progress < self.progress_s:
+            return
+        self._last_progress = self.now()
+        done = sum(1 for r in self._requests if r.finished is not None)
+        ev = sum(r.events_done for r in self._requests)
+        self.progress(f"t={self.now():.1f}s admitted={sum(1 for r in self._requests if r.admitted is not None)} "
+                      f"done={done}/{len(self._requests)} events={ev} used={self.used / 2**30:.2f}GiB")
+
+    def _check(self):
+        self._report()
+        for w in self.workers.values():
+            if w.error is not None:
+                self._closing = True
+                raise RuntimeError(f"worker {w.key} failed") from w.error
+
+
+def summary(requests, wall_s=None):
+    """p50/p99 completion latency (arrival to result) and aggregate events/s."""
+    lat = np.array([r.latency for r in requests if r.latency is not None])
+    ev = sum(r.events_done for r in requests)
+    t_end = max((r.finished for r in requests if r.finished is not None), default=0.0)
+    t_start = min((r.arrival for r in requests), default=0.0)
+    span = wall_s if wall_s is not None else (t_end - t_start)
+    return dict(requests=len(requests), completed=int(lat.size), events=int(ev),
+                p50_latency_s=float(np.percentile(lat, 50)) if lat.size else None,
+                p99_latency_s=float(np.percentile(lat, 99)) if lat.size else None,
+                events_per_s=ev / span if span > 0 else None, span_s=span)

@@ -246,6 +246,9 @@ class Context:
         check(gpu().mgk_time_ls_pass(self._h, int(pass_id), 1 if fp32 else 0, int(reps), C.byref(ms)))
         return ms.value
 
+    def device_bytes(self):
+        return int(gpu().mgk_device_bytes(self._h))
+
     def last_solve_stats(self):
         a = C.c_int(0)
         b = C.c_int(0)
@@ -352,6 +355,10 @@ class Simulation:
         host().mgk_sim_device_ctx(self._h, C.byref(h))
         return h
 
+    def device_bytes(self):
+        """Device bytes this simulation's context holds."""
+        return int(gpu().mgk_device_bytes(self.device_context_handle()))
+
     def time_ls_pass(self, pass_id, fp32=False, reps=20):
         ms = C.c_double(0.0)
         check(gpu().mgk_time_ls_pass(self.device_context_handle(), int(pass_id), 1 if fp32 else 0, int(reps),
@@ -383,9 +390,21 @@ class SimulationBatch:
         except Exception:
             pass
 
-    def event_steps(self, n, drv=(0, 0)):
-        """n attempts for every simulation; returns one RunStats per simulation."""
+    def event_steps(self, n, drv=(0, 0), caps=None):
+        """Up to n attempts for every simulation, simulation b stopping after
+        caps[b] events (caps=None: no cap); one RunStats per simulation."""
         st = (abi.RunStats * len(self.sims))()
-        check_host(host().mgk_sim_batch_event_steps(self._h, int(n), drv[0], drv[1], st))
+        cp = None
+        if caps is not None:
+            cp = (C.c_longlong * len(self.sims))(*[int(x) for x in caps])
+        check_host(host().mgk_sim_batch_event_steps(self._h, int(n), drv[0], drv[1], cp, st))
         return list(st)
+
+
+def estimate_bytes(shape, M, per_voxel_catalog=False):
+    """Device bytes a context of this grid and mode count holds once its KMC
+    state and catalog are set (mgk_estimate_bytes)."""
+    dims = len(shape)
+    nz = shape[2] if dims == 3 else 1
+    return int(gpu().mgk_estimate_bytes(dims, shape[0], shape[1], nz, int(M), 1 if per_voxel_catalog else 0))
 

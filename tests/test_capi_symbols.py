@@ -13,7 +13,7 @@ LIB = os.path.join(ROOT, "mgkmc_cpp_amd", "lib")
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+[\s\*]+(mgk_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:\w+\s+)*\w+[\s\*]+(mgk_\w+)\s*\(", text, flags=re.M)))
 
 
 @pytest.mark.parametrize("header,lib", [("mgkmc_gpu.h", "libmgkmc_gpu.so"), ("mgkmc_host.h", "libmgkmc_host.so")])
@@ -35,3 +35,17 @@ def test_abi_version_and_error_path_without_device():
     # argument validation happens before any device call
     rc = g.mgk_create(4, 8, 8, 1, 1.0, 0, 0, None, None, ctypes.byref(ctypes.c_void_p()))
     assert rc == 1 and b"null" in g.mgk_last_error()
+
+
+def test_footprint_estimate_without_device():
+    # pure arithmetic: per-voxel catalogs add C*N*M doubles over a shared one,
+    # the KMC state grows with M, and bad shapes are rejected
+    from mgkmc_cpp_amd import _lib
+    g = _lib.gpu()
+    n3 = 32 * 32 * 32
+    shared = g.mgk_estimate_bytes(3, 32, 32, 32, 12, 0)
+    per_voxel = g.mgk_estimate_bytes(3, 32, 32, 32, 12, 1)
+    assert per_voxel - shared == 6 * n3 * 12 * 8 - 6 * 12 * 8
+    assert g.mgk_estimate_bytes(3, 32, 32, 32, 0, 0) < shared < g.mgk_estimate_bytes(3, 32, 32, 32, 24, 0)
+    assert g.mgk_estimate_bytes(2, 64, 64, 1, 6, 0) < g.mgk_estimate_bytes(3, 16, 16, 16, 6, 0) * 16
+    assert g.mgk_estimate_bytes(4, 8, 8, 8, 6, 0) == -1

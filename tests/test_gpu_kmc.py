@@ -287,16 +287,16 @@ def test_device_event_without_rate_or_with_unstable_modes():
         assert np.array_equal(a.get_field(f), before[f]), f
 
 
-@pytest.mark.parametrize("kw,events", [
-    (dict(), 25),                                                          # 2D fixed catalog, cascade mode
-    (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM, neighbor_softening_fraction=0.2), 25),
-    (dict(dims=3, nx=8, ny=8, nz=8, M=12), 10),
+@pytest.mark.parametrize("kw,events,seeds", [
+    (dict(), 25, (42, 7, 1234)),                                           # 2D fixed catalog, cascade mode
+    (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM, neighbor_softening_fraction=0.2), 25, (42, 7, 1234)),
+    (dict(dims=3, nx=8, ny=8, nz=8, M=12), 10, (42, 7, 1234)),
+    (dict(), 15, (42,)),                                                   # a batch of one
 ])
-def test_batch_equals_separate_trajectories(kw, events):
+def test_batch_equals_separate_trajectories(kw, events, seeds):
     # trajectories advanced together (one launch sequence per step) equal the
     # same trajectories run one by one, bit for bit in FP64, and the oracle
     from mgkmc_cpp_amd import SimulationBatch
-    seeds = (42, 7, 1234)
     cfgs = [_cfg(seed=sd, **kw) for sd in seeds]
     N = cfgs[0].nx * cfgs[0].ny * (cfgs[0].nz if cfgs[0].dims == 3 else 1)
     mats = [_field(N, seed=17 + i) for i in range(len(seeds))]
