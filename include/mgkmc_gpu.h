@@ -236,6 +236,10 @@ typedef struct mgk_batch mgk_batch;
 int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out);
 int mgk_batch_destroy(mgk_batch* b);
 int mgk_batch_size(const mgk_batch* b);
+/* on != 0 (the default): the launches of an event call that do not depend
+ * on its outcome are captured once per launch structure into a HIP graph and
+ * replayed; 0: launched one by one (same results). */
+int mgk_batch_set_graphs(mgk_batch* b, int on);
 int mgk_batch_kmc_event(mgk_batch* b, const mgk_event_in* in, const int* active, double* eps_macro,
                         double* sig_macro, mgk_event_out* out);
 

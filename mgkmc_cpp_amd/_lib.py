@@ -61,6 +61,7 @@ def gpu():
         _sig(lib, "mgk_batch_create", C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)])
         _sig(lib, "mgk_batch_destroy", C.c_int, [P])
         _sig(lib, "mgk_batch_size", C.c_int, [P])
+        _sig(lib, "mgk_batch_set_graphs", C.c_int, [P, C.c_int])
         _sig(lib, "mgk_batch_kmc_event", C.c_int,
              [P, C.POINTER(abi.EventIn), IP, DP, DP, C.POINTER(abi.EventOut)])
         _sig(lib, "mgk_sync", C.c_int, [P])

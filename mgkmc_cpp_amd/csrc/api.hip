@@ -9,6 +9,8 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mgkmc_gpu.h"
@@ -26,6 +28,12 @@ int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+// Calls that allocate, free or synchronise the whole device (hipMalloc,
+// hipFree, hipDeviceSynchronize, synchronous copies) must not overlap a
+// stream capture in another thread: they would invalidate it.  Entry points
+// that make them, and every graph capture, hold this lock.
+std::recursive_mutex g_device_mu;
 
 template <typename F> int guarded(F&& f) {
     try {
@@ -463,6 +471,7 @@ int mgk_abi_version(void) { return MGK_ABI_VERSION; }
 int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, int precision, const double* E,
                const double* nu, mgk_ctx** out) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!out || !E || !nu) throw ArgError("null argument");
         *out = nullptr;
         if (dims != 2 && dims != 3) throw ArgError("dims must be 2 or 3");
@@ -564,6 +573,7 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
 int mgk_destroy(mgk_ctx* ctx) {
     if (!ctx) return MGK_OK;
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         delete ctx;
         return MGK_OK;
@@ -583,6 +593,7 @@ int mgk_ls_solve(mgk_ctx* ctx, const double* eps_bar, double* eps, int warm_star
                  double* sig_out, double* eps_macro, double* sig_macro, int max_iter, double tol, int* iters_out,
                  double* diff_out) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx || !eps_bar || !eps || !eps_macro || !sig_macro) throw ArgError("null argument");
         if (max_iter < 0) throw ArgError("max_iter < 0");
         mgk_ctx& c = *ctx;
@@ -661,6 +672,7 @@ static bool kmc_field(int which) {
 
 int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx || !host) throw ArgError("null argument");
         mgk_ctx& c = *ctx;
         if (field_len(c, which) < 0) throw ArgError("unknown field");
@@ -730,6 +742,7 @@ int mgk_set_field(mgk_ctx* ctx, int which, const double* host) {
 
 int mgk_get_field(mgk_ctx* ctx, int which, double* host) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx || !host) throw ArgError("null argument");
         mgk_ctx& c = *ctx;
         if (field_len(c, which) < 0) throw ArgError("unknown field");
@@ -787,6 +800,7 @@ int mgk_get_field(mgk_ctx* ctx, int which, double* host) {
 
 int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx || !kp) throw ArgError("null argument");
         if (kp->M < 1) throw ArgError("M must be >= 1");
         mgk_ctx& c = *ctx;
@@ -926,6 +940,7 @@ int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
 int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_catalog, const double* new_Q0,
                    int first_mode, int n_modes) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx) throw ArgError("null context");
         mgk_ctx& c = *ctx;
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
@@ -1103,6 +1118,7 @@ int mgk_last_selection(mgk_ctx* ctx, int* flat_idx, double* q_sel) {
 
 int mgk_find_unstable(mgk_ctx* ctx, int* idx_out, double* q_out, int cap, int* n_out) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctx || !n_out) throw ArgError("null argument");
         mgk_ctx& c = *ctx;
         if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
@@ -1243,8 +1259,21 @@ struct mgk_batch {
     double *d_stage = nullptr, *h_stage = nullptr;
     size_t stage_per = 0;   // M * (C + 1) doubles per trajectory
 
+    // The first round of an event call (uploads from the pinned buffers, event
+    // kernels, the predicted iterations, final pass, read-back) as HIP
+    // graphs, one per launch structure.  Every per-call value is read from
+    // device memory the graph's own copies fill, so a graph replays as is.
+    bool use_graphs = true;
+    std::map<std::vector<int>, hipGraphExec_t> graphs;
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
     ~mgk_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         auto f = [](void* p) { if (p) (void)hipFree(p); };
         auto fh = [](void* p) { if (p) (void)hipHostFree(p); };
         f(d_lsp); f(d_kp); f(d_ea); f(d_times); f(d_ctl); f(d_kctl); f(d_stage);
@@ -1330,42 +1359,77 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     }
     if (n_active == 0) throw ArgError("no active trajectory in the batch call");
     hipStream_t s = b.stream;
-    MGK_HIP_CHECK(hipMemcpyAsync(b.d_lsp, b.h_lsp, sizeof(LsParams) * B, hipMemcpyHostToDevice, s));
-    MGK_HIP_CHECK(hipMemcpyAsync(b.d_kp, b.h_kp, sizeof(KmcParams) * B, hipMemcpyHostToDevice, s));
-    MGK_HIP_CHECK(hipMemcpyAsync(b.d_ea, b.h_ea, sizeof(EventArgs) * B, hipMemcpyHostToDevice, s));
-    MGK_HIP_CHECK(hipMemcpyAsync(b.d_times, b.h_times, sizeof(double) * B, hipMemcpyHostToDevice, s));
-    MGK_HIP_CHECK(hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(Ctl) * B, hipMemcpyHostToDevice, s));
-    if (n_modes > 0)
-        MGK_HIP_CHECK(hipMemcpyAsync(b.d_stage, b.h_stage, sizeof(double) * b.stage_per * B, hipMemcpyHostToDevice, s));
     const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0], true};
     const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true};
-    kmc_enqueue_rates(KL, 0.0, b.d_times, s);
-    kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, s);
-    kmc_enqueue_select(KL, 0.0, b.d_ea, s);
-    kmc_enqueue_select_fallback(KL, 1, s);
-    kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, s, 1);
-    if (!all_keep) ls_enqueue_start(L, LS_START_MEAN, s);   // trajectories with a known mean keep their shift
-    // iterations until every trajectory's control block says done
-    int enqueued = 0;
-    int nb = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
-    while (true) {
-        for (int q = 0; q < nb; ++q) {
-            const int k = enqueued + q;
+    const int nb0 = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
+    auto iterations = [&](int from, int count, hipStream_t st) {
+        for (int q = 0; q < count; ++q) {
+            const int k = from + q;
             const int flags = (k == 0 ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
-            ls_enqueue_iteration(L, LS_FULL, flags, s);
+            ls_enqueue_iteration(L, LS_FULL, flags, st);
             if (refine)
                 for (int j = 0; j < J; ++j)
-                    ls_enqueue_iteration(L, LS_CG, j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), s);
+                    ls_enqueue_iteration(L, LS_CG, j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), st);
         }
-        enqueued += nb;
-        ls_enqueue_final(L, s);
-        MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, s));
-        MGK_HIP_CHECK(hipMemcpyAsync(b.h_kctl, b.d_kctl, sizeof(KmcCtl) * B, hipMemcpyDeviceToHost, s));
-        MGK_HIP_CHECK(hipStreamSynchronize(s));
+        ls_enqueue_final(L, st);
+        MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, st));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.h_kctl, b.d_kctl, sizeof(KmcCtl) * B, hipMemcpyDeviceToHost, st));
+    };
+    auto first_round = [&](hipStream_t st) {
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_lsp, b.h_lsp, sizeof(LsParams) * B, hipMemcpyHostToDevice, st));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_kp, b.h_kp, sizeof(KmcParams) * B, hipMemcpyHostToDevice, st));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_ea, b.h_ea, sizeof(EventArgs) * B, hipMemcpyHostToDevice, st));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_times, b.h_times, sizeof(double) * B, hipMemcpyHostToDevice, st));
+        MGK_HIP_CHECK(hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(Ctl) * B, hipMemcpyHostToDevice, st));
+        if (n_modes > 0)
+            MGK_HIP_CHECK(hipMemcpyAsync(b.d_stage, b.h_stage, sizeof(double) * b.stage_per * B, hipMemcpyHostToDevice, st));
+        kmc_enqueue_rates(KL, 0.0, b.d_times, st);
+        kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, st);
+        kmc_enqueue_select(KL, 0.0, b.d_ea, st);
+        kmc_enqueue_select_fallback(KL, 1, st);
+        kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, st, 1);
+        if (!all_keep) ls_enqueue_start(L, LS_START_MEAN, st);   // trajectories with a known mean keep their shift
+        iterations(0, nb0, st);
+    };
+    if (b.use_graphs) {
+        const std::vector<int> key{J, nb0, all_keep ? 1 : 0, n_modes, first_mode, has_cat, has_q0, refine ? 1 : 0};
+        auto it = b.graphs.find(key);
+        if (it == b.graphs.end()) {
+            std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+            if (b.graphs.size() >= 64) b.drop_graphs();
+            hipGraph_t graph = nullptr;
+            MGK_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                first_round(s);
+            } catch (...) {
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(s, &partial);
+                if (partial) (void)hipGraphDestroy(partial);
+                throw;
+            }
+            MGK_HIP_CHECK(hipStreamEndCapture(s, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            MGK_HIP_CHECK(e);
+            it = b.graphs.emplace(key, exec).first;
+        }
+        MGK_HIP_CHECK(hipGraphLaunch(it->second, s));
+    } else {
+        first_round(s);
+    }
+    MGK_HIP_CHECK(hipStreamSynchronize(s));
+    // further iterations (eager) until every trajectory's control block says done
+    int enqueued = nb0;
+    int nb = nb0;
+    while (true) {
         bool all_done = true;
         for (int y = 0; y < B; ++y) all_done = all_done && b.h_ctl[y].done;
         if (all_done || enqueued >= max_iter) break;
         nb = std::min(max_iter - enqueued, std::max(refine ? 1 : 2, nb / 2));
+        iterations(enqueued, nb, s);
+        enqueued += nb;
+        MGK_HIP_CHECK(hipStreamSynchronize(s));
     }
     for (int y = 0; y < B; ++y) {
         mgk_ctx& c = *b.c[y];
@@ -1395,6 +1459,7 @@ extern "C" {
 
 int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out) {
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         if (!ctxs || !out || B < 1) throw ArgError("bad argument");
         *out = nullptr;
         const mgk_ctx* a = ctxs[0];
@@ -1434,12 +1499,24 @@ int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out) {
 int mgk_batch_destroy(mgk_batch* b) {
     if (!b) return MGK_OK;
     return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
         delete b;
         return MGK_OK;
     });
 }
 
 int mgk_batch_size(const mgk_batch* b) { return b ? b->B : 0; }
+
+int mgk_batch_set_graphs(mgk_batch* b, int on) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!b) throw ArgError("null batch");
+        MGK_HIP_CHECK(hipStreamSynchronize(b->stream));
+        if (!on) b->drop_graphs();
+        b->use_graphs = on != 0;
+        return MGK_OK;
+    });
+}
 
 int mgk_batch_kmc_event(mgk_batch* b, const mgk_event_in* in, const int* active, double* eps_macro, double* sig_macro,
                         mgk_event_out* out) {
