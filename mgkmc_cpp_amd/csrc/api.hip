@@ -476,21 +476,15 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
         *out = nullptr;
         if (dims != 2 && dims != 3) throw ArgError("dims must be 2 or 3");
         if (dims == 2) nz = 1;
-        if (nx < 2 || ny < 2 || nz < 1 || (dims == 3 && nz < 2)) throw ArgError("grid too small");
+        if (nx < 1 || ny < 1 || nz < 1) throw ArgError("grid too small");
+        if (dims == 2 && (nx < 2 || ny < 2)) throw ArgError("grid too small");
+        if (dims == 3 && (nx == 1) + (ny == 1) + (nz == 1) > 1) throw ArgError("grid too small: at most one 3D axis of length 1");
         if (precision != MGK_FP64 && precision != MGK_FP32_REFINE) throw ArgError("bad precision");
-        Geom g;
-        g.dims = dims;
-        g.nf = nx;
-        g.nm = (dims == 3) ? ny : 1;
-        g.nl = (dims == 3) ? nz : ny;
+        const Geom g = make_geom(dims, nx, ny, nz);
         if (g.nl % 2 != 0) throw UnsupportedError("the contiguous axis length must be even");
         for (int n : {g.nf, g.nl})
             if (!ls_length_supported(n)) throw UnsupportedError("FFT length " + std::to_string(n) + " not built");
-        if (dims == 3 && !ls_length_supported(g.nm)) throw UnsupportedError("FFT length " + std::to_string(g.nm) + " not built");
-        g.nlh = g.nl / 2 + 1;
-        g.N = (long)g.nf * g.nm * g.nl;
-        g.NS = (long)g.nf * g.nm * g.nlh;
-        g.C = (dims == 2) ? 3 : 6;
+        if (g.nm > 1 && !ls_length_supported(g.nm)) throw UnsupportedError("FFT length " + std::to_string(g.nm) + " not built");
 
         mgk_ctx* c = new mgk_ctx();
         try {
@@ -1167,7 +1161,7 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
     return guarded([&]() -> int {
         if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
-        if (pass < 0 || pass > 5 || (ctx->g.dims == 2 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        if (pass < 0 || pass > 5 || (ctx->g.nm == 1 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
         if (pass == 5 && (!fp32 || !ctx->corr)) throw ArgError("pass 5 needs an FP32+refine context");
         mgk_ctx& c = *ctx;
         // a control block that lets every pass run (not done, shift off)
@@ -1196,15 +1190,7 @@ long long mgk_device_bytes(const mgk_ctx* ctx) { return ctx ? (long long)ctx->de
 
 long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_voxel_catalog) {
     if ((dims != 2 && dims != 3) || nx < 1 || ny < 1 || (dims == 3 && nz < 1) || M < 0) return -1;
-    Geom g{};
-    g.dims = dims;
-    g.nf = nx;
-    g.nm = (dims == 2) ? 1 : ny;
-    g.nl = (dims == 2) ? ny : nz;
-    g.nlh = g.nl / 2 + 1;
-    g.N = (long)g.nf * g.nm * g.nl;
-    g.NS = (long)g.nf * g.nm * g.nlh;
-    g.C = (dims == 2) ? 3 : 6;
+    const Geom g = make_geom(dims, nx, ny, nz);
     const long long N = g.N, C = g.C, NS = g.NS;
     auto r = [](long long n, long long sz) { return std::max<long long>(n, 1) * sz; };
     long long b = 0;
@@ -1468,7 +1454,7 @@ int mgk_batch_create(mgk_ctx* const* ctxs, int B, mgk_batch** out) {
             const mgk_ctx* c = ctxs[y];
             if (!c) throw ArgError("null context");
             if (!c->kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
-            if (c->g.dims != a->g.dims || c->g.nf != a->g.nf || c->g.nm != a->g.nm || c->g.nl != a->g.nl ||
+            if (c->g.dims != a->g.dims || c->g.nx != a->g.nx || c->g.ny != a->g.ny || c->g.nz != a->g.nz ||
                 c->precision != a->precision || c->kp.M != a->kp.M)
                 throw ArgError("the trajectories of a batch share shape, precision and mode count");
             for (int z = 0; z < y; ++z)

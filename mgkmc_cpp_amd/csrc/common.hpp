@@ -11,6 +11,12 @@ namespace mgk {
 // F (first, x), M (middle, y in 3D, absent in 2D) and L (last, contiguous:
 // z in 3D, y in 2D), matching the reference's voxel order idx = x*ny + y and
 // idx = x*ny*nz + y*nz + z (linear_elastic_solver.hpp:81, :357).
+// A 3D grid with one axis of length 1 keeps that memory order but puts the
+// length-1 axis at M, so the transform runs over the two others (the M
+// passes are then left out): nz = 1 gives (F, M, L) = (x, z, y), nx = 1
+// gives (y, x, z), ny = 1 keeps (x, y, z).
+enum GeomPerm { PERM_XYZ = 0, PERM_XZY = 1, PERM_YXZ = 2 };
+
 struct Geom {
     int dims = 2;
     int nf = 1, nm = 1, nl = 1;
@@ -18,8 +24,28 @@ struct Geom {
     long N = 1;        // voxels
     long NS = 1;       // complex bins of the half spectrum = nf*nm*nlh
     int C = 3;         // symmetric components stored: 3 (2D) or 6 (3D)
+    int nx = 1, ny = 1, nz = 1;   // the grid as given
+    int perm = PERM_XYZ;          // physical direction of each FFT axis
     __host__ __device__ long nlines() const { return (long)nf * nm; }
 };
+
+// Geometry of an (nx, ny, nz) grid (nz ignored in 2D); no size checks.
+inline Geom make_geom(int dims, int nx, int ny, int nz) {
+    Geom g;
+    g.dims = dims;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = (dims == 2) ? 1 : nz;
+    if (dims == 2) { g.nf = nx; g.nm = 1; g.nl = ny; }
+    else if (nz == 1) { g.nf = nx; g.nm = 1; g.nl = ny; g.perm = PERM_XZY; }
+    else if (nx == 1) { g.nf = ny; g.nm = 1; g.nl = nz; g.perm = PERM_YXZ; }
+    else { g.nf = nx; g.nm = ny; g.nl = nz; }
+    g.nlh = g.nl / 2 + 1;
+    g.N = (long)g.nf * g.nm * g.nl;
+    g.NS = (long)g.nf * g.nm * g.nlh;
+    g.C = (dims == 2) ? 3 : 6;
+    return g;
+}
 
 // Symmetric component order on the device.
 // 2D: 0 xx, 1 yy, 2 xy.   3D: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz.

@@ -438,7 +438,7 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
     p.last_t[v] = time;
     if (ph.nfrac > 0.0) {
         int x, y, z = 0;
-        const int nx = g.nf, ny = (D == 2) ? g.nl : g.nm, nz = (D == 2) ? 1 : g.nl;
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
         if (D == 2) { y = (int)(v % ny); x = (int)(v / ny); }
         else { z = (int)(v % nz); const long t = v / nz; y = (int)(t % ny); x = (int)(t / ny); }
         const int zr = (D == 3) ? 1 : 0;

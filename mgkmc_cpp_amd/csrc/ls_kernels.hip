@@ -496,6 +496,15 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
         if (D == 3) {
             qa[1] = sfreq(km, g.nm, nm_);
             qa[2] = sfreq(kl, g.nl, nl_);
+            // FFT axes (F, M, L) to directions (x, y, z); M has length 1
+            // (frequency 0, no Nyquist bin) when perm is not PERM_XYZ
+            if (g.perm == PERM_XZY) {
+                const SR t = qa[1]; qa[1] = qa[2]; qa[2] = t;
+                const bool b = nm_; nm_ = nl_; nl_ = b;
+            } else if (g.perm == PERM_YXZ) {
+                const SR t = qa[0]; qa[0] = qa[1]; qa[1] = t;
+                const bool b = nf_; nf_ = nm_; nm_ = b;
+            }
         } else {
             nm_ = false;
             qa[1] = sfreq(kl, g.nl, nl_);
@@ -943,17 +952,17 @@ int ls_partial_slots(const Geom& g) {
 
 template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int flags, hipStream_t s) {
     launch_fwd_last<double, C, LS_FULL, BATCH>(p, flags, s);
-    if (C == 6) launch_mid<double, -1, BATCH>(p, s, LS_FULL);
+    if (C == 6 && p.g.nm > 1) launch_mid<double, -1, BATCH>(p, s, LS_FULL);
     launch_first_gamma<double, C, BATCH>(p, s, LS_FULL);
-    if (C == 6) launch_mid<double, +1, BATCH>(p, s, LS_FULL);
+    if (C == 6 && p.g.nm > 1) launch_mid<double, +1, BATCH>(p, s, LS_FULL);
     launch_inv_last<double, C, LS_FULL, BATCH>(p, flags, s);
 }
 
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
     launch_fwd_last<float, C, LS_CG, BATCH>(p, flags, s);
-    if (C == 6) launch_mid<float, -1, BATCH>(p, s, LS_CG);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG);
     launch_first_gamma<float, C, BATCH>(p, s, LS_CG);
-    if (C == 6) launch_mid<float, +1, BATCH>(p, s, LS_CG);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, +1, BATCH>(p, s, LS_CG);
     launch_inv_last<float, C, LS_CG, BATCH>(p, flags, s);
 }
 
@@ -997,9 +1006,13 @@ template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int p
         case 0:
             if (full) launch_fwd_last<double, C, LS_FULL, BATCH>(p, 0, s); else launch_fwd_last<float, C, LS_CG, BATCH>(p, 0, s);
             break;
-        case 1: if (full) launch_mid<double, -1, BATCH>(p, s, mode); else launch_mid<float, -1, BATCH>(p, s, mode); break;
+        case 1:
+            if (p.g.nm == 1) throw ArgError("pass not present for this grid");
+            if (full) launch_mid<double, -1, BATCH>(p, s, mode); else launch_mid<float, -1, BATCH>(p, s, mode); break;
         case 2: if (full) launch_first_gamma<double, C, BATCH>(p, s, mode); else launch_first_gamma<float, C, BATCH>(p, s, mode); break;
-        case 3: if (full) launch_mid<double, +1, BATCH>(p, s, mode); else launch_mid<float, +1, BATCH>(p, s, mode); break;
+        case 3:
+            if (p.g.nm == 1) throw ArgError("pass not present for this grid");
+            if (full) launch_mid<double, +1, BATCH>(p, s, mode); else launch_mid<float, +1, BATCH>(p, s, mode); break;
         case 4:
             if (full) launch_inv_last_only<double, C, LS_FULL, BATCH>(p, 0, s);
             else launch_inv_last_only<float, C, LS_CG, BATCH>(p, 0, s);

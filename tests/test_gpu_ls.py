@@ -29,8 +29,10 @@ def _bar(D, seed=1, sym=True):
     return 0.5 * (b + b.T) if sym else b
 
 
+# (32, 32, 1), (16, 1, 32), (1, 32, 16): 3D grids with one axis of length 1
+# (the reference's DAMASK comparison runs 32 x 32 x 1)
 CASES = [(8, 8), (16, 16), (32, 32), (48, 32), (64, 64), (128, 128), (8, 8, 8), (16, 16, 16), (32, 32, 32),
-         (16, 48, 32), (64, 64, 64)]
+         (16, 48, 32), (64, 64, 64), (32, 32, 1), (16, 1, 32), (1, 32, 16)]
 
 
 @pytest.mark.parametrize("shape", CASES)
@@ -107,3 +109,28 @@ def test_fp32_refine_error_bound():
     err = _rel(got["sig"], ref["sig"])
     print("fp32+refine iters", got["iters"], ctx.last_solve_stats(), "rel err", err)
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("precision", [abi.MGK_FP64, abi.MGK_FP32_REFINE])
+@pytest.mark.parametrize("thin", [(32, 16, 1), (32, 1, 16), (1, 32, 16)])
+def test_thin_3d_equals_plane_strain(thin, precision):
+    # fields uniform along a length-1 axis with no macro strain on it: the
+    # 3D solve is the 2D plane-strain solve of the other two axes, and the
+    # out-of-plane normal stress is lam (e_11 + e_22)
+    keep = [a for a in range(3) if thin[a] > 1]
+    shape2 = tuple(thin[a] for a in keep)
+    E, nu = _field(shape2, 17)
+    b2 = np.array([[1e-3, 2e-4], [2e-4, -3e-4]])
+    b3 = np.zeros((3, 3))
+    b3[np.ix_(keep, keep)] = b2
+    r2 = Context(shape2, E, nu, precision=precision).ls_solve(b2, tol=1e-10)
+    r3 = Context(thin, E, nu, precision=precision).ls_solve(b3, tol=1e-10)
+    e3 = r3["eps"][:, keep][:, :, keep]
+    s3 = r3["sig"][:, keep][:, :, keep]
+    out = [a for a in range(3) if a not in keep][0]
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    bound = 1e-12 if precision == abi.MGK_FP64 else 1e-6
+    assert r3["iters"] == r2["iters"] or precision != abi.MGK_FP64
+    assert _rel(e3, r2["eps"]) < bound and _rel(s3, r2["sig"]) < bound
+    assert np.abs(r3["eps"][:, out, :]).max() < bound * np.abs(b2).max()
+    assert _rel(r3["sig"][:, out, out], lam * (r2["eps"][:, 0, 0] + r2["eps"][:, 1, 1])) < bound
