@@ -145,6 +145,12 @@ int mgk_update_rates(mgk_ctx* ctx, double time, double* total_rate,
  * Outputs the flat mode index and Q of that mode (is_instab = q_sel <= thr). */
 int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel);
 
+/* The reference's selection computed exactly as written (kmc_solver.hpp:808-826,
+ * 3D :1820-1838): the left-to-right FP64 total, target = u2 * total and the
+ * first included mode whose left-to-right prefix reaches it, with no parallel
+ * shortcut.  Same outputs as mgk_select; one wave, O(N*M) sequential adds. */
+int mgk_select_exact(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel);
+
 /* apply_flip_soa_2d/3d + prev_strain_dir update + redraw upload
  * (kmc_solver.hpp:423-462, :835-864; 3D :1424-1471, :1850-1875).
  * new_catalog (n_modes*D*D, AoS) and new_Q0 (n_modes) may be NULL; when
@@ -187,7 +193,13 @@ int mgk_last_selection(mgk_ctx* ctx, int* flat_idx, double* q_sel);
  *                       reference runs run_cascade first; nothing changed)
  *   MGK_EVENT_NO_RATE   R is not > 0 (nothing drawn in the reference)
  *   MGK_EVENT_REJECTED  t_wait >= remaining (only u1 drawn in the reference)
- * remaining = +inf accepts every finite t_wait. */
+ * remaining = +inf accepts every finite t_wait.
+ * Selection: on an FP64 context exactly as mgk_select (the reference's index).
+ * On an FP32+refine context the parallel prefix decides alone: targets within
+ * the rounding window of a boundary (about 1e-9 of R at 3e6 modes, where
+ * mgk_select reruns the left-to-right pass) are left to it, since that
+ * context's barriers already differ from the FP64 ones by the solve's
+ * tolerance.  The choice is deterministic either way. */
 enum mgk_event_outcome {
     MGK_EVENT_DONE = 0,
     MGK_EVENT_UNSTABLE = 1,

@@ -52,6 +52,7 @@ def gpu():
         _sig(lib, "mgk_elastic_run", C.c_int, [P, DP, DP, C.c_int, C.c_double, IP, DP])
         _sig(lib, "mgk_update_rates", C.c_int, [P, C.c_double, DP, IP, IP])
         _sig(lib, "mgk_select", C.c_int, [P, C.c_double, IP, DP])
+        _sig(lib, "mgk_select_exact", C.c_int, [P, C.c_double, IP, DP])
         _sig(lib, "mgk_apply_flip", C.c_int, [P, C.c_int, C.c_double, DP, DP, C.c_int, C.c_int])
         _sig(lib, "mgk_find_unstable", C.c_int, [P, IP, DP, C.c_int, IP])
         _sig(lib, "mgk_select_flip", C.c_int, [P, C.c_double, C.c_double, DP, DP, C.c_int, C.c_int])

@@ -931,6 +931,21 @@ int mgk_select(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
     });
 }
 
+int mgk_select_exact(mgk_ctx* ctx, double u2, int* flat_idx, double* q_sel) {
+    return guarded([&]() -> int {
+        if (!ctx || !flat_idx) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        kmc_enqueue_select(c.kml(), u2, nullptr, c.stream);   // sets u2 and the go flag
+        kmc_enqueue_select_fallback(c.kml(), 0, c.stream);
+        MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        *flat_idx = c.hkctl->sel;
+        if (q_sel) *q_sel = c.hkctl->q_sel;
+        return MGK_OK;
+    });
+}
+
 int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_catalog, const double* new_Q0,
                    int first_mode, int n_modes) {
     return guarded([&]() -> int {
@@ -1069,7 +1084,7 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         kmc_enqueue_rates(KL, in->time, nullptr, c.stream);
         kmc_enqueue_event_prep(KL, c.ctl, nullptr, ea, nullptr, c.stream);
         kmc_enqueue_select(KL, in->u2, nullptr, c.stream);
-        kmc_enqueue_select_fallback(KL, 1, c.stream);
+        if (c.precision == MGK_FP64) kmc_enqueue_select_fallback(KL, 1, c.stream);
         if (n_modes > 0)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
@@ -1372,7 +1387,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         kmc_enqueue_rates(KL, 0.0, b.d_times, st);
         kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, st);
         kmc_enqueue_select(KL, 0.0, b.d_ea, st);
-        kmc_enqueue_select_fallback(KL, 1, st);
+        if (!refine) kmc_enqueue_select_fallback(KL, 1, st);
         kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, st, 1);
         if (!all_keep) ls_enqueue_start(L, LS_START_MEAN, st);   // trajectories with a known mean keep their shift
         iterations(0, nb0, st);

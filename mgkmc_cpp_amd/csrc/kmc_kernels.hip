@@ -359,37 +359,121 @@ __global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const Kmc
     kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
 }
 
-// Exact left-to-right pass of the reference (single lane): the total in flat
-// order, target = u2 * total, then the first prefix >= target.  With
-// only_if_ambiguous it runs only when k_select flagged its answer.
+// Rates of the kRateBlock modes of chunk b into dst, excluded modes as `excl`,
+// by `lanes` lanes starting at lane index t.
+__device__ __forceinline__ void chunk_rates(const KmcParams& p, int b, long nmodes, double excl, double* dst, int t,
+                                            int lanes) {
+    constexpr int MAXPER = kRateBlock / 64;
+    double q[MAXPER];
+#pragma unroll
+    for (int k = 0; k < MAXPER; ++k) {
+        const int j = t + k * lanes;
+        const long i = (long)b * kRateBlock + j;
+        q[k] = (j < kRateBlock && i < nmodes) ? p.Q[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < MAXPER; ++k) {
+        const int j = t + k * lanes;
+        if (j >= kRateBlock) break;
+        const long i = (long)b * kRateBlock + j;
+        bool incl = false;
+        const double r = (i < nmodes) ? mode_rate(p.ph, q[k], incl) : 0.0;
+        dst[j] = incl ? r : excl;
+    }
+}
+
+// Exact left-to-right pass of the reference (kmc_solver.hpp:790-826, :1803-1845):
+// the total in flat order, target = u2 * total, then the first included mode
+// whose running sum reaches the target.  The additions are the only
+// sequential part: wave 0 runs them in flat order over a chunk of kRateBlock
+// rates in LDS (adding 0 for an excluded mode leaves the sum exact) while
+// waves 1-3 evaluate the next chunk's rates into the other LDS buffer.  Pass 1
+// keeps the running sum before every chunk in bsum (free once k_rates_scan
+// has read it); pass 2 restarts at the first chunk whose end reaches the
+// target.  With only_if_ambiguous it runs only when k_select flagged its answer.
 template <bool BATCH>
-__global__ void k_select_seq(const KmcParams p_arg, const KmcParams* __restrict__ PK, int only_if_ambiguous) {
+__global__ __launch_bounds__(256) void k_select_seq(const KmcParams p_arg, const KmcParams* __restrict__ PK,
+                                                    int only_if_ambiguous) {
     const KmcParams& p = param_sel<BATCH>(p_arg, PK);
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
     if (!kc->go || (only_if_ambiguous && !kc->ambiguous)) return;
+    __shared__ double rb[2][kRateBlock];
+    __shared__ double s_tot;
+    __shared__ int s_bs;
+    __shared__ long s_sel;
     const long nmodes = p.g.N * p.ph.M;
+    const int nb = (int)((nmodes + kRateBlock - 1) / kRateBlock);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int G = 16;   // rates read ahead of the adds
+    if (wid > 0) chunk_rates(p, 0, nmodes, 0.0, rb[0], threadIdx.x - 64, 192);
+    __syncthreads();
     double tot = 0.0;
-    long last = -1;
-    for (long i = 0; i < nmodes; ++i) {
-        bool incl;
-        const double r = mode_rate(p.ph, p.Q[i], incl);
-        if (incl) { tot += r; last = i; }
+    for (int b = 0; b < nb; ++b) {
+        if (wid == 0) {
+            if (lane == 0) p.bsum[b] = tot;
+            const double* r = rb[b & 1];
+            double cur[G], nxt[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) cur[k] = r[k];
+            for (int g0 = 0; g0 < kRateBlock; g0 += G) {
+                if (g0 + G < kRateBlock) {
+#pragma unroll
+                    for (int k = 0; k < G; ++k) nxt[k] = r[g0 + G + k];
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) tot += cur[k];
+#pragma unroll
+                for (int k = 0; k < G; ++k) cur[k] = nxt[k];
+            }
+        } else if (b + 1 < nb) {
+            chunk_rates(p, b + 1, nmodes, 0.0, rb[(b + 1) & 1], threadIdx.x - 64, 192);
+        }
+        __syncthreads();
     }
+    if (threadIdx.x == 0) s_tot = tot;
+    __syncthreads();
+    tot = s_tot;
     const double target = kc->u2 * tot;
-    double cur = 0.0;
-    long sel = last;
-    for (long i = 0; i < nmodes; ++i) {
-        bool incl;
-        const double r = mode_rate(p.ph, p.Q[i], incl);
-        if (!incl) continue;
-        cur += r;
-        if (cur >= target) { sel = i; break; }
+    // first chunk whose end (the start of the next, or the total) reaches the target
+    if (wid == 0) {
+        int bs = nb;
+        for (int b = lane; b < nb; b += 64) {
+            const double e = (b + 1 < nb) ? p.bsum[b + 1] : tot;
+            if (e >= target) { bs = b; break; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) bs = min(bs, __shfl_xor(bs, off, 64));
+        // no chunk reaches a target above the total (u2 >= 1): scan everything,
+        // ending at the last included mode
+        if (lane == 0) { s_bs = (bs < nb) ? bs : 0; s_sel = -1; }
     }
-    kc->target = target;
-    kc->sel = (int)sel;
-    kc->ambiguous = 0;
-    kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+    __syncthreads();
+    const int bs = s_bs;
+    double run = (bs < nb) ? p.bsum[bs] : 0.0;
+    long last = -1;
+    for (int b = bs; b < nb; ++b) {
+        chunk_rates(p, b, nmodes, -1.0, rb[0], threadIdx.x, 256);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int j = 0; j < kRateBlock; ++j) {
+                const double r = rb[0][j];
+                if (r < 0.0) continue;   // not included
+                last = (long)b * kRateBlock + j;
+                run += r;
+                if (run >= target) { s_sel = last; break; }
+            }
+            if (s_sel < 0 && b == nb - 1) s_sel = last;   // the reference's default: the last included mode
+        }
+        __syncthreads();
+        if (s_sel >= 0) break;
+    }
+    if (threadIdx.x == 0) {
+        const long sel = s_sel;
+        kc->target = target;
+        kc->sel = (int)sel;
+        kc->ambiguous = 0;
+        kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -607,8 +691,8 @@ void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipS
 }
 
 void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s) {
-    if (l.batch) hipLaunchKernelGGL(k_select_seq<true>, dim3(1, l.B), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
-    else hipLaunchKernelGGL(k_select_seq<false>, dim3(1, 1), dim3(64), 0, s, l.h, l.P, only_if_ambiguous);
+    if (l.batch) hipLaunchKernelGGL(k_select_seq<true>, dim3(1, l.B), dim3(256), 0, s, l.h, l.P, only_if_ambiguous);
+    else hipLaunchKernelGGL(k_select_seq<false>, dim3(1, 1), dim3(256), 0, s, l.h, l.P, only_if_ambiguous);
     MGK_HIP_CHECK(hipGetLastError());
 }
 

@@ -695,7 +695,9 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
 
 // Reduction of pass E's block partials and the control-block update.  A kernel
 // of its own: the kernel boundary makes every block's partials visible, with
-// no per-block device-scope fence (which writes back the XCD's L2).
+// no per-block device-scope fence (which writes back the XCD's L2).  A
+// last-block reduction inside pass E (release fence and ticket per block)
+// measured 12% fewer FP32+refine events/s at 64^3 than this launch.
 template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);

@@ -152,10 +152,13 @@ class Context:
         check(gpu().mgk_update_rates(self._h, float(time), C.byref(tot), C.byref(nu), C.byref(ni)))
         return tot.value, nu.value, ni.value
 
-    def select(self, u2):
+    def select(self, u2, exact=False):
+        """Selected flat mode index and its Q; exact=True runs the reference's
+        left-to-right pass itself (mgk_select_exact)."""
         idx = C.c_int(-1)
         q = C.c_double(0.0)
-        check(gpu().mgk_select(self._h, float(u2), C.byref(idx), C.byref(q)))
+        f = gpu().mgk_select_exact if exact else gpu().mgk_select
+        check(f(self._h, float(u2), C.byref(idx), C.byref(q)))
         return idx.value, q.value
 
     def apply_flip(self, flat, time, new_catalog=None, new_Q0=None, first_mode=0):

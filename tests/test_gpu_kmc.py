@@ -77,6 +77,34 @@ def test_rates_and_selection_fed_oracle_state(dims, directional):
         assert got == want, (u2, got, want)
 
 
+@pytest.mark.parametrize("kw", [
+    dict(nx=48, ny=32, M=7, instability_kmc=1),                                  # 10752 modes: 10.5 chunks
+    dict(nx=48, ny=32, M=7, instability_kmc=0, b_mean=0.3, b_std=0.3, b_min_cutoff=-1.0),   # excluded modes
+    dict(dims=3, nx=16, ny=16, nz=16, M=12, instability_kmc=1),                  # 49152 modes: 48 chunks
+])
+def test_exact_selection_pass_matches_oracle(kw):
+    # the reference's left-to-right selection run as written on the device
+    # (one wave), against the oracle's, over several chunks of 1024 modes
+    c = _cfg(**kw)
+    N = c.nx * c.ny * (c.nz if c.dims == 3 else 1)
+    E, nu = _field(N)
+    osim = po.OracleSim(c, E, nu)
+    osim.elastic_run()
+    ctx = _device_from_oracle(osim, c, E, nu)
+    _, _, t = osim.macro()
+    osim.update_barriers()
+    total_o, rates_o, idx_o = osim.rates()
+    ctx.update_rates(t)
+    assert kw.get("instability_kmc") or len(rates_o) < N * c.M
+    rng = np.random.default_rng(5)
+    us = [0.0, float(np.nextafter(1.0, 0.0))] + list(rng.random(40))
+    for u2 in us:
+        want = idx_o[po.select_sequential(rates_o, u2 * total_o)]
+        got, q = ctx.select(u2, exact=True)
+        assert got == want, (u2, got, want)
+        assert got == ctx.select(u2)[0]
+
+
 def test_selection_on_exact_boundaries():
     # T = 0 and instability_mode "kmc": rates are exactly V*nu0 (Q <= 0) or 0,
     # so prefix sums are exact and targets can sit on a boundary.
