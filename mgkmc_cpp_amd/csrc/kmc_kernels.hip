@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, c
         const long i = base + k * kRateThreads + threadIdx.x;
         ok[k] = i < nmodes;
         mi[k] = ok[k] ? i : 0;
-        vi[k] = mi[k] / ph.M;
+        vi[k] = (long)((unsigned)mi[k] / (unsigned)ph.M);   // N*M < 2^31 (mgk_kmc_init)
         gp[k] = p.soft[vi[k]];
         gt[k] = p.soft[g.N + vi[k]];
         tl[k] = p.last_t[vi[k]];

@@ -816,21 +816,34 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
 #pragma unroll
     for (int c = 0; c < 2 * C; ++c) acc[c] = 0.0;
     const bool absorb = p.corr && p.ctl->corr_valid;   // the solve ended inside an FP32 CG cycle
+    const bool has_epsp = p.epsp != nullptr;
+    // absent optional fields read a valid stand-in, and every load of a voxel
+    // is issued before its first store (the compiler keeps loads behind
+    // stores to fields it cannot prove distinct)
+    const float* src_cr = absorb ? p.corr : reinterpret_cast<const float*>(p.eps);
+    const double* src_ep = has_epsp ? p.epsp : p.eps;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x) {
-        double e[C], ee[C];
+        double e[C], ee[C], ep[C];
+        float cr[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             e[c] = p.eps[c * g.N + v];
+            cr[c] = src_cr[c * g.N + v];
+            ep[c] = src_ep[c * g.N + v];
+        }
+        const double mu_v = p.mu[v], lam_v = p.lam[v];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
             if (absorb) {
-                e[c] += (double)p.corr[c * g.N + v];
+                e[c] += (double)cr[c];
                 p.eps[c * g.N + v] = e[c];
             }
-            ee[c] = p.epsp ? e[c] - p.epsp[c * g.N + v] : e[c];
+            ee[c] = has_epsp ? e[c] - ep[c] : e[c];
         }
         double tre = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) tre += ee[c];
-        const double tm = 2.0 * p.mu[v], lt = p.lam[v] * tre;
+        const double tm = 2.0 * mu_v, lt = lam_v * tre;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const double s = tm * ee[c] + (c < D ? lt : 0.0);
