@@ -282,6 +282,18 @@ int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
 long long mgk_device_bytes(const mgk_ctx* ctx);
 long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_voxel_catalog);
 
+/* Trajectory state snapshot (no counterpart in the reference, whose
+ * output.checkpoint_interval is parsed and ignored, main.cpp:252-258): the
+ * strain, stress and plastic-strain fields, the KMC fields (Q, Q0, catalog,
+ * softening, last event times, previous directions) and the solver's
+ * warm-start and CG-tuning state, as one host buffer of mgk_state_size bytes.
+ * Loading needs a context of the same grid, material (checked by hash) and
+ * mode count; a trajectory continued from a loaded state repeats the saved
+ * one bit for bit.  For checkpoints, preemption and eviction. */
+long long mgk_state_size(const mgk_ctx* ctx);
+int mgk_state_save(mgk_ctx* ctx, void* buf, long long bytes);
+int mgk_state_load(mgk_ctx* ctx, const void* buf, long long bytes);
+
 #ifdef __cplusplus
 }
 #endif

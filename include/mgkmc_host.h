@@ -116,6 +116,18 @@ int  mgk_sim_run(mgk_sim* s, const mgk_run_config* r, mgk_run_stats* stats);
  * solve -- the body of the accepted-event branch of run_simulation without
  * the remaining-time test. */
 int  mgk_sim_event_steps(mgk_sim* s, int n, int drv_i, int drv_j, mgk_run_stats* stats);
+/* Checkpoints (no counterpart in the reference, which parses
+ * output.checkpoint_interval and ignores it, main.cpp:252-258).
+ * mgk_sim_set_checkpoint: the checkpoint_interval and checkpoint_path that
+ * mgk_sim_run passes to run_simulation ("none", "end" or a positive integer
+ * k: a checkpoint after every k-th elastic step; the file is checkpoint_path
+ * under the output directory, ".mgkc" appended, "{step}" replaced by the
+ * elastic step count).  mgk_sim_save / mgk_sim_load write and read one file
+ * now; a run_simulation after a load of a checkpoint it wrote continues where
+ * it stopped, and the trajectory equals the uninterrupted one bit for bit. */
+int  mgk_sim_set_checkpoint(mgk_sim* s, const char* interval, const char* path);
+int  mgk_sim_save(mgk_sim* s, const char* path);
+int  mgk_sim_load(mgk_sim* s, const char* path);
 int  mgk_sim_elastic_run(mgk_sim* s);
 int  mgk_sim_get_field(mgk_sim* s, int which, double* host);
 int  mgk_sim_set_field(mgk_sim* s, int which, const double* host);

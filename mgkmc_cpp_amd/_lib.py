@@ -53,6 +53,9 @@ def gpu():
         _sig(lib, "mgk_update_rates", C.c_int, [P, C.c_double, DP, IP, IP])
         _sig(lib, "mgk_select", C.c_int, [P, C.c_double, IP, DP])
         _sig(lib, "mgk_select_exact", C.c_int, [P, C.c_double, IP, DP])
+        _sig(lib, "mgk_state_size", C.c_longlong, [P])
+        _sig(lib, "mgk_state_save", C.c_int, [P, C.c_void_p, C.c_longlong])
+        _sig(lib, "mgk_state_load", C.c_int, [P, C.c_void_p, C.c_longlong])
         _sig(lib, "mgk_apply_flip", C.c_int, [P, C.c_int, C.c_double, DP, DP, C.c_int, C.c_int])
         _sig(lib, "mgk_find_unstable", C.c_int, [P, IP, DP, C.c_int, IP])
         _sig(lib, "mgk_select_flip", C.c_int, [P, C.c_double, C.c_double, DP, DP, C.c_int, C.c_int])
@@ -91,6 +94,9 @@ def host():
         _sig(lib, "mgk_sim_destroy", C.c_int, [P])
         _sig(lib, "mgk_sim_run", C.c_int, [P, C.POINTER(abi.RunConfig), C.POINTER(abi.RunStats)])
         _sig(lib, "mgk_sim_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
+        _sig(lib, "mgk_sim_set_checkpoint", C.c_int, [P, C.c_char_p, C.c_char_p])
+        _sig(lib, "mgk_sim_save", C.c_int, [P, C.c_char_p])
+        _sig(lib, "mgk_sim_load", C.c_int, [P, C.c_char_p])
         _sig(lib, "mgk_sim_elastic_run", C.c_int, [P])
         _sig(lib, "mgk_sim_get_field", C.c_int, [P, C.c_int, DP])
         _sig(lib, "mgk_sim_set_field", C.c_int, [P, C.c_int, DP])

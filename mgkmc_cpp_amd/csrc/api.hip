@@ -72,6 +72,13 @@ template <typename R> std::vector<R> twiddle_table(int n) {
     return t;
 }
 
+// 64-bit FNV-1a over a byte range, continuing from h.
+unsigned long long fnv1a(const void* data, size_t n, unsigned long long h = 1469598103934665603ull) {
+    const unsigned char* b = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
 void lame(int dims, int plane_mode, double E, double nu, double& lam, double& mu) {
     // compute_lame_2d / compute_lame_3d (linear_elastic_solver.hpp:36-45, :315-319)
     mu = E / (2.0 * (1.0 + nu));
@@ -95,6 +102,7 @@ struct mgk_ctx {
     float *dlam32 = nullptr, *dmu32 = nullptr;
     double lam0 = 0, mu0 = 0, mean_mu = 0;
     std::vector<double> h_mu;
+    unsigned long long material_hash = 0;   // FNV-1a of E and nu, checked when a state is loaded
 
     // strain state (symmetric SoA) + uniform antisymmetric part on the host
     double *eps = nullptr, *sig = nullptr, *epsp = nullptr;
@@ -513,6 +521,7 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             }
             c->mean_mu = msum / N;
             c->h_mu = hm;
+            c->material_hash = fnv1a(nu, N * sizeof(double), fnv1a(E, N * sizeof(double)));
             c->E = c->dal<double>(N);
             c->nu = c->dal<double>(N);
             c->lam = c->dal<double>(N);
@@ -1197,6 +1206,143 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         float ms = 0.f;
         MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         *avg_ms = (double)ms / reps;
+        return MGK_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Trajectory state snapshot: header, then the fields in device order.
+// ---------------------------------------------------------------------------
+namespace {
+struct StateHeader {
+    char magic[8];
+    int version;
+    int dims, nx, ny, nz, C, M;
+    int cat_shared, epsp_set, mean_valid;
+    long long cat_n;                    // doubles of catalog storage
+    unsigned long long material_hash;
+    double W[3];
+    double mean_eps[6];
+    int predicted, inner_per_outer, inner_hist[4];
+    unsigned inner_pos;
+    int last_iters, last_inner;
+};
+const char kStateMagic[8] = {'M', 'G', 'K', 'S', 'T', 'A', 'T', 0};
+
+struct StateArray {
+    void* dev;
+    long long n;   // doubles
+};
+
+std::vector<StateArray> state_arrays(mgk_ctx& c, int M, long long cat_n) {
+    const long long N = c.g.N, C = c.g.C;
+    std::vector<StateArray> a = {{c.eps, C * N}, {c.sig, C * N}, {c.epsp, C * N}};
+    if (M > 0) {
+        a.push_back({c.K.Q, N * M});
+        a.push_back({c.K.Q0, N * M});
+        a.push_back({c.K.cat, cat_n});
+        a.push_back({c.K.soft, 2 * N});
+        a.push_back({c.K.last_t, N});
+        a.push_back({c.K.prev_dir, C * N});
+    }
+    return a;
+}
+
+long long cat_doubles(const mgk_ctx& c) {
+    if (!c.kmc) return 0;
+    return (long long)c.g.C * (c.K.cat_shared ? c.kp.M : c.g.N * c.kp.M);
+}
+}  // namespace
+
+long long mgk_state_size(const mgk_ctx* ctx) {
+    if (!ctx) return -1;
+    mgk_ctx& c = const_cast<mgk_ctx&>(*ctx);
+    long long b = sizeof(StateHeader);
+    for (const StateArray& a : state_arrays(c, c.kmc ? c.kp.M : 0, cat_doubles(c))) b += a.n * (long long)sizeof(double);
+    return b;
+}
+
+int mgk_state_save(mgk_ctx* ctx, void* buf, long long bytes) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || !buf) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (bytes < mgk_state_size(ctx)) throw ArgError("state buffer too small (mgk_state_size)");
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        StateHeader h{};
+        std::memcpy(h.magic, kStateMagic, sizeof(h.magic));
+        h.version = 1;
+        h.dims = c.g.dims; h.nx = c.g.nx; h.ny = c.g.ny; h.nz = c.g.nz; h.C = c.g.C;
+        h.M = c.kmc ? c.kp.M : 0;
+        h.cat_shared = c.kmc ? c.K.cat_shared : 0;
+        h.epsp_set = c.epsp_set ? 1 : 0;
+        h.mean_valid = c.mean_valid ? 1 : 0;
+        h.cat_n = cat_doubles(c);
+        h.material_hash = c.material_hash;
+        for (int k = 0; k < 3; ++k) h.W[k] = c.W[k];
+        for (int k = 0; k < 6; ++k) h.mean_eps[k] = c.mean_eps[k];
+        h.predicted = c.predicted; h.inner_per_outer = c.inner_per_outer;
+        for (int k = 0; k < 4; ++k) h.inner_hist[k] = c.inner_hist[k];
+        h.inner_pos = c.inner_pos; h.last_iters = c.last_iters; h.last_inner = c.last_inner;
+        char* out = static_cast<char*>(buf);
+        std::memcpy(out, &h, sizeof(h));
+        out += sizeof(h);
+        for (const StateArray& a : state_arrays(c, h.M, h.cat_n)) {
+            const size_t nb = (size_t)a.n * sizeof(double);
+            MGK_HIP_CHECK(hipMemcpy(out, a.dev, nb, hipMemcpyDeviceToHost));
+            out += nb;
+        }
+        return MGK_OK;
+    });
+}
+
+int mgk_state_load(mgk_ctx* ctx, const void* buf, long long bytes) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || !buf) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        StateHeader h;
+        if (bytes < (long long)sizeof(h)) throw ArgError("state buffer too small");
+        std::memcpy(&h, buf, sizeof(h));
+        if (std::memcmp(h.magic, kStateMagic, sizeof(h.magic)) != 0 || h.version != 1)
+            throw ArgError("not a trajectory state (bad magic or version)");
+        if (h.dims != c.g.dims || h.nx != c.g.nx || h.ny != c.g.ny || h.nz != c.g.nz || h.C != c.g.C)
+            throw ArgError("state is for another grid");
+        if (h.material_hash != c.material_hash) throw ArgError("state is for another material (E, nu)");
+        if (h.M > 0 && (!c.kmc || c.kp.M != h.M)) throw ArgError("state holds KMC fields for another mode count");
+        if (h.M == 0 && c.kmc) throw ArgError("state holds no KMC fields");
+        const long long want = (long long)c.g.C * (h.cat_shared ? h.M : c.g.N * h.M);
+        if (h.M > 0 && h.cat_n != want) throw ArgError("state catalog size is inconsistent");
+        long long need = sizeof(h);
+        if (h.M > 0) c.ensure_cat((size_t)h.cat_n);
+        const std::vector<StateArray> arrays = state_arrays(c, h.M, h.cat_n);
+        for (const StateArray& a : arrays) need += a.n * (long long)sizeof(double);
+        if (bytes < need) throw ArgError("state buffer truncated");
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        const char* in = static_cast<const char*>(buf) + sizeof(h);
+        for (const StateArray& a : arrays) {
+            const size_t nb = (size_t)a.n * sizeof(double);
+            MGK_HIP_CHECK(hipMemcpy(a.dev, in, nb, hipMemcpyHostToDevice));
+            in += nb;
+        }
+        if (h.M > 0) {
+            c.K.cat_shared = h.cat_shared;
+            if (h.cat_shared) {
+                c.shared_cat.resize((size_t)h.cat_n);
+                MGK_HIP_CHECK(hipMemcpy(c.shared_cat.data(), c.K.cat, (size_t)h.cat_n * sizeof(double), hipMemcpyDeviceToHost));
+            } else {
+                c.shared_cat.clear();
+            }
+        }
+        c.epsp_set = h.epsp_set != 0;
+        c.mean_valid = h.mean_valid != 0;
+        for (int k = 0; k < 3; ++k) c.W[k] = h.W[k];
+        for (int k = 0; k < 6; ++k) c.mean_eps[k] = h.mean_eps[k];
+        c.predicted = h.predicted; c.inner_per_outer = h.inner_per_outer;
+        for (int k = 0; k < 4; ++k) c.inner_hist[k] = h.inner_hist[k];
+        c.inner_pos = h.inner_pos; c.last_iters = h.last_iters; c.last_inner = h.last_inner;
+        c.kctl_pending = false;
+        MGK_HIP_CHECK(hipDeviceSynchronize());
         return MGK_OK;
     });
 }

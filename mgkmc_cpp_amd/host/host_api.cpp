@@ -54,6 +54,7 @@ BarrierGenerator barrier_from(const mgk_sim_config& c) {
 
 struct mgk_sim {
     int dims = 2;
+    std::string checkpoint_interval = "none", checkpoint_path = "checkpoint";
     std::unique_ptr<KmcSimulation2D> s2;
     std::unique_ptr<KmcSimulation3D> s3;
     template <typename F> auto with(F&& f) { return dims == 2 ? f(*s2) : f(*s3); }
@@ -181,9 +182,41 @@ int mgk_sim_run(mgk_sim* s, const mgk_run_config* r, mgk_run_stats* stats) {
         return s->with([&](auto& sim) {
             sim.max_events = r->max_events;
             sim.run_simulation(r->n_global_steps, r->step_size, {r->drv_i, r->drv_j}, targets, r->mixed_tol,
-                               r->mixed_max_iter, "none", "checkpoint", "none", true, false, false,
+                               r->mixed_max_iter, s->checkpoint_interval, s->checkpoint_path, "none", true, false, false,
                                "summary_log.txt", false, false, r->max_kmc_steps_pct);
             fill_stats(sim.counters, sim.time, stats);
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_set_checkpoint(mgk_sim* s, const char* interval, const char* path) {
+    return guard([&]() -> int {
+        if (!s || !interval || !path) throw std::runtime_error("null argument");
+        return s->with([&](auto& sim) {
+            sim.checkpoint_every(interval);   // validates
+            s->checkpoint_interval = interval;
+            s->checkpoint_path = path;
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_save(mgk_sim* s, const char* path) {
+    return guard([&]() -> int {
+        if (!s || !path) throw std::runtime_error("null argument");
+        return s->with([&](auto& sim) {
+            sim.save_checkpoint(path);
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_load(mgk_sim* s, const char* path) {
+    return guard([&]() -> int {
+        if (!s || !path) throw std::runtime_error("null argument");
+        return s->with([&](auto& sim) {
+            sim.load_checkpoint(path);
             return MGK_OK;
         });
     });

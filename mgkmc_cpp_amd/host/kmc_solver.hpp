@@ -644,6 +644,160 @@ public:
         return complete_event(d, out, is_instab, t_wait_out);
     }
 
+    // --- checkpoint and resume (no counterpart in the reference: main.cpp:252-258
+    // parses output.checkpoint_interval and ignores it) -----------------------
+    // A checkpoint holds the device state (mgk_state_save), the host state the
+    // trajectory depends on (time, macros, the mt19937 stream, counters, the
+    // event history) and the loop position of run_simulation.  run_simulation
+    // continues a loaded checkpoint where it was written, bit for bit.
+    struct LoopState {
+        bool valid = false;
+        int elastic_steps_done = 0, total_kmc_steps = 0, step = 1, seq = 0;
+        bool mid_step = false;    // written inside a strain step (event cap, termination)
+        double remaining = 0.0;   // time left in that strain step
+    };
+    LoopState resume_;
+
+    void save_checkpoint(const std::string& path, const LoopState& at = LoopState{}) {
+        const long long nb = mgk_state_size(ctx);
+        if (nb < 0) throw std::runtime_error("mgk_state_size failed");
+        std::vector<char> dev((size_t)nb);
+        check(mgk_state_save(ctx, dev.data(), nb), "mgk_state_save");
+        std::ostringstream rs;
+        rs << rng;
+        const std::string rng_text = rs.str();
+        const std::string tmp = path + ".tmp";
+        const std::filesystem::path dir = std::filesystem::path(path).parent_path();
+        if (!dir.empty()) std::filesystem::create_directories(dir);
+        {
+            std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+            if (!f) throw std::runtime_error("cannot write checkpoint " + tmp);
+            auto put = [&](const void* p, size_t n) { f.write(static_cast<const char*>(p), (std::streamsize)n); };
+            const char magic[8] = {'M', 'G', 'K', 'C', 'H', 'K', 'P', 'T'};
+            const int hdr[8] = {1, D, nx, ny, nz, M, at.valid ? 1 : 0, flips_since_sync};
+            put(magic, 8);
+            put(hdr, sizeof(hdr));
+            const int loop[5] = {at.elastic_steps_done, at.total_kmc_steps, at.step, at.seq, at.mid_step ? 1 : 0};
+            put(loop, sizeof(loop));
+            put(&at.remaining, sizeof(double));
+            put(&time, sizeof(time));
+            put(eps_macro.data(), sizeof(double) * D * D);
+            put(sig_macro.data(), sizeof(double) * D * D);
+            const long long cnt[7] = {counters.kmc_events, counters.cascades, counters.cascade_flips,
+                                      counters.elastic_steps, counters.ls_solves, counters.ls_iters,
+                                      counters.ls_iters_fp32};
+            const int flags[2] = {counters.terminated ? 1 : 0, counters.event_cap_reached ? 1 : 0};
+            put(cnt, sizeof(cnt));
+            put(flags, sizeof(flags));
+            const long long nr = (long long)rng_text.size();
+            put(&nr, sizeof(nr));
+            put(rng_text.data(), rng_text.size());
+            const long long ne = (long long)events.size();
+            put(&ne, sizeof(ne));
+            if (ne) put(events.data(), sizeof(EventRecord) * events.size());
+            put(&nb, sizeof(nb));
+            put(dev.data(), dev.size());
+            if (!f) throw std::runtime_error("write error on checkpoint " + tmp);
+        }
+        std::filesystem::rename(tmp, path);
+    }
+
+    void load_checkpoint(const std::string& path) {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw std::runtime_error("cannot read checkpoint " + path);
+        auto get = [&](void* p, size_t n) {
+            f.read(static_cast<char*>(p), (std::streamsize)n);
+            if (!f) throw std::runtime_error("truncated checkpoint " + path);
+        };
+        char magic[8];
+        int hdr[8], loop[5];
+        double rem = 0.0;
+        get(magic, 8);
+        if (std::memcmp(magic, "MGKCHKPT", 8) != 0) throw std::runtime_error(path + " is not a checkpoint");
+        get(hdr, sizeof(hdr));
+        if (hdr[0] != 1 || hdr[1] != D || hdr[2] != nx || hdr[3] != ny || hdr[4] != nz || hdr[5] != M)
+            throw std::runtime_error("checkpoint " + path + " is for another version, grid or mode count");
+        get(loop, sizeof(loop));
+        get(&rem, sizeof(rem));
+        double t;
+        Mat<D> em, sm;
+        get(&t, sizeof(t));
+        get(em.data(), sizeof(double) * D * D);
+        get(sm.data(), sizeof(double) * D * D);
+        long long cnt[7];
+        int flags[2];
+        get(cnt, sizeof(cnt));
+        get(flags, sizeof(flags));
+        long long nr = 0;
+        get(&nr, sizeof(nr));
+        if (nr < 0 || nr > (1 << 20)) throw std::runtime_error("corrupt checkpoint " + path);
+        std::string rng_text((size_t)nr, '\0');
+        get(&rng_text[0], (size_t)nr);
+        long long ne = 0;
+        get(&ne, sizeof(ne));
+        if (ne < 0) throw std::runtime_error("corrupt checkpoint " + path);
+        std::vector<EventRecord> ev((size_t)ne);
+        if (ne) get(ev.data(), sizeof(EventRecord) * ev.size());
+        long long nb = 0;
+        get(&nb, sizeof(nb));
+        if (nb <= 0) throw std::runtime_error("corrupt checkpoint " + path);
+        std::vector<char> dev((size_t)nb);
+        get(dev.data(), dev.size());
+        check(mgk_state_load(ctx, dev.data(), nb), "mgk_state_load");
+        std::istringstream rs(rng_text);
+        rs >> rng;
+        if (!rs) throw std::runtime_error("corrupt generator state in " + path);
+        time = t;
+        eps_macro = em;
+        sig_macro = sm;
+        flips_since_sync = hdr[7];
+        counters.kmc_events = cnt[0]; counters.cascades = cnt[1]; counters.cascade_flips = cnt[2];
+        counters.elastic_steps = cnt[3]; counters.ls_solves = cnt[4]; counters.ls_iters = cnt[5];
+        counters.ls_iters_fp32 = cnt[6];
+        counters.terminated = flags[0] != 0;
+        counters.event_cap_reached = flags[1] != 0;
+        events = std::move(ev);
+        resume_ = LoopState{};
+        if (hdr[6]) {
+            resume_.valid = true;
+            resume_.elastic_steps_done = loop[0];
+            resume_.total_kmc_steps = loop[1];
+            resume_.step = loop[2];
+            resume_.seq = loop[3];
+            resume_.mid_step = loop[4] != 0;
+            resume_.remaining = rem;
+        }
+        sync_from_device();
+    }
+
+    // The file a checkpoint of run_simulation goes to: checkpoint_path under
+    // output_dir (unless absolute), ".mgkc" appended, "{step}" replaced by the
+    // elastic step count.
+    std::string checkpoint_file(const std::string& checkpoint_path, int elastic_steps_done) const {
+        std::string name = checkpoint_path;
+        const size_t k = name.find("{step}");
+        if (k != std::string::npos) name.replace(k, 6, std::to_string(elastic_steps_done));
+        std::filesystem::path p(name + ".mgkc");
+        if (p.is_relative()) p = std::filesystem::path(output_dir) / p;
+        return p.string();
+    }
+
+    // checkpoint_interval: "none" (default), a positive integer k (a checkpoint
+    // after every k-th elastic step) or "end" (one when the run returns).
+    static int checkpoint_every(const std::string& interval) {
+        std::string v = interval;
+        std::transform(v.begin(), v.end(), v.begin(), ::tolower);
+        if (v == "none" || v.empty()) return 0;
+        if (v == "end") return -1;
+        try {
+            size_t used = 0;
+            const int k = std::stoi(v, &used);
+            if (used == v.size() && k > 0) return k;
+        } catch (...) {
+        }
+        throw std::runtime_error("checkpoint_interval must be 'none', 'end' or a positive integer: " + interval);
+    }
+
     // run_simulation (kmc_solver.hpp:721-1037, :1735-2062), fast patching off.
     void run_simulation(int n_global_steps, double step_size, std::pair<int, int> component,
                         const std::map<std::pair<int, int>, double>& stress_targets, double mixed_tol = 1e-4,
@@ -652,27 +806,54 @@ public:
                         bool vtk_elastic_only = true, bool track_cascades = false, bool enable_console_log = true,
                         std::string summary_filename = "summary_log.txt", bool enable_summary_log = true,
                         bool enable_global_log = true, double max_kmc_steps_pct = 0.3) {
-        (void)checkpoint_interval; (void)checkpoint_path; (void)vtk_elastic_only; (void)track_cascades;
+        (void)vtk_elastic_only; (void)track_cascades;
         if (vtk_interval != "none") std::cerr << " [mgkmc] VTU export is not built on the device path; skipped\n";
+        const int ck_every = checkpoint_every(checkpoint_interval);
         init_logs(summary_filename, enable_summary_log, enable_global_log);
         const auto start = std::chrono::high_resolution_clock::now();
-        counters = RunCounters{};
         int elastic_steps_done = 0, total_kmc_steps = 0, step = 1, seq = 0;
+        double remaining = 0.0;
+        bool mid_step = false;
+        const bool resumed = resume_.valid;
+        if (resumed) {
+            // a loaded checkpoint: its counters, loop position and solved state
+            elastic_steps_done = resume_.elastic_steps_done;
+            total_kmc_steps = resume_.total_kmc_steps;
+            step = resume_.step;
+            seq = resume_.seq;
+            mid_step = resume_.mid_step;
+            remaining = resume_.remaining;
+            resume_ = LoopState{};
+        } else {
+            counters = RunCounters{};
+        }
         const int max_seq = static_cast<int>(max_kmc_steps_pct * N());
-        elastic_run();
+        if (!resumed) elastic_run();
         Mat<D> su = Mat<D>::Zero();
         su(component.first, component.second) = 1.0;
         const double dt_step = std::abs(step_size) / strain_rate;
-        double remaining = 0.0;
         std::uniform_real_distribution<double> uniform_dist(0.0, 1.0);
-        auto finish = [&]() {
+        auto loop_state = [&](bool inside) {
+            LoopState l;
+            l.valid = true;
+            l.elastic_steps_done = elastic_steps_done;
+            l.total_kmc_steps = total_kmc_steps;
+            l.step = step;
+            l.seq = seq;
+            l.mid_step = inside;
+            l.remaining = inside ? remaining : 0.0;
+            return l;
+        };
+        auto finish = [&](bool inside) {
             counters.kmc_events = total_kmc_steps;
             counters.elastic_steps = elastic_steps_done;
             sync_from_device();
             close_logs();
+            if (ck_every < 0) save_checkpoint(checkpoint_file(checkpoint_path, elastic_steps_done), loop_state(inside));
         };
         while (elastic_steps_done < n_global_steps) {
-            remaining += dt_step;
+            if (!mid_step) remaining += dt_step;
+            mid_step = false;
             while (remaining > 0.0) {
                 int inst = 0;
                 double t_wait = 0.0;
@@ -695,7 +876,7 @@ public:
                         std::cout << "\n[TERMINATE] KMC instability sequence limit reached! " << seq
                                   << " steps." << std::endl;
                         counters.terminated = true;
-                        finish();
+                        finish(true);
                         return;
                     }
                     do_logging(step, elastic_steps_done, total_kmc_steps, inst ? "KMC_INSTAB" : "KMC", 0, 1,
@@ -704,7 +885,7 @@ public:
                     step++;
                     if (max_events > 0 && total_kmc_steps >= max_events) {
                         counters.event_cap_reached = true;
-                        finish();
+                        finish(true);
                         return;
                     }
                     continue;
@@ -738,8 +919,13 @@ public:
             do_logging(step, elastic_steps_done, total_kmc_steps, "ELASTIC", 0, 0, component, enable_console_log,
                        start);
             step++;
+            if (ck_every > 0 && elastic_steps_done % ck_every == 0) {
+                counters.kmc_events = total_kmc_steps;
+                counters.elastic_steps = elastic_steps_done;
+                save_checkpoint(checkpoint_file(checkpoint_path, elastic_steps_done), loop_state(false));
+            }
         }
-        finish();
+        finish(false);
         const double dur = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - start).count();
         if (enable_console_log) std::cout << "\nTotal Duration: " << std::fixed << std::setprecision(2) << dur << " seconds\n";
     }

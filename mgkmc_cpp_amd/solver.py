@@ -40,6 +40,18 @@ def _mat_from(buf, D):
     return np.asarray(buf, dtype=np.float64).reshape(D, D).T.copy()
 
 
+def _state_save(h):
+    n = int(gpu().mgk_state_size(h))
+    buf = C.create_string_buffer(n)
+    check(gpu().mgk_state_save(h, buf, n))
+    return buf.raw
+
+
+def _state_load(h, data):
+    buf = C.create_string_buffer(bytes(data), len(data))
+    check(gpu().mgk_state_load(h, buf, len(data)))
+
+
 class Context:
     """One trajectory's device state (mgk_ctx)."""
 
@@ -252,6 +264,13 @@ class Context:
     def device_bytes(self):
         return int(gpu().mgk_device_bytes(self._h))
 
+    def state_save(self):
+        """The trajectory state (mgk_state_save) as bytes."""
+        return _state_save(self._h)
+
+    def state_load(self, data):
+        _state_load(self._h, data)
+
     def last_solve_stats(self):
         a = C.c_int(0)
         b = C.c_int(0)
@@ -310,6 +329,17 @@ class Simulation:
         check_host(host().mgk_sim_run(self._h, C.byref(run_config), C.byref(st)))
         return st
 
+    def set_checkpoint(self, interval, path):
+        """checkpoint_interval ("none", "end" or k) and checkpoint_path of run()."""
+        check_host(host().mgk_sim_set_checkpoint(self._h, str(interval).encode(), str(path).encode()))
+
+    def save(self, path):
+        check_host(host().mgk_sim_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        """Restore a checkpoint; a following run() continues where it was written."""
+        check_host(host().mgk_sim_load(self._h, str(path).encode()))
+
     def event_steps(self, n, drv=(0, 0)):
         st = abi.RunStats()
         check_host(host().mgk_sim_event_steps(self._h, int(n), drv[0], drv[1], C.byref(st)))
@@ -361,6 +391,14 @@ class Simulation:
     def device_bytes(self):
         """Device bytes this simulation's context holds."""
         return int(gpu().mgk_device_bytes(self.device_context_handle()))
+
+    def state_save(self):
+        """The device state of the trajectory (mgk_state_save) as bytes; the
+        host state (generator, macros, time) is in save()/load() files."""
+        return _state_save(self.device_context_handle())
+
+    def state_load(self, data):
+        _state_load(self.device_context_handle(), data)
 
     def time_ls_pass(self, pass_id, fp32=False, reps=20):
         ms = C.c_double(0.0)
