@@ -126,6 +126,21 @@ int  mgk_sim_event_steps(mgk_sim* s, int n, int drv_i, int drv_j, mgk_run_stats*
  * now; a run_simulation after a load of a checkpoint it wrote continues where
  * it stopped, and the trajectory equals the uninterrupted one bit for bit. */
 int  mgk_sim_set_checkpoint(mgk_sim* s, const char* interval, const char* path);
+/* Outputs of mgk_sim_run (off by default): run_simulation's output directory,
+ * vtk_interval and vtk_elastic_only (kmc_solver.hpp:721-729; "none",
+ * "current", "last" or an interval), and the summary and global logs
+ * (do_logging, :566-719).  VTU files are written by a host thread from a copy
+ * of the fields. */
+int  mgk_sim_set_output(mgk_sim* s, const char* output_dir, const char* vtk_interval, int vtk_elastic_only,
+                        int enable_summary_log, int enable_global_log);
+/* The current fields as one VTU file (export_to_vtu[_2d], vtu_writer.hpp:169-393). */
+int  mgk_sim_export_vtu(mgk_sim* s, const char* path);
+/* export_to_vtu / export_to_vtu_2d on host arrays (no device): eps, sig and
+ * eps_plastic (nullable) N x D x D column-major per cell, E and nu N values,
+ * soft (nullable) N x 2; voxel order x*ny*nz + y*nz + z. */
+int  mgk_vtu_write(const char* path, int dims, int nx, int ny, int nz, double pixel, const double* eps,
+                   const double* sig, const double* E, const double* nu, const double* eps_plastic,
+                   const double* soft);
 int  mgk_sim_save(mgk_sim* s, const char* path);
 int  mgk_sim_load(mgk_sim* s, const char* path);
 int  mgk_sim_elastic_run(mgk_sim* s);

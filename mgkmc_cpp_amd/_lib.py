@@ -96,6 +96,10 @@ def host():
         _sig(lib, "mgk_sim_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
         _sig(lib, "mgk_sim_set_checkpoint", C.c_int, [P, C.c_char_p, C.c_char_p])
         _sig(lib, "mgk_sim_save", C.c_int, [P, C.c_char_p])
+        _sig(lib, "mgk_sim_set_output", C.c_int, [P, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "mgk_sim_export_vtu", C.c_int, [P, C.c_char_p])
+        _sig(lib, "mgk_vtu_write", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, DP, DP, DP, DP,
+                                             DP, DP])
         _sig(lib, "mgk_sim_load", C.c_int, [P, C.c_char_p])
         _sig(lib, "mgk_sim_elastic_run", C.c_int, [P])
         _sig(lib, "mgk_sim_get_field", C.c_int, [P, C.c_int, DP])

@@ -56,7 +56,7 @@ def build(verbose=False):
     os.replace(tmp, gpu_so)
     host_so = os.path.join(LIB, "libmgkmc_host.so")
     tmp = host_so + ".tmp"
-    _run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
+    _run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-Wno-unused-function",
           os.path.join(HOST, "host_api.cpp"), "-o", tmp,
           "-L" + LIB, "-lmgkmc_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, host_so)

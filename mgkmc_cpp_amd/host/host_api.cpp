@@ -55,6 +55,8 @@ BarrierGenerator barrier_from(const mgk_sim_config& c) {
 struct mgk_sim {
     int dims = 2;
     std::string checkpoint_interval = "none", checkpoint_path = "checkpoint";
+    std::string vtk_interval = "none", summary_filename = "summary_log.txt";
+    bool vtk_elastic_only = true, enable_summary_log = false, enable_global_log = false;
     std::unique_ptr<KmcSimulation2D> s2;
     std::unique_ptr<KmcSimulation3D> s3;
     template <typename F> auto with(F&& f) { return dims == 2 ? f(*s2) : f(*s3); }
@@ -182,8 +184,9 @@ int mgk_sim_run(mgk_sim* s, const mgk_run_config* r, mgk_run_stats* stats) {
         return s->with([&](auto& sim) {
             sim.max_events = r->max_events;
             sim.run_simulation(r->n_global_steps, r->step_size, {r->drv_i, r->drv_j}, targets, r->mixed_tol,
-                               r->mixed_max_iter, s->checkpoint_interval, s->checkpoint_path, "none", true, false, false,
-                               "summary_log.txt", false, false, r->max_kmc_steps_pct);
+                               r->mixed_max_iter, s->checkpoint_interval, s->checkpoint_path, s->vtk_interval,
+                               s->vtk_elastic_only, false, false, s->summary_filename, s->enable_summary_log,
+                               s->enable_global_log, r->max_kmc_steps_pct);
             fill_stats(sim.counters, sim.time, stats);
             return MGK_OK;
         });
@@ -199,6 +202,65 @@ int mgk_sim_set_checkpoint(mgk_sim* s, const char* interval, const char* path) {
             s->checkpoint_path = path;
             return MGK_OK;
         });
+    });
+}
+
+int mgk_sim_set_output(mgk_sim* s, const char* output_dir, const char* vtk_interval, int vtk_elastic_only,
+                       int enable_summary_log, int enable_global_log) {
+    return guard([&]() -> int {
+        if (!s || !output_dir || !vtk_interval) throw std::runtime_error("null argument");
+        return s->with([&](auto& sim) {
+            sim.output_dir = output_dir;
+            s->vtk_interval = vtk_interval;
+            s->vtk_elastic_only = vtk_elastic_only != 0;
+            s->enable_summary_log = enable_summary_log != 0;
+            s->enable_global_log = enable_global_log != 0;
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_sim_export_vtu(mgk_sim* s, const char* path) {
+    return guard([&]() -> int {
+        if (!s || !path) throw std::runtime_error("null argument");
+        return s->with([&](auto& sim) {
+            sim.export_vtu(path);
+            sim.wait_vtu();
+            return MGK_OK;
+        });
+    });
+}
+
+int mgk_vtu_write(const char* path, int dims, int nx, int ny, int nz, double pixel, const double* eps,
+                  const double* sig, const double* E, const double* nu, const double* eps_plastic,
+                  const double* soft) {
+    return guard([&]() -> int {
+        if (!path || !eps || !sig || !E || !nu) throw std::runtime_error("null argument");
+        if (dims != 2 && dims != 3) throw std::runtime_error("dims must be 2 or 3");
+        if (dims == 2) nz = 1;
+        const long N = (long)nx * ny * nz;
+        std::vector<double> Ev(E, E + N), nuv(nu, nu + N);
+        std::vector<mgkt::Vector2d> sp;
+        if (soft) {
+            sp.resize(N);
+            std::memcpy(sp.data(), soft, sizeof(double) * 2 * N);
+        }
+        auto tensors = [N](const double* a, auto tag) {
+            using T = decltype(tag);
+            std::vector<T> v;
+            if (a) {
+                v.resize(N);
+                std::memcpy(v.data(), a, sizeof(T) * N);
+            }
+            return v;
+        };
+        if (dims == 2)
+            export_to_vtu_2d(path, nx, ny, tensors(eps, mgkt::Matrix2d{}), tensors(sig, mgkt::Matrix2d{}), Ev, nuv,
+                             pixel, {}, tensors(eps_plastic, mgkt::Matrix2d{}), sp);
+        else
+            export_to_vtu(path, nx, ny, nz, tensors(eps, mgkt::Matrix3d{}), tensors(sig, mgkt::Matrix3d{}), Ev, nuv,
+                          pixel, {}, tensors(eps_plastic, mgkt::Matrix3d{}), sp);
+        return MGK_OK;
     });
 }
 

@@ -34,11 +34,13 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mgkmc_gpu.h"
 #include "linear_elastic_solver.hpp"
 #include "mgk_tensor.hpp"
+#include "vtu_writer.hpp"
 
 std::string get_current_timestamp();
 
@@ -455,14 +457,13 @@ public:
         return {local_step, total_flips};
     }
 
+    // init_logs (kmc_solver.hpp:566-602): the summary and global log headers.
     void init_logs(std::string summary_filename, bool enable_summary_log, bool enable_global_log) {
         if (!enable_summary_log && !enable_global_log) return;
         std::filesystem::create_directories(output_dir);
         if (enable_summary_log) {
             f_summary.open(output_dir + "/" + summary_filename);
-            std::string header =
-                "Timestamp Elapsed(s) Step Type Eps_xx Sig_xx(GPa) KMC Cascade Flips SimTime(s)\n";
-            f_summary << header << std::string(header.length() - 1, '-') << "\n";
+            f_summary << kSummaryHeader << std::string(std::strlen(kSummaryHeader) - 1, '-') << "\n";
         }
         if (enable_global_log) {
             f_global.open(output_dir + "/global_log.txt");
@@ -478,16 +479,91 @@ public:
         }
     }
     void close_logs() {
+        wait_vtu();
         if (f_summary.is_open()) f_summary.close();
         if (f_global.is_open()) f_global.close();
     }
 
-    // do_logging (kmc_solver.hpp:606-719): the summary and global lines; VTU
-    // export is not built on the device path.
+    static constexpr const char* kSummaryHeader =
+        "Timestamp              Elapsed(s)   Step     Type            Eps_xx       Sig_xx(GPa)     KMC      "
+        "Cascade  Flips    SimTime(s)\n";
+
+    // --- VTU output ----------------------------------------------------------
+    // The fields are copied from the device into the host mirrors, and a
+    // writer thread formats the file from a snapshot of them, so the run goes
+    // on while the ASCII is written (at most two files in flight).
+    std::vector<std::thread> vtu_jobs_;
+    std::string vtu_error_;
+
+    void export_vtu(const std::string& name) {
+        check(mgk_get_field(ctx, MGK_FIELD_EPS, eps_field[0].data()), "get eps");
+        check(mgk_get_field(ctx, MGK_FIELD_SIG, sig_field[0].data()), "get sig");
+        check(mgk_get_field(ctx, MGK_FIELD_EPS_PLASTIC, eps_plastic[0].data()), "get eps_plastic");
+        check(mgk_get_field(ctx, MGK_FIELD_SOFT, soft_prop[0].data()), "get soft_prop");
+        while (vtu_jobs_.size() >= 2) {
+            vtu_jobs_.front().join();
+            vtu_jobs_.erase(vtu_jobs_.begin());
+        }
+        vtu_jobs_.emplace_back([this, name, e = eps_field, s = sig_field, ep = eps_plastic, sp = soft_prop,
+                                E = E_field, nu = nu_field]() {
+            try {
+                if constexpr (D == 2) export_to_vtu_2d(name, nx, ny, e, s, E, nu, pixel, {}, ep, sp);
+                else export_to_vtu(name, nx, ny, nz, e, s, E, nu, pixel, {}, ep, sp);
+            } catch (const std::exception& ex) {
+                vtu_error_ = ex.what();
+            }
+        });
+    }
+    void wait_vtu() {
+        for (auto& t : vtu_jobs_) t.join();
+        vtu_jobs_.clear();
+        if (!vtu_error_.empty()) {
+            const std::string e = vtu_error_;
+            vtu_error_.clear();
+            throw std::runtime_error(e);
+        }
+    }
+    std::string vtu_step_name(int step, const char* suffix = "") const {
+        std::ostringstream ss;
+        ss << output_dir << "/vtk_step_" << std::setfill('0') << std::setw(6) << step << suffix << ".vtu";
+        return ss.str();
+    }
+
+    // do_logging (kmc_solver.hpp:606-719, 3D :1630-1733): the summary and
+    // global lines and the VTU files vtk_interval_str asks for ("none",
+    // "current", "last" or an interval of logged steps, counted in elastic
+    // steps when vtk_elastic_only).
     void do_logging(int global_step, int elastic_steps_done, int total_kmc_steps, std::string step_type,
                     int cascade_steps, int cascade_flips, std::pair<int, int> component, bool enable_console_log,
-                    std::chrono::high_resolution_clock::time_point start_time_total) {
-        if (step_type == "INIT") return;
+                    std::chrono::high_resolution_clock::time_point start_time_total,
+                    std::string vtk_interval_str = "none", bool vtk_elastic_only = true) {
+        if (step_type == "INIT") {
+            if (vtk_interval_str == "current") export_vtu(output_dir + "/step.vtu");
+            else if (vtk_interval_str != "none" && vtk_interval_str != "last") export_vtu(vtu_step_name(0));
+            return;
+        }
+        log_lines(global_step, elastic_steps_done, total_kmc_steps, step_type, cascade_steps, cascade_flips,
+                  component, enable_console_log, start_time_total);
+        if (vtk_interval_str == "none") return;
+        if (vtk_interval_str == "current") {
+            export_vtu(output_dir + "/step.vtu");
+            return;
+        }
+        int interval = 0;
+        try {
+            interval = std::stoi(vtk_interval_str);
+        } catch (...) {
+            return;
+        }
+        if (interval <= 0) return;
+        const bool due = vtk_elastic_only ? (step_type == "ELASTIC" && elastic_steps_done % interval == 0)
+                                          : (global_step % interval == 0);
+        if (due) export_vtu(vtu_step_name(global_step));
+    }
+
+    void log_lines(int global_step, int elastic_steps_done, int total_kmc_steps, const std::string& step_type,
+                   int cascade_steps, int cascade_flips, std::pair<int, int> component, bool enable_console_log,
+                   std::chrono::high_resolution_clock::time_point start_time_total) {
         if (!f_summary.is_open() && !f_global.is_open() && !enable_console_log) return;
         const double elapsed =
             std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - start_time_total).count();
@@ -806,8 +882,7 @@ public:
                         bool vtk_elastic_only = true, bool track_cascades = false, bool enable_console_log = true,
                         std::string summary_filename = "summary_log.txt", bool enable_summary_log = true,
                         bool enable_global_log = true, double max_kmc_steps_pct = 0.3) {
-        (void)vtk_elastic_only; (void)track_cascades;
-        if (vtk_interval != "none") std::cerr << " [mgkmc] VTU export is not built on the device path; skipped\n";
+        (void)track_cascades;
         const int ck_every = checkpoint_every(checkpoint_interval);
         init_logs(summary_filename, enable_summary_log, enable_global_log);
         const auto start = std::chrono::high_resolution_clock::now();
@@ -828,7 +903,13 @@ public:
             counters = RunCounters{};
         }
         const int max_seq = static_cast<int>(max_kmc_steps_pct * N());
-        if (!resumed) elastic_run();
+        if (!vtk_interval.empty() && vtk_interval != "none") std::filesystem::create_directories(output_dir);
+        if (!resumed) {
+            elastic_run();
+            if (enable_console_log) std::cout << kSummaryHeader << std::string(std::strlen(kSummaryHeader) - 1, '-') << "\n";
+            do_logging(0, elastic_steps_done, total_kmc_steps, "INIT", 0, 0, component, enable_console_log, start,
+                       vtk_interval, vtk_elastic_only);
+        }
         Mat<D> su = Mat<D>::Zero();
         su(component.first, component.second) = 1.0;
         const double dt_step = std::abs(step_size) / strain_rate;
@@ -864,7 +945,7 @@ public:
                         counters.cascades++;
                         counters.cascade_flips += cs.second;
                         do_logging(step, elastic_steps_done, total_kmc_steps, "CASCADE", cs.first, cs.second,
-                                   component, enable_console_log, start);
+                                   component, enable_console_log, start, vtk_interval, vtk_elastic_only);
                         step++;
                     }
                     continue;
@@ -880,7 +961,7 @@ public:
                         return;
                     }
                     do_logging(step, elastic_steps_done, total_kmc_steps, inst ? "KMC_INSTAB" : "KMC", 0, 1,
-                               component, enable_console_log, start);
+                               component, enable_console_log, start, vtk_interval, vtk_elastic_only);
                     total_kmc_steps++;
                     step++;
                     if (max_events > 0 && total_kmc_steps >= max_events) {
@@ -917,7 +998,7 @@ public:
             }
             elastic_steps_done++;
             do_logging(step, elastic_steps_done, total_kmc_steps, "ELASTIC", 0, 0, component, enable_console_log,
-                       start);
+                       start, vtk_interval, vtk_elastic_only);
             step++;
             if (ck_every > 0 && elastic_steps_done % ck_every == 0) {
                 counters.kmc_events = total_kmc_steps;
@@ -925,9 +1006,29 @@ public:
                 save_checkpoint(checkpoint_file(checkpoint_path, elastic_steps_done), loop_state(false));
             }
         }
-        finish(false);
+        // the last VTU unless the interval already wrote it (:998-1015)
+        bool final_vtu = (vtk_interval == "last");
+        if (!final_vtu && vtk_interval != "none") {
+            try {
+                const int k = std::stoi(vtk_interval);
+                final_vtu = k > 0 && (step - 1) % k != 0;
+            } catch (...) {
+            }
+        }
+        if (final_vtu) export_vtu(vtu_step_name(step - 1, "_final"));
+        // run time (:1017-1035)
         const double dur = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - start).count();
-        if (enable_console_log) std::cout << "\nTotal Duration: " << std::fixed << std::setprecision(2) << dur << " seconds\n";
+        const int h = static_cast<int>(dur / 3600.0);
+        const int m = static_cast<int>((dur - h * 3600.0) / 60.0);
+        const double sec = dur - h * 3600.0 - m * 60.0;
+        std::ostringstream fin;
+        fin << "\nSimulation Finish Time: " << get_current_timestamp() << "\n"
+            << "Total Duration: " << std::fixed << std::setprecision(2) << dur << " seconds (" << h << "h "
+            << std::setw(2) << std::setfill('0') << m << "m " << std::setw(2) << std::setfill('0')
+            << static_cast<int>(sec) << "s)\n";
+        if (f_summary.is_open()) f_summary << fin.str();
+        if (enable_console_log) std::cout << fin.str();
+        finish(false);
     }
 
     // Forced event steps: the accepted-event body without the remaining-time

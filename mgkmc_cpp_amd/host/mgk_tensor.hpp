@@ -58,9 +58,11 @@ template <int R, int C> struct Matrix {
 using Matrix2d = Matrix<2, 2>;
 using Matrix3d = Matrix<3, 3>;
 using Vector2d = Matrix<2, 1>;
+using Vector3d = Matrix<3, 1>;
 
 static_assert(sizeof(Matrix2d) == 4 * sizeof(double), "layout");
 static_assert(sizeof(Matrix3d) == 9 * sizeof(double), "layout");
 static_assert(sizeof(Vector2d) == 2 * sizeof(double), "layout");
+static_assert(sizeof(Vector3d) == 3 * sizeof(double), "layout");
 
 }  // namespace mgkt

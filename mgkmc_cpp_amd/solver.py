@@ -333,6 +333,17 @@ class Simulation:
         """checkpoint_interval ("none", "end" or k) and checkpoint_path of run()."""
         check_host(host().mgk_sim_set_checkpoint(self._h, str(interval).encode(), str(path).encode()))
 
+    def set_output(self, output_dir, vtk_interval="none", vtk_elastic_only=True, summary_log=False,
+                   global_log=False):
+        """Outputs of run(): output directory, VTU interval ("none",
+        "current", "last" or k) and the summary / global logs."""
+        check_host(host().mgk_sim_set_output(self._h, str(output_dir).encode(), str(vtk_interval).encode(),
+                                             int(bool(vtk_elastic_only)), int(bool(summary_log)),
+                                             int(bool(global_log))))
+
+    def export_vtu(self, path):
+        check_host(host().mgk_sim_export_vtu(self._h, str(path).encode()))
+
     def save(self, path):
         check_host(host().mgk_sim_save(self._h, str(path).encode()))
 
@@ -440,6 +451,23 @@ class SimulationBatch:
             cp = (C.c_longlong * len(self.sims))(*[int(x) for x in caps])
         check_host(host().mgk_sim_batch_event_steps(self._h, int(n), drv[0], drv[1], cp, st))
         return list(st)
+
+
+def write_vtu(path, shape, eps, sig, E, nu, pixel=1.0, eps_plastic=None, soft=None):
+    """export_to_vtu / export_to_vtu_2d (vtu_writer.hpp:169-393) on host
+    arrays: eps, sig, eps_plastic (N, D, D); E, nu (N,); soft (N, 2)."""
+    shape = tuple(int(s) for s in shape)
+    D = len(shape)
+    N = int(np.prod(shape))
+    nz = shape[2] if D == 3 else 1
+    e = _aos(eps, N, D)
+    s = _aos(sig, N, D)
+    ep = _aos(eps_plastic, N, D) if eps_plastic is not None else None
+    E = np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(-1))
+    nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(-1))
+    sp = np.ascontiguousarray(np.asarray(soft, dtype=np.float64).reshape(N, 2)) if soft is not None else None
+    check_host(host().mgk_vtu_write(str(path).encode(), D, shape[0], shape[1], nz, float(pixel), _dp(e), _dp(s),
+                                    _dp(E), _dp(nu), _dp(ep), _dp(sp)))
 
 
 def estimate_bytes(shape, M, per_voxel_catalog=False):
