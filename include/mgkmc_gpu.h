@@ -220,6 +220,8 @@ typedef struct mgk_event_in {
     int max_iter;                /* LS solve (>= 1) */
     double tol;
     int cascade_check;           /* instability_mode == "cascade" */
+    int patch_mode;              /* 0: LS solve after the flip; 1: fast patching instead of the solve;
+                                    2: fast patching, then the solve (a sync); see mgk_patch_init */
 } mgk_event_in;
 typedef struct mgk_event_out {
     int outcome;                 /* mgk_event_outcome */
@@ -281,6 +283,26 @@ int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
  * admit work by HBM footprint. */
 long long mgk_device_bytes(const mgk_ctx* ctx);
 long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_voxel_catalog);
+
+/* Fast patching (dynamics.fast_patching; kmc_solver.hpp:276-332, :1256-1331):
+ * an approximate stress update that replaces most LS solves.
+ * mgk_patch_init: the stress responses of unit deviatoric eigenstrains at the
+ *   grid centre (2 bases in 2D: xx = -yy, xy; 5 in 3D: xx = -yy, yy = -zz, xy,
+ *   xz, yz), each an FP64 cold solve with tol 1e-6 and 200 iterations, cropped
+ *   to a (2R+1)^d window (z window 1 when nz = 1) plus the mean the window
+ *   misses.  0 <= radius <= every grid length.
+ * mgk_patch_unit: sigma_macro_unit, the macro stress of a unit strain in the
+ *   driven component (D x D column-major; cold solve, tol 1e-6, 200).
+ * mgk_patch_apply: sig += sigma_macro_unit * pred (pred != 0), then the window
+ *   superposition for a strain flipped at `voxel` (coefficients xx, xy in 2D;
+ *   xx, -zz, xy, xz, yz in 3D) and the missing-mean shift; voxel < 0 applies
+ *   the predictor alone.  Only the symmetric part of sigma_macro_unit is
+ *   stored (the antisymmetric part does not enter any barrier).
+ * mgk_kmc_event with patch_mode 1 or 2 applies the same update after the
+ * device-decided flip with pred = strain_rate * t_wait (t_wait > 0). */
+int mgk_patch_init(mgk_ctx* ctx, int radius);
+int mgk_patch_unit(mgk_ctx* ctx, const double* strain_unit);
+int mgk_patch_apply(mgk_ctx* ctx, int voxel, const double* strain, double pred);
 
 /* Trajectory state snapshot (no counterpart in the reference, whose
  * output.checkpoint_interval is parsed and ignored, main.cpp:252-258): the

@@ -57,6 +57,12 @@ typedef struct mgk_sim_config {
     int    ls_max_iter;          /* reference: 200 (kmc_solver.hpp:337) */
     double ls_tol;               /* reference: 1e-6 (kmc_solver.hpp:337) */
     int    precision;            /* mgk_precision (device driver only) */
+    /* fast patching (dynamics.fast_patching, main.cpp:310-312; kmc_solver.hpp:276-332):
+     * stress update by superposed unit-eigenstrain kernels, full solve every
+     * sync_interval flips */
+    int    fast_patching;        /* reference default: 0 */
+    int    patch_radius;         /* reference default: 5 */
+    int    sync_interval;        /* reference default: 100 */
 } mgk_sim_config;
 
 typedef struct mgk_run_config {

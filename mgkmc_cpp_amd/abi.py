@@ -56,6 +56,7 @@ class EventIn(C.Structure):
         ("max_iter", C.c_int),
         ("tol", C.c_double),
         ("cascade_check", C.c_int),
+        ("patch_mode", C.c_int),
     ]
 
 
@@ -116,6 +117,9 @@ class SimConfig(C.Structure):
         ("ls_max_iter", C.c_int),
         ("ls_tol", C.c_double),
         ("precision", C.c_int),
+        ("fast_patching", C.c_int),
+        ("patch_radius", C.c_int),
+        ("sync_interval", C.c_int),
     ]
 
 
@@ -175,7 +179,7 @@ def default_sim_config(**kw):
         temperature=0.0, strain_rate=1.0, stability_threshold=0.0, nu0=1e13, instability_kmc=0,
         scale_rate_by_volume=1, redraw_directions=1, redraw_barriers=1, use_3d_barriers=0, seed=42,
         catalog_mode=CATALOG_RANDOM, catalog_magnitude=0.1, ls_max_iter=200, ls_tol=1e-6,
-        precision=MGK_FP64,
+        precision=MGK_FP64, fast_patching=0, patch_radius=5, sync_interval=100,
     )
     vals.update(kw)
     for k, v in vals.items():

@@ -156,6 +156,11 @@ struct mgk_ctx {
     EventArgs* d_ea = nullptr;
     EventArgs* h_ea = nullptr;
 
+    // fast patching: kernels [nb][C][W], missing means [nb][C], sigma_macro_unit [C]
+    double *pk = nullptr, *pkm = nullptr, *psu = nullptr;
+    PatchParams pp{};
+    bool patch_ready = false, unit_ready = false;
+
     // device bytes this context holds (every allocation goes through dal)
     size_t dev_bytes = 0;
     size_t kmc_bytes = 0;   // the part held by the KMC state
@@ -193,7 +198,7 @@ struct mgk_ctx {
         f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.bsum); f(K.bincl);
         f(K.bunst); f(K.bunst_off); f(K.binc); f(kctl); f(kstage); f(unst_idx); f(unst_q);
         fh(hkctl); fh(hkstage);
-        f(d_lsp); f(d_kp); f(d_ea);
+        f(d_lsp); f(d_kp); f(d_ea); f(pk); f(pkm); f(psu);
         fh(h_lsp); fh(h_kp); fh(h_ea);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1069,6 +1074,10 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         const int Dd = c.D();
         if (in->drv_i < 0 || in->drv_i >= Dd || in->drv_j < 0 || in->drv_j >= Dd) throw ArgError("driven component out of range");
         if (in->max_iter < 1) throw ArgError("max_iter must be >= 1");
+        const int pm = in->patch_mode;
+        if (pm < 0 || pm > 2) throw ArgError("patch_mode must be 0, 1 or 2");
+        if (pm && !c.patch_ready) throw ArgError("fast patching not initialised (mgk_patch_init)");
+        if (pm && !c.unit_ready) throw ArgError("sigma_macro_unit not set (mgk_patch_unit)");
         int has_cat = 0, has_q0 = 0;
         const int n_modes = stage_redraw(c, in->new_catalog, in->new_Q0, in->first_mode, in->n_modes, &has_cat, &has_q0);
         // LS control block: the device fills the imposed strain and the shift
@@ -1098,6 +1107,29 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
         kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1);
+        if (pm) kmc_enqueue_patch(KL, c.pp, -1, nullptr, 0.0, 1, in->strain_rate, c.stream);
+        if (pm == 1) {
+            // patched, not solved: the control block alone comes back; eps_macro
+            // advances by strain_rate * t_wait and sig_macro stays (:811-814)
+            MGK_HIP_CHECK(hipMemcpyAsync(c.hkctl, c.kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, c.stream));
+            MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+            c.kctl_pending = false;
+            const KmcCtl& k = *c.hkctl;
+            std::memset(out, 0, sizeof(*out));
+            out->outcome = k.outcome;
+            out->total_rate = k.total_rate;
+            out->n_unstable = k.n_unstable;
+            out->n_included = k.n_included;
+            out->t_wait = k.t_wait;
+            out->flat_idx = k.go ? k.sel : -1;
+            out->q_sel = k.go ? k.q_sel : 0.0;
+            if (k.outcome == MGK_EVENT_DONE) {
+                const double x = in->strain_rate * k.t_wait;
+                const int drv = in->drv_i + Dd * in->drv_j;
+                for (int q = 0; q < Dd * Dd; ++q) eps_macro[q] += ((q == drv) ? 1.0 : 0.0) * x;
+            }
+            return MGK_OK;
+        }
         ls_enqueue_start(L, start, c.stream);
         c.kctl_pending = true;
         double diff = 0.0;
@@ -1206,6 +1238,158 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         float ms = 0.f;
         MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         *avg_ms = (double)ms / reps;
+        return MGK_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Fast patching (kmc_solver.hpp:276-332, :1256-1331, :867-878)
+// ---------------------------------------------------------------------------
+namespace {
+// A scratch FP64 context on the same grid and material, for the unit solves.
+mgk_ctx* scratch_like(mgk_ctx& c) {
+    const long N = c.g.N;
+    std::vector<double> E(N), nu(N);
+    MGK_HIP_CHECK(hipMemcpy(E.data(), c.E, N * sizeof(double), hipMemcpyDeviceToHost));
+    MGK_HIP_CHECK(hipMemcpy(nu.data(), c.nu, N * sizeof(double), hipMemcpyDeviceToHost));
+    mgk_ctx* t = nullptr;
+    if (mgk_create(c.g.dims, c.g.nx, c.g.ny, c.g.nz, c.pixel, c.plane_mode, MGK_FP64, E.data(), nu.data(), &t) != MGK_OK)
+        throw HipError(std::string("scratch context: ") + mgk_last_error());
+    return t;
+}
+// (i, j) of symmetric component k
+void comp_ij(int dims, int k, int& i, int& j) {
+    static const int i2[3] = {0, 1, 0}, j2[3] = {0, 1, 1};
+    static const int i3[6] = {0, 1, 2, 0, 0, 1}, j3[6] = {0, 1, 2, 1, 2, 2};
+    i = dims == 2 ? i2[k] : i3[k];
+    j = dims == 2 ? j2[k] : j3[k];
+}
+}  // namespace
+
+int mgk_patch_init(mgk_ctx* ctx, int radius) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx) throw ArgError("null context");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const Geom& g = c.g;
+        const bool zwin = g.dims == 3 && g.nz > 1;
+        if (radius < 0 || radius > g.nx || radius > g.ny || (zwin && radius > g.nz))
+            throw ArgError("patch radius must lie in [0, every grid length]");
+        const int D = g.dims, F = D * D, C = g.C, nb = (D == 2) ? 2 : 5;
+        const int R = radius, side = 2 * R + 1, sz = zwin ? side : 1, Z = zwin ? R : 0, W = side * side * sz;
+        const long N = g.N;
+        const int cx = g.nx / 2, cy = g.ny / 2, cz = (D == 3) ? g.nz / 2 : 0;
+        auto vid = [&](int x, int y, int z) { return D == 2 ? (long)x * g.ny + y : ((long)x * g.ny + y) * g.nz + z; };
+        std::vector<double> K((size_t)nb * C * W), Km((size_t)nb * C);
+        mgk_ctx* t = scratch_like(c);
+        try {
+            std::vector<double> ep((size_t)N * F), e((size_t)N * F), sg((size_t)N * F);
+            for (int b = 0; b < nb; ++b) {
+                // unit deviatoric basis b at the centre voxel, column-major
+                std::fill(ep.begin(), ep.end(), 0.0);
+                double* B = &ep[(size_t)vid(cx, cy, cz) * F];
+                auto at = [&](int i, int j) -> double& { return B[i + D * j]; };
+                if (b == 0) { at(0, 0) = 1.0; at(1, 1) = -1.0; }
+                else if (D == 2) { at(0, 1) = 1.0; at(1, 0) = 1.0; }
+                else if (b == 1) { at(1, 1) = 1.0; at(2, 2) = -1.0; }
+                else if (b == 2) { at(0, 1) = 1.0; at(1, 0) = 1.0; }
+                else if (b == 3) { at(0, 2) = 1.0; at(2, 0) = 1.0; }
+                else { at(1, 2) = 1.0; at(2, 1) = 1.0; }
+                double bar[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, em[9], sm[9], diff;
+                int it;
+                if (mgk_ls_solve(t, bar, e.data(), 0, ep.data(), sg.data(), em, sm, 200, 1e-6, &it, &diff) != MGK_OK)
+                    throw HipError(std::string("patch kernel solve: ") + mgk_last_error());
+                // crop the window around the centre (periodic) and the mean it misses
+                double sum_crop[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sum_sig[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int dx = -R; dx <= R; ++dx)
+                    for (int dy = -R; dy <= R; ++dy)
+                        for (int dz = -Z; dz <= Z; ++dz) {
+                            const int sx = (cx + dx + g.nx) % g.nx, sy = (cy + dy + g.ny) % g.ny;
+                            const int szz = (D == 3) ? (cz + dz + g.nz) % g.nz : 0;
+                            const double* val = &sg[(size_t)vid(sx, sy, szz) * F];
+                            const int w = ((dx + R) * side + (dy + R)) * sz + (dz + Z);
+                            for (int k = 0; k < C; ++k) {
+                                int i, j;
+                                comp_ij(D, k, i, j);
+                                K[((size_t)b * C + k) * W + w] = val[i + D * j];
+                            }
+                            for (int f = 0; f < F; ++f) sum_crop[f] += val[f];
+                        }
+                for (long v = 0; v < N; ++v)
+                    for (int f = 0; f < F; ++f) sum_sig[f] += sg[(size_t)v * F + f];
+                for (int k = 0; k < C; ++k) {
+                    int i, j;
+                    comp_ij(D, k, i, j);
+                    const int f = i + D * j;
+                    Km[(size_t)b * C + k] = sum_sig[f] / N - sum_crop[f] / N;
+                }
+            }
+        } catch (...) {
+            mgk_destroy(t);
+            throw;
+        }
+        mgk_destroy(t);
+        if (!c.pk) {
+            c.pk = c.dal<double>(K.size());
+            c.pkm = c.dal<double>(Km.size());
+            c.psu = c.dal<double>(6);
+        } else if (c.pp.nb * C * c.pp.W != (int)K.size()) {
+            MGK_HIP_CHECK(hipFree(c.pk));
+            c.dev_bytes -= std::max<size_t>((size_t)c.pp.nb * C * c.pp.W, 1) * sizeof(double);
+            c.pk = c.dal<double>(K.size());
+        }
+        MGK_HIP_CHECK(hipMemcpy(c.pk, K.data(), K.size() * sizeof(double), hipMemcpyHostToDevice));
+        MGK_HIP_CHECK(hipMemcpy(c.pkm, Km.data(), Km.size() * sizeof(double), hipMemcpyHostToDevice));
+        c.pp = PatchParams{c.pk, c.pkm, c.psu, R, side, sz, Z, nb, W};
+        c.patch_ready = true;
+        return MGK_OK;
+    });
+}
+
+int mgk_patch_unit(mgk_ctx* ctx, const double* strain_unit) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || !strain_unit) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.patch_ready) throw ArgError("fast patching not initialised (mgk_patch_init)");
+        const int D = c.g.dims, C = c.g.C;
+        std::vector<double> e((size_t)c.g.N * D * D);
+        double em[9], sm[9], diff;
+        int it;
+        mgk_ctx* t = scratch_like(c);
+        const int rc = mgk_ls_solve(t, strain_unit, e.data(), 0, nullptr, nullptr, em, sm, 200, 1e-6, &it, &diff);
+        mgk_destroy(t);
+        if (rc != MGK_OK) throw HipError(std::string("sigma_macro_unit solve: ") + mgk_last_error());
+        double su[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < C; ++k) {
+            int i, j;
+            comp_ij(D, k, i, j);
+            su[k] = (i == j) ? sm[i + D * j] : 0.5 * (sm[i + D * j] + sm[j + D * i]);
+        }
+        MGK_HIP_CHECK(hipMemcpy(c.psu, su, sizeof(su), hipMemcpyHostToDevice));
+        c.unit_ready = true;
+        return MGK_OK;
+    });
+}
+
+int mgk_patch_apply(mgk_ctx* ctx, int voxel, const double* strain, double pred) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        mgk_ctx& c = *ctx;
+        if (!c.patch_ready) throw ArgError("fast patching not initialised (mgk_patch_init)");
+        if (pred != 0.0 && !c.unit_ready) throw ArgError("sigma_macro_unit not set (mgk_patch_unit)");
+        if (voxel >= c.g.N) throw ArgError("voxel out of range");
+        if (voxel >= 0 && !strain) throw ArgError("null strain");
+        const int D = c.g.dims;
+        double cs[6] = {0, 0, 0, 0, 0, 0};
+        if (voxel >= 0)
+            for (int k = 0; k < c.g.C; ++k) {
+                int i, j;
+                comp_ij(D, k, i, j);
+                cs[k] = strain[i + D * j];
+            }
+        kmc_enqueue_patch(c.kml(), c.pp, voxel, cs, pred, 0, 0.0, c.stream);
         return MGK_OK;
     });
 }
@@ -1455,6 +1639,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         if (on) {
             if (e.drv_i < 0 || e.drv_i >= Dd || e.drv_j < 0 || e.drv_j >= Dd) throw ArgError("driven component out of range");
             if (e.max_iter < 1) throw ArgError("max_iter must be >= 1");
+            if (e.patch_mode) throw ArgError("fast patching runs on single trajectories (mgk_kmc_event)");
             int hc = 0, hq = 0;
             const int n = stage_redraw(c, e.new_catalog, e.new_Q0, e.first_mode, e.n_modes, &hc, &hq,
                                        b.h_stage + (size_t)y * b.stage_per);

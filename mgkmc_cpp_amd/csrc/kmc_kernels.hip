@@ -506,6 +506,9 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
     for (int k = 0; k < C; ++k) c[k] = p.cat_shared ? p.cat[k * ph.M + flat % ph.M] : p.cat[k * nmodes + flat];
 #pragma unroll
     for (int k = 0; k < C; ++k) p.eps_p[k * g.N + v] += c[k];
+#pragma unroll
+    for (int k = 0; k < C; ++k) p.kctl->flip_c[k] = c[k];
+    p.kctl->flip_voxel = (int)v;
     double sum_sq;
     if constexpr (D == 2) {
         const double e11 = c[0], e22 = c[1], e12 = c[2];
@@ -730,6 +733,102 @@ void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, con
         else
             hipLaunchKernelGGL((k_event_prep<3, 6, false>), dim3(1, 1), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
     }
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Fast patching (kmc_solver.hpp:525-547, :866-913, :941-949; 3D :1535-1573,
+// :1877-1939, :1967-1972).  One lane per voxel applies, in the reference's
+// order: the predictor sig += sigma_unit * (strain_rate * t_wait), the window
+// cells whose periodic image is this voxel (dx, dy, dz ascending, as the
+// reference's loops meet them), then the missing-mean shift.  Basis
+// coefficients: 2D (c_xx, c_xy); 3D (c_xx, -c_zz, c_xy, c_xz, c_yz).
+// ---------------------------------------------------------------------------
+// Offsets d in [-R, R] with (x0 + d) mod n == x, ascending: the first and
+// their count (d = r + k n with r = (x - x0) mod n).
+__device__ __forceinline__ void window_hits(int x0, int x, int n, int R, int& d0, int& cnt) {
+    int r = (x - x0) % n;
+    if (r < 0) r += n;
+    d0 = r - ((R + r) / n) * n;   // smallest k with r + k n >= -R
+    cnt = (d0 <= R) ? (R - d0) / n + 1 : 0;
+}
+
+template <int D, int C>
+__global__ __launch_bounds__(256) void k_patch(const KmcParams p, const PatchParams pp, int voxel, double c0, double c1,
+                                               double c2, double c3, double c4, double c5, double pred, int from_ctl,
+                                               double strain_rate) {
+    const Geom g = p.g;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.N) return;
+    double cs[6] = {c0, c1, c2, c3, c4, c5};
+    if (from_ctl) {
+        const KmcCtl* kc = p.kctl;
+        if (!kc->go) return;
+        voxel = kc->flip_voxel;
+#pragma unroll
+        for (int k = 0; k < C; ++k) cs[k] = kc->flip_c[k];
+        pred = (kc->t_wait > 0.0) ? strain_rate * kc->t_wait : 0.0;
+    }
+    double a[5];
+    if constexpr (D == 2) { a[0] = cs[0]; a[1] = cs[2]; }
+    else { a[0] = cs[0]; a[1] = -cs[2]; a[2] = cs[3]; a[3] = cs[4]; a[4] = cs[5]; }
+    constexpr int NB = (D == 2) ? 2 : 5;
+    double s[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) s[k] = p.sig[k * g.N + v];
+    if (pred != 0.0) {
+#pragma unroll
+        for (int k = 0; k < C; ++k) s[k] += pp.sunit[k] * pred;
+    }
+    if (voxel >= 0) {
+        int x, y, z = 0, x0, y0, z0 = 0;
+        if (D == 2) { y = (int)(v % g.ny); x = (int)(v / g.ny); y0 = voxel % g.ny; x0 = voxel / g.ny; }
+        else {
+            z = (int)(v % g.nz); const long t = v / g.nz; y = (int)(t % g.ny); x = (int)(t / g.ny);
+            z0 = voxel % g.nz; const int t0 = voxel / g.nz; y0 = t0 % g.ny; x0 = t0 / g.ny;
+        }
+        int dx0, nxh, dy0, nyh, dz0 = 0, nzh = 1;
+        window_hits(x0, x, g.nx, pp.R, dx0, nxh);
+        window_hits(y0, y, g.ny, pp.R, dy0, nyh);
+        if (D == 3) window_hits(z0, z, g.nz, pp.zr, dz0, nzh);
+        for (int ix = 0; ix < nxh; ++ix)
+            for (int iy = 0; iy < nyh; ++iy)
+                for (int iz = 0; iz < nzh; ++iz) {
+                    const int dx = dx0 + ix * g.nx, dy = dy0 + iy * g.ny, dz = dz0 + iz * g.nz;
+                    const int w = ((dx + pp.R) * pp.side + (dy + pp.R)) * pp.side_z + (dz + pp.zr);
+#pragma unroll
+                    for (int k = 0; k < C; ++k) {
+                        double pv = a[0] * pp.K[(0 * C + k) * pp.W + w];
+#pragma unroll
+                        for (int b = 1; b < NB; ++b) pv += a[b] * pp.K[(b * C + k) * pp.W + w];
+                        s[k] += pv;
+                    }
+                }
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            double m = a[0] * pp.Km[0 * C + k];
+#pragma unroll
+            for (int b = 1; b < NB; ++b) m += a[b] * pp.Km[b * C + k];
+            s[k] += m;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) const_cast<double*>(p.sig)[k * g.N + v] = s[k];   // the one pass that writes sig
+}
+
+void kmc_enqueue_patch(const KmcLaunch& l, const PatchParams& pp, int voxel, const double* c, double pred,
+                       int from_ctl, double strain_rate, hipStream_t s) {
+    if (l.batch) throw ArgError("fast patching runs on single trajectories");
+    double cs[6] = {0, 0, 0, 0, 0, 0};
+    if (c)
+        for (int k = 0; k < l.g.C; ++k) cs[k] = c[k];
+    const int nb = (int)((l.g.N + 255) / 256);
+    if (l.g.dims == 2)
+        hipLaunchKernelGGL((k_patch<2, 3>), dim3(nb), dim3(256), 0, s, l.h, pp, voxel, cs[0], cs[1], cs[2], cs[3],
+                           cs[4], cs[5], pred, from_ctl, strain_rate);
+    else
+        hipLaunchKernelGGL((k_patch<3, 6>), dim3(nb), dim3(256), 0, s, l.h, pp, voxel, cs[0], cs[1], cs[2], cs[3],
+                           cs[4], cs[5], pred, from_ctl, strain_rate);
     MGK_HIP_CHECK(hipGetLastError());
 }
 

@@ -20,6 +20,20 @@ struct KmcCtl {
     int nblocks;
     int go;                // 1: selection and flip proceed (0: no event this attempt)
     int outcome;           // MGK_EVENT_* of a device-decided event
+    double flip_c[6];      // symmetric components of the last flipped strain (before the redraw)
+    int flip_voxel;        // its voxel
+    int pad_;
+};
+
+// Fast-patching kernels in device memory (kmc_solver.hpp:276-332): per basis
+// b, the stress response K[b][c][w] of a unit deviatoric eigenstrain, on a
+// window of (2R+1)^2 or (2R+1)^2 (2R+1 | 1) cells, and the missing mean
+// Km[b][c]; sunit[c] is sigma_macro_unit (:867-878).
+struct PatchParams {
+    const double* K;
+    const double* Km;
+    const double* sunit;
+    int R, side, side_z, zr, nb, W;
 };
 
 // Arguments of the device-side event decision (one attempt of the
@@ -99,6 +113,13 @@ void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipS
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s, int time_from_ctl = 0);
 void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s);
+// Fast-patching stress update: sig += sunit * pred (pred != 0), the kernel
+// superposition around voxel for the flipped strain c (symmetric
+// components), and the missing-mean shift.  voxel < 0: the predictor only.
+// from_ctl: voxel, strain and pred = strain_rate * t_wait (t_wait > 0) from
+// the control block of the last device-decided event (nothing if no event).
+void kmc_enqueue_patch(const KmcLaunch& l, const PatchParams& pp, int voxel, const double* c, double pred,
+                       int from_ctl, double strain_rate, hipStream_t s);
 // decision of one event attempt per trajectory after kmc_enqueue_rates; on
 // "no event" it marks that trajectory's LS control block done so a solve
 // enqueued behind it returns at once.  B = 1: ctl and ea_one; a batch: the

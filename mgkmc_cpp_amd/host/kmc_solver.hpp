@@ -239,8 +239,6 @@ protected:
           cascade_timing(ctiming), scale_rate_by_volume(scale_vol), redraw_directions(redraw_d),
           redraw_barriers(redraw_b), use_3d_barriers(use3d), barrier_gen(bg), rng(seed),
           catalog_fixed(fixed_catalog), catalog_magnitude(fixed_mag) {
-        if (fast_patching_enabled)
-            throw std::runtime_error("fast patching (dynamics.fast_patching) is not built on the device path");
         E_field = E_;
         nu_field = nu_;
         const long n = N();
@@ -289,6 +287,28 @@ protected:
         check(mgk_kmc_init(ctx, &kp), "mgk_kmc_init");
         check(mgk_set_field(ctx, MGK_FIELD_Q0, Q0.data()), "mgk_set_field(Q0)");
         check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "mgk_set_field(catalog)");
+        if (fast_patching_enabled) precompute_patch_kernels();
+    }
+
+public:
+    // --- fast patching (kmc_solver.hpp:276-332, :1256-1331) ----------------
+    // The kernels live on the device (mgk_patch_init); sigma_macro_unit is
+    // solved there before the first patched event (mgk_patch_unit) and counts
+    // as initialised, as in the reference, once an event has used it.
+    bool sigma_macro_unit_initialized = false;
+    bool unit_on_device_ = false;
+    void precompute_patch_kernels() {
+        const char* who = (D == 2) ? "KmcSimulation2D" : "KmcSimulation3D";
+        std::cout << "\n [" << who << "] Pre-computing Stress Patches (Radius=" << patch_radius << ")..." << std::endl;
+        check(mgk_patch_init(ctx, patch_radius), "mgk_patch_init");
+        std::cout << " [" << who << "] Fast Patching Kernels Ready." << std::endl;
+    }
+    void ensure_unit_stress(std::pair<int, int> component) {
+        if (unit_on_device_) return;
+        Mat<D> su = Mat<D>::Zero();
+        su(component.first, component.second) = 1.0;
+        check(mgk_patch_unit(ctx, su.data()), "mgk_patch_unit");
+        unit_on_device_ = true;
     }
 
 public:
@@ -390,6 +410,7 @@ public:
 
     // redraw after a flip (kmc_solver.hpp:839-864, cascade :499-523) + flip
     void flip_and_redraw(long voxel, long flat) {
+        const Mat<D> flipped = catalog[flat];   // the strain before its redraw (:488, :1501)
         std::normal_distribution<double> standard_normal(0.0, 1.0);   // fresh per event (:840, :499)
         std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
         redraw_cat_.resize((size_t)M);
@@ -424,9 +445,10 @@ public:
             if (has_cat) catalog[mi] = redraw_cat_[j];
             if (has_q0) Q0[mi] = redraw_q0_[j];
         }
+        if (fast_patching_enabled) check(mgk_patch_apply(ctx, (int)voxel, flipped.data(), 0.0), "mgk_patch_apply");
     }
 
-    // run_cascade (kmc_solver.hpp:464-564, :1473-1590), fast patching off
+    // run_cascade (kmc_solver.hpp:464-564, :1473-1590)
     std::pair<int, int> run_cascade(int step, std::pair<int, int> component) {
         (void)step;
         (void)component;
@@ -449,11 +471,12 @@ public:
                 batch++;
                 flip_and_redraw(v, mi);
             }
-            elastic_run();
+            if (!fast_patching_enabled) elastic_run();
             total_flips += batch;
             local_step++;
             if (local_step > limit) break;
         }
+        if (fast_patching_enabled && total_flips > 0) elastic_run();
         return {local_step, total_flips};
     }
 
@@ -653,6 +676,11 @@ public:
         in.max_iter = ls_max_iter;
         in.tol = ls_tol;
         in.cascade_check = (instability_mode == "cascade") ? 1 : 0;
+        if (fast_patching_enabled) {
+            // patch; the flip that reaches sync_interval also solves (:908-912)
+            ensure_unit_stress(component);
+            in.patch_mode = (flips_since_sync + 1 >= sync_interval) ? 2 : 1;
+        }
     }
 
     // Second half: the generator is rewound to where the reference would
@@ -678,11 +706,18 @@ public:
         }
         const double t_wait = -std::log(d.r_val) / total_rate_;   // the reference's expression (= out.t_wait)
         time += t_wait;
-        int inner = 0;
-        mgk_last_solve_stats(ctx, nullptr, &inner);
-        counters.ls_solves++;
-        counters.ls_iters += out.iters;
-        counters.ls_iters_fp32 += inner;
+        const bool solved = !fast_patching_enabled || flips_since_sync + 1 >= sync_interval;
+        if (fast_patching_enabled) {
+            sigma_macro_unit_initialized = true;
+            flips_since_sync = solved ? 0 : flips_since_sync + 1;
+        }
+        if (solved) {
+            int inner = 0;
+            mgk_last_solve_stats(ctx, nullptr, &inner);
+            counters.ls_solves++;
+            counters.ls_iters += out.iters;
+            counters.ls_iters_fp32 += inner;
+        }
         last_iters_ = out.iters;
         const long flat = out.flat_idx;
         const long v = flat / M;
@@ -750,7 +785,8 @@ public:
             if (!f) throw std::runtime_error("cannot write checkpoint " + tmp);
             auto put = [&](const void* p, size_t n) { f.write(static_cast<const char*>(p), (std::streamsize)n); };
             const char magic[8] = {'M', 'G', 'K', 'C', 'H', 'K', 'P', 'T'};
-            const int hdr[8] = {1, D, nx, ny, nz, M, at.valid ? 1 : 0, flips_since_sync};
+            const int hdr[9] = {1, D, nx, ny, nz, M, at.valid ? 1 : 0, flips_since_sync,
+                                sigma_macro_unit_initialized ? 1 : 0};
             put(magic, 8);
             put(hdr, sizeof(hdr));
             const int loop[5] = {at.elastic_steps_done, at.total_kmc_steps, at.step, at.seq, at.mid_step ? 1 : 0};
@@ -786,7 +822,7 @@ public:
             if (!f) throw std::runtime_error("truncated checkpoint " + path);
         };
         char magic[8];
-        int hdr[8], loop[5];
+        int hdr[9], loop[5];
         double rem = 0.0;
         get(magic, 8);
         if (std::memcmp(magic, "MGKCHKPT", 8) != 0) throw std::runtime_error(path + " is not a checkpoint");
@@ -827,6 +863,8 @@ public:
         eps_macro = em;
         sig_macro = sm;
         flips_since_sync = hdr[7];
+        sigma_macro_unit_initialized = hdr[8] != 0;
+        if (sigma_macro_unit_initialized && fast_patching_enabled) unit_on_device_ = false;   // solved again on use
         counters.kmc_events = cnt[0]; counters.cascades = cnt[1]; counters.cascade_flips = cnt[2];
         counters.elastic_steps = cnt[3]; counters.ls_solves = cnt[4]; counters.ls_iters = cnt[5];
         counters.ls_iters_fp32 = cnt[6];
@@ -972,6 +1010,10 @@ public:
                     continue;
                 }
                 // no event (no rate, or t_wait >= remaining): to the end of the strain step
+                if (fast_patching_enabled && sigma_macro_unit_initialized) {   // :941-949
+                    ensure_unit_stress(component);
+                    check(mgk_patch_apply(ctx, -1, nullptr, strain_rate * remaining), "mgk_patch_apply");
+                }
                 eps_macro += su * (strain_rate * remaining);
                 time += remaining;
                 remaining = 0.0;

@@ -403,6 +403,14 @@ class Simulation:
         """Device bytes this simulation's context holds."""
         return int(gpu().mgk_device_bytes(self.device_context_handle()))
 
+    def patch(self, voxel, strain, pred=0.0):
+        """Fast-patching stress update (mgk_patch_apply) for a strain (D, D)
+        flipped at a voxel; voxel < 0: the predictor sig += sigma_unit * pred."""
+        h = self.device_context_handle()
+        D = self.config.dims
+        st = _mat_aos(strain, D) if strain is not None else None
+        check(gpu().mgk_patch_apply(h, int(voxel), _dp(st), float(pred)))
+
     def state_save(self):
         """The device state of the trajectory (mgk_state_save) as bytes; the
         host state (generator, macros, time) is in save()/load() files."""

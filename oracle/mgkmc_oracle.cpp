@@ -516,6 +516,12 @@ struct Sim {
     std::vector<mgk_event_record> events;
     long long ls_solves = 0, ls_iters = 0;
     int last_iters = 0;
+    // fast patching state (kmc_solver.hpp:154-160, :276-332)
+    std::vector<std::vector<Mat<D>>> patch_kernels;   // [basis][window cell]
+    std::vector<Mat<D>> patch_missing_mean;           // [basis]
+    Mat<D> sigma_macro_unit{};
+    bool sigma_macro_unit_initialized = false;
+    int flips_since_sync = 0;
 
     long vidx(int x, int y, int z) const { return D == 2 ? (long)x * n[1] + y : ((long)x * n[1] + y) * n[2] + z; }
 
@@ -599,6 +605,111 @@ struct Sim {
         double Es = 0.0, ns = 0.0;
         for (long i = 0; i < N; ++i) { Es += E[i]; ns += nu[i]; }
         init_gamma(Es / N, ns / N);
+        if (c.fast_patching) precompute_patch_kernels();
+    }
+
+    // --- fast patching (kmc_solver.hpp:276-332, 3D :1256-1331) -------------
+    // Deviatoric unit bases: 2D xx = -yy, xy; 3D xx = -yy, yy = -zz, xy, xz, yz.
+    static int n_bases() { return D == 2 ? 2 : 5; }
+    static Mat<D> basis(int b) {
+        Mat<D> a{};
+        auto at = [&](int i, int j) -> double& { return a[i + D * j]; };
+        if (D == 2) {
+            if (b == 0) { at(0, 0) = 1.0; at(1, 1) = -1.0; }
+            else { at(0, 1) = 1.0; at(1, 0) = 1.0; }
+        } else {
+            if (b == 0) { at(0, 0) = 1.0; at(1, 1) = -1.0; }
+            else if (b == 1) { at(1, 1) = 1.0; at(2, 2) = -1.0; }
+            else if (b == 2) { at(0, 1) = 1.0; at(1, 0) = 1.0; }
+            else if (b == 3) { at(0, 2) = 1.0; at(2, 0) = 1.0; }
+            else { at(1, 2) = 1.0; at(2, 1) = 1.0; }
+        }
+        return a;
+    }
+    // Coefficients of a flipped strain on the bases (:885-886, :1896-1900).
+    static void patch_coefs(const Mat<D>& C, double* a) {
+        if (D == 2) { a[0] = C[0]; a[1] = C[D]; }
+        else { a[0] = C[0]; a[1] = -C[8]; a[2] = C[3]; a[3] = C[6]; a[4] = C[7]; }
+    }
+    int side() const { return 2 * c.patch_radius + 1; }
+    int side_z() const { return (D == 3 && n[2] > 1) ? side() : 1; }
+    int zr() const { return (D == 3 && n[2] > 1) ? c.patch_radius : 0; }
+
+    void precompute_patch_kernels() {
+        const int R = c.patch_radius, sd = side(), sz = side_z(), Z = zr();
+        const int cx = n[0] / 2, cy = n[1] / 2, cz = (D == 3) ? n[2] / 2 : 0;
+        patch_kernels.assign(n_bases(), std::vector<Mat<D>>((size_t)sd * sd * sz, Mat<D>{}));
+        patch_missing_mean.assign(n_bases(), Mat<D>{});
+        for (int b = 0; b < n_bases(); ++b) {
+            std::vector<Mat<D>> ep(N, Mat<D>{}), e(N, Mat<D>{}), sg(N, Mat<D>{});
+            ep[vidx(cx, cy, cz)] = basis(b);
+            Mat<D> bar{}, em{}, sm{};
+            double diff;
+            ls_solve<D>(n, E.data(), nu.data(), c.plane_mode == MGK_PLANE_STRESS, bar.data(), G, e[0].data(), false,
+                        ep[0].data(), sg[0].data(), em.data(), sm.data(), 200, 1e-6, &diff);
+            Mat<D> sum_crop{};
+            for (int dx = -R; dx <= R; ++dx)
+                for (int dy = -R; dy <= R; ++dy)
+                    for (int dz = -Z; dz <= Z; ++dz) {
+                        const int sx = (cx + dx + n[0]) % n[0], sy = (cy + dy + n[1]) % n[1];
+                        const int szz = (D == 3) ? (cz + dz + n[2]) % n[2] : 0;
+                        const Mat<D>& val = sg[vidx(sx, sy, szz)];
+                        patch_kernels[b][((size_t)(dx + R) * sd + (dy + R)) * sz + (dz + Z)] = val;
+                        for (int k = 0; k < D * D; ++k) sum_crop[k] += val[k];
+                    }
+            Mat<D> sum_sig{};
+            for (long i = 0; i < N; ++i)
+                for (int k = 0; k < D * D; ++k) sum_sig[k] += sg[i][k];
+            for (int k = 0; k < D * D; ++k) patch_missing_mean[b][k] = sum_sig[k] / N - sum_crop[k] / N;
+        }
+    }
+
+    // sigma_macro_unit (:867-878): macro stress of a unit strain in the driven
+    // component (cold solve, tol 1e-6, 200 iterations)
+    void init_sigma_macro_unit(const Mat<D>& strain_unit) {
+        std::vector<Mat<D>> e(N, Mat<D>{}), sg(N, Mat<D>{});
+        Mat<D> em{}, sm{};
+        double diff;
+        ls_solve<D>(n, E.data(), nu.data(), c.plane_mode == MGK_PLANE_STRESS, strain_unit.data(), G, e[0].data(),
+                    false, nullptr, sg[0].data(), em.data(), sm.data(), 200, 1e-6, &diff);
+        sigma_macro_unit = sm;
+        sigma_macro_unit_initialized = true;
+    }
+    // sig += sigma_macro_unit * (strain_rate * dt) everywhere (:889-893, :944-948)
+    void predictor(double dt) {
+        const double x = c.strain_rate * dt;
+        for (long i = 0; i < N; ++i)
+            for (int k = 0; k < D * D; ++k) sig[i][k] += sigma_macro_unit[k] * x;
+    }
+    // kernel superposition around voxel v and the missing-mean shift (:895-913)
+    void apply_patch(long v, const Mat<D>& C) {
+        double a[5];
+        patch_coefs(C, a);
+        const int R = c.patch_radius, sd = side(), sz = side_z(), Z = zr();
+        int x, y, z = 0;
+        if (D == 2) { y = (int)(v % n[1]); x = (int)(v / n[1]); }
+        else { z = (int)(v % n[2]); long t = v / n[2]; y = (int)(t % n[1]); x = (int)(t / n[1]); }
+        for (int dx = -R; dx <= R; ++dx)
+            for (int dy = -R; dy <= R; ++dy)
+                for (int dz = -Z; dz <= Z; ++dz) {
+                    const int px = (x + dx + n[0]) % n[0], py = (y + dy + n[1]) % n[1];
+                    const int pz = (D == 3) ? (z + dz + n[2]) % n[2] : 0;
+                    const size_t ci = ((size_t)(dx + R) * sd + (dy + R)) * sz + (dz + Z);
+                    Mat<D>& s_ = sig[vidx(px, py, pz)];
+                    for (int k = 0; k < D * D; ++k) {
+                        double pv = a[0] * patch_kernels[0][ci][k];
+                        for (int b = 1; b < n_bases(); ++b) pv += a[b] * patch_kernels[b][ci][k];
+                        s_[k] += pv;
+                    }
+                }
+        Mat<D> shift{};
+        for (int k = 0; k < D * D; ++k) {
+            double m = a[0] * patch_missing_mean[0][k];
+            for (int b = 1; b < n_bases(); ++b) m += a[b] * patch_missing_mean[b][k];
+            shift[k] = m;
+        }
+        for (long i = 0; i < N; ++i)
+            for (int k = 0; k < D * D; ++k) sig[i][k] += shift[k];
     }
 
     // init_gamma (:259-274, :1234-1254)
@@ -729,7 +840,7 @@ struct Sim {
         }
     }
 
-    // run_cascade (:464-564, :1473-1590) without fast patching
+    // run_cascade (:464-564, :1473-1590)
     std::pair<int, int> run_cascade() {
         int total = 0, local = 0;
         long limit = N;
@@ -751,12 +862,14 @@ struct Sim {
                 apply_flip(v, m);
                 prev_dir[v] = C;
                 redraw(v, mi);
+                if (c.fast_patching) apply_patch(v, C);
             }
-            elastic_run();
+            if (!c.fast_patching) elastic_run();
             total += batch;
             local++;
             if (local > limit) break;
         }
+        if (c.fast_patching && total > 0) elastic_run();
         return {local, total};
     }
 
@@ -807,7 +920,20 @@ struct Sim {
         apply_flip(v, m);
         prev_dir[v] = C;
         redraw(v, flat);
-        elastic_run();
+        last_iters = 0;
+        if (c.fast_patching) {
+            // :866-913, 3D :1876-1937
+            if (!sigma_macro_unit_initialized) init_sigma_macro_unit(strain_unit);
+            if (t_wait > 0.0) predictor(t_wait);
+            apply_patch(v, C);
+            flips_since_sync++;
+            if (flips_since_sync >= c.sync_interval) {
+                elastic_run();
+                flips_since_sync = 0;
+            }
+        } else {
+            elastic_run();
+        }
         mgk_event_record e{};
         e.flat_idx = flat;
         e.is_instab = is_instab;
@@ -819,8 +945,8 @@ struct Sim {
         return is_instab;
     }
 
-    // run_simulation (:721-1037, :1735-2062): logging, VTU and fast
-    // patching off; max_events is the event-cap extension.
+    // run_simulation (:721-1037, :1735-2062): logging and VTU off; max_events
+    // is the event-cap extension.
     void run(const mgk_run_config& r, mgk_run_stats& st) {
         std::memset(&st, 0, sizeof(st));
         int elastic_done = 0;
@@ -867,6 +993,7 @@ struct Sim {
                         continue;
                     }
                 }
+                if (c.fast_patching && sigma_macro_unit_initialized) predictor(remaining);   // :941-949
                 for (int k = 0; k < D * D; ++k) eps_macro[k] += su[k] * (c.strain_rate * remaining);
                 time += remaining;
                 remaining = 0.0;
@@ -1072,6 +1199,18 @@ int orc_select_sequential(const double* rates, int n, double target) {
         if (cur >= target) { sel = i; break; }
     }
     return sel;
+}
+
+// kernel superposition of a flipped strain at a voxel (fast patching); the
+// strain is D x D column-major; fails (-1) without fast patching
+int orc_sim_patch(orc_sim* s, long voxel, const double* strain) {
+    return ORC_DISPATCH(s, ([&]() {
+        if (!S->c.fast_patching) return -1;
+        auto C = S->sigma_macro_unit;   // a Mat<D>
+        for (int k = 0; k < (int)C.size(); ++k) C[k] = strain[k];
+        S->apply_patch(voxel, C);
+        return 0;
+    })());
 }
 
 int orc_sim_apply_flip(orc_sim* s, int flat, double time) {

@@ -68,6 +68,7 @@ def lib(ref=False):
     _sig(L, "orc_sim_rates", C.c_int, [P, DP, IP, DP, IP])
     _sig(L, "orc_select_sequential", C.c_int, [DP, C.c_int, C.c_double])
     _sig(L, "orc_sim_apply_flip", C.c_int, [P, C.c_int, C.c_double])
+    _sig(L, "orc_sim_patch", C.c_int, [P, C.c_long, DP])
     _sig(L, "orc_sim_get_field", C.c_int, [P, C.c_int, DP])
     _sig(L, "orc_sim_set_field", C.c_int, [P, C.c_int, DP])
     _sig(L, "orc_sim_get_macro", C.c_int, [P, DP, DP, DP])
@@ -231,6 +232,11 @@ class OracleSim:
 
     def apply_flip(self, flat, time):
         self._lib.orc_sim_apply_flip(self._h, int(flat), float(time))
+
+    def patch(self, voxel, strain):
+        """Fast-patching stress update for a flipped strain (D, D) at a voxel."""
+        if self._lib.orc_sim_patch(self._h, int(voxel), _dp(_mat_aos(strain, self.D))) != 0:
+            raise RuntimeError("fast patching is off")
 
     def _shape(self, which):
         D, N, M = self.D, self.N, self.M
