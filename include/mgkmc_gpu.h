@@ -304,6 +304,30 @@ int mgk_patch_init(mgk_ctx* ctx, int radius);
 int mgk_patch_unit(mgk_ctx* ctx, const double* strain_unit);
 int mgk_patch_apply(mgk_ctx* ctx, int voxel, const double* strain, double pred);
 
+/* Input generation on the device (no counterpart in the reference, which
+ * draws on the host: constructor draws kmc_solver.hpp:216-240, :1194-1215;
+ * generate_correlated_field_cpp material_generator.hpp:130-258).  Random
+ * numbers are Philox4x32-10 (counter-based, keyed by seed), so the draws follow
+ * the reference's distributions but not its std::mt19937 stream: for
+ * statistically equivalent trajectories (batches, serving), not for parity.
+ * mgk_kmc_draw: Q0 of every (voxel, mode) from the BarrierGenerator and, with
+ *   draw_catalog, the per-(voxel, mode) STZ strains (planar, or
+ *   stz_mode_glass_3d with use_3d_barriers) into the context's KMC state.
+ * mgk_correlated_field: the reference's field (noise, Gaussian filter
+ *   exp(-2 pi^2 corr^2 |f|^2), standardise, scale, clip) into host `out`;
+ *   noise = N host normals (the result is then the reference's field to
+ *   rounding) or NULL for device normals. */
+typedef struct mgk_draw_params {
+    int    barrier_type;         /* mgk_barrier_type values of include/mgkmc_host.h */
+    double b_mean, b_std, b_min_cutoff, b_max_cutoff, b_loc, b_scale, b_epsilon, b_ratio;
+    double gamma0;
+    int    use_3d_barriers;
+} mgk_draw_params;
+int mgk_kmc_draw(mgk_ctx* ctx, const mgk_draw_params* p, unsigned long long seed, int draw_catalog);
+int mgk_correlated_field(int dims, int nx, int ny, int nz, double mean, double std, double corr, int has_min,
+                         double clip_min, int has_max, double clip_max, unsigned long long seed, const double* noise,
+                         double* out);
+
 /* Trajectory state snapshot (no counterpart in the reference, whose
  * output.checkpoint_interval is parsed and ignored, main.cpp:252-258): the
  * strain, stress and plastic-strain fields, the KMC fields (Q, Q0, catalog,

@@ -63,6 +63,11 @@ typedef struct mgk_sim_config {
     int    fast_patching;        /* reference default: 0 */
     int    patch_radius;         /* reference default: 5 */
     int    sync_interval;        /* reference default: 100 */
+    /* extension: constructor draws (Q0, random catalog) on the device with
+     * Philox4x32-10 (mgk_kmc_draw) instead of the host std::mt19937 -- the
+     * reference's distributions, not its stream; the generator stays seeded
+     * for the redraws that follow */
+    int    device_draws;
 } mgk_sim_config;
 
 typedef struct mgk_run_config {

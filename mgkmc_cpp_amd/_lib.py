@@ -55,6 +55,10 @@ def gpu():
         _sig(lib, "mgk_select_exact", C.c_int, [P, C.c_double, IP, DP])
         _sig(lib, "mgk_state_size", C.c_longlong, [P])
         _sig(lib, "mgk_patch_init", C.c_int, [P, C.c_int])
+        _sig(lib, "mgk_kmc_draw", C.c_int, [P, C.c_void_p, C.c_ulonglong, C.c_int])
+        _sig(lib, "mgk_correlated_field", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                    C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
+                                                    C.c_ulonglong, DP, DP])
         _sig(lib, "mgk_patch_unit", C.c_int, [P, DP])
         _sig(lib, "mgk_patch_apply", C.c_int, [P, C.c_int, DP, C.c_double])
         _sig(lib, "mgk_state_save", C.c_int, [P, C.c_void_p, C.c_longlong])

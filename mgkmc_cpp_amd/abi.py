@@ -42,6 +42,17 @@ EVENT_REJECTED = 3
 EVENT_SKIPPED = 4
 
 
+class DrawParams(C.Structure):
+    """mgk_draw_params (include/mgkmc_gpu.h)."""
+    _fields_ = [
+        ("barrier_type", C.c_int),
+        ("b_mean", C.c_double), ("b_std", C.c_double), ("b_min_cutoff", C.c_double), ("b_max_cutoff", C.c_double),
+        ("b_loc", C.c_double), ("b_scale", C.c_double), ("b_epsilon", C.c_double), ("b_ratio", C.c_double),
+        ("gamma0", C.c_double),
+        ("use_3d_barriers", C.c_int),
+    ]
+
+
 class EventIn(C.Structure):
     _fields_ = [
         ("time", C.c_double),
@@ -120,6 +131,7 @@ class SimConfig(C.Structure):
         ("fast_patching", C.c_int),
         ("patch_radius", C.c_int),
         ("sync_interval", C.c_int),
+        ("device_draws", C.c_int),
     ]
 
 
@@ -179,7 +191,7 @@ def default_sim_config(**kw):
         temperature=0.0, strain_rate=1.0, stability_threshold=0.0, nu0=1e13, instability_kmc=0,
         scale_rate_by_volume=1, redraw_directions=1, redraw_barriers=1, use_3d_barriers=0, seed=42,
         catalog_mode=CATALOG_RANDOM, catalog_magnitude=0.1, ls_max_iter=200, ls_tol=1e-6,
-        precision=MGK_FP64, fast_patching=0, patch_radius=5, sync_interval=100,
+        precision=MGK_FP64, fast_patching=0, patch_radius=5, sync_interval=100, device_draws=0,
     )
     vals.update(kw)
     for k, v in vals.items():

@@ -16,7 +16,7 @@ HOST = os.path.join(HERE, "host")
 LIB = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["api.hip", "ls_kernels.hip", "kmc_kernels.hip"]
+HIP_SOURCES = ["api.hip", "ls_kernels.hip", "kmc_kernels.hip", "gen_kernels.hip"]
 ARCH = os.environ.get("MGK_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

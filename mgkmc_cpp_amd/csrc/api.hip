@@ -15,6 +15,7 @@
 
 #include "../../include/mgkmc_gpu.h"
 #include "common.hpp"
+#include "gen_kernels.hpp"
 #include "kmc_kernels.hpp"
 #include "ls_kernels.hpp"
 
@@ -1238,6 +1239,48 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         float ms = 0.f;
         MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         *avg_ms = (double)ms / reps;
+        return MGK_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Input generation on the device (gen_kernels.hip)
+// ---------------------------------------------------------------------------
+int mgk_kmc_draw(mgk_ctx* ctx, const mgk_draw_params* p, unsigned long long seed, int draw_catalog) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || !p) throw ArgError("null argument");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const long nm = c.g.N * c.kp.M;
+        DrawParams d{p->barrier_type, p->b_mean, p->b_std, p->b_min_cutoff, p->b_max_cutoff, p->b_loc, p->b_scale,
+                     p->b_epsilon, p->b_ratio, p->gamma0, (c.g.dims == 3 && p->use_3d_barriers) ? 1 : 0};
+        double* cat = nullptr;
+        if (draw_catalog) {
+            c.ensure_cat((size_t)c.g.C * nm);
+            cat = c.K.cat;
+        }
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        gen_enqueue_modes(c.K.Q0, cat, nm, c.g.C, d, seed, c.stream);
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (draw_catalog) {
+            c.K.cat_shared = 0;
+            c.shared_cat.clear();
+        }
+        return MGK_OK;
+    });
+}
+
+int mgk_correlated_field(int dims, int nx, int ny, int nz, double mean, double std, double corr, int has_min,
+                         double clip_min, int has_max, double clip_max, unsigned long long seed, const double* noise,
+                         double* out) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!out) throw ArgError("null argument");
+        if ((dims != 2 && dims != 3) || nx < 1 || ny < 1 || (dims == 3 && nz < 1)) throw ArgError("bad grid");
+        const int n[3] = {nx, ny, dims == 3 ? nz : 1};
+        FieldParams f{mean, std, corr, has_min ? 1 : 0, has_max ? 1 : 0, clip_min, clip_max};
+        gen_correlated_field(n, dims, f, seed, noise, out);
         return MGK_OK;
     });
 }

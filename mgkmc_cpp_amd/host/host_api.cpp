@@ -108,6 +108,7 @@ void mgk_sim_config_defaults(mgk_sim_config* c) {
     c->fast_patching = 0;
     c->patch_radius = 5;
     c->sync_interval = 100;
+    c->device_draws = 0;
 }
 
 void mgk_run_config_defaults(mgk_run_config* r) {
@@ -138,14 +139,14 @@ int mgk_sim_create(const mgk_sim_config* c, const double* E, const double* nu, m
                 c->neighbor_softening_fraction, c->q_act_temp, "output", c->temperature, c->strain_rate,
                 c->stability_threshold, c->nu0, c->plane_mode == MGK_PLANE_STRESS ? "plane_stress" : "plane_strain",
                 c->fast_patching != 0, c->patch_radius, c->sync_interval, imode, "none", c->scale_rate_by_volume != 0, c->redraw_directions != 0,
-                c->redraw_barriers != 0, c->seed, c->precision, fixed, c->catalog_magnitude);
+                c->redraw_barriers != 0, c->seed, c->precision, fixed, c->catalog_magnitude, c->device_draws != 0);
         } else if (c->dims == 3) {
             s->s3 = std::make_unique<KmcSimulation3D>(
                 c->nx, c->ny, c->nz, c->M, c->gamma0, Ev, nv, c->pixel, bg, scheme, c->softening_cap, c->jp, c->jt,
                 c->neighbor_softening_fraction, c->q_act_temp, "output", c->temperature, c->strain_rate,
                 c->stability_threshold, c->nu0, c->fast_patching != 0, c->patch_radius, c->sync_interval, imode, "none", c->scale_rate_by_volume != 0,
                 c->redraw_directions != 0, c->redraw_barriers != 0, c->use_3d_barriers != 0, c->seed, c->precision,
-                fixed, c->catalog_magnitude);
+                fixed, c->catalog_magnitude, c->device_draws != 0);
         } else {
             throw std::invalid_argument("dims must be 2 or 3");
         }

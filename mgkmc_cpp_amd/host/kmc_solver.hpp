@@ -207,6 +207,7 @@ public:
     long long max_events = -1;
     bool catalog_fixed = false;
     double catalog_magnitude = 0.1;
+    bool device_draws = false;   // constructor draws on the device (mgk_kmc_draw): the reference's distributions, not its stream
     RunCounters counters;
     std::vector<EventRecord> events;
     bool record_events = false;
@@ -230,7 +231,7 @@ protected:
                    std::string output_dir_, double T, double rate, double thr, double nu0_, std::string plane_mode_,
                    bool fp_enabled, int fp_radius, int fp_sync, std::string instab, std::string ctiming,
                    bool scale_vol, bool redraw_d, bool redraw_b, bool use3d, int seed, int precision,
-                   bool fixed_catalog, double fixed_mag)
+                   bool fixed_catalog, double fixed_mag, bool device_draws_ = false)
         : nx(nx_), ny(ny_), nz(nz_), M(M_), gamma0(gamma0_), pixel(pixel_), volume(pixel_ * pixel_ * pixel_),
           output_dir(output_dir_), softening_scheme(softening_scheme_), softening_cap(cap), jp(jp_), jt(jt_),
           neighbor_softening_fraction(nfrac), temperature(T), strain_rate(rate), stability_threshold(thr),
@@ -238,7 +239,7 @@ protected:
           patch_radius(fp_radius), sync_interval(fp_sync), flips_since_sync(0), instability_mode(instab),
           cascade_timing(ctiming), scale_rate_by_volume(scale_vol), redraw_directions(redraw_d),
           redraw_barriers(redraw_b), use_3d_barriers(use3d), barrier_gen(bg), rng(seed),
-          catalog_fixed(fixed_catalog), catalog_magnitude(fixed_mag) {
+          catalog_fixed(fixed_catalog), catalog_magnitude(fixed_mag), device_draws(device_draws_) {
         E_field = E_;
         nu_field = nu_;
         const long n = N();
@@ -261,12 +262,17 @@ protected:
         catalog.resize(n * M);
         prev_strain_dir.assign(n, Mat<D>::Zero());
         std::normal_distribution<double> standard_normal(0.0, 1.0);   // :223 (2D draws share it)
-        for (long v = 0; v < n; ++v)
-            for (int m = 0; m < M; ++m) {
-                const long mi = v * M + m;
-                Q0[mi] = barrier_gen.generate(rng);
-                catalog[mi] = draw_mode(D == 2 ? &standard_normal : nullptr, m);
-            }
+        if (!device_draws) {
+            for (long v = 0; v < n; ++v)
+                for (int m = 0; m < M; ++m) {
+                    const long mi = v * M + m;
+                    Q0[mi] = barrier_gen.generate(rng);
+                    catalog[mi] = draw_mode(D == 2 ? &standard_normal : nullptr, m);
+                }
+        } else if (catalog_fixed) {
+            for (long v = 0; v < n; ++v)
+                for (int m = 0; m < M; ++m) catalog[v * M + m] = fixed_shear<D>(m, M, catalog_magnitude);
+        }
         check(mgk_create(D, nx, ny, nz, pixel, plane_code(plane_mode), precision, E_field.data(), nu_field.data(),
                          &ctx),
               "mgk_create");
@@ -285,9 +291,36 @@ protected:
         kp.instability_kmc = (instability_mode == "kmc") ? 1 : 0;
         kp.scale_rate_by_volume = scale_rate_by_volume ? 1 : 0;
         check(mgk_kmc_init(ctx, &kp), "mgk_kmc_init");
-        check(mgk_set_field(ctx, MGK_FIELD_Q0, Q0.data()), "mgk_set_field(Q0)");
-        check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "mgk_set_field(catalog)");
+        if (!device_draws) {
+            check(mgk_set_field(ctx, MGK_FIELD_Q0, Q0.data()), "mgk_set_field(Q0)");
+            check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "mgk_set_field(catalog)");
+        } else {
+            // the reference's distributions on the device; host mirrors follow
+            if (catalog_fixed) check(mgk_set_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "mgk_set_field(catalog)");
+            mgk_draw_params dp{};
+            dp.barrier_type = barrier_code(barrier_gen.type);
+            dp.b_mean = barrier_gen.mean;
+            dp.b_std = barrier_gen.std_val;
+            dp.b_min_cutoff = barrier_gen.min_cutoff;
+            dp.b_max_cutoff = barrier_gen.max_cutoff;
+            dp.b_loc = barrier_gen.loc;
+            dp.b_scale = barrier_gen.scale;
+            dp.b_epsilon = barrier_gen.epsilon;
+            dp.b_ratio = barrier_gen.ratio;
+            dp.gamma0 = gamma0;
+            dp.use_3d_barriers = use_3d_barriers ? 1 : 0;
+            check(mgk_kmc_draw(ctx, &dp, (unsigned long long)(unsigned)seed, catalog_fixed ? 0 : 1), "mgk_kmc_draw");
+            check(mgk_get_field(ctx, MGK_FIELD_Q0, Q0.data()), "get Q0");
+            if (!catalog_fixed) check(mgk_get_field(ctx, MGK_FIELD_CATALOG, catalog[0].data()), "get catalog");
+        }
         if (fast_patching_enabled) precompute_patch_kernels();
+    }
+    static int barrier_code(const std::string& t) {
+        if (t == "gaussian") return 0;
+        if (t == "rayleigh") return 1;
+        if (t == "modified_rayleigh") return 2;
+        if (t == "modified_rayleigh_with_exponential") return 3;
+        return 4;
     }
 
 public:
@@ -1169,13 +1202,14 @@ public:
                     int patch_radius_ = 5, int sync_interval_ = 100, std::string instability_mode_ = "cascade",
                     std::string cascade_timing_ = "none", bool scale_rate_by_volume_ = true,
                     bool redraw_directions_ = true, bool redraw_barriers_ = true, int seed_ = 42,
-                    int precision_ = MGK_FP64, bool catalog_fixed_ = false, double catalog_magnitude_ = 0.1)
+                    int precision_ = MGK_FP64, bool catalog_fixed_ = false, double catalog_magnitude_ = 0.1,
+                    bool device_draws_ = false)
         : KmcSimulationT<2>(nx_, ny_, 1, M_, gamma0_, E_field_, nu_field_, pixel_, barrier_gen_, softening_scheme_,
                             softening_cap_, jp_, jt_, neighbor_softening_fraction_, q_act_temp_, output_dir_,
                             temperature_, strain_rate_, stability_threshold_, nu0_, plane_mode_,
                             fast_patching_enabled_, patch_radius_, sync_interval_, instability_mode_,
                             cascade_timing_, scale_rate_by_volume_, redraw_directions_, redraw_barriers_, false,
-                            seed_, precision_, catalog_fixed_, catalog_magnitude_) {}
+                            seed_, precision_, catalog_fixed_, catalog_magnitude_, device_draws_) {}
     void apply_flip_soa_2d(int x, int y, int m) { apply_flip(((long)x * ny + y) * M + m); }
 };
 
@@ -1193,12 +1227,12 @@ public:
                     std::string cascade_timing_ = "none", bool scale_rate_by_volume_ = true,
                     bool redraw_directions_ = true, bool redraw_barriers_ = true, bool use_3d_barriers_ = false,
                     int seed_ = 42, int precision_ = MGK_FP64, bool catalog_fixed_ = false,
-                    double catalog_magnitude_ = 0.1)
+                    double catalog_magnitude_ = 0.1, bool device_draws_ = false)
         : KmcSimulationT<3>(nx_, ny_, nz_, M_, gamma0_, E_field_, nu_field_, pixel_, barrier_gen_,
                             softening_scheme_, softening_cap_, jp_, jt_, neighbor_softening_fraction_, q_act_temp_,
                             output_dir_, temperature_, strain_rate_, stability_threshold_, nu0_, "plane_strain",
                             fast_patching_enabled_, patch_radius_, sync_interval_, instability_mode_,
                             cascade_timing_, scale_rate_by_volume_, redraw_directions_, redraw_barriers_,
-                            use_3d_barriers_, seed_, precision_, catalog_fixed_, catalog_magnitude_) {}
+                            use_3d_barriers_, seed_, precision_, catalog_fixed_, catalog_magnitude_, device_draws_) {}
     void apply_flip_soa_3d(int x, int y, int z, int m) { apply_flip((((long)x * ny + y) * nz + z) * M + m); }
 };

@@ -264,6 +264,11 @@ class Context:
     def device_bytes(self):
         return int(gpu().mgk_device_bytes(self._h))
 
+    def kmc_draw(self, params, seed, draw_catalog=True):
+        """Q0 and (draw_catalog) the STZ catalog drawn on the device
+        (mgk_kmc_draw); params: abi.DrawParams."""
+        check(gpu().mgk_kmc_draw(self._h, C.byref(params), int(seed) & (2 ** 64 - 1), int(bool(draw_catalog))))
+
     def state_save(self):
         """The trajectory state (mgk_state_save) as bytes."""
         return _state_save(self._h)
@@ -459,6 +464,23 @@ class SimulationBatch:
             cp = (C.c_longlong * len(self.sims))(*[int(x) for x in caps])
         check_host(host().mgk_sim_batch_event_steps(self._h, int(n), drv[0], drv[1], cp, st))
         return list(st)
+
+
+def correlated_field_device(shape, mean, std, corr, seed, clip_min=None, clip_max=None, noise=None):
+    """generate_correlated_field_cpp (material_generator.hpp:130-258) on the
+    device: noise (N normals, e.g. fields.box_muller_normals for the
+    reference's own) or Philox normals keyed by seed, the Gaussian filter as
+    three circular convolutions, standardisation, scale and clips."""
+    shape = tuple(int(s) for s in shape)
+    D = len(shape)
+    N = int(np.prod(shape))
+    out = np.empty(N)
+    nz_ = np.ascontiguousarray(np.asarray(noise, dtype=np.float64).reshape(-1)) if noise is not None else None
+    check(gpu().mgk_correlated_field(D, shape[0], shape[1], shape[2] if D == 3 else 1, float(mean), float(std),
+                                     float(corr), int(clip_min is not None), float(clip_min or 0.0),
+                                     int(clip_max is not None), float(clip_max or 0.0), int(seed) & (2 ** 64 - 1),
+                                     _dp(nz_), _dp(out)))
+    return out
 
 
 def write_vtu(path, shape, eps, sig, E, nu, pixel=1.0, eps_plastic=None, soft=None):
