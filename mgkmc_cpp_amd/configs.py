@@ -27,9 +27,14 @@ E_COV = 0.15
 NU = 0.35
 
 
-def material(shape, corr, seed, heterogeneous=True):
+def material(shape, corr, seed, heterogeneous=True, device=False):
+    """E (Pa) and nu; device=True draws the field on the GPU
+    (mgk_correlated_field: the same filter, Philox noise)."""
     N = int(np.prod(shape))
-    if heterogeneous:
+    if heterogeneous and device:
+        from .solver import correlated_field_device
+        E = correlated_field_device(shape, E_MEAN_GPA, E_COV * E_MEAN_GPA, corr, seed) * 1e9
+    elif heterogeneous:
         E = correlated_field(shape, E_MEAN_GPA, E_COV * E_MEAN_GPA, corr, seed) * 1e9
     else:
         E = np.full(N, E_MEAN_GPA * 1e9)
@@ -78,10 +83,11 @@ def c3_requests(n=256, events=2000, seed=7, base_seed=1000, precision=abi.MGK_FP
     trajectory i seeded base_seed + i.  The batch and serving workloads run
     instability_mode "kmc" (unstable modes are rated events), so the work of
     a request is its event count; "cascade" mode can flip for up to N rounds
-    between two events."""
+    between two events.  Their inputs (E field, barriers, STZ strains) are
+    drawn on the device."""
     from .serving import Request
     return [Request(rid=i, shape=g, events=events, seed=base_seed + i, arrival=0.0, precision=precision,
-                    instability_kmc=1) for i, g in enumerate(c3_grids(n, seed))]
+                    instability_kmc=1, device_inputs=True) for i, g in enumerate(c3_grids(n, seed))]
 
 
 def c4_requests(n, rate=50.0, seed=11, events=(500, 5000), grids=C4_GRIDS, precision=abi.MGK_FP32_REFINE):
@@ -94,7 +100,8 @@ def c4_requests(n, rate=50.0, seed=11, events=(500, 5000), grids=C4_GRIDS, preci
     g = rng.integers(0, len(grids), n)
     ev = rng.integers(events[0], events[1] + 1, n)
     return [Request(rid=i, shape=tuple(grids[g[i]]), events=int(ev[i]), seed=100000 + seed * 1000 + i,
-                    arrival=float(t[i]), precision=precision, instability_kmc=1) for i in range(n)]
+                    arrival=float(t[i]), precision=precision, instability_kmc=1, device_inputs=True)
+            for i in range(n)]
 
 
 def uniaxial_run(events, step_size=1e-4):

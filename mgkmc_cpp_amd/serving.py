@@ -36,6 +36,7 @@ class Request:
     arrival: float = 0.0               # seconds after the start of the run
     precision: int = abi.MGK_FP32_REFINE
     instability_kmc: int = 0           # instability_mode "kmc" (1) or "cascade" (0)
+    device_inputs: bool = False        # E field and constructor draws generated on the GPU
     # filled by the server
     admitted: Optional[float] = None
     started: Optional[float] = None
@@ -50,10 +51,13 @@ class Request:
 
 def request_simulation(req):
     """The trajectory of a request: the benchmark workloads' material and
-    catalog (correlation length 3 voxels in 2D, 4 in 3D) with its own seed."""
+    catalog (correlation length 3 voxels in 2D, 4 in 3D) with its own seed;
+    with device_inputs the field and the constructor draws come from the GPU
+    (the reference's distributions, Philox streams)."""
     corr = 3.0 if len(req.shape) == 2 else 4.0
-    cfg = configs.sim_config(req.shape, precision=req.precision, seed=req.seed, instability_kmc=req.instability_kmc)
-    E, nu = configs.material(req.shape, corr, req.seed)
+    cfg = configs.sim_config(req.shape, precision=req.precision, seed=req.seed, instability_kmc=req.instability_kmc,
+                             device_draws=int(req.device_inputs))
+    E, nu = configs.material(req.shape, corr, req.seed, device=req.device_inputs)
     return Simulation(cfg, E, nu)
 
 

@@ -42,10 +42,11 @@ def _alone(req):
     return out
 
 
-def test_served_requests_equal_standalone_runs():
+@pytest.mark.parametrize("device_inputs", [False, True])
+def test_served_requests_equal_standalone_runs(device_inputs):
     shapes = [(16, 16), (8, 8, 8), (16, 16), (32, 32), (8, 8, 8), (16, 16)]
-    reqs = [Request(rid=i, shape=s, events=12 + 3 * i, seed=50 + i, arrival=0.02 * i, precision=abi.MGK_FP64)
-            for i, s in enumerate(shapes)]
+    reqs = [Request(rid=i, shape=s, events=12 + 3 * i, seed=50 + i, arrival=0.02 * i, precision=abi.MGK_FP64,
+                    device_inputs=device_inputs) for i, s in enumerate(shapes)]
     got = {}
 
     def keep(req, sim):
