@@ -197,8 +197,8 @@ struct mgk_ctx {
         for (int a = 0; a < 3; ++a) { f(tw64[a]); f(tw32[a]); }
         fh(hctl); fh(hstage);
         f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.bsum); f(K.bincl);
-        f(K.bunst); f(K.bunst_off); f(K.binc); f(kctl); f(kstage); f(unst_idx); f(unst_q);
-        fh(hkctl); fh(hkstage);
+        f(K.bunst); f(K.bunst_off); f(K.binc); f(kstage); f(unst_idx); f(unst_q);   // kctl: inside ctl's block
+        fh(hkstage);
         f(d_lsp); f(d_kp); f(d_ea); f(pk); f(pkm); f(psu);
         fh(h_lsp); fh(h_kp); fh(h_ea);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -403,11 +403,10 @@ struct mgk_ctx {
                 // the final pass runs with the batch: one host sync per solve
                 // when the prediction holds; it is repeated if more iterations follow
                 ls_enqueue_final(p, stream);
-                MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-                if (kctl_pending) {
-                    MGK_HIP_CHECK(hipMemcpyAsync(hkctl, kctl, sizeof(KmcCtl), hipMemcpyDeviceToHost, stream));
-                    kctl_pending = false;
-                }
+                // the KMC control block sits right after the LS one (CtlPair)
+                MGK_HIP_CHECK(hipMemcpyAsync(hctl, ctl, kctl_pending ? sizeof(CtlPair) : sizeof(Ctl),
+                                             hipMemcpyDeviceToHost, stream));
+                kctl_pending = false;
                 MGK_HIP_CHECK(hipStreamSynchronize(stream));
                 if (hctl->done || enqueued >= max_iter) { final_done = true; break; }
                 batch = std::min(max_iter - enqueued, std::max(refine ? 1 : 2, batch / 2));
@@ -551,8 +550,15 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             c->cg = c->dal<float>((size_t)4 * C * N);
             c->corr = c->dal<float>((size_t)C * N);
             c->partials = c->dal<double>((size_t)ls_partial_slots(g) + 64);
-            c->ctl = c->dal<Ctl>(1);
-            c->hctl = halloc<Ctl>(1);
+            // the LS and KMC control blocks side by side, so one copy brings both back
+            CtlPair* pair = c->dal<CtlPair>(1);
+            CtlPair* hpair = halloc<CtlPair>(1);
+            MGK_HIP_CHECK(hipMemset(pair, 0, sizeof(CtlPair)));
+            std::memset(hpair, 0, sizeof(CtlPair));
+            c->ctl = &pair->ls;
+            c->hctl = &hpair->ls;
+            c->kctl = &pair->kmc;
+            c->hkctl = &hpair->kmc;
             c->d_lsp = c->dal<LsParams>(2);
             c->h_lsp = halloc<LsParams>(2);
             c->d_kp = c->dal<KmcParams>(1);
@@ -819,8 +825,7 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         if (c.kmc) {
             auto f = [](void* p) { if (p) (void)hipFree(p); };
             f(c.K.Q); f(c.K.Q0); f(c.K.cat); f(c.K.soft); f(c.K.last_t); f(c.K.prev_dir); f(c.K.bsum);
-            f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kctl); f(c.kstage); f(c.unst_idx); f(c.unst_q);
-            if (c.hkctl) (void)hipHostFree(c.hkctl);
+            f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kstage); f(c.unst_idx); f(c.unst_q);
             if (c.hkstage) (void)hipHostFree(c.hkstage);
             c.dev_bytes -= c.kmc_bytes;
             c.kmc_bytes = 0;
@@ -860,9 +865,7 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         K.bunst = c.dal<int>(nb, true);
         K.bunst_off = c.dal<int>(nb, true);
         K.binc = c.dal<int>(nb, true);
-        c.kctl = c.dal<KmcCtl>(1, true);
-        c.hkctl = halloc<KmcCtl>(1);
-        K.kctl = c.kctl;
+        K.kctl = c.kctl;   // allocated with the LS control block
         c.kstage = c.dal<double>((size_t)kp->M * (C + 1), true);
         c.hkstage = halloc<double>((size_t)kp->M * (C + 1));
         K.staging = c.kstage;
@@ -1081,8 +1084,8 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         if (pm && !c.unit_ready) throw ArgError("sigma_macro_unit not set (mgk_patch_unit)");
         int has_cat = 0, has_q0 = 0;
         const int n_modes = stage_redraw(c, in->new_catalog, in->new_Q0, in->first_mode, in->n_modes, &has_cat, &has_q0);
-        // LS control block: the device fills the imposed strain and the shift
-        c.init_ctl(in->max_iter, in->tol);
+        // LS control block: reset on the device by k_event_prep (init_ctl),
+        // which also fills the imposed strain and the shift
         const int start = c.start_mode(true);
         EventArgs ea{};
         ea.time = in->time;
@@ -1096,18 +1099,23 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         ea.keep_shift = (start == LS_START_KEEP) ? 1 : 0;
         ea.u2 = in->u2;
         ea.active = 1;
-        c.hctl->keep_shift = ea.keep_shift;
-        MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
+        ea.init_ctl = 1;
+        ea.max_iter = in->max_iter;
+        ea.tol = in->tol;
+        for (int k = 0; k < 6; ++k) ea.mean_eps[k] = c.mean_eps[k];
         const KmcLaunch KL = c.kml();
         const LsLaunch L = c.lsl();
         kmc_enqueue_rates(KL, in->time, nullptr, c.stream);
         kmc_enqueue_event_prep(KL, c.ctl, nullptr, ea, nullptr, c.stream);
         kmc_enqueue_select(KL, in->u2, nullptr, c.stream);
         if (c.precision == MGK_FP64) kmc_enqueue_select_fallback(KL, 1, c.stream);
-        if (n_modes > 0)
+        // redraw values: kernel arguments when they fit, else the staging copy
+        const bool by_args = n_modes > 0 && kmc_redraw_fits_args(n_modes, c.g.C);
+        if (n_modes > 0 && !by_args)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1);
+        kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1,
+                         by_args ? c.hkstage : nullptr);
         if (pm) kmc_enqueue_patch(KL, c.pp, -1, nullptr, 0.0, 1, in->strain_rate, c.stream);
         if (pm == 1) {
             // patched, not solved: the control block alone comes back; eps_macro
@@ -1586,7 +1594,7 @@ long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_vo
     b += 3 * r(C * N, 8);                                 // eps, sig, eps_p
     b += r(2 * C * NS, 8) + r(4 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
     b += r((long long)ls_partial_slots(g) + 64, 8);
-    b += (long long)sizeof(Ctl) + 2 * (long long)sizeof(LsParams) + (long long)sizeof(KmcParams) +
+    b += (long long)sizeof(CtlPair) + 2 * (long long)sizeof(LsParams) + (long long)sizeof(KmcParams) +
          (long long)sizeof(EventArgs);
     const int lens[3] = {g.nf, g.nm, g.nl};
     for (int a = 0; a < 3; ++a) b += r(2LL * lens[a], 8) + r(2LL * lens[a], 4);   // twiddles
@@ -1596,7 +1604,7 @@ long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_vo
         b += per_voxel_catalog ? r(C * nm, 8) : r(C * M, 8);
         b += r(2 * N, 8) + r(N, 8) + r(C * N, 8);         // softening, last event time, previous direction
         b += 2 * r(nb, 8) + 3 * r(nb, 4);                 // block sums and counts
-        b += (long long)sizeof(KmcCtl) + r((long long)M * (C + 1), 8);
+        b += r((long long)M * (C + 1), 8);
         b += r(nm, 4) + r(nm, 8);                         // unstable list
     }
     return b;

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_select_seq(const KmcParams p_arg, const
 // ---------------------------------------------------------------------------
 template <int D, int C, bool BATCH>
 __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, int flat, double time, int first_mode,
-                       int n_modes, int has_cat, int has_q0, int time_from_ctl) {
+                       int n_modes, int has_cat, int has_q0, int time_from_ctl, const RedrawArgs ra, int ra_on) {
     const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     // flat < 0: the mode the last selection chose, if that attempt produced
@@ -501,6 +501,7 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long v = flat / ph.M;
+    const double* stage = ra_on ? ra.v : p.staging;
     double c[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) c[k] = p.cat_shared ? p.cat[k * ph.M + flat % ph.M] : p.cat[k * nmodes + flat];
@@ -547,8 +548,8 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
     for (int j = 0; j < n_modes; ++j) {
         const long mi = v * ph.M + first_mode + j;
         if (has_cat && !p.cat_shared)
-            for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = p.staging[j * C + k];
-        if (has_q0) p.Q0[mi] = p.staging[n_modes * C + j];
+            for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = stage[j * C + k];
+        if (has_q0) p.Q0[mi] = stage[n_modes * C + j];
     }
 }
 
@@ -610,6 +611,16 @@ __global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict_
     Ctl* ctl = BATCH ? P[blockIdx.y].ctl : ctl_arg;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     KmcCtl* kc = p.kctl;
+    if (ea.init_ctl) {
+        // the host's control-block reset for a solve: zero, then max_iter, tol,
+        // the field mean and the warm-start mode
+        double* w = reinterpret_cast<double*>(ctl);
+        for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(double)); ++k) w[k] = 0.0;
+        ctl->max_iter = ea.max_iter;
+        ctl->tol = ea.tol;
+        ctl->keep_shift = ea.keep_shift;
+        for (int c = 0; c < 6; ++c) ctl->mean_eps[c] = ea.mean_eps[c];
+    }
     const double R = kc->total_rate;
     int outcome;
     double tw = 0.0;
@@ -701,20 +712,29 @@ void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipS
 
 template <int D, int C, bool BATCH>
 static void enqueue_flip_t(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
-                           int has_q0, hipStream_t s, int time_from_ctl) {
+                           int has_q0, hipStream_t s, int time_from_ctl, const RedrawArgs& ra, int ra_on) {
     hipLaunchKernelGGL((k_flip<D, C, BATCH>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, flat, time, first_mode, n_modes,
-                       has_cat, has_q0, time_from_ctl);
+                       has_cat, has_q0, time_from_ctl, ra, ra_on);
 }
 
+bool kmc_redraw_fits_args(int n_modes, int C) { return n_modes * (C + 1) <= kRedrawArgs; }
+
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
-                      int has_q0, hipStream_t s, int time_from_ctl) {
+                      int has_q0, hipStream_t s, int time_from_ctl, const double* host_redraw) {
     if (l.batch && (flat >= 0 || !time_from_ctl)) throw ArgError("a batch flips its device-selected modes");
+    RedrawArgs ra{};
+    int ra_on = 0;
+    if (host_redraw) {
+        if (l.batch || !kmc_redraw_fits_args(n_modes, l.g.C)) throw ArgError("redraw values too many for kernel arguments");
+        for (int k = 0; k < n_modes * (l.g.C + 1); ++k) ra.v[k] = host_redraw[k];
+        ra_on = 1;
+    }
     if (l.g.dims == 2) {
-        if (l.batch) enqueue_flip_t<2, 3, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
-        else enqueue_flip_t<2, 3, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        if (l.batch) enqueue_flip_t<2, 3, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, 0);
+        else enqueue_flip_t<2, 3, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, ra_on);
     } else {
-        if (l.batch) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
-        else enqueue_flip_t<3, 6, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl);
+        if (l.batch) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, 0);
+        else enqueue_flip_t<3, 6, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, ra_on);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }

@@ -25,6 +25,14 @@ struct KmcCtl {
     int pad_;
 };
 
+// The LS and KMC control blocks of one context, adjacent so that one copy
+// reads both back.
+struct CtlPair {
+    Ctl ls;
+    KmcCtl kmc;
+};
+static_assert(sizeof(Ctl) % 8 == 0, "Ctl is a whole number of doubles");
+
 // Fast-patching kernels in device memory (kmc_solver.hpp:276-332): per basis
 // b, the stress response K[b][c][w] of a unit deviatoric eigenstrain, on a
 // window of (2R+1)^2 or (2R+1)^2 (2R+1 | 1) cells, and the missing mean
@@ -50,6 +58,17 @@ struct EventArgs {
     int cascade_check;     // instability_mode == "cascade": unstable modes pre-empt the event
     int keep_shift;        // the field mean is known: shift = eps_bar - mean (else a mean pass follows)
     int active;            // 0: no attempt for this trajectory (MGK_EVENT_SKIPPED)
+    int init_ctl;          // 1: the attempt resets the LS control block itself (max_iter, tol, mean)
+    int max_iter;
+    double tol;
+    double mean_eps[6];    // field mean of the last solve (used when keep_shift)
+};
+
+// Redraw values passed by value to the flip (no staging copy): n_modes
+// catalog rows of C, then n_modes Q0 values.
+constexpr int kRedrawArgs = 112;
+struct RedrawArgs {
+    double v[kRedrawArgs];
 };
 
 // Per-trajectory KMC physics scalars (device copy of mgk_kmc_params plus
@@ -110,8 +129,11 @@ void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hip
 void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipStream_t s);
 void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s);
 // time_from_ctl: the flip time is the device-decided event's time_new
+// host_redraw: redraw values passed by value (single trajectory, at most
+// kRedrawArgs doubles) instead of the device staging buffer
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
-                      int has_q0, hipStream_t s, int time_from_ctl = 0);
+                      int has_q0, hipStream_t s, int time_from_ctl = 0, const double* host_redraw = nullptr);
+bool kmc_redraw_fits_args(int n_modes, int C);
 void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s);
 // Fast-patching stress update: sig += sunit * pred (pred != 0), the kernel
 // superposition around voxel for the flipped strain c (symmetric
