@@ -394,10 +394,11 @@ struct mgk_ctx {
                     const int k = enqueued + b;
                     const int flags = ((k == 0 && warm) ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
                     ls_enqueue_iteration(p, LS_FULL, flags, stream);
-                    if (refine)
+                    if (refine) {
+                        const int sp = spec_from();
                         for (int j = 0; j < J; ++j)
-                            ls_enqueue_iteration(p, LS_CG, j == 0 ? LS_FLAG_CG_FIRST
-                                                          : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), stream);
+                            ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp), stream);
+                    }
                 }
                 enqueued += batch;
                 // the final pass runs with the batch: one host sync per solve
@@ -455,6 +456,21 @@ struct mgk_ctx {
         for (int w = 0; w < ns; ++w) sk_sig[w] = 2.0 * mean_mu * bar_skew[w];
         join(hctl->mean_eps, bar_skew, eps_macro);
         join(hctl->mean_sig, sk_sig, sig_macro);
+    }
+
+    // CG steps from which a step is speculative: the fewest any of the last
+    // few single-cycle solves used (no history: none).  A speculative step
+    // tests the control block before loading its inputs, so that past
+    // convergence it costs little more than its launch.
+    int spec_from() const {
+        int mn = 1 << 30;
+        for (int h : inner_hist)
+            if (h > 0) mn = std::min(mn, h);
+        return mn;
+    }
+    static int cg_flags(int j, int spec_from) {
+        const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
+        return f | (j >= spec_from ? LS_FLAG_SPEC : 0);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -1675,7 +1691,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     const Geom g = c0.g;
     const int Dd = c0.D(), F = Dd * Dd, C = g.C;
     const bool refine = (c0.precision == MGK_FP32_REFINE);
-    int max_iter = 0, J = 1, predicted = 1;
+    int max_iter = 0, J = 1, predicted = 1, sp = 1 << 30;
     double tol = 0.0;
     int first_mode = 0, n_modes = -1, has_cat = 0, has_q0 = 0;
     bool all_keep = true;
@@ -1738,6 +1754,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         ea.active = on ? 1 : 0;
         b.h_times[y] = on ? e.time : 0.0;
         J = std::max(J, c.inner_per_outer);
+        sp = std::min(sp, c.spec_from());
         predicted = std::max(predicted, c.predicted);
     }
     if (n_active == 0) throw ArgError("no active trajectory in the batch call");
@@ -1751,8 +1768,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
             const int flags = (k == 0 ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
             ls_enqueue_iteration(L, LS_FULL, flags, st);
             if (refine)
-                for (int j = 0; j < J; ++j)
-                    ls_enqueue_iteration(L, LS_CG, j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE), st);
+                for (int j = 0; j < J; ++j) ls_enqueue_iteration(L, LS_CG, mgk_ctx::cg_flags(j, sp), st);
         }
         ls_enqueue_final(L, st);
         MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, st));
@@ -1775,7 +1791,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         iterations(0, nb0, st);
     };
     if (b.use_graphs) {
-        const std::vector<int> key{J, nb0, all_keep ? 1 : 0, n_modes, first_mode, has_cat, has_q0, refine ? 1 : 0};
+        const std::vector<int> key{J, nb0, all_keep ? 1 : 0, n_modes, first_mode, has_cat, has_q0, refine ? 1 : 0, std::min(sp, J)};
         auto it = b.graphs.find(key);
         if (it == b.graphs.end()) {
             std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);

@@ -38,11 +38,93 @@ __device__ __forceinline__ double mode_rate(const KmcPhys& ph, double q, bool& i
 
 // ---------------------------------------------------------------------------
 // update_barriers + rates (kmc_solver.hpp:341-410, :783-806; 3D :1340-1411,
-// :1796-1818).  One lane per (voxel, mode), kRateBlock modes per block of
-// kRateThreads lanes: every input of a lane's modes is loaded before any is
-// used.  The catalog is per (voxel, mode) or, when every voxel holds the same
-// M strains, one shared copy of M entries.
+// :1796-1818), split into the voxel part (softening state) and the mode part
+// (directional factor, resolved work, barrier), each in the reference's
+// operation order.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double voxel_gbase(const KmcPhys& ph, double time, double gp, double g_t, double t_last) {
+    double g_t_curr = 0.0;
+    if (g_t > 0.0) {
+        if (ph.tau_inf) g_t_curr = g_t;
+        else if (t_last == -INFINITY) g_t_curr = 0.0;
+        else {
+            double dt = time - t_last;
+            if (dt < 0.0) dt = 0.0;
+            g_t_curr = g_t * exp(-dt / ph.tau);
+        }
+    }
+    double g_base = gp + g_t_curr;
+    if (ph.cap > 0.0 && g_base > ph.cap) g_base = ph.cap;
+    return g_base;
+}
+
+// Q of one mode.  pd: the voxel's previous direction (read when
+// directional); eg: exp(-g_base), used when not directional (then
+// g_eff = 1.0 * g_base = g_base exactly).
+template <int D, int C, class S, class V, class P>
+__device__ __forceinline__ double mode_barrier(const KmcPhys& ph, double g_base, double eg, double q0, const S& sig,
+                                               const V& cat, const P& pd) {
+    const double GPa_nm3_to_eV = 6.241509;
+    double decay = eg;
+    if (ph.directional) {
+        double modifier = 1.0;
+        double dot = 0.0, nmq = 0.0, npq = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                const int c = comp_of(D, a, b);
+                const double vm = cat(c), vp = pd(c);
+                dot += vm * vp;
+                nmq += vm * vm;
+                npq += vp * vp;
+            }
+        const double npv = sqrt(npq), nmv = sqrt(nmq);
+        if (!(npv < 1e-12 || nmv < 1e-12)) {
+            const double ct = dot / (nmv * npv);
+            modifier = ((1.0 + ct) * (1.0 + ct)) / 4.0;
+        }
+        const double g_eff = modifier * g_base;
+        decay = exp(-g_eff);
+    }
+    double w_sum = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            const int c = comp_of(D, a, b);
+            w_sum += sig(c) * cat(c);
+        }
+    const double w = 0.5 * ph.volume * (w_sum / 1e9) * GPa_nm3_to_eV;
+    return q0 * decay - w;
+}
+
+// Block totals of rates, unstable and included counts (thread 0 writes).
+__device__ __forceinline__ void rate_block_totals(const KmcParams& p, double lsum, int lun, int linc) {
+    constexpr int NW = kRateThreads / 64;
+    __shared__ double sd[NW];
+    __shared__ int su[NW], si[NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    lsum = wsum(lsum);
+    lun = wsum(lun);
+    linc = wsum(linc);
+    if (lane == 0) { sd[wid] = lsum; su[wid] = lun; si[wid] = linc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        int tu = 0, ti = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) { t += sd[w]; tu += su[w]; ti += si[w]; }
+        p.bsum[blockIdx.x] = t;
+        p.bunst[blockIdx.x] = tu;
+        p.binc[blockIdx.x] = ti;
+    }
+}
+
+// One lane per (voxel, mode), kRateBlock modes per block of kRateThreads
+// lanes; every input of a lane's modes is loaded before any is used.  The
+// catalog is per (voxel, mode) or, when every voxel holds the same M strains,
+// one shared copy of M entries.  For M < kStageMinM.
 template <int D, int C, bool BATCH>
 __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, const KmcParams* __restrict__ PK,
                                                         double time, const double* __restrict__ times) {
@@ -53,7 +135,6 @@ __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, c
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long base = (long)blockIdx.x * kRateBlock;
-    const double GPa_nm3_to_eV = 6.241509;
 
     long mi[K], vi[K];
     bool ok[K];
@@ -86,75 +167,109 @@ __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, c
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!ok[k]) continue;
-        // softening state of the voxel
-        const double g_t = gt[k];
-        double g_t_curr = 0.0;
-        if (g_t > 0.0) {
-            const double t_last = tl[k];
-            if (ph.tau_inf) g_t_curr = g_t;
-            else if (t_last == -INFINITY) g_t_curr = 0.0;
-            else {
-                double dt = time - t_last;
-                if (dt < 0.0) dt = 0.0;
-                g_t_curr = g_t * exp(-dt / ph.tau);
-            }
-        }
-        double g_base = gp[k] + g_t_curr;
-        if (ph.cap > 0.0 && g_base > ph.cap) g_base = ph.cap;
-        double modifier = 1.0;
-        if (ph.directional) {
-            double dot = 0.0, nmq = 0.0, npq = 0.0;
-#pragma unroll
-            for (int a = 0; a < D; ++a)
-#pragma unroll
-                for (int b = 0; b < D; ++b) {
-                    const int c = comp_of(D, a, b);
-                    const double vm = cat[k][c], vp = pd[k][c];
-                    dot += vm * vp;
-                    nmq += vm * vm;
-                    npq += vp * vp;
-                }
-            const double npv = sqrt(npq), nmv = sqrt(nmq);
-            if (!(npv < 1e-12 || nmv < 1e-12)) {
-                const double ct = dot / (nmv * npv);
-                modifier = ((1.0 + ct) * (1.0 + ct)) / 4.0;
-            }
-        }
-        const double g_eff = modifier * g_base;
-        double w_sum = 0.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int b = 0; b < D; ++b) {
-                const int c = comp_of(D, a, b);
-                w_sum += s[k][c] * cat[k][c];
-            }
-        const double w = 0.5 * ph.volume * (w_sum / 1e9) * GPa_nm3_to_eV;
-        const double q = q0[k] * exp(-g_eff) - w;
+        const double g_base = voxel_gbase(ph, time, gp[k], gt[k], tl[k]);
+        const double eg = ph.directional ? 0.0 : exp(-g_base);
+        const double q = mode_barrier<D, C>(ph, g_base, eg, q0[k], [&](int c) { return s[k][c]; },
+                                            [&](int c) { return cat[k][c]; }, [&](int c) { return pd[k][c]; });
         p.Q[mi[k]] = q;
         bool incl;
         const double r = mode_rate(ph, q, incl);
         if (incl) { lsum += r; linc++; }
         if (q < ph.thr) lun++;
     }
-    constexpr int NW = kRateThreads / 64;
-    __shared__ double sd[NW];
-    __shared__ int su[NW], si[NW];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    lsum = wsum(lsum);
-    lun = wsum(lun);
-    linc = wsum(linc);
-    if (lane == 0) { sd[wid] = lsum; su[wid] = lun; si[wid] = linc; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        int tu = 0, ti = 0;
+    rate_block_totals(p, lsum, lun, linc);
+}
+
+// The same pass for M >= kStageMinM: the block's voxels (at most
+// kRateBlock / M + 2) are staged in LDS once -- stress, previous direction,
+// softening base and its exponential -- so a lane loads only its modes' Q0
+// (and catalog rows when they are per voxel).  Same arithmetic, same bits.
+template <int D, int C, bool BATCH>
+__global__ __launch_bounds__(kRateThreads) void k_rates_staged(const KmcParams p_arg, const KmcParams* __restrict__ PK,
+                                                               double time, const double* __restrict__ times) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    if (BATCH) time = times[blockIdx.y];
+    constexpr int K = kRateBlock / kRateThreads;
+    constexpr int NV = kRateBlock / kStageMinM + 2;
+    static_assert(NV <= kRateThreads, "one staged voxel per lane");
+    const Geom g = p.g;
+    const KmcPhys ph = p.ph;
+    const long nmodes = g.N * ph.M;
+    const long base = (long)blockIdx.x * kRateBlock;
+    __shared__ double s_sig[C][NV], s_pd[C][NV], s_gb[NV], s_eg[NV];
+    __shared__ double s_cat[C][kStageCat];
+
+    const unsigned M = (unsigned)ph.M;
+    const long v0 = (long)((unsigned)base / M);
+    const long last = (base + kRateBlock < nmodes ? base + kRateBlock : nmodes) - 1;
+    const long vend = (long)((unsigned)last / M);
+    const int nv = (int)(vend - v0 + 1);
+    const bool cat_lds = p.cat_shared && ph.M <= kStageCat;   // M*C <= kRateThreads entries
+    // per-mode inputs
+    long mi[K];
+    int vj[K];
+    bool ok[K];
+    double q0[K], cat[K][C];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) { t += sd[w]; tu += su[w]; ti += si[w]; }
-        p.bsum[blockIdx.x] = t;
-        p.bunst[blockIdx.x] = tu;
-        p.binc[blockIdx.x] = ti;
+    for (int k = 0; k < K; ++k) {
+        const long i = base + k * kRateThreads + threadIdx.x;
+        ok[k] = i < nmodes;
+        mi[k] = ok[k] ? i : base;
+        const long v = (long)((unsigned)mi[k] / M);
+        vj[k] = (int)(v - v0);
+        q0[k] = p.Q0[mi[k]];
+        if (!p.cat_shared) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * nmodes + mi[k]];
+        } else if (!cat_lds) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * ph.M + (mi[k] - v * ph.M)];
+        }
     }
+    // per-voxel inputs, one voxel per lane
+    const int j = threadIdx.x;
+    const long v = v0 + (j < nv ? j : 0);
+    double gp = p.soft[v], gt = p.soft[g.N + v], tl = p.last_t[v], sv[C], pv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) sv[c] = p.sig[c * g.N + v];
+    if (ph.directional) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) pv[c] = p.prev_dir[c * g.N + v];
+    }
+    double cv = 0.0;
+    if (cat_lds && j < ph.M * C) cv = p.cat[j];   // [C][M] order
+    if (j < nv) {
+        const double g_base = voxel_gbase(ph, time, gp, gt, tl);
+        s_gb[j] = g_base;
+        s_eg[j] = ph.directional ? 0.0 : exp(-g_base);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            s_sig[c][j] = sv[c];
+            if (ph.directional) s_pd[c][j] = pv[c];
+        }
+    }
+    if (cat_lds && j < ph.M * C) s_cat[j / ph.M][j % ph.M] = cv;
+    __syncthreads();
+
+    double lsum = 0.0;
+    int lun = 0, linc = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!ok[k]) continue;
+        const int jj = vj[k];
+        const int m = (int)(mi[k] - (v0 + jj) * (long)ph.M);
+        const double q = cat_lds
+            ? mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c][jj]; },
+                                 [&](int c) { return s_cat[c][m]; }, [&](int c) { return s_pd[c][jj]; })
+            : mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c][jj]; },
+                                 [&](int c) { return cat[k][c]; }, [&](int c) { return s_pd[c][jj]; });
+        p.Q[mi[k]] = q;
+        bool incl;
+        const double r = mode_rate(ph, q, incl);
+        if (incl) { lsum += r; linc++; }
+        if (q < ph.thr) lun++;
+    }
+    rate_block_totals(p, lsum, lun, linc);
 }
 
 // Inclusive wave scan (lane order) of a double and two ints.
@@ -529,19 +644,51 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
         const int nx = g.nx, ny = g.ny, nz = g.nz;
         if (D == 2) { y = (int)(v % ny); x = (int)(v / ny); }
         else { z = (int)(v % nz); const long t = v / nz; y = (int)(t % ny); x = (int)(t / ny); }
-        const int zr = (D == 3) ? 1 : 0;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -zr; dz <= zr; ++dz) {
-                    if (dx == 0 && dy == 0 && dz == 0) continue;
-                    const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny;
-                    const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
-                    const long nb = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
-                    double gn = p.soft[nb] + ph.nfrac * ph.jp * sum_sq;
-                    if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
-                    p.soft[nb] = gn;
-                    p.soft[g.N + nb] += ph.nfrac * ph.jt * sum_sq;
-                }
+        constexpr int zr = (D == 3) ? 1 : 0;
+        if (nx >= 3 && ny >= 3 && (D == 2 || nz >= 3)) {
+            // every neighbour is a distinct voxel other than v: load them all,
+            // then update (the same arithmetic per neighbour)
+            constexpr int NB = (D == 2) ? 8 : 26;
+            long nbs[NB];
+            double g0[NB], g1[NB];
+            int q = 0;
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dz = -zr; dz <= zr; ++dz) {
+                        if (dx == 0 && dy == 0 && dz == 0) continue;
+                        const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny;
+                        const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
+                        nbs[q++] = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
+                    }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                g0[k] = p.soft[nbs[k]];
+                g1[k] = p.soft[g.N + nbs[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                double gn = g0[k] + ph.nfrac * ph.jp * sum_sq;
+                if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
+                p.soft[nbs[k]] = gn;
+                p.soft[g.N + nbs[k]] = g1[k] + ph.nfrac * ph.jt * sum_sq;
+            }
+        } else {
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -zr; dz <= zr; ++dz) {
+                        if (dx == 0 && dy == 0 && dz == 0) continue;
+                        const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny;
+                        const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
+                        const long nb = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
+                        double gn = p.soft[nb] + ph.nfrac * ph.jp * sum_sq;
+                        if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
+                        p.soft[nb] = gn;
+                        p.soft[g.N + nb] += ph.nfrac * ph.jt * sum_sq;
+                    }
+        }
     }
 #pragma unroll
     for (int k = 0; k < C; ++k) p.prev_dir[k * g.N + v] = c[k];
@@ -677,10 +824,18 @@ int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlo
 
 template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
     const int nb = kmc_nblocks(l.g.N * l.M);
-    if (l.g.dims == 2)
-        hipLaunchKernelGGL((k_rates<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
-    else
-        hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+    const bool staged = l.M >= kStageMinM;
+    if (l.g.dims == 2) {
+        if (staged)
+            hipLaunchKernelGGL((k_rates_staged<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        else
+            hipLaunchKernelGGL((k_rates<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+    } else {
+        if (staged)
+            hipLaunchKernelGGL((k_rates_staged<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        else
+            hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+    }
     hipLaunchKernelGGL(k_rates_scan<BATCH>, dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, nb);
 }
 

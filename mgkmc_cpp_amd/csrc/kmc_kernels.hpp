@@ -108,6 +108,8 @@ struct KmcParams {
 
 constexpr int kRateBlock = 1024;   // modes per rate block
 constexpr int kRateThreads = 512;  // lanes per rate block (2 modes each)
+constexpr int kStageMinM = 4;      // from this many modes per voxel the rate pass stages voxels in LDS
+constexpr int kStageCat = 64;      // shared catalogs of at most this many modes are staged too
 
 // One launch over B trajectories of one shape and mode count: row
 // blockIdx.y of every grid works on the trajectory whose parameter block is

@@ -250,6 +250,7 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    if ((flags & LS_FLAG_SPEC) && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
     const Geom g = p.g;
@@ -417,13 +418,15 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
 // gather and store touch consecutive columns of one row.
 // ---------------------------------------------------------------------------
 template <typename SR, int NM, int DIR, bool BATCH>
-__global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, const LsParams* __restrict__ P, int mode) {
+__global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
+                                                        int speculative) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int T = FftPlan<NM>::threads;
     constexpr int L = mid_cols<SR, NM>();
     constexpr int NMP = padded<NM>();
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    if (speculative && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* work = tw + NM;
     const Geom g = p.g;
@@ -450,7 +453,8 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
 template <typename SR, int NF, int C, bool BATCH>
-__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode) {
+__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
+                                                                int speculative) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int D = (C == 3) ? 2 : 3;
     constexpr int T = FftPlan<NF>::threads;
@@ -459,6 +463,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    if (speculative && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* work = tw + NF;
     const Geom g = p.g;
@@ -560,6 +565,7 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    if ((flags & LS_FLAG_SPEC) && stop) return;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
     __shared__ double red[8 * 4 * 2];
@@ -924,7 +930,7 @@ static void launch_inv_last(const LsLaunch& p, int flags, hipStream_t s) {
 }
 
 template <typename SR, int DIR, bool BATCH>
-static void launch_mid(const LsLaunch& p, hipStream_t s, int mode) {
+static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0) {
     const int nm = p.g.nm;
     const long total = (long)p.g.nf * p.g.nlh * p.g.C;
 #define MGK_CASE(NN)                                                                                       \
@@ -932,7 +938,7 @@ static void launch_mid(const LsLaunch& p, hipStream_t s, int mode) {
         constexpr int L = mid_cols<SR, NN>();                                                              \
         const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);                              \
         hipLaunchKernelGGL((k_mid<SR, NN, DIR, BATCH>), dim3(blocks_for(total, L), p.B), dim3(L * FftPlan<NN>::threads), \
-                           sm, s, p.h, p.P, mode);                                                                \
+                           sm, s, p.h, p.P, mode, spec);                                                          \
         return;                                                                                            \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -941,7 +947,7 @@ static void launch_mid(const LsLaunch& p, hipStream_t s, int mode) {
 }
 
 template <typename SR, int C, bool BATCH>
-static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode) {
+static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int spec = 0) {
     const int nf = p.g.nf;
     const long nxi = (long)p.g.nm * p.g.nlh;
 #define MGK_CASE(NN)                                                                                 \
@@ -949,7 +955,7 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode) {
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
         hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH>), dim3(blocks_for(nxi, G), p.B),                     \
-                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode);                      \
+                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode, spec);                \
         return;                                                                                      \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -974,10 +980,11 @@ template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int f
 }
 
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
+    const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
     launch_fwd_last<float, C, LS_CG, BATCH>(p, flags, s);
-    if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG);
-    launch_first_gamma<float, C, BATCH>(p, s, LS_CG);
-    if (C == 6 && p.g.nm > 1) launch_mid<float, +1, BATCH>(p, s, LS_CG);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG, spec);
+    launch_first_gamma<float, C, BATCH>(p, s, LS_CG, spec);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, +1, BATCH>(p, s, LS_CG, spec);
     launch_inv_last<float, C, LS_CG, BATCH>(p, flags, s);
 }
 

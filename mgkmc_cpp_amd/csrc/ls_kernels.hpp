@@ -12,7 +12,8 @@ enum {
     LS_FLAG_CORR = 2,         // FULL: add the pending FP32 correction (refinement path)
     LS_FLAG_CG_FIRST = 4,     // CG step 0 of a cycle: operator on r0 only
     LS_FLAG_CG_UPDATE1 = 8,   // CG step 1: p = r, s = w, x = alpha p (no history)
-    LS_FLAG_CG_UPDATE = 16    // CG step >= 2: full recurrence
+    LS_FLAG_CG_UPDATE = 16,   // CG step >= 2: full recurrence
+    LS_FLAG_SPEC = 32         // CG step past the likely count: test `done` before loading anything
 };
 
 // how a solve starts
