@@ -33,7 +33,20 @@ PRELOAD_STRAIN = 0.01
 
 
 PASS_NAMES = {0: "k_fwd_last", 1: "k_mid<fwd>", 2: "k_first_gamma", 3: "k_mid<inv>", 4: "k_inv_last",
-              5: "k_fwd_last+cg_update"}
+              5: "k_fwd_last+cg_update", 6: "k_fwd_plane+cg_update", 7: "k_first_gamma<plane>", 8: "k_inv_plane"}
+
+
+def plane_passes():
+    """The FP32 CG steps run as plane passes on the 64^3 grid (square
+    64 x 64 planes) unless MGK_LS_PLANE=0."""
+    return os.environ.get("MGK_LS_PLANE", "1") != "0"
+
+
+def _passes(fp32):
+    """Pipeline passes of one precision as the event loop runs them."""
+    if not fp32:
+        return [0, 1, 2, 3, 4]
+    return [6, 7, 8] if plane_passes() else [0, 1, 2, 3, 4, 5]
 
 
 def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
@@ -44,9 +57,9 @@ def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
     spec = 6 * h * cs                # one half-spectrum tensor field per voxel
     if pass_id == 0:                 # y, dlam, dmu | eps, eps_p, lam, mu  -> spectrum
         return ((6 * 4 + 2 * 4) if fp32 else (6 * 8 + 6 * 8 + 2 * 8)) + spec
-    if pass_id == 5:                 # r w p s x read, p s x r written, dlam dmu -> spectrum
+    if pass_id in (5, 6):            # r w p s x read, p s x r written, dlam dmu -> spectrum
         return 5 * 6 * 4 + 4 * 6 * 4 + 2 * 4 + spec
-    if pass_id == 4:                 # spectrum -> w (reads r) | eps_new (reads eps, writes the FP32 increment)
+    if pass_id in (4, 8):            # spectrum -> w (reads r) | eps_new (reads eps, writes the FP32 increment)
         return spec + (2 * 6 * 4 if fp32 else 2 * 6 * 8 + 6 * 4)
     return 2 * spec                  # B, C, D: read and write the spectrum
 
@@ -59,7 +72,7 @@ def _launches_per_event(pass_id, fp32, info):
         return cycles
     if pass_id == 0:
         return cycles
-    if pass_id == 5:
+    if pass_id in (5, 6):
         return max(steps - cycles, 0.0)
     return steps
 
@@ -97,9 +110,7 @@ def kernel_roofline(sim, info32):
     N = int(np.prod(SHAPE))
     rows = []
     for fp32 in (False, True):
-        for pass_id in range(6):
-            if pass_id == 5 and not fp32:
-                continue
+        for pass_id in _passes(fp32):
             ms = sim.time_ls_pass(pass_id, fp32=fp32, reps=20)
             b = _algorithmic_bytes_per_voxel(pass_id, fp32, nl) * N
             n = _launches_per_event(pass_id, fp32, info32)
@@ -180,9 +191,7 @@ def probe():
     E, nu = configs.material(SHAPE, 4.0, 42)
     ctx = Context(SHAPE, E, nu, precision=abi.MGK_FP32_REFINE)
     for fp32 in (False, True):
-        for pass_id in range(6):
-            if pass_id == 5 and not fp32:
-                continue
+        for pass_id in _passes(fp32):
             ctx.time_ls_pass(pass_id, fp32=fp32, reps=4)
             print(f"[probe] pass {pass_id} {'fp32' if fp32 else 'fp64'}: 5 launches", flush=True)
     ctx.close()

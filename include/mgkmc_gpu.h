@@ -267,7 +267,10 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms);
  * 1: forward c2c along the middle axis (3D), 2: c2c + Green operator +
  * inverse c2c along the first axis, 3: inverse c2c along the middle axis,
  * 4: c2r along the last axis + update, 5: pass 0 with the FP32 CG vector
- * update, FP32+refine contexts only), FP64 (fp32 == 0) or FP32, launched
+ * update, FP32+refine contexts only; the plane passes of 3D grids with square
+ * nm = nl planes, FP32 only: 6 forward plane with the CG vector update
+ * (FP32+refine contexts), 7 pass 2 on the plane layout, 8 inverse plane
+ * without its reduction), FP64 (fp32 == 0) or FP32, launched
  * reps times back to back on the context's stream between two HIP events.
  * Leaves the spectral buffer, the FP32 CG vectors and (pass 4 in FP64) the
  * strain field undefined: a measurement context, not a simulation one. */

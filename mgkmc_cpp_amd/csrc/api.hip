@@ -112,7 +112,7 @@ struct mgk_ctx {
 
     // spectra and FP32 refinement buffers
     void* spec = nullptr;
-    float *cg = nullptr, *corr = nullptr;   // CG vectors r, w, p, s (4 x C x N) and iterate x
+    float *cg = nullptr, *corr = nullptr;   // CG vectors r, w, p, s, r', s' (6 x C x N) and iterate x
     double* partials = nullptr;
     Ctl* ctl = nullptr;
     Ctl* hctl = nullptr;
@@ -250,6 +250,8 @@ struct mgk_ctx {
         p.w32 = cg + CN;
         p.p32 = cg + 2 * CN;
         p.s32 = cg + 3 * CN;
+        p.r32b = cg + 4 * CN;
+        p.s32b = cg + 5 * CN;
         p.corr = (precision == MGK_FP32_REFINE) ? corr : nullptr;
         p.dlam32 = dlam32;
         p.dmu32 = dmu32;
@@ -272,7 +274,7 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(d_lsp + slot, h_lsp + slot, sizeof(LsParams), hipMemcpyHostToDevice, stream));
             lsp_up[slot] = true;
         }
-        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false};
+        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false, ls_plane_path(g)};
     }
     KmcLaunch kml() {
         if (!kp_up || std::memcmp(h_kp, &K, sizeof(KmcParams)) != 0) {
@@ -470,7 +472,9 @@ struct mgk_ctx {
     }
     static int cg_flags(int j, int spec_from) {
         const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
-        return f | (j >= spec_from ? LS_FLAG_SPEC : 0);
+        // step j >= 1 writes r and s to the copy step j-1 did not: step 1 to
+        // the second, step 2 back to the first, ...
+        return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j >= 2 && j % 2 == 0) ? LS_FLAG_RB : 0);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -563,7 +567,7 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             MGK_HIP_CHECK(hipMemset(c->sig, 0, sizeof(double) * C * N));
             MGK_HIP_CHECK(hipMemset(c->epsp, 0, sizeof(double) * C * N));
             c->spec = c->dal<double>((size_t)2 * C * g.NS);
-            c->cg = c->dal<float>((size_t)4 * C * N);
+            c->cg = c->dal<float>((size_t)6 * C * N);
             c->corr = c->dal<float>((size_t)C * N);
             c->partials = c->dal<double>((size_t)ls_partial_slots(g) + 64);
             // the LS and KMC control blocks side by side, so one copy brings both back
@@ -1242,8 +1246,9 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
     return guarded([&]() -> int {
         if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
-        if (pass < 0 || pass > 5 || (ctx->g.nm == 1 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
-        if (pass == 5 && (!fp32 || !ctx->corr)) throw ArgError("pass 5 needs an FP32+refine context");
+        if (pass < 0 || pass > 8 || (ctx->g.nm == 1 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        if ((pass == 5 || pass == 6) && (!fp32 || !ctx->corr)) throw ArgError("passes 5 and 6 need an FP32+refine context");
+        if (pass >= 6 && (!fp32 || !ls_plane_supported(ctx->g))) throw ArgError("plane passes: FP32 on a supported 3D grid");
         mgk_ctx& c = *ctx;
         // a control block that lets every pass run (not done, shift off)
         std::memset(c.hctl, 0, sizeof(Ctl));
@@ -1251,7 +1256,7 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
         c.hctl->max_iter = 1 << 30;
         MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
         LsParams q = c.ls_params();
-        q.corr = (pass == 5) ? c.corr : nullptr;
+        q.corr = (pass == 5 || pass == 6) ? c.corr : nullptr;
         const int mode = fp32 ? LS_CG : LS_FULL;
         q.r32 = fp32 ? c.cg : nullptr;   // the FP32 passes read and write the CG vectors
         const LsLaunch p = c.lsl(1, &q);
@@ -1608,7 +1613,7 @@ long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_vo
     long long b = 0;
     b += 4 * r(N, 8) + 2 * r(N, 4);                      // E, nu, lam, mu; dlam, dmu
     b += 3 * r(C * N, 8);                                 // eps, sig, eps_p
-    b += r(2 * C * NS, 8) + r(4 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
+    b += r(2 * C * NS, 8) + r(6 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
     b += r((long long)ls_partial_slots(g) + 64, 8);
     b += (long long)sizeof(CtlPair) + 2 * (long long)sizeof(LsParams) + (long long)sizeof(KmcParams) +
          (long long)sizeof(EventArgs);
@@ -1760,7 +1765,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     if (n_active == 0) throw ArgError("no active trajectory in the batch call");
     hipStream_t s = b.stream;
     const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0], true};
-    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true};
+    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true, ls_plane_path(g)};
     const int nb0 = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
     auto iterations = [&](int from, int count, hipStream_t st) {
         for (int q = 0; q < count; ++q) {

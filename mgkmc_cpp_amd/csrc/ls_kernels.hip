@@ -48,6 +48,7 @@
 #include "fft_lds.hpp"
 #include "ls_kernels.hpp"
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace mgk {
@@ -202,6 +203,10 @@ constexpr int pow2floor(int x) { int p = 1; while (2 * p <= x) p *= 2; return p;
 // Lanes per block of the column passes B, C and D.
 template <typename SR, int C> constexpr int col_threads() { return (sizeof(SR) == 4 && C == 6) ? 512 : 256; }
 constexpr int kColThreadsMax = 512;
+// Points per tile of the plane passes' spectrum layout: point xi of component
+// c in plane f at f*nxi*6 + (xi / kPlaneTile)*6*kPlaneTile + c*kPlaneTile +
+// xi % kPlaneTile (nxi a multiple of kPlaneTile).
+constexpr int kPlaneTile = 8;
 
 // Line pairs per block of the L passes (two real lines share one complex FFT
 // line): a power of two whose FFT lanes fit 256 and whose LDS fits the budget.
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // f).  A block owns G frequency points (m, l-bin) with all C components, i.e.
 // L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
-template <typename SR, int NF, int C, bool BATCH>
+template <typename SR, int NF, int C, bool BATCH, bool PLANE = false>
 __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
                                                                 int speculative) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
@@ -470,10 +475,18 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     const long nxi = (long)g.nm * g.nlh;
     const int l = threadIdx.x % L, t = threadIdx.x / L;
     const long xi0 = (long)blockIdx.x * G;
-    const bool active = xi0 + l / C < nxi;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
+    // Both layouts keep the C components of a run of points contiguous, so
+    // lane l reads column xi0*C + l either way: interleaved, lane l is
+    // (point l / C, component l % C); plane-tiled (tiles of kPlaneTile points
+    // per component), lane l is (point (l / (C*T8))*T8 + l % T8, component
+    // (l % (C*T8)) / T8).
+    constexpr int T8 = kPlaneTile;
+    const int gl = PLANE ? (l / (C * T8)) * T8 + l % T8 : l / C;
+    const int cl = PLANE ? (l % (C * T8)) / T8 : l % C;
+    const bool active = xi0 + gl < nxi;
     const LineRef<SR> column{spec + (active ? xi0 * C + l : 0), nxi * C};
-    const LineRef<SR> mine{work + l * NFP, 1};
+    const LineRef<SR> mine{work + (gl * C + cl) * NFP, 1};
     cx v[NF / T];
     line_gather<NF, SR>(column, t, active, v);
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
@@ -699,6 +712,267 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
     }
 }
 
+// ---------------------------------------------------------------------------
+// Plane passes (3D CG steps on square nm = nl = NP planes).  One block per
+// (component c, plane f) does pass A's work for its component and both the L
+// and the M transform in LDS, so passes B and D disappear:
+//   k_fwd_plane: CG vector update of component c, tau_c = (C - C0):r, r2c
+//                along L (two rows packed per complex line), c2c along M;
+//   k_inv_plane: c2c along M, c2r along L, w_c = r_c - eps_tilde_c and the
+//                block's share of the C0-energy products.
+// The spectrum is tiled: runs of kPlaneTile points per component, the six
+// components' runs side by side (pass C with PLANE = true reads it as
+// contiguously as the interleaved layout).
+// A normal component needs the trace of r: its block also updates the other
+// two normal components (the same float operations as their own blocks) and
+// keeps only their sum.  Since blocks read other components' r and s while
+// their owners write them, a step reads r and s from one copy and writes the
+// other (LS_FLAG_RB: this step starts in r32b/s32b).  Block x is (c, f) =
+// (x / nf, x % nf): with nf a multiple of 8, the six blocks of one plane share
+// an XCD and its L2.
+// ---------------------------------------------------------------------------
+constexpr int kPlaneThreads = 512;
+
+template <int NP> constexpr size_t plane_lds_bytes() {
+    return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<float>);
+}
+
+// One component's CG update at one voxel (pass A's recurrence).
+struct CgUpd {
+    float al, be;
+    bool upd, upd1;
+    __device__ __forceinline__ float rn(float r, float w, float s) const {
+        if (!upd) return r;
+        const float ss = upd1 ? w : w + be * s;
+        return r - al * ss;
+    }
+};
+
+template <int NP, bool BATCH>
+__global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
+    constexpr int VOX = NM * NL;
+    constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int D = 3;
+    using cx = cplx<float>;
+    const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
+    if ((flags & LS_FLAG_SPEC) && stop) return;
+    cx* tw_l = reinterpret_cast<cx*>(smem_raw);
+    cx* tw_m = tw_l + NL;
+    cx* buf = tw_m + NM;               // NM/2 packed row pairs of NLP
+    cx* pl = buf + (NM / 2) * NLP;     // NLH columns of NMP (the half spectrum, transposed)
+    const Geom g = p.g;
+    const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
+    const bool normal = c < D;
+    const int o0 = (c == 0) ? 1 : 0, o1 = (c == 2) ? 1 : 2;   // the other two normal components
+    const long plane0 = (long)f * VOX;
+    const long N = g.N;
+    const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
+    const bool upd1 = flags & LS_FLAG_CG_UPDATE1;
+    const bool rb = flags & LS_FLAG_RB;
+    const float* r_in = rb ? p.r32b : p.r32;
+    const float* s_in = rb ? p.s32b : p.s32;
+    float* r_out = rb ? p.r32 : p.r32b;
+    float* s_out = rb ? p.s32 : p.s32b;
+
+    float r[VPT], w[VPT], pv[VPT], sv[VPT], xv[VPT], dm[VPT], dl[VPT];
+    float ra[2][VPT], wa[2][VPT], sa[2][VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + kPlaneThreads * k;
+        const long v = plane0 + (vox < VOX ? vox : 0);
+        r[k] = r_in[c * N + v];
+        dm[k] = p.dmu32[v];
+        if (normal) dl[k] = p.dlam32[v];
+        w[k] = pv[k] = sv[k] = xv[k] = 0.f;
+        ra[0][k] = ra[1][k] = wa[0][k] = wa[1][k] = sa[0][k] = sa[1][k] = 0.f;
+        if (upd) {
+            w[k] = p.w32[c * N + v];
+            pv[k] = p.p32[c * N + v];
+            sv[k] = s_in[c * N + v];
+            xv[k] = p.corr[c * N + v];
+        }
+        if (normal) {
+            ra[0][k] = r_in[o0 * N + v];
+            ra[1][k] = r_in[o1 * N + v];
+            if (upd) {
+                wa[0][k] = p.w32[o0 * N + v];
+                wa[1][k] = p.w32[o1 * N + v];
+                sa[0][k] = s_in[o0 * N + v];
+                sa[1][k] = s_in[o1 * N + v];
+            }
+        }
+    }
+    stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
+    stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
+    if (stop) return;
+
+    const CgUpd u{(float)p.ctl->cg_alpha, (float)p.ctl->cg_beta, upd, upd1};
+    float* bufr = reinterpret_cast<float*>(buf);
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + kPlaneThreads * k;
+        if (vox >= VOX) continue;
+        const long o = c * N + plane0 + vox;
+        const float y = u.rn(r[k], w[k], sv[k]);
+        if (upd) {
+            const float pp = upd1 ? r[k] : r[k] + u.be * pv[k];
+            const float ss = upd1 ? w[k] : w[k] + u.be * sv[k];
+            // x is the correction of eps_1 = eps_0 + r0 (see pass A)
+            const float xx = upd1 ? (u.al - 1.0f) * r[k] : xv[k] + u.al * pp;
+            p.p32[o] = pp;
+            s_out[o] = ss;
+            p.corr[o] = xx;
+            r_out[o] = y;
+        }
+        float tau = 2.0f * dm[k] * y;
+        if (normal) {
+            float yy[3];
+            yy[c] = y;
+            yy[o0] = u.rn(ra[0][k], wa[0][k], sa[0][k]);
+            yy[o1] = u.rn(ra[1][k], wa[1][k], sa[1][k]);
+            float tr = 0.f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) tr += yy[a];
+            tau = 2.0f * dm[k] * y + dl[k] * tr;
+        }
+        const int m = vox / NL, j = vox % NL;
+        bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = tau;
+    }
+    __syncthreads();
+    lds_fft<NL, -1, float>(buf, NM / 2, NLP, 1, tw_l);
+    // split each packed row pair into the two half spectra, transposed
+    for (int idx = threadIdx.x; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+        const int pr = idx / NLH, kb = idx % NLH;
+        const cx z = buf[pr * NLP + kb];
+        const cx zm = buf[pr * NLP + (NL - kb) % NL];
+        pl[kb * NMP + 2 * pr] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
+        pl[kb * NMP + 2 * pr + 1] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
+    }
+    __syncthreads();
+    lds_fft<NM, -1, float>(pl, NLH, NMP, 1, tw_m);
+    cx* spec = reinterpret_cast<cx*>(p.spec32) + (long)f * NM * NLH * 6 + c * kPlaneTile;
+    for (int idx = threadIdx.x; idx < NM * NLH; idx += kPlaneThreads) {
+        const int m = idx / NLH, kb = idx % NLH;
+        spec[(idx / kPlaneTile) * (6 * kPlaneTile) + idx % kPlaneTile] = pl[kb * NMP + m];
+    }
+}
+
+template <int NP, bool BATCH>
+__global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
+    constexpr int VOX = NM * NL;
+    constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int SPT = (NM * NLH + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int D = 3;
+    using cx = cplx<float>;
+    const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
+    if ((flags & LS_FLAG_SPEC) && stop) return;
+    cx* tw_l = reinterpret_cast<cx*>(smem_raw);
+    cx* tw_m = tw_l + NL;
+    cx* buf = tw_m + NM;
+    cx* pl = buf + (NM / 2) * NLP;
+    __shared__ double red[8 * 4 * 2];
+    const Geom g = p.g;
+    const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
+    const bool normal = c < D;
+    const int o0 = (c == 0) ? 1 : 0, o1 = (c == 2) ? 1 : 2;
+    const long plane0 = (long)f * VOX;
+    const long N = g.N;
+    // r as this step's forward plane pass left it
+    const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
+    const bool rb = ((flags & LS_FLAG_RB) != 0) != upd;
+    const float* r_cur = rb ? p.r32b : p.r32;
+
+    const cx* spec = reinterpret_cast<const cx*>(p.spec32) + (long)f * NM * NLH * 6 + c * kPlaneTile;
+    cx sp[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = threadIdx.x + kPlaneThreads * k;
+        const int i = idx < NM * NLH ? idx : 0;
+        sp[k] = spec[(i / kPlaneTile) * (6 * kPlaneTile) + i % kPlaneTile];
+    }
+    float r[VPT], ra[2][VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + kPlaneThreads * k;
+        const long v = plane0 + (vox < VOX ? vox : 0);
+        r[k] = r_cur[c * N + v];
+        ra[0][k] = ra[1][k] = 0.f;
+        if (normal) {
+            ra[0][k] = r_cur[o0 * N + v];
+            ra[1][k] = r_cur[o1 * N + v];
+        }
+    }
+    stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
+    stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
+    if (stop) return;
+
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = threadIdx.x + kPlaneThreads * k;
+        if (idx < NM * NLH) pl[(idx % NLH) * NMP + idx / NLH] = sp[k];
+    }
+    __syncthreads();
+    lds_fft<NM, +1, float>(pl, NLH, NMP, 1, tw_m);
+    // pack row pairs into complex lines for the c2r along L
+    for (int idx = threadIdx.x; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+        const int pr = idx / NLH, kb = idx % NLH;
+        const cx A = pl[kb * NMP + 2 * pr], Bv = pl[kb * NMP + 2 * pr + 1];
+        cx* line = buf + pr * NLP;
+        if (kb == 0 || 2 * kb == NL) {
+            line[kb] = {A.x, Bv.x};   // self-conjugate bins: real parts only (c2r)
+        } else {
+            line[kb] = {A.x - Bv.y, A.y + Bv.x};
+            line[NL - kb] = {A.x + Bv.y, Bv.x - A.y};
+        }
+    }
+    __syncthreads();
+    lds_fft<NL, +1, float>(buf, NM / 2, NLP, 1, tw_l);
+
+    // w_c = r_c - eps_tilde_c; this component's share of (r, r)_C0, (w, r)_C0
+    // and ||r||_W^2 (the trace terms lam0 tr(r) r_c, lam0 tr(r) w_c summed
+    // over the normal components give lam0 tr(r)^2 and lam0 tr(r) tr(w))
+    const double mu2 = 2.0 * p.mu0, lam0 = p.lam0;
+    const double wt = normal ? 1.0 : 2.0;
+    const float* bufr = reinterpret_cast<const float*>(buf);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int vox = threadIdx.x + kPlaneThreads * k;
+        if (vox >= VOX) continue;
+        const int m = vox / NL, j = vox % NL;
+        const float et = bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
+        const float rv = r[k];
+        const float wv = rv - et;
+        p.w32[c * N + plane0 + vox] = wv;
+        const double rd = (double)rv, wd = (double)wv;
+        double a0 = mu2 * (wt * rd * rd), a1 = mu2 * (wt * wd * rd);
+        if (normal) {
+            float yy[3];
+            yy[c] = rv;
+            yy[o0] = ra[0][k];
+            yy[o1] = ra[1][k];
+            double trr = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) trr += yy[a];
+            a0 += lam0 * trr * rd;
+            a1 += lam0 * trr * wd;
+        }
+        acc[0] += a0;
+        acc[1] += a1;
+        acc[2] += wt * rd * rd;
+    }
+    block_sum<3>(acc, red);
+    if (threadIdx.x == 0) {
+        p.partials[3 * blockIdx.x] = acc[0];
+        p.partials[3 * blockIdx.x + 1] = acc[1];
+        p.partials[3 * blockIdx.x + 2] = acc[2];
+    }
+}
+
 // Reduction of pass E's block partials and the control-block update.  A kernel
 // of its own: the kernel boundary makes every block's partials visible, with
 // no per-block device-scope fence (which writes back the XCD's L2).  A
@@ -730,7 +1004,9 @@ __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const L
             // reference's basic scheme would measure at this iterate.
             const double gam = tot[0], del = tot[1];
             const bool first = flags & LS_FLAG_CG_FIRST;
+            const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
             ctl->inner += 1;
+            ctl->r_in_b = ((flags & LS_FLAG_PLANE) && (((flags & LS_FLAG_RB) != 0) != upd)) ? 1 : 0;
             if (!first) {
                 ctl->corr_valid = 1;
                 const double diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
@@ -821,20 +1097,27 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
     double acc[2 * C];
 #pragma unroll
     for (int c = 0; c < 2 * C; ++c) acc[c] = 0.0;
-    const bool absorb = p.corr && p.ctl->corr_valid;   // the solve ended inside an FP32 CG cycle
+    // The solve ended inside an FP32 CG cycle: eps absorbs the iterate's
+    // correction x and the residual r of the last step, which is the increment
+    // the basic scheme would add at that iterate -- so the returned field
+    // includes its last increment, as the reference's loop does (update, then
+    // test).
+    const bool absorb = p.corr && p.ctl->corr_valid;
     const bool has_epsp = p.epsp != nullptr;
     // absent optional fields read a valid stand-in, and every load of a voxel
     // is issued before its first store (the compiler keeps loads behind
     // stores to fields it cannot prove distinct)
     const float* src_cr = absorb ? p.corr : reinterpret_cast<const float*>(p.eps);
+    const float* src_r = absorb ? (p.ctl->r_in_b ? p.r32b : p.r32) : reinterpret_cast<const float*>(p.eps);
     const double* src_ep = has_epsp ? p.epsp : p.eps;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x) {
         double e[C], ee[C], ep[C];
-        float cr[C];
+        float cr[C], rr[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             e[c] = p.eps[c * g.N + v];
             cr[c] = src_cr[c * g.N + v];
+            rr[c] = src_r[c * g.N + v];
             ep[c] = src_ep[c * g.N + v];
         }
         const double mu_v = p.mu[v], lam_v = p.lam[v];
@@ -842,6 +1125,7 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
         for (int c = 0; c < C; ++c) {
             if (absorb) {
                 e[c] += (double)cr[c];
+                e[c] += (double)rr[c];
                 p.eps[c * g.N + v] = e[c];
             }
             ee[c] = has_epsp ? e[c] - ep[c] : e[c];
@@ -946,7 +1230,7 @@ static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0)
     throw UnsupportedError("FFT length not supported on the middle axis");
 }
 
-template <typename SR, int C, bool BATCH>
+template <typename SR, int C, bool BATCH, bool PLANE = false>
 static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int spec = 0) {
     const int nf = p.g.nf;
     const long nxi = (long)p.g.nm * p.g.nlh;
@@ -954,7 +1238,7 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int s
     if (nf == NN) {                                                                                  \
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
-        hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH>), dim3(blocks_for(nxi, G), p.B),                     \
+        hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH, PLANE>), dim3(blocks_for(nxi, G), p.B),              \
                            dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode, spec);                \
         return;                                                                                      \
     }
@@ -979,8 +1263,68 @@ template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int f
     launch_inv_last<double, C, LS_FULL, BATCH>(p, flags, s);
 }
 
+#define MGK_PLANE_CASES(X) X(8) X(16) X(32) X(48) X(64)
+
+bool ls_plane_path(const Geom& g) {
+    const char* e = std::getenv("MGK_LS_PLANE");   // read per launch set-up: a test can compare both paths
+    return !(e && e[0] == '0') && ls_plane_supported(g);
+}
+
+static bool plane_gamma_tiles(int nf) {
+#define MGK_CASE(NN) if (nf == NN) return gamma_points<float, NN, 6>() % kPlaneTile == 0;
+    MGK_LEN_CASES(MGK_CASE)
+#undef MGK_CASE
+    return false;
+}
+
+bool ls_plane_supported(const Geom& g) {
+    if (g.C != 6 || g.nm <= 1 || g.nm != g.nl) return false;
+    if (((long)g.nm * g.nlh) % kPlaneTile != 0 || !plane_gamma_tiles(g.nf)) return false;
+#define MGK_CASE(NN) if (g.nm == NN) return true;
+    MGK_PLANE_CASES(MGK_CASE)
+#undef MGK_CASE
+    return false;
+}
+
+template <bool BATCH> static void launch_fwd_plane(const LsLaunch& p, int flags, hipStream_t s) {
+#define MGK_CASE(NN)                                                                                  \
+    if (p.g.nm == NN) {                                                                               \
+        hipLaunchKernelGGL((k_fwd_plane<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads),       \
+                           plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
+        return;                                                                                       \
+    }
+    MGK_PLANE_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("plane length not supported");
+}
+
+template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags, hipStream_t s, bool reduce = true) {
+    const int nb = 6 * p.g.nf;
+#define MGK_CASE(NN)                                                                                  \
+    if (p.g.nm == NN) {                                                                               \
+        hipLaunchKernelGGL((k_inv_plane<NN, BATCH>), dim3(nb, p.B), dim3(kPlaneThreads),              \
+                           plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
+        if (reduce)                                                                                   \
+            hipLaunchKernelGGL((k_ls_reduce<LS_CG, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb, flags); \
+        return;                                                                                       \
+    }
+    MGK_PLANE_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("plane length not supported");
+}
+
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
     const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
+    // the five-pass pipeline keeps r and s in one copy
+    flags = p.plane ? (flags | LS_FLAG_PLANE) : (flags & ~(LS_FLAG_RB | LS_FLAG_PLANE));
+    if constexpr (C == 6) {
+        if (p.plane) {
+            launch_fwd_plane<BATCH>(p, flags, s);
+            launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG, spec);
+            launch_inv_plane<BATCH>(p, flags, s);
+            return;
+        }
+    }
     launch_fwd_last<float, C, LS_CG, BATCH>(p, flags, s);
     if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG, spec);
     launch_first_gamma<float, C, BATCH>(p, s, LS_CG, spec);
@@ -1042,6 +1386,18 @@ template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int p
         case 5:
             if (full) throw ArgError("the CG update pass runs in FP32 only");
             launch_fwd_last<float, C, LS_CG, BATCH>(p, LS_FLAG_CG_UPDATE, s);
+            break;
+        case 6:
+        case 7:
+        case 8:
+            if constexpr (C == 6) {
+                if (full || !ls_plane_supported(p.g)) throw ArgError("plane passes run in FP32 on supported 3D grids");
+                if (pass == 6) launch_fwd_plane<BATCH>(p, LS_FLAG_CG_UPDATE, s);
+                else if (pass == 7) launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG);
+                else launch_inv_plane<BATCH>(p, 0, s, false);
+            } else {
+                throw ArgError("plane passes are 3D only");
+            }
             break;
         default: throw ArgError("unknown pass");
     }

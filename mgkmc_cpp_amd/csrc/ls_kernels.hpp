@@ -13,7 +13,9 @@ enum {
     LS_FLAG_CG_FIRST = 4,     // CG step 0 of a cycle: operator on r0 only
     LS_FLAG_CG_UPDATE1 = 8,   // CG step 1: p = r, s = w, x = alpha p (no history)
     LS_FLAG_CG_UPDATE = 16,   // CG step >= 2: full recurrence
-    LS_FLAG_SPEC = 32         // CG step past the likely count: test `done` before loading anything
+    LS_FLAG_SPEC = 32,        // CG step past the likely count: test `done` before loading anything
+    LS_FLAG_RB = 64,          // plane passes: r and s start this step in r32b / s32b (CG steps 2, 4, ...)
+    LS_FLAG_PLANE = 128       // set by the launcher: the step runs as plane passes
 };
 
 // how a solve starts
@@ -41,6 +43,8 @@ struct LsParams {
     float* w32;             // CG w = A r
     float* p32;             // CG direction p
     float* s32;             // CG s = A p
+    float* r32b;            // second copies of r and s: the plane passes alternate
+    float* s32b;            //   between the two (LS_FLAG_RB)
     float* corr;            // CG iterate x: FP32 correction of eps, or null
     const float* dlam32;    // lam - lam0, mu - mu0 in FP32
     const float* dmu32;
@@ -60,15 +64,22 @@ struct LsLaunch {
     int B;
     LsParams h;   // host copy of a single context's parameter block (passed by value)
     bool batch;   // a batch (any B, also 1): parameter blocks from P
+    bool plane = false;   // CG steps run the plane passes (ls_plane_path)
 };
 
 bool ls_length_supported(int n);
+// 3D grids whose CG steps run as plane passes (square nm = nl planes of a
+// supported length; the environment variable MGK_LS_PLANE=0 turns them off)
+bool ls_plane_supported(const Geom& g);
+bool ls_plane_path(const Geom& g);
 int ls_partial_slots(const Geom& g);
 void ls_enqueue_start(const LsLaunch& l, int start, hipStream_t s);
 void ls_enqueue_iteration(const LsLaunch& l, int mode, int flags, hipStream_t s);
 void ls_enqueue_final(const LsLaunch& l, hipStream_t s);
 // one pass of the pipeline alone, for kernel timing: 0 A, 1 B, 2 C, 3 D,
-// 4 E (without its reduction), 5 A with the CG vector update (FP32)
+// 4 E (without its reduction), 5 A with the CG vector update (FP32); plane
+// passes (FP32): 6 forward plane with the CG update, 7 C on the plane
+// layout, 8 inverse plane (without its reduction)
 void ls_enqueue_pass(const LsLaunch& l, int pass, int mode, hipStream_t s);
 
 }  // namespace mgk

@@ -111,6 +111,35 @@ def test_fp32_refine_error_bound():
     assert err < 1e-6
 
 
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (48, 48, 48), (64, 64, 64), (16, 48, 32), (32, 16, 16)])
+def test_fp32_refine_plane_and_line_passes(shape, monkeypatch):
+    # Square nm = nl planes run the CG steps as plane passes (two launches per
+    # step fewer); MGK_LS_PLANE=0 forces the five-pass pipeline.  Both are
+    # measured against the converged solution (the oracle at tol 1e-11): the
+    # error of an FP32+refine solve at tol 1e-8 stays within 1e-6, or within
+    # twice the FP64 path's own error at the same tolerance where the field
+    # makes that larger (uncorrelated cov-0.15 fields at 64^3 reach 1e-6).
+    E, nu = _field(shape, 23)
+    bar = _bar(3, 4)
+    ref = po.ls_solve(shape, E, nu, bar, tol=1e-11)
+    c64 = Context(shape, E, nu)
+    e64 = _rel(c64.ls_solve(bar, tol=1e-8)["sig"], ref["sig"])
+    c64.close()
+    bound = max(1e-6, 2.0 * e64)
+    out = {}
+    for plane in ("1", "0"):
+        monkeypatch.setenv("MGK_LS_PLANE", plane)
+        ctx = Context(shape, E, nu, precision=abi.MGK_FP32_REFINE)
+        got = ctx.ls_solve(bar, tol=1e-8)
+        out[plane] = got
+        err = _rel(got["sig"], ref["sig"])
+        print(shape, "plane", plane, "iters", got["iters"], ctx.last_solve_stats(), "rel err", err, "fp64 err", e64)
+        assert err < bound
+        assert _rel(got["sig_macro"], ref["sig_macro"]) < bound
+        ctx.close()
+    assert _rel(out["1"]["sig"], out["0"]["sig"]) < 2.0 * bound
+
+
 @pytest.mark.parametrize("precision", [abi.MGK_FP64, abi.MGK_FP32_REFINE])
 @pytest.mark.parametrize("thin", [(32, 16, 1), (32, 1, 16), (1, 32, 16)])
 def test_thin_3d_equals_plane_strain(thin, precision):
