@@ -112,7 +112,7 @@ struct mgk_ctx {
 
     // spectra and FP32 refinement buffers
     void* spec = nullptr;
-    float *cg = nullptr, *corr = nullptr;   // CG vectors r, w, p, s, r', s' (6 x C x N) and iterate x
+    float *cg = nullptr, *corr = nullptr;   // CG vectors r, w, p, s (4 x C x N) and iterate x
     double* partials = nullptr;
     Ctl* ctl = nullptr;
     Ctl* hctl = nullptr;
@@ -250,8 +250,7 @@ struct mgk_ctx {
         p.w32 = cg + CN;
         p.p32 = cg + 2 * CN;
         p.s32 = cg + 3 * CN;
-        p.r32b = cg + 4 * CN;
-        p.s32b = cg + 5 * CN;
+        p.mandel = ls_plane_path(g) ? 1 : 0;
         p.corr = (precision == MGK_FP32_REFINE) ? corr : nullptr;
         p.dlam32 = dlam32;
         p.dmu32 = dmu32;
@@ -274,7 +273,7 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(d_lsp + slot, h_lsp + slot, sizeof(LsParams), hipMemcpyHostToDevice, stream));
             lsp_up[slot] = true;
         }
-        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false, ls_plane_path(g)};
+        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false, h_lsp[slot].mandel != 0};
     }
     KmcLaunch kml() {
         if (!kp_up || std::memcmp(h_kp, &K, sizeof(KmcParams)) != 0) {
@@ -472,9 +471,7 @@ struct mgk_ctx {
     }
     static int cg_flags(int j, int spec_from) {
         const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
-        // step j >= 1 writes r and s to the copy step j-1 did not: step 1 to
-        // the second, step 2 back to the first, ...
-        return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j >= 2 && j % 2 == 0) ? LS_FLAG_RB : 0);
+        return f | (j >= spec_from ? LS_FLAG_SPEC : 0);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -567,7 +564,7 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             MGK_HIP_CHECK(hipMemset(c->sig, 0, sizeof(double) * C * N));
             MGK_HIP_CHECK(hipMemset(c->epsp, 0, sizeof(double) * C * N));
             c->spec = c->dal<double>((size_t)2 * C * g.NS);
-            c->cg = c->dal<float>((size_t)6 * C * N);
+            c->cg = c->dal<float>((size_t)4 * C * N);
             c->corr = c->dal<float>((size_t)C * N);
             c->partials = c->dal<double>((size_t)ls_partial_slots(g) + 64);
             // the LS and KMC control blocks side by side, so one copy brings both back
@@ -1613,7 +1610,7 @@ long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_vo
     long long b = 0;
     b += 4 * r(N, 8) + 2 * r(N, 4);                      // E, nu, lam, mu; dlam, dmu
     b += 3 * r(C * N, 8);                                 // eps, sig, eps_p
-    b += r(2 * C * NS, 8) + r(6 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
+    b += r(2 * C * NS, 8) + r(4 * C * N, 4) + r(C * N, 4);   // spectrum, CG vectors, correction
     b += r((long long)ls_partial_slots(g) + 64, 8);
     b += (long long)sizeof(CtlPair) + 2 * (long long)sizeof(LsParams) + (long long)sizeof(KmcParams) +
          (long long)sizeof(EventArgs);

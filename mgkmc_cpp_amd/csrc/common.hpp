@@ -79,7 +79,7 @@ struct Ctl {
     double eps_norm;       // ||eps|| after the last FP64 iteration
     double cg_alpha, cg_beta, cg_gamma;
     int cg_stop;           // CG breakdown: the rest of the cycle is skipped
-    int r_in_b;            // the CG residual r of the last step is in r32b (plane passes), else r32
+    int pad2;
 };
 
 struct Status {

@@ -166,6 +166,51 @@ __device__ __forceinline__ void gamma_apply(const cplx<R>* t, cplx<R>* o, const 
 }
 
 // ---------------------------------------------------------------------------
+// Mandel (orthonormal) components of a symmetric 3x3 tensor T, m_i = T : e_i
+// with e1 = I/sqrt3, e2 = (xx - yy)/sqrt2, e3 = (xx + yy - 2 zz)/sqrt6,
+// e4..e6 = sqrt2 sym(xy), sqrt2 sym(xz), sqrt2 sym(yz).  An isotropic
+// stiffness is diagonal there (3 lam + 2 mu on e1, 2 mu on the rest) and the
+// Frobenius product is the plain dot product, so the plane passes' CG vectors
+// use these components: no component's update needs another's.
+// ---------------------------------------------------------------------------
+template <typename R> __device__ __forceinline__ void mandel_to_cart(const R* m, R* t) {
+    const R a = m[0] * (R)0.57735026918962576451, b = m[1] * (R)0.70710678118654752440,
+            c = m[2] * (R)0.40824829046386301637;
+    t[0] = (a + b) + c;
+    t[1] = (a - b) + c;
+    t[2] = a - (R)2 * c;
+    t[3] = m[3] * (R)0.70710678118654752440;
+    t[4] = m[4] * (R)0.70710678118654752440;
+    t[5] = m[5] * (R)0.70710678118654752440;
+}
+template <typename R> __device__ __forceinline__ void cart_to_mandel(const R* t, R* m) {
+    m[0] = ((t[0] + t[1]) + t[2]) * (R)0.57735026918962576451;
+    m[1] = (t[0] - t[1]) * (R)0.70710678118654752440;
+    m[2] = ((t[0] + t[1]) - (R)2 * t[2]) * (R)0.40824829046386301637;
+    m[3] = t[3] * (R)1.41421356237309504880;
+    m[4] = t[4] * (R)1.41421356237309504880;
+    m[5] = t[5] * (R)1.41421356237309504880;
+}
+template <typename R> __device__ __forceinline__ void mandel_to_cart_c(const cplx<R>* m, cplx<R>* t) {
+    R mx[6], my[6], tx[6], ty[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { mx[c] = m[c].x; my[c] = m[c].y; }
+    mandel_to_cart(mx, tx);
+    mandel_to_cart(my, ty);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t[c] = {tx[c], ty[c]};
+}
+template <typename R> __device__ __forceinline__ void cart_to_mandel_c(const cplx<R>* t, cplx<R>* m) {
+    R tx[6], ty[6], mx[6], my[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { tx[c] = t[c].x; ty[c] = t[c].y; }
+    cart_to_mandel(tx, mx);
+    cart_to_mandel(ty, my);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) m[c] = {mx[c], my[c]};
+}
+
+// ---------------------------------------------------------------------------
 // Block shapes (compile time, so every per-thread loop unrolls and a thread
 // issues all of its global loads before it waits on any of them).
 // ---------------------------------------------------------------------------
@@ -339,10 +384,21 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
             const int li = vox / NL, j = vox % NL;
             double ee[C], ek[C];
             double tre = 0.0, tr0 = 0.0;
+            double crc[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) crc[c] = (double)cr[k][c];
+            if constexpr (C == 6) {
+                if (add_corr && p.mandel) {   // the CG iterate x in Mandel components
+                    double t[6];
+                    mandel_to_cart(crc, t);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) crc[c] = t[c];
+                }
+            }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 ek[c] = e[k][c] + shift[c];
-                if (add_corr) ek[c] += (double)cr[k][c];
+                if (add_corr) ek[c] += crc[c];
                 ee[c] = has_epsp ? ek[c] - ep[k][c] : ek[c];
             }
 #pragma unroll
@@ -530,6 +586,12 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
         cx tt[C], o[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) tt[c] = work[(gg * C + c) * NFP + f];
+        if constexpr (PLANE && C == 6) {   // tau_hat in Mandel components
+            cx t6[6];
+            mandel_to_cart_c(tt, t6);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) tt[c] = t6[c];
+        }
         SR k2 = (SR)0;
 #pragma unroll
         for (int d = 0; d < D; ++d) k2 += qa[d] * qa[d];
@@ -552,6 +614,12 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
 #pragma unroll
                 for (int c = 0; c < C; ++c) o[c] = {(SR)0.5 * (o[c].x + o2[c].x), (SR)0.5 * (o[c].y + o2[c].y)};
             }
+        }
+        if constexpr (PLANE && C == 6) {
+            cx m6[6];
+            cart_to_mandel_c(o, m6);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) o[c] = m6[c];
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) work[(gg * C + c) * NFP + f] = {-o[c].x * scale, -o[c].y * scale};
@@ -665,19 +733,42 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
             if (!ok[k]) continue;
             const int vox = threadIdx.x + 256 * k;
             const int li = vox / NL, j = vox % NL;
+            double crc[C], dd[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) crc[c] = (double)cr[k][c];
+            if constexpr (C == 6) {
+                if (add_corr && p.mandel) {
+                    double t[6];
+                    mandel_to_cart(crc, t);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) crc[c] = t[c];
+                }
+            }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const SR et = bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)];
                 const double w = (c < D) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
                 double o = old[k][c] + shift[c];
-                if (add_corr) o += (double)cr[k][c];
+                if (add_corr) o += crc[c];
                 const double nw = bar[c] + (double)et;
                 const double d = nw - o;
                 acc[0] += w * d * d;
                 acc[1] += w * o * o;
                 acc[2] += w * nw * nw;
                 p.eps[c * g.N + vid[k]] = nw;
-                if (p.r32) p.r32[c * g.N + vid[k]] = (float)d;
+                dd[c] = d;
+            }
+            if (p.r32) {   // the increment starts the CG cycle (Mandel components for the plane passes)
+                if constexpr (C == 6) {
+                    if (p.mandel) {
+                        double m[6];
+                        cart_to_mandel(dd, m);
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) dd[c] = m[c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < C; ++c) p.r32[c * g.N + vid[k]] = (float)dd[c];
             }
         }
     } else {
@@ -720,33 +811,23 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
 //                along L (two rows packed per complex line), c2c along M;
 //   k_inv_plane: c2c along M, c2r along L, w_c = r_c - eps_tilde_c and the
 //                block's share of the C0-energy products.
-// The spectrum is tiled: runs of kPlaneTile points per component, the six
-// components' runs side by side (pass C with PLANE = true reads it as
-// contiguously as the interleaved layout).
-// A normal component needs the trace of r: its block also updates the other
-// two normal components (the same float operations as their own blocks) and
-// keeps only their sum.  Since blocks read other components' r and s while
-// their owners write them, a step reads r and s from one copy and writes the
-// other (LS_FLAG_RB: this step starts in r32b/s32b).  Block x is (c, f) =
-// (x / nf, x % nf): with nf a multiple of 8, the six blocks of one plane share
-// an XCD and its L2.
+// The CG vectors hold Mandel components (mandel_to_cart), in which
+// (C - C0) and C0 are diagonal: a block reads and writes its own component
+// only.  The FP64 passes convert where they meet the CG vectors (the
+// increment that starts a cycle, the absorbed iterate), pass C around the
+// Green operator.  The spectrum is tiled: runs of kPlaneTile points per
+// component, the six components' runs side by side, so pass C (PLANE = true)
+// reads it as contiguously as the interleaved layout.  Block x is (c, f) =
+// (x / nf, x % nf).
 // ---------------------------------------------------------------------------
-constexpr int kPlaneThreads = 512;
+#ifndef MGK_PLANE_THREADS
+#define MGK_PLANE_THREADS 512
+#endif
+constexpr int kPlaneThreads = MGK_PLANE_THREADS;   // lanes per plane block (a build-time tuning knob)
 
 template <int NP> constexpr size_t plane_lds_bytes() {
     return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<float>);
 }
-
-// One component's CG update at one voxel (pass A's recurrence).
-struct CgUpd {
-    float al, be;
-    bool upd, upd1;
-    __device__ __forceinline__ float rn(float r, float w, float s) const {
-        if (!upd) return r;
-        const float ss = upd1 ? w : w + be * s;
-        return r - al * ss;
-    }
-};
 
 template <int NP, bool BATCH>
 __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
@@ -754,7 +835,6 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
     constexpr int VOX = NM * NL;
     constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
-    constexpr int D = 3;
     using cx = cplx<float>;
     const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
@@ -764,79 +844,50 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     cx* pl = buf + (NM / 2) * NLP;     // NLH columns of NMP (the half spectrum, transposed)
     const Geom g = p.g;
     const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
-    const bool normal = c < D;
-    const int o0 = (c == 0) ? 1 : 0, o1 = (c == 2) ? 1 : 2;   // the other two normal components
-    const long plane0 = (long)f * VOX;
-    const long N = g.N;
+    const long base = (long)c * g.N + (long)f * VOX;   // component c of plane f
     const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
     const bool upd1 = flags & LS_FLAG_CG_UPDATE1;
-    const bool rb = flags & LS_FLAG_RB;
-    const float* r_in = rb ? p.r32b : p.r32;
-    const float* s_in = rb ? p.s32b : p.s32;
-    float* r_out = rb ? p.r32 : p.r32b;
-    float* s_out = rb ? p.s32 : p.s32b;
 
     float r[VPT], w[VPT], pv[VPT], sv[VPT], xv[VPT], dm[VPT], dl[VPT];
-    float ra[2][VPT], wa[2][VPT], sa[2][VPT];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
         const int vox = threadIdx.x + kPlaneThreads * k;
-        const long v = plane0 + (vox < VOX ? vox : 0);
-        r[k] = r_in[c * N + v];
-        dm[k] = p.dmu32[v];
-        if (normal) dl[k] = p.dlam32[v];
+        const int vv = vox < VOX ? vox : 0;
+        r[k] = p.r32[base + vv];
+        dm[k] = p.dmu32[(long)f * VOX + vv];
+        dl[k] = (c == 0) ? p.dlam32[(long)f * VOX + vv] : 0.f;
         w[k] = pv[k] = sv[k] = xv[k] = 0.f;
-        ra[0][k] = ra[1][k] = wa[0][k] = wa[1][k] = sa[0][k] = sa[1][k] = 0.f;
         if (upd) {
-            w[k] = p.w32[c * N + v];
-            pv[k] = p.p32[c * N + v];
-            sv[k] = s_in[c * N + v];
-            xv[k] = p.corr[c * N + v];
-        }
-        if (normal) {
-            ra[0][k] = r_in[o0 * N + v];
-            ra[1][k] = r_in[o1 * N + v];
-            if (upd) {
-                wa[0][k] = p.w32[o0 * N + v];
-                wa[1][k] = p.w32[o1 * N + v];
-                sa[0][k] = s_in[o0 * N + v];
-                sa[1][k] = s_in[o1 * N + v];
-            }
+            w[k] = p.w32[base + vv];
+            pv[k] = p.p32[base + vv];
+            sv[k] = p.s32[base + vv];
+            xv[k] = p.corr[base + vv];
         }
     }
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
     if (stop) return;
 
-    const CgUpd u{(float)p.ctl->cg_alpha, (float)p.ctl->cg_beta, upd, upd1};
+    const float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
     float* bufr = reinterpret_cast<float*>(buf);
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
         const int vox = threadIdx.x + kPlaneThreads * k;
         if (vox >= VOX) continue;
-        const long o = c * N + plane0 + vox;
-        const float y = u.rn(r[k], w[k], sv[k]);
+        float y = r[k];
         if (upd) {
-            const float pp = upd1 ? r[k] : r[k] + u.be * pv[k];
-            const float ss = upd1 ? w[k] : w[k] + u.be * sv[k];
+            const float pp = upd1 ? r[k] : r[k] + be * pv[k];
+            const float ss = upd1 ? w[k] : w[k] + be * sv[k];
             // x is the correction of eps_1 = eps_0 + r0 (see pass A)
-            const float xx = upd1 ? (u.al - 1.0f) * r[k] : xv[k] + u.al * pp;
-            p.p32[o] = pp;
-            s_out[o] = ss;
-            p.corr[o] = xx;
-            r_out[o] = y;
+            const float xx = upd1 ? (al - 1.0f) * r[k] : xv[k] + al * pp;
+            y = r[k] - al * ss;
+            p.p32[base + vox] = pp;
+            p.s32[base + vox] = ss;
+            p.corr[base + vox] = xx;
+            p.r32[base + vox] = y;
         }
-        float tau = 2.0f * dm[k] * y;
-        if (normal) {
-            float yy[3];
-            yy[c] = y;
-            yy[o0] = u.rn(ra[0][k], wa[0][k], sa[0][k]);
-            yy[o1] = u.rn(ra[1][k], wa[1][k], sa[1][k]);
-            float tr = 0.f;
-#pragma unroll
-            for (int a = 0; a < D; ++a) tr += yy[a];
-            tau = 2.0f * dm[k] * y + dl[k] * tr;
-        }
+        // (C - C0) in Mandel components: 3 dlam + 2 dmu on e1, 2 dmu elsewhere
+        const float tau = (c == 0) ? (3.0f * dl[k] + 2.0f * dm[k]) * y : 2.0f * dm[k] * y;
         const int m = vox / NL, j = vox % NL;
         bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = tau;
     }
@@ -866,7 +917,6 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     constexpr int VOX = NM * NL;
     constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
     constexpr int SPT = (NM * NLH + kPlaneThreads - 1) / kPlaneThreads;
-    constexpr int D = 3;
     using cx = cplx<float>;
     const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
@@ -874,17 +924,10 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     cx* tw_m = tw_l + NL;
     cx* buf = tw_m + NM;
     cx* pl = buf + (NM / 2) * NLP;
-    __shared__ double red[8 * 4 * 2];
+    __shared__ double red[16 * 3];
     const Geom g = p.g;
     const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
-    const bool normal = c < D;
-    const int o0 = (c == 0) ? 1 : 0, o1 = (c == 2) ? 1 : 2;
-    const long plane0 = (long)f * VOX;
-    const long N = g.N;
-    // r as this step's forward plane pass left it
-    const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
-    const bool rb = ((flags & LS_FLAG_RB) != 0) != upd;
-    const float* r_cur = rb ? p.r32b : p.r32;
+    const long base = (long)c * g.N + (long)f * VOX;
 
     const cx* spec = reinterpret_cast<const cx*>(p.spec32) + (long)f * NM * NLH * 6 + c * kPlaneTile;
     cx sp[SPT];
@@ -894,17 +937,11 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         const int i = idx < NM * NLH ? idx : 0;
         sp[k] = spec[(i / kPlaneTile) * (6 * kPlaneTile) + i % kPlaneTile];
     }
-    float r[VPT], ra[2][VPT];
+    float r[VPT];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
         const int vox = threadIdx.x + kPlaneThreads * k;
-        const long v = plane0 + (vox < VOX ? vox : 0);
-        r[k] = r_cur[c * N + v];
-        ra[0][k] = ra[1][k] = 0.f;
-        if (normal) {
-            ra[0][k] = r_cur[o0 * N + v];
-            ra[1][k] = r_cur[o1 * N + v];
-        }
+        r[k] = p.r32[base + (vox < VOX ? vox : 0)];
     }
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
@@ -932,11 +969,9 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     __syncthreads();
     lds_fft<NL, +1, float>(buf, NM / 2, NLP, 1, tw_l);
 
-    // w_c = r_c - eps_tilde_c; this component's share of (r, r)_C0, (w, r)_C0
-    // and ||r||_W^2 (the trace terms lam0 tr(r) r_c, lam0 tr(r) w_c summed
-    // over the normal components give lam0 tr(r)^2 and lam0 tr(r) tr(w))
-    const double mu2 = 2.0 * p.mu0, lam0 = p.lam0;
-    const double wt = normal ? 1.0 : 2.0;
+    // w_c = r_c - eps_tilde_c and this component's share of (r, r)_C0,
+    // (w, r)_C0 (C0 = 3 lam0 + 2 mu0 on e1, 2 mu0 elsewhere) and ||r||^2
+    const double c0 = (c == 0) ? 3.0 * p.lam0 + 2.0 * p.mu0 : 2.0 * p.mu0;
     const float* bufr = reinterpret_cast<const float*>(buf);
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -947,24 +982,14 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         const float et = bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
         const float rv = r[k];
         const float wv = rv - et;
-        p.w32[c * N + plane0 + vox] = wv;
+        p.w32[base + vox] = wv;
         const double rd = (double)rv, wd = (double)wv;
-        double a0 = mu2 * (wt * rd * rd), a1 = mu2 * (wt * wd * rd);
-        if (normal) {
-            float yy[3];
-            yy[c] = rv;
-            yy[o0] = ra[0][k];
-            yy[o1] = ra[1][k];
-            double trr = 0.0;
-#pragma unroll
-            for (int a = 0; a < D; ++a) trr += yy[a];
-            a0 += lam0 * trr * rd;
-            a1 += lam0 * trr * wd;
-        }
-        acc[0] += a0;
-        acc[1] += a1;
-        acc[2] += wt * rd * rd;
+        acc[0] += rd * rd;
+        acc[1] += wd * rd;
+        acc[2] += rd * rd;
     }
+    acc[0] *= c0;
+    acc[1] *= c0;
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
         p.partials[3 * blockIdx.x] = acc[0];
@@ -1004,9 +1029,7 @@ __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const L
             // reference's basic scheme would measure at this iterate.
             const double gam = tot[0], del = tot[1];
             const bool first = flags & LS_FLAG_CG_FIRST;
-            const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
             ctl->inner += 1;
-            ctl->r_in_b = ((flags & LS_FLAG_PLANE) && (((flags & LS_FLAG_RB) != 0) != upd)) ? 1 : 0;
             if (!first) {
                 ctl->corr_valid = 1;
                 const double diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
@@ -1108,7 +1131,7 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
     // is issued before its first store (the compiler keeps loads behind
     // stores to fields it cannot prove distinct)
     const float* src_cr = absorb ? p.corr : reinterpret_cast<const float*>(p.eps);
-    const float* src_r = absorb ? (p.ctl->r_in_b ? p.r32b : p.r32) : reinterpret_cast<const float*>(p.eps);
+    const float* src_r = absorb ? p.r32 : reinterpret_cast<const float*>(p.eps);
     const double* src_ep = has_epsp ? p.epsp : p.eps;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < g.N; v += (long)gridDim.x * blockDim.x) {
         double e[C], ee[C], ep[C];
@@ -1121,11 +1144,28 @@ __global__ __launch_bounds__(256) void k_final_stress(const LsParams p_arg, cons
             ep[c] = src_ep[c * g.N + v];
         }
         const double mu_v = p.mu[v], lam_v = p.lam[v];
+        double xc[C], rc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            xc[c] = (double)cr[c];
+            rc[c] = (double)rr[c];
+        }
+        if constexpr (C == 6) {
+            if (absorb && p.mandel) {
+                double t[6];
+                mandel_to_cart(xc, t);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) xc[c] = t[c];
+                mandel_to_cart(rc, t);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) rc[c] = t[c];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             if (absorb) {
-                e[c] += (double)cr[c];
-                e[c] += (double)rr[c];
+                e[c] += xc[c];
+                e[c] += rc[c];
                 p.eps[c * g.N + v] = e[c];
             }
             ee[c] = has_epsp ? e[c] - ep[c] : e[c];
@@ -1315,8 +1355,6 @@ template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags,
 
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
     const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
-    // the five-pass pipeline keeps r and s in one copy
-    flags = p.plane ? (flags | LS_FLAG_PLANE) : (flags & ~(LS_FLAG_RB | LS_FLAG_PLANE));
     if constexpr (C == 6) {
         if (p.plane) {
             launch_fwd_plane<BATCH>(p, flags, s);

@@ -13,9 +13,7 @@ enum {
     LS_FLAG_CG_FIRST = 4,     // CG step 0 of a cycle: operator on r0 only
     LS_FLAG_CG_UPDATE1 = 8,   // CG step 1: p = r, s = w, x = alpha p (no history)
     LS_FLAG_CG_UPDATE = 16,   // CG step >= 2: full recurrence
-    LS_FLAG_SPEC = 32,        // CG step past the likely count: test `done` before loading anything
-    LS_FLAG_RB = 64,          // plane passes: r and s start this step in r32b / s32b (CG steps 2, 4, ...)
-    LS_FLAG_PLANE = 128       // set by the launcher: the step runs as plane passes
+    LS_FLAG_SPEC = 32         // CG step past the likely count: test `done` before loading anything
 };
 
 // how a solve starts
@@ -43,8 +41,7 @@ struct LsParams {
     float* w32;             // CG w = A r
     float* p32;             // CG direction p
     float* s32;             // CG s = A p
-    float* r32b;            // second copies of r and s: the plane passes alternate
-    float* s32b;            //   between the two (LS_FLAG_RB)
+    int mandel;             // the CG vectors (r, w, p, s, x) hold Mandel components (plane passes)
     float* corr;            // CG iterate x: FP32 correction of eps, or null
     const float* dlam32;    // lam - lam0, mu - mu0 in FP32
     const float* dmu32;
