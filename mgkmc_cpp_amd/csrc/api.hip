@@ -1122,17 +1122,22 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         for (int k = 0; k < 6; ++k) ea.mean_eps[k] = c.mean_eps[k];
         const KmcLaunch KL = c.kml();
         const LsLaunch L = c.lsl();
-        kmc_enqueue_rates(KL, in->time, nullptr, c.stream);
-        kmc_enqueue_event_prep(KL, c.ctl, nullptr, ea, nullptr, c.stream);
-        kmc_enqueue_select(KL, in->u2, nullptr, c.stream);
-        if (c.precision == MGK_FP64) kmc_enqueue_select_fallback(KL, 1, c.stream);
         // redraw values: kernel arguments when they fit, else the staging copy
         const bool by_args = n_modes > 0 && kmc_redraw_fits_args(n_modes, c.g.C);
         if (n_modes > 0 && !by_args)
             MGK_HIP_CHECK(hipMemcpyAsync(c.kstage, c.hkstage, sizeof(double) * n_modes * (c.g.C + 1),
                                          hipMemcpyHostToDevice, c.stream));
-        kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1,
-                         by_args ? c.hkstage : nullptr);
+        // rates, then one block for scan, decision, selection and flip; FP64
+        // runs the exact selection pass before its flip
+        const bool exact = (c.precision == MGK_FP64);
+        kmc_enqueue_rates(KL, in->time, nullptr, c.stream, false);
+        kmc_enqueue_event_chain(KL, c.ctl, nullptr, ea, nullptr, !exact, in->first_mode, n_modes, has_cat, has_q0,
+                                by_args ? c.hkstage : nullptr, c.stream);
+        if (exact) {
+            kmc_enqueue_select_fallback(KL, 1, c.stream);
+            kmc_enqueue_flip(KL, -1, in->time, in->first_mode, n_modes, has_cat, has_q0, c.stream, 1,
+                             by_args ? c.hkstage : nullptr);
+        }
         if (pm) kmc_enqueue_patch(KL, c.pp, -1, nullptr, 0.0, 1, in->strain_rate, c.stream);
         if (pm == 1) {
             // patched, not solved: the control block alone comes back; eps_macro
@@ -1784,11 +1789,13 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         MGK_HIP_CHECK(hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(Ctl) * B, hipMemcpyHostToDevice, st));
         if (n_modes > 0)
             MGK_HIP_CHECK(hipMemcpyAsync(b.d_stage, b.h_stage, sizeof(double) * b.stage_per * B, hipMemcpyHostToDevice, st));
-        kmc_enqueue_rates(KL, 0.0, b.d_times, st);
-        kmc_enqueue_event_prep(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, st);
-        kmc_enqueue_select(KL, 0.0, b.d_ea, st);
-        if (!refine) kmc_enqueue_select_fallback(KL, 1, st);
-        kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, st, 1);
+        kmc_enqueue_rates(KL, 0.0, b.d_times, st, false);
+        kmc_enqueue_event_chain(KL, nullptr, b.d_lsp, b.h_ea[0], b.d_ea, refine, first_mode, std::max(n_modes, 0),
+                                has_cat, has_q0, nullptr, st);
+        if (!refine) {
+            kmc_enqueue_select_fallback(KL, 1, st);
+            kmc_enqueue_flip(KL, -1, 0.0, first_mode, std::max(n_modes, 0), has_cat, has_q0, st, 1);
+        }
         if (!all_keep) ls_enqueue_start(L, LS_START_MEAN, st);   // trajectories with a known mean keep their shift
         iterations(0, nb0, st);
     };

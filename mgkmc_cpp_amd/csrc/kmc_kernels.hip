@@ -286,9 +286,7 @@ __device__ __forceinline__ void wave_scan(double& v, int& a, int& b) {
 // One block: inclusive prefix of the block sums (chunked, fixed order), the
 // exclusive prefix of unstable counts and the totals.  A kernel of its own so
 // that the rate blocks need no device-scope fence.
-template <bool BATCH>
-__global__ __launch_bounds__(1024) void k_rates_scan(const KmcParams p_arg, const KmcParams* __restrict__ PK, int nb) {
-    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+__device__ __forceinline__ void rates_scan_body(const KmcParams& p, int nb) {
     constexpr int G = 8;   // block sums per lane loaded together
     const int chunk = (nb + 1023) / 1024;
     const int b0 = threadIdx.x * chunk;
@@ -376,16 +374,20 @@ __global__ __launch_bounds__(1024) void k_rates_scan(const KmcParams p_arg, cons
     }
 }
 
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_rates_scan(const KmcParams p_arg, const KmcParams* __restrict__ PK, int nb) {
+    rates_scan_body(param_sel<BATCH>(p_arg, PK), nb);
+}
+
 // ---------------------------------------------------------------------------
 // Selection: block by a two-level search over the inclusive block prefixes
 // (non-decreasing), mode by an in-block scan; a rounding-bound guard flags
 // targets too close to call.
 // ---------------------------------------------------------------------------
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const KmcParams* __restrict__ PK, double u2,
-                                                const EventArgs* __restrict__ EA) {
-    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
-    if (BATCH) u2 = EA[blockIdx.y].u2;
+// The lanes 0..255 of the block work; every lane of a larger block takes
+// part in the barriers.
+__device__ __forceinline__ void select_body(const KmcParams& p, double u2) {
+    const bool act = threadIdx.x < 256;
     const long nmodes = p.g.N * p.ph.M;
     KmcCtl* kc = p.kctl;
     if (!kc->go) return;
@@ -408,10 +410,10 @@ __global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const Kmc
     {
         const int lo = threadIdx.x * chunk;
         const int last = min(nb, lo + chunk) - 1;
-        if (lo < nb && p.bincl[last] >= target) atomicMin(&s_chunk, (int)threadIdx.x);
+        if (act && lo < nb && p.bincl[last] >= target) atomicMin(&s_chunk, (int)threadIdx.x);
     }
     __syncthreads();
-    if (s_chunk != 0x7fffffff) {
+    if (act && s_chunk != 0x7fffffff) {
         for (int k = threadIdx.x; k < chunk; k += 256) {
             const int bb = s_chunk * chunk + k;
             if (bb < nb && p.bincl[bb] >= target) atomicMin(&s_b, bb);
@@ -429,25 +431,26 @@ __global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const Kmc
     for (int k = 0; k < 4; ++k) {
         const long i = base + threadIdx.x * 4 + k;
         bool incl = false;
-        const double r = (i < nmodes) ? mode_rate(p.ph, p.Q[i], incl) : 0.0;
+        const double r = (act && i < nmodes) ? mode_rate(p.ph, p.Q[i], incl) : 0.0;
         run += incl ? r : 0.0;
         loc[k] = run;
         inc[k] = incl;
     }
-    s_tot[threadIdx.x] = run;
+    if (act) s_tot[threadIdx.x] = run;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        double a = ((int)threadIdx.x >= off) ? s_tot[threadIdx.x - off] : 0.0;
+        double a = (act && (int)threadIdx.x >= off) ? s_tot[threadIdx.x - off] : 0.0;
         __syncthreads();
-        s_tot[threadIdx.x] += a;
+        if (act) s_tot[threadIdx.x] += a;
         __syncthreads();
     }
-    const double excl = (threadIdx.x > 0) ? s_tot[threadIdx.x - 1] : 0.0;
+    const double excl = (act && threadIdx.x > 0) ? s_tot[threadIdx.x - 1] : 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int li = threadIdx.x * 4 + k;
         const long i = base + li;
         const double S = prev + (excl + loc[k]);
+        if (!act) continue;
         s_scan[li] = S;
         if (i < nmodes && inc[k] && S >= target) atomicMin(&s_first, li);
     }
@@ -472,6 +475,13 @@ __global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const Kmc
     kc->sel = sel;
     kc->ambiguous = amb;
     kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const KmcParams* __restrict__ PK, double u2,
+                                                const EventArgs* __restrict__ EA) {
+    if (BATCH) u2 = EA[blockIdx.y].u2;
+    select_body(param_sel<BATCH>(p_arg, PK), u2);
 }
 
 // Rates of the kRateBlock modes of chunk b into dst, excluded modes as `excl`,
@@ -597,11 +607,9 @@ __global__ __launch_bounds__(256) void k_select_seq(const KmcParams p_arg, const
 // reference's order on one lane: periodic wraps on small grids hit the same
 // voxel several times and each hit accumulates.
 // ---------------------------------------------------------------------------
-template <int D, int C, bool BATCH>
-__global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, int flat, double time, int first_mode,
-                       int n_modes, int has_cat, int has_q0, int time_from_ctl, const RedrawArgs ra, int ra_on) {
-    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <int D, int C>
+__device__ __forceinline__ void flip_body(const KmcParams& p, int flat, double time, int first_mode, int n_modes,
+                                          int has_cat, int has_q0, int time_from_ctl, const RedrawArgs& ra, int ra_on) {
     // flat < 0: the mode the last selection chose, if that attempt produced
     // an event; first_mode < 0: redraw only that mode (both redraw switches
     // off, kmc_solver.hpp:856-864)
@@ -646,34 +654,34 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
         else { z = (int)(v % nz); const long t = v / nz; y = (int)(t % ny); x = (int)(t / ny); }
         constexpr int zr = (D == 3) ? 1 : 0;
         if (nx >= 3 && ny >= 3 && (D == 2 || nz >= 3)) {
-            // every neighbour is a distinct voxel other than v: load them all,
-            // then update (the same arithmetic per neighbour)
-            constexpr int NB = (D == 2) ? 8 : 26;
-            long nbs[NB];
-            double g0[NB], g1[NB];
-            int q = 0;
+            // every neighbour is a distinct voxel other than v: per dx plane,
+            // load its neighbours together, then update them (the same
+            // arithmetic per neighbour)
+            constexpr int NP = (D == 2) ? 3 : 9;   // cells per dx plane
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx)
+            for (int dx = -1; dx <= 1; ++dx) {
+                long nbs[NP];
+                double g0[NP], g1[NP];
+                const int xn = (x + dx + nx) % nx;
 #pragma unroll
-                for (int dy = -1; dy <= 1; ++dy)
+                for (int q = 0; q < NP; ++q) {
+                    const int dy = (D == 2) ? q - 1 : q / 3 - 1, dz = (D == 2) ? 0 : q % 3 - 1;
+                    const int yn = (y + dy + ny) % ny;
+                    const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
+                    nbs[q] = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
+                    const bool centre = (dx == 0 && dy == 0 && dz == 0);
+                    g0[q] = p.soft[centre ? v : nbs[q]];
+                    g1[q] = p.soft[g.N + (centre ? v : nbs[q])];
+                }
 #pragma unroll
-                    for (int dz = -zr; dz <= zr; ++dz) {
-                        if (dx == 0 && dy == 0 && dz == 0) continue;
-                        const int xn = (x + dx + nx) % nx, yn = (y + dy + ny) % ny;
-                        const int zn = (D == 3) ? (z + dz + nz) % nz : 0;
-                        nbs[q++] = (D == 2) ? (long)xn * ny + yn : ((long)xn * ny + yn) * nz + zn;
-                    }
-#pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                g0[k] = p.soft[nbs[k]];
-                g1[k] = p.soft[g.N + nbs[k]];
-            }
-#pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                double gn = g0[k] + ph.nfrac * ph.jp * sum_sq;
-                if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
-                p.soft[nbs[k]] = gn;
-                p.soft[g.N + nbs[k]] = g1[k] + ph.nfrac * ph.jt * sum_sq;
+                for (int q = 0; q < NP; ++q) {
+                    const int dy = (D == 2) ? q - 1 : q / 3 - 1, dz = (D == 2) ? 0 : q % 3 - 1;
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    double gn = g0[q] + ph.nfrac * ph.jp * sum_sq;
+                    if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
+                    p.soft[nbs[q]] = gn;
+                    p.soft[g.N + nbs[q]] = g1[q] + ph.nfrac * ph.jt * sum_sq;
+                }
             }
         } else {
             for (int dx = -1; dx <= 1; ++dx)
@@ -698,6 +706,14 @@ __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, 
             for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = stage[j * C + k];
         if (has_q0) p.Q0[mi] = stage[n_modes * C + j];
     }
+}
+
+template <int D, int C, bool BATCH>
+__global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, int flat, double time, int first_mode,
+                       int n_modes, int has_cat, int has_q0, int time_from_ctl, const RedrawArgs ra, int ra_on) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    flip_body<D, C>(param_sel<BATCH>(p_arg, PK), flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl, ra,
+                    ra_on);
 }
 
 // ---------------------------------------------------------------------------
@@ -750,13 +766,8 @@ __global__ __launch_bounds__(256) void k_unstable(const KmcParams p_arg, const K
 // Every operation is the host's own, in its order (no contraction in this
 // file), so time_new and the imposed strain equal the host's bit for bit.
 // ---------------------------------------------------------------------------
-template <int D, int C, bool BATCH>
-__global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict__ PK, Ctl* ctl_arg,
-                             const LsParams* __restrict__ P, const EventArgs ea_arg, const EventArgs* __restrict__ EA) {
-    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
-    const EventArgs& ea = param_sel<BATCH>(ea_arg, EA);
-    Ctl* ctl = BATCH ? P[blockIdx.y].ctl : ctl_arg;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <int D, int C>
+__device__ __forceinline__ void event_prep_body(const KmcParams& p, Ctl* ctl, const EventArgs& ea) {
     KmcCtl* kc = p.kctl;
     if (ea.init_ctl) {
         // the host's control-block reset for a solve: zero, then max_iter, tol,
@@ -817,12 +828,47 @@ __global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict_
     }
 }
 
+template <int D, int C, bool BATCH>
+__global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict__ PK, Ctl* ctl_arg,
+                             const LsParams* __restrict__ P, const EventArgs ea_arg, const EventArgs* __restrict__ EA) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    event_prep_body<D, C>(param_sel<BATCH>(p_arg, PK), BATCH ? P[blockIdx.y].ctl : ctl_arg,
+                          param_sel<BATCH>(ea_arg, EA));
+}
+
+// ---------------------------------------------------------------------------
+// One block per trajectory for everything between the rate pass and the LS
+// batch of a device-decided event: the block-sum scan, the event decision,
+// the selection and (FLIP) the flip with its redraw -- the same code as the
+// four separate kernels, in one launch.  FP64 contexts leave the flip out:
+// the exact selection pass runs between.
+// ---------------------------------------------------------------------------
+template <int D, int C, bool BATCH, bool FLIP>
+__global__ __launch_bounds__(1024) void k_event_chain(const KmcParams p_arg, const KmcParams* __restrict__ PK,
+                                                      Ctl* ctl_arg, const LsParams* __restrict__ P,
+                                                      const EventArgs ea_arg, const EventArgs* __restrict__ EA, int nb,
+                                                      int first_mode, int n_modes, int has_cat, int has_q0,
+                                                      const RedrawArgs ra, int ra_on) {
+    const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    const EventArgs& ea = param_sel<BATCH>(ea_arg, EA);
+    rates_scan_body(p, nb);
+    __syncthreads();
+    if (threadIdx.x == 0) event_prep_body<D, C>(p, BATCH ? P[blockIdx.y].ctl : ctl_arg, ea);
+    __syncthreads();
+    select_body(p, ea.u2);
+    if constexpr (FLIP) {
+        __syncthreads();
+        if (threadIdx.x == 0) flip_body<D, C>(p, -1, ea.time, first_mode, n_modes, has_cat, has_q0, 1, ra, ra_on);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlock); }
 
-template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
+template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double time, const double* times, hipStream_t s,
+                                                  bool with_scan) {
     const int nb = kmc_nblocks(l.g.N * l.M);
     const bool staged = l.M >= kStageMinM;
     if (l.g.dims == 2) {
@@ -836,15 +882,15 @@ template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double tim
         else
             hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
     }
-    hipLaunchKernelGGL(k_rates_scan<BATCH>, dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, nb);
+    if (with_scan) hipLaunchKernelGGL(k_rates_scan<BATCH>, dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, nb);
 }
 
-void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s) {
+void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s, bool with_scan) {
     if (l.batch) {
         if (!times) throw ArgError("a batch needs per-trajectory times");
-        enqueue_rates_t<true>(l, time, times, s);
+        enqueue_rates_t<true>(l, time, times, s, with_scan);
     } else {
-        enqueue_rates_t<false>(l, time, times, s);
+        enqueue_rates_t<false>(l, time, times, s, with_scan);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }
@@ -890,6 +936,40 @@ void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode,
     } else {
         if (l.batch) enqueue_flip_t<3, 6, true>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, 0);
         else enqueue_flip_t<3, 6, false>(l, flat, time, first_mode, n_modes, has_cat, has_q0, s, time_from_ctl, ra, ra_on);
+    }
+    MGK_HIP_CHECK(hipGetLastError());
+}
+
+template <int D, int C, bool BATCH>
+static void enqueue_chain_t(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
+                            const EventArgs* ea, bool flip, int first_mode, int n_modes, int has_cat, int has_q0,
+                            const RedrawArgs& ra, int ra_on, hipStream_t s) {
+    const int nb = kmc_nblocks(l.g.N * l.M);
+    if (flip)
+        hipLaunchKernelGGL((k_event_chain<D, C, BATCH, true>), dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, ctl, P, ea_one,
+                           ea, nb, first_mode, n_modes, has_cat, has_q0, ra, ra_on);
+    else
+        hipLaunchKernelGGL((k_event_chain<D, C, BATCH, false>), dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, ctl, P,
+                           ea_one, ea, nb, first_mode, n_modes, has_cat, has_q0, ra, ra_on);
+}
+
+void kmc_enqueue_event_chain(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
+                             const EventArgs* ea, bool flip, int first_mode, int n_modes, int has_cat, int has_q0,
+                             const double* host_redraw, hipStream_t s) {
+    if (l.batch ? (!P || !ea) : (!ctl || l.B != 1)) throw ArgError("event decision: missing control blocks");
+    RedrawArgs ra{};
+    int ra_on = 0;
+    if (host_redraw) {
+        if (l.batch || !kmc_redraw_fits_args(n_modes, l.g.C)) throw ArgError("redraw values too many for kernel arguments");
+        for (int k = 0; k < n_modes * (l.g.C + 1); ++k) ra.v[k] = host_redraw[k];
+        ra_on = 1;
+    }
+    if (l.g.dims == 2) {
+        if (l.batch) enqueue_chain_t<2, 3, true>(l, ctl, P, ea_one, ea, flip, first_mode, n_modes, has_cat, has_q0, ra, 0, s);
+        else enqueue_chain_t<2, 3, false>(l, ctl, P, ea_one, ea, flip, first_mode, n_modes, has_cat, has_q0, ra, ra_on, s);
+    } else {
+        if (l.batch) enqueue_chain_t<3, 6, true>(l, ctl, P, ea_one, ea, flip, first_mode, n_modes, has_cat, has_q0, ra, 0, s);
+        else enqueue_chain_t<3, 6, false>(l, ctl, P, ea_one, ea, flip, first_mode, n_modes, has_cat, has_q0, ra, ra_on, s);
     }
     MGK_HIP_CHECK(hipGetLastError());
 }

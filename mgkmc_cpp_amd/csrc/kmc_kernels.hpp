@@ -126,7 +126,8 @@ struct KmcLaunch {
 
 int kmc_nblocks(long nmodes);
 // times: per-trajectory times (device, B entries) or null for `time`
-void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s);
+// with_scan = false leaves the block-sum scan to kmc_enqueue_event_chain
+void kmc_enqueue_rates(const KmcLaunch& l, double time, const double* times, hipStream_t s, bool with_scan = true);
 // ea: per-trajectory event arguments (device) whose u2 replaces `u2`, or null
 void kmc_enqueue_select(const KmcLaunch& l, double u2, const EventArgs* ea, hipStream_t s);
 void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipStream_t s);
@@ -149,6 +150,12 @@ void kmc_enqueue_patch(const KmcLaunch& l, const PatchParams& pp, int voxel, con
 // enqueued behind it returns at once.  B = 1: ctl and ea_one; a batch: the
 // control blocks P[y].ctl and the arguments ea[y] (device).
 struct LsParams;
+// scan + event decision + selection (+ flip with its redraw) in one launch
+// after kmc_enqueue_rates(..., with_scan = false); host_redraw as for
+// kmc_enqueue_flip (single trajectory), else the staging buffer
+void kmc_enqueue_event_chain(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
+                             const EventArgs* ea, bool flip, int first_mode, int n_modes, int has_cat, int has_q0,
+                             const double* host_redraw, hipStream_t s);
 void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
                             const EventArgs* ea, hipStream_t s);
 
