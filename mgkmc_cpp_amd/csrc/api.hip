@@ -398,7 +398,7 @@ struct mgk_ctx {
                     if (refine) {
                         const int sp = spec_from();
                         for (int j = 0; j < J; ++j)
-                            ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp), stream);
+                            ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp, J), stream);
                     }
                 }
                 enqueued += batch;
@@ -469,9 +469,9 @@ struct mgk_ctx {
             if (h > 0) mn = std::min(mn, h);
         return mn;
     }
-    static int cg_flags(int j, int spec_from) {
+    static int cg_flags(int j, int spec_from, int J) {
         const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
-        return f | (j >= spec_from ? LS_FLAG_SPEC : 0);
+        return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j & 1) ? LS_FLAG_ODD : 0) | (j == J - 1 ? LS_FLAG_LAST : 0);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -1775,7 +1775,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
             const int flags = (k == 0 ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
             ls_enqueue_iteration(L, LS_FULL, flags, st);
             if (refine)
-                for (int j = 0; j < J; ++j) ls_enqueue_iteration(L, LS_CG, mgk_ctx::cg_flags(j, sp), st);
+                for (int j = 0; j < J; ++j) ls_enqueue_iteration(L, LS_CG, mgk_ctx::cg_flags(j, sp, J), st);
         }
         ls_enqueue_final(L, st);
         MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, st));

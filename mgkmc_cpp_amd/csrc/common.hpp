@@ -80,6 +80,11 @@ struct Ctl {
     double cg_alpha, cg_beta, cg_gamma;
     int cg_stop;           // CG breakdown: the rest of the cycle is skipped
     int pad2;
+    // CG scalars and step count by step parity, for reductions folded into
+    // the next step's first kernel: the reduction of step j writes slot j & 1
+    // and reads slot (j - 1) & 1, which no block of that kernel writes
+    double cgs_gamma[2], cgs_alpha[2], cgs_beta[2];
+    int cgs_inner[2];
 };
 
 struct Status {

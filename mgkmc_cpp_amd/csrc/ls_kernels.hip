@@ -804,6 +804,63 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
 }
 
 // ---------------------------------------------------------------------------
+// The CG reduction of one step from its three partial sums: the stopping test
+// on ||r||_W / ||eps|| (the relative increment the reference's basic scheme
+// would measure at this iterate) and the Chronopoulos-Gear scalars (alpha,
+// beta) of the next update.  `slots`: the step's scalars and count go to slot
+// `slot_new` and the previous ones come from the other slot (reductions
+// folded into the plane passes), else the single fields.  Only `write`
+// updates the control block.
+// ---------------------------------------------------------------------------
+struct CgDecision {
+    double al, be;
+    int halt;   // converged, at the iteration cap, or broken down
+};
+
+__device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&tot)[3], bool first, bool slots,
+                                                     int slot_new, bool write) {
+    const double gam = tot[0], del = tot[1];
+    const int inner = (slots ? ctl->cgs_inner[slot_new ^ 1] : ctl->inner) + 1;
+    bool done = false;
+    double diff = 0.0;
+    if (!first) {
+        diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
+        done = diff < ctl->tol || ctl->iters + inner >= ctl->max_iter;
+    }
+    double al, be;
+    if (first) {
+        be = 0.0;
+        al = gam / del;
+    } else {
+        const double g_old = slots ? ctl->cgs_gamma[slot_new ^ 1] : ctl->cg_gamma;
+        const double a_old = slots ? ctl->cgs_alpha[slot_new ^ 1] : ctl->cg_alpha;
+        be = gam / g_old;
+        al = gam / (del - be * gam / a_old);
+    }
+    const bool stop = !(al > 0.0) || !isfinite(al) || !isfinite(be);   // breakdown: back to FP64
+    if (write) {
+        ctl->inner = inner;
+        if (!first) {
+            ctl->corr_valid = 1;
+            ctl->diff = diff;
+            if (ctl->iters + inner < 32) ctl->diff_hist[ctl->iters + inner] = diff;
+            if (done) ctl->done = 1;
+        }
+        if (stop) ctl->cg_stop = 1;
+        ctl->cg_alpha = al;
+        ctl->cg_beta = be;
+        ctl->cg_gamma = gam;
+        if (slots) {
+            ctl->cgs_inner[slot_new] = inner;
+            ctl->cgs_alpha[slot_new] = al;
+            ctl->cgs_beta[slot_new] = be;
+            ctl->cgs_gamma[slot_new] = gam;
+        }
+    }
+    return {al, be, (done || stop) ? 1 : 0};
+}
+
+// ---------------------------------------------------------------------------
 // Plane passes (3D CG steps on square nm = nl = NP planes).  One block per
 // (component c, plane f) does pass A's work for its component and both the L
 // and the M transform in LDS, so passes B and D disappear:
@@ -847,6 +904,24 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     const long base = (long)c * g.N + (long)f * VOX;   // component c of plane f
     const bool upd = flags & (LS_FLAG_CG_UPDATE1 | LS_FLAG_CG_UPDATE);
     const bool upd1 = flags & LS_FLAG_CG_UPDATE1;
+    // An updating step folds in the reduction of the step before it: every
+    // block reduces the same partials in the same order (so all take the same
+    // decision and scalars); block 0 writes the control block, into the slot
+    // no block of this launch reads.
+    __shared__ double red[16 * 3];
+    __shared__ double s_al, s_be;
+    __shared__ int s_halt;
+    constexpr int PU = (6 * 256 + kPlaneThreads - 1) / kPlaneThreads;   // partial rows per lane (nf <= 256)
+    double part[3][PU];
+    const int nbp = 6 * g.nf;   // blocks of the previous step's inverse plane pass
+    if (upd) {
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int b = threadIdx.x + kPlaneThreads * u;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) part[k][u] = p.partials[3 * (b < nbp ? b : 0) + k];
+        }
+    }
 
     float r[VPT], w[VPT], pv[VPT], sv[VPT], xv[VPT], dm[VPT], dl[VPT];
 #pragma unroll
@@ -868,7 +943,31 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
     if (stop) return;
 
-    const float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
+    float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
+    if (upd) {
+        double tot[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double v = 0.0;
+#pragma unroll
+            for (int u = 0; u < PU; ++u)
+                if ((int)threadIdx.x + kPlaneThreads * u < nbp) v += part[k][u];
+            tot[k] = v;
+        }
+        block_sum<3>(tot, red);
+        if (threadIdx.x == 0) {
+            const bool probe = flags & LS_FLAG_PROBE;
+            const int slot_new = (flags & LS_FLAG_ODD) ? 0 : 1;   // the previous step's parity
+            const CgDecision d = cg_step_update(p.ctl, tot, upd1, true, slot_new, blockIdx.x == 0 && !probe);
+            s_al = d.al;
+            s_be = d.be;
+            s_halt = probe ? 0 : d.halt;
+        }
+        __syncthreads();
+        if (s_halt) return;
+        al = (float)s_al;
+        be = (float)s_be;
+    }
     float* bufr = reinterpret_cast<float*>(buf);
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
@@ -1023,32 +1122,10 @@ __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const L
             ctl->corr_valid = 0;   // a pending correction was absorbed into eps
             ctl->cg_stop = 0;
             if (diff < ctl->tol || ctl->iters + ctl->inner >= ctl->max_iter) ctl->done = 1;
+            ctl->cgs_inner[1] = ctl->inner;   // the count a cycle's first CG reduction starts from
         } else {
-            // Chronopoulos-Gear scalars (alpha, beta) for the next update, and the
-            // stopping test on ||r||_W / ||eps||, the relative increment the
-            // reference's basic scheme would measure at this iterate.
-            const double gam = tot[0], del = tot[1];
-            const bool first = flags & LS_FLAG_CG_FIRST;
-            ctl->inner += 1;
-            if (!first) {
-                ctl->corr_valid = 1;
-                const double diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
-                ctl->diff = diff;
-                if (ctl->iters + ctl->inner < 32) ctl->diff_hist[ctl->iters + ctl->inner] = diff;
-                if (diff < ctl->tol || ctl->iters + ctl->inner >= ctl->max_iter) ctl->done = 1;
-            }
-            double al, be;
-            if (first) {
-                be = 0.0;
-                al = gam / del;
-            } else {
-                be = gam / ctl->cg_gamma;
-                al = gam / (del - be * gam / ctl->cg_alpha);
-            }
-            if (!(al > 0.0) || !isfinite(al) || !isfinite(be)) ctl->cg_stop = 1;   // breakdown: back to FP64
-            ctl->cg_alpha = al;
-            ctl->cg_beta = be;
-            ctl->cg_gamma = gam;
+            const bool slots = flags & LS_FLAG_PLANE_SLOTS;
+            cg_step_update(ctl, tot, (flags & LS_FLAG_CG_FIRST) != 0, slots, (flags & LS_FLAG_ODD) ? 1 : 0, true);
         }
     }
 }
@@ -1357,9 +1434,11 @@ template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int 
     const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
     if constexpr (C == 6) {
         if (p.plane) {
+            // the next step's forward plane pass folds in this step's
+            // reduction; the cycle's last step keeps a reduction kernel
             launch_fwd_plane<BATCH>(p, flags, s);
             launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG, spec);
-            launch_inv_plane<BATCH>(p, flags, s);
+            launch_inv_plane<BATCH>(p, flags | LS_FLAG_PLANE_SLOTS, s, (flags & LS_FLAG_LAST) != 0);
             return;
         }
     }
@@ -1430,7 +1509,7 @@ template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int p
         case 8:
             if constexpr (C == 6) {
                 if (full || !ls_plane_supported(p.g)) throw ArgError("plane passes run in FP32 on supported 3D grids");
-                if (pass == 6) launch_fwd_plane<BATCH>(p, LS_FLAG_CG_UPDATE, s);
+                if (pass == 6) launch_fwd_plane<BATCH>(p, LS_FLAG_CG_UPDATE | LS_FLAG_PROBE, s);
                 else if (pass == 7) launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG);
                 else launch_inv_plane<BATCH>(p, 0, s, false);
             } else {

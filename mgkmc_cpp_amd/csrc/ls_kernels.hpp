@@ -13,7 +13,11 @@ enum {
     LS_FLAG_CG_FIRST = 4,     // CG step 0 of a cycle: operator on r0 only
     LS_FLAG_CG_UPDATE1 = 8,   // CG step 1: p = r, s = w, x = alpha p (no history)
     LS_FLAG_CG_UPDATE = 16,   // CG step >= 2: full recurrence
-    LS_FLAG_SPEC = 32         // CG step past the likely count: test `done` before loading anything
+    LS_FLAG_SPEC = 32,        // CG step past the likely count: test `done` before loading anything
+    LS_FLAG_ODD = 64,         // CG step index within the cycle is odd (the reduction slots)
+    LS_FLAG_LAST = 128,       // last CG step of the cycle: its reduction is a kernel of its own
+    LS_FLAG_PROBE = 256,      // timing launch: a folded reduction is loaded and computed, not applied
+    LS_FLAG_PLANE_SLOTS = 512 // set by the launcher: the CG reduction uses the parity slots (plane passes)
 };
 
 // how a solve starts
