@@ -477,6 +477,88 @@ __device__ __forceinline__ void select_body(const KmcParams& p, double u2) {
     kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
 }
 
+// The selection for a 1024-lane block that has just run rates_scan_body:
+// each lane searches the inclusive prefixes of its own scan chunk (its own
+// writes), and the chosen block's 1024 modes get one lane each, with a wave
+// scan and a scan of the 16 wave totals.  The summation order differs from
+// select_body's; the rounding-bound test holds for any order of the 1024
+// terms.
+__device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) {
+    const long nmodes = p.g.N * p.ph.M;
+    KmcCtl* kc = p.kctl;
+    if (!kc->go) return;
+    const int nb = kc->nblocks;
+    const double total = kc->total_rate;
+    const double target = u2 * total;
+    __shared__ int s_b1, s_first1;
+    __shared__ double s_scan1[kRateBlock];
+    __shared__ double s_wave[16];
+    if (threadIdx.x == 0) {
+        s_b1 = nb - 1;   // no prefix reaches the target: the last block
+        s_first1 = kRateBlock;
+    }
+    __syncthreads();
+    {
+        const int chunk = (nb + 1023) / 1024;   // rates_scan_body's chunks
+        const int b0 = threadIdx.x * chunk;
+        for (int k = 0; k < chunk; ++k) {
+            const int b = b0 + k;
+            if (b >= nb) break;
+            if (p.bincl[b] >= target) {
+                atomicMin(&s_b1, b);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    const int b = s_b1;
+    const long base = (long)b * kRateBlock;
+    const double prev = (b > 0) ? p.bincl[b - 1] : 0.0;
+    const long i = base + threadIdx.x;
+    bool incl = false;
+    const double r = (i < nmodes) ? mode_rate(p.ph, p.Q[i], incl) : 0.0;
+    double v = incl ? r : 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) s_wave[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            const double t = s_wave[w];
+            s_wave[w] = run;   // exclusive wave offsets
+            run += t;
+        }
+    }
+    __syncthreads();
+    const double S = prev + (s_wave[wid] + v);
+    s_scan1[threadIdx.x] = S;
+    if (i < nmodes && incl && S >= target) atomicMin(&s_first1, (int)threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double u = 1.1102230246251565e-16;   // 2^-53
+    int sel = -1, amb = 1;
+    if (s_first1 < kRateBlock) {
+        const int li = s_first1;
+        const long im = base + li;
+        const double Sm = s_scan1[li];
+        const double Sp = (li > 0) ? s_scan1[li - 1] : prev;
+        const double mS = 2.0 * ((double)im + 2048.0) * u * Sm + 2.0 * (double)nmodes * u * total;
+        const double mP = 2.0 * ((double)im + 2048.0) * u * Sp + 2.0 * (double)nmodes * u * total;
+        sel = (int)im;
+        amb = (Sm - target <= mS) || (target - Sp <= mP) ? 1 : 0;
+    }
+    kc->u2 = u2;
+    kc->target = target;
+    kc->sel = sel;
+    kc->ambiguous = amb;
+    kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_select(const KmcParams p_arg, const KmcParams* __restrict__ PK, double u2,
                                                 const EventArgs* __restrict__ EA) {
@@ -839,8 +921,8 @@ __global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict_
 // ---------------------------------------------------------------------------
 // One block per trajectory for everything between the rate pass and the LS
 // batch of a device-decided event: the block-sum scan, the event decision,
-// the selection and (FLIP) the flip with its redraw -- the same code as the
-// four separate kernels, in one launch.  FP64 contexts leave the flip out:
+// the selection and (FLIP) the flip with its redraw, in one launch (the
+// selection as select_body_1024).  FP64 contexts leave the flip out:
 // the exact selection pass runs between.
 // ---------------------------------------------------------------------------
 template <int D, int C, bool BATCH, bool FLIP>
@@ -855,7 +937,7 @@ __global__ __launch_bounds__(1024) void k_event_chain(const KmcParams p_arg, con
     __syncthreads();
     if (threadIdx.x == 0) event_prep_body<D, C>(p, BATCH ? P[blockIdx.y].ctl : ctl_arg, ea);
     __syncthreads();
-    select_body(p, ea.u2);
+    select_body_1024(p, ea.u2);
     if constexpr (FLIP) {
         __syncthreads();
         if (threadIdx.x == 0) flip_body<D, C>(p, -1, ea.time, first_mode, n_modes, has_cat, has_q0, 1, ra, ra_on);
