@@ -315,6 +315,47 @@ def test_device_event_without_rate_or_with_unstable_modes():
         assert np.array_equal(a.get_field(f), before[f]), f
 
 
+def test_batch_matches_separate_fp32_refine_plane_passes():
+    # FP32+refine on 16^3 (square planes: the CG steps run as plane passes
+    # with folded reductions).  A batch runs every trajectory's CG cycles with
+    # the batch's longest predicted cycle, so a solve can end in another cycle
+    # than when run alone: the same events, times and fields within the
+    # FP32+refine bar, not bit for bit (FP64 batches are, see below).
+    from mgkmc_cpp_amd import SimulationBatch
+    seeds, events = (42, 7, 1234), 8
+    cfgs = [_cfg(seed=sd, dims=3, nx=16, ny=16, nz=16, M=12, precision=abi.MGK_FP32_REFINE) for sd in seeds]
+    mats = [_field(16 ** 3, seed=17 + i) for i in range(len(seeds))]
+    single = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
+    batched = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
+    for s in single + batched:
+        s.elastic_run()
+    st_single = [s.event_steps(events) for s in single]
+    batch = SimulationBatch(batched)
+    st_batch = batch.event_steps(events)
+    for a, b, sa, sb in zip(single, batched, st_single, st_batch):
+        assert sa.kmc_events == sb.kmc_events == events
+        assert sb.ls_iters_fp32 > 0
+        ea, eb = a.events(), b.events()
+        assert [e.flat_idx for e in ea] == [e.flat_idx for e in eb]
+        ta, tb = np.array([e.time for e in ea]), np.array([e.time for e in eb])
+        rt = np.abs(ta - tb) / np.abs(ta)
+        assert rt.max() < 1e-6, rt
+        for f in (abi.FIELD_EPS, abi.FIELD_SIG):
+            assert _rel(b.get_field(f), a.get_field(f)) < 1e-6, f
+        for f in (abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_Q0):
+            assert np.array_equal(a.get_field(f), b.get_field(f)), f
+    # against the FP64 oracle: the same events; the stress within 1e-5 of max
+    # |sigma| (2.7e-6 measured).  Both solves stop at tol 1e-8 from warm
+    # starts, each with its own truncation error near the flipped voxels; a
+    # single cold solve is within 1e-6 (tests/test_gpu_ls.py).
+    osim = po.OracleSim(cfgs[0], *mats[0])
+    osim.elastic_run()
+    osim.event_steps(events)
+    assert [e.flat_idx for e in osim.events()] == [e.flat_idx for e in batched[0].events()]
+    assert _rel(batched[0].get_field(abi.FIELD_SIG), osim.get_field(abi.FIELD_SIG)) < 1e-5
+    batch.close()
+
+
 @pytest.mark.parametrize("kw,events,seeds", [
     (dict(), 25, (42, 7, 1234)),                                           # 2D fixed catalog, cascade mode
     (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM, neighbor_softening_fraction=0.2), 25, (42, 7, 1234)),
