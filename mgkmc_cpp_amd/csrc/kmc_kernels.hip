@@ -7,7 +7,9 @@
 // sums and an in-block scan; when the target lies within a rounding bound of
 // a prefix boundary -- the only case where the parallel order could choose a
 // different mode than the reference's left-to-right sum -- it reports
-// "ambiguous" and the host runs the exact sequential pass instead.
+// "ambiguous" and the exact sequential pass (k_select_seq) decides instead.
+// A device-decided event runs the scan, decision, selection and flip in one
+// block (k_event_chain).
 #include "common.hpp"
 #include "kmc_kernels.hpp"
 #include "ls_kernels.hpp"

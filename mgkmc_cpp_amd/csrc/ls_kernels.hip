@@ -10,16 +10,23 @@
 //   C  first_gamma: c2c along F, eps_hat = -Gamma0_eff(xi):tau_hat / N on the
 //                   fly (no stored Green tensor), inverse c2c along F
 //   D  mid   inv  : c2c along M (3D only)
-//   E  inv_last   : c2r along L, eps_new = eps_bar + eps_tilde, the
-//                   relative-increment norms, and a last-block reduction that
-//                   sets the convergence flag in the control block
+//   E  inv_last   : c2r along L, eps_new = eps_bar + eps_tilde and the
+//                   relative-increment norms (block partials)
+//   k_ls_reduce   : one block reduces the partials and sets the convergence
+//                   flag in the control block
+//
+// On 3D grids with square nm = nl planes the FP32 CG steps run as plane
+// passes instead: k_fwd_plane (A and B for one component of one f-plane),
+// pass C on a tiled spectrum, k_inv_plane (D and E); the forward plane pass
+// folds in the previous step's reduction.
 //
 // Every kernel returns at once when the control block says the solve is done,
 // so the host enqueues iterations in batches without a sync per iteration.
 //
 // Spectrum layout: component-interleaved half spectrum,
 //   spec[((f*nm + m)*nlh + l)*C + c]   (complex),
-// so every pass moves contiguous rows of (bin, component) pairs.
+// so every pass moves contiguous rows of (bin, component) pairs (the plane
+// passes tile it: runs of kPlaneTile bins per component).
 //
 // The same passes run in two arithmetic modes:
 //   FULL  (FP64 spectra): the reference's iteration on eps.
