@@ -229,6 +229,101 @@ def cpu_baseline(E, nu, budget_s=15.0):
                        f"{'the reference pocketfft (oracle/_ref)' if ref else 'the oracle FFT'}")
 
 
+def field_errors(E, nu, events=40):
+    """FP32+refine against the FP64 path on the same workload and state: the
+    same number of events from the same pre-loaded state, then the largest
+    difference of each field relative to the largest FP64 value (eps, sigma,
+    eps_p, Q), and of the total rate R and the event times over the events."""
+    from mgkmc_cpp_amd import abi
+    sims = {}
+    for prec in (abi.MGK_FP64, abi.MGK_FP32_REFINE):
+        sims[prec], _ = run_gpu(prec, events, 0, E, nu)
+    a, b = sims[abi.MGK_FP32_REFINE], sims[abi.MGK_FP64]
+    ea, eb = a.events(), b.events()
+    same = 0
+    for x, y in zip(ea, eb):
+        if x.flat_idx != y.flat_idx:
+            break
+        same += 1
+
+    def rel(fa, fb):
+        return float(np.abs(fa - fb).max() / max(np.abs(fb).max(), 1e-300))
+
+    out = dict(events=events, same_events=same, norm="max |fp32_refine - fp64| / max |fp64|")
+    if same == len(ea) == len(eb):
+        out["rel_err"] = {name: rel(a.get_field(f), b.get_field(f)) for name, f in
+                          (("eps", abi.FIELD_EPS), ("sig", abi.FIELD_SIG), ("eps_plastic", abi.FIELD_EPS_PLASTIC),
+                           ("Q", abi.FIELD_Q))}
+        out["rel_err"]["R"] = max(abs(x.total_rate - y.total_rate) / y.total_rate for x, y in zip(ea, eb))
+        out["rel_err"]["time"] = max(abs(x.time - y.time) / abs(y.time) for x, y in zip(ea, eb))
+        out["rel_err"] = {k: float("%.3g" % v) for k, v in out["rel_err"].items()}
+    for sim in sims.values():
+        sim.close()
+    return out
+
+
+def whole_run(E, nu, events=1000, steps=1000):
+    """run_simulation on the 64^3 workload (FP32+refine, strain steps of 1e-4
+    along xx as configs.uniaxial_run, logging off) until `events` KMC events
+    or `steps` elastic steps: events/s with the elastic steps included."""
+    from mgkmc_cpp_amd import Simulation, abi, configs
+    cfg = configs.sim_config(SHAPE, precision=abi.MGK_FP32_REFINE, seed=42)
+    sim = Simulation(cfg, E, nu)
+    run = abi.default_run_config(n_global_steps=steps, **configs.uniaxial_run(events))
+    t0 = time.perf_counter()
+    st = sim.run(run)
+    dt = time.perf_counter() - t0
+    sim.close()
+    return dict(events=int(st.kmc_events), elastic_steps=int(st.elastic_steps), cascades=int(st.cascades),
+                seconds=round(dt, 3), events_per_s=round(st.kmc_events / dt, 1) if dt > 0 else None,
+                ls_solves=int(st.ls_solves), workload=f"run_simulation, 64^3 FP32+refine, strain steps of 1e-4 "
+                f"along xx from zero, up to {events} events or {steps} elastic steps, logging off")
+
+
+def cpu_worker(seed, budget_s):
+    """One CPU-baseline trajectory (the oracle on one core) for the aggregate
+    leg: prints {events, seconds}.  Touches no GPU."""
+    from mgkmc_cpp_amd import abi, configs
+    from oracle import pyoracle as po
+    E, nu = configs.material(SHAPE, 4.0, 42)
+    cfg = configs.sim_config(SHAPE, precision=abi.MGK_FP64, seed=seed)
+    osim = po.OracleSim(cfg, E, nu, ref=os.path.exists(po.REF_LIB))
+    e = np.zeros((3, 3))
+    e[0, 0] = PRELOAD_STRAIN
+    osim.set_macro(e, 0.0)
+    osim.elastic_run()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s and n < 50:
+        st = osim.event_steps(1)
+        n += int(st.kmc_events)
+        if st.kmc_events == 0:
+            break
+    print(json.dumps(dict(events=n, seconds=time.perf_counter() - t0)), flush=True)
+
+
+def cpu_aggregate(budget_s):
+    """P independent oracle trajectories (seeds 42..42+P-1), one per core, run
+    at once: aggregate CPU events/s.  P = the host's cores, at most 16 (a GPU
+    box's CPU share)."""
+    P = max(1, min(16, os.cpu_count() or 1))
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(42 + i),
+                               "--cpu-budget", str(budget_s)], stdout=subprocess.PIPE, text=True,
+                              env=dict(os.environ, OMP_NUM_THREADS="1")) for i in range(P)]
+    t0 = time.perf_counter()
+    res = []
+    for pr in procs:
+        outp, _ = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError("CPU worker failed")
+        res.append(json.loads(outp.strip().splitlines()[-1]))
+    wall = time.perf_counter() - t0
+    n = sum(r["events"] for r in res)
+    return dict(value=round(n / wall, 4), unit="events/s", cores=P, kind="port",
+                sample=f"{P} independent FP64 trajectories of the same 64^3 workload (seeds 42..{41 + P}), "
+                       f"one per core, {n} events in {wall:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,7 +337,11 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=48, help="requests of the bounded c4 sample")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="HBM admission budget of the serving loop")
+    ap.add_argument("--cpu-worker", type=int, default=None, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_worker is not None:
+        cpu_worker(a.cpu_worker, a.cpu_budget)
+        return
     if a.probe:
         probe()
         return
@@ -268,10 +367,19 @@ def main():
     roof["bytes_per_launch"] = int(top["bytes"])
     roof["avg_launch_ms"] = round(top["ms"], 5)
     sim32.close()
-    cpu = None
+    print("[bench] FP32+refine against FP64: per-field errors", file=sys.stderr, flush=True)
+    errs = field_errors(E, nu)
+    print(f"[bench] field errors {errs}", file=sys.stderr, flush=True)
+    print("[bench] whole run with elastic steps", file=sys.stderr, flush=True)
+    whole = whole_run(E, nu)
+    print(f"[bench] whole run {whole}", file=sys.stderr, flush=True)
+    cpu = cpu_all = None
     if rank == 0 and not a.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(E, nu, a.cpu_budget)
+        print(f"[bench] CPU baseline {cpu}", file=sys.stderr, flush=True)
+        cpu_all = cpu_aggregate(a.cpu_budget)
+        print(f"[bench] CPU aggregate {cpu_all}", file=sys.stderr, flush=True)
     serving_sample = None
     if not a.no_serving:
         # bounded c4 sample: Poisson arrivals at 50/s, grids up to 64^3, events
@@ -305,6 +413,9 @@ def main():
                                "fp32_refine_fp32": round(i32["fp32_iters_per_event"], 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_aggregate": cpu_all,
+        "fp32_refine_errors_vs_fp64": errs,
+        "whole_run": whole,
         "serving": serving_sample,
     }
     if rank == 0:
