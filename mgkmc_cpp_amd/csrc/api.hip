@@ -471,7 +471,8 @@ struct mgk_ctx {
     }
     static int cg_flags(int j, int spec_from, int J) {
         const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
-        return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j & 1) ? LS_FLAG_ODD : 0) | (j == J - 1 ? LS_FLAG_LAST : 0);
+        return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j & 1) ? LS_FLAG_ODD : 0) | (j == J - 1 ? LS_FLAG_LAST : 0) |
+               (j << LS_STEP_SHIFT);
     }
 
     // CG steps per refinement cycle for the next solve: one more than the
@@ -1101,7 +1102,7 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         if (pm && !c.unit_ready) throw ArgError("sigma_macro_unit not set (mgk_patch_unit)");
         int has_cat = 0, has_q0 = 0;
         const int n_modes = stage_redraw(c, in->new_catalog, in->new_Q0, in->first_mode, in->n_modes, &has_cat, &has_q0);
-        // LS control block: reset on the device by k_event_prep (init_ctl),
+        // LS control block: reset on the device by the event decision (init_ctl),
         // which also fills the imposed strain and the shift
         const int start = c.start_mode(true);
         EventArgs ea{};
@@ -1730,6 +1731,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         c.init_ctl(on ? e.max_iter : 1, on ? e.tol : 0.0);
         Ctl& h = b.h_ctl[y];
         h = *c.hctl;
+        h.cycle_len = c.inner_per_outer;   // this trajectory's own cycle, as when it runs alone
         const int keep = c.mean_valid ? 1 : 0;
         if (on) all_keep = all_keep && keep;
         h.keep_shift = keep;

@@ -79,7 +79,7 @@ struct Ctl {
     double eps_norm;       // ||eps|| after the last FP64 iteration
     double cg_alpha, cg_beta, cg_gamma;
     int cg_stop;           // CG breakdown: the rest of the cycle is skipped
-    int pad2;
+    int cycle_len;         // CG steps of this solve's cycles (0: as many as enqueued)
     // CG scalars and step count by step parity, for reductions folded into
     // the next step's first kernel: the reduction of step j writes slot j & 1
     // and reads slot (j - 1) & 1, which no block of that kernel writes

@@ -859,6 +859,7 @@ __device__ __forceinline__ void event_prep_body(const KmcParams& p, Ctl* ctl, co
         double* w = reinterpret_cast<double*>(ctl);
         for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(double)); ++k) w[k] = 0.0;
         ctl->max_iter = ea.max_iter;
+        ctl->cycle_len = ea.cycle_len;
         ctl->tol = ea.tol;
         ctl->keep_shift = ea.keep_shift;
         for (int c = 0; c < 6; ++c) ctl->mean_eps[c] = ea.mean_eps[c];
@@ -910,14 +911,6 @@ __device__ __forceinline__ void event_prep_body(const KmcParams& p, Ctl* ctl, co
         ctl->eps_bar[c] = sym[c];
         if (ea.keep_shift) ctl->shift[c] = sym[c] - ctl->mean_eps[c];
     }
-}
-
-template <int D, int C, bool BATCH>
-__global__ void k_event_prep(const KmcParams p_arg, const KmcParams* __restrict__ PK, Ctl* ctl_arg,
-                             const LsParams* __restrict__ P, const EventArgs ea_arg, const EventArgs* __restrict__ EA) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    event_prep_body<D, C>(param_sel<BATCH>(p_arg, PK), BATCH ? P[blockIdx.y].ctl : ctl_arg,
-                          param_sel<BATCH>(ea_arg, EA));
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,22 +1051,6 @@ void kmc_enqueue_event_chain(const KmcLaunch& l, Ctl* ctl, const LsParams* P, co
     MGK_HIP_CHECK(hipGetLastError());
 }
 
-void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
-                            const EventArgs* ea, hipStream_t s) {
-    if (l.batch ? (!P || !ea) : (!ctl || l.B != 1)) throw ArgError("event decision: missing control blocks");
-    if (l.batch) {
-        if (l.g.dims == 2)
-            hipLaunchKernelGGL((k_event_prep<2, 3, true>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
-        else
-            hipLaunchKernelGGL((k_event_prep<3, 6, true>), dim3(1, l.B), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
-    } else {
-        if (l.g.dims == 2)
-            hipLaunchKernelGGL((k_event_prep<2, 3, false>), dim3(1, 1), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
-        else
-            hipLaunchKernelGGL((k_event_prep<3, 6, false>), dim3(1, 1), dim3(64), 0, s, l.h, l.P, ctl, P, ea_one, ea);
-    }
-    MGK_HIP_CHECK(hipGetLastError());
-}
 
 // ---------------------------------------------------------------------------
 // Fast patching (kmc_solver.hpp:525-547, :866-913, :941-949; 3D :1535-1573,

@@ -60,6 +60,8 @@ struct EventArgs {
     int active;            // 0: no attempt for this trajectory (MGK_EVENT_SKIPPED)
     int init_ctl;          // 1: the attempt resets the LS control block itself (max_iter, tol, mean)
     int max_iter;
+    int cycle_len;         // CG steps per refinement cycle of this trajectory (0: all enqueued)
+    int pad_;
     double tol;
     double mean_eps[6];    // field mean of the last solve (used when keep_shift)
 };
@@ -156,7 +158,5 @@ struct LsParams;
 void kmc_enqueue_event_chain(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
                              const EventArgs* ea, bool flip, int first_mode, int n_modes, int has_cat, int has_q0,
                              const double* host_redraw, hipStream_t s);
-void kmc_enqueue_event_prep(const KmcLaunch& l, Ctl* ctl, const LsParams* P, const EventArgs& ea_one,
-                            const EventArgs* ea, hipStream_t s);
 
 }  // namespace mgk

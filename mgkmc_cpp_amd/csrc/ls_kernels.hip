@@ -73,6 +73,13 @@ template <bool BATCH, typename T> __device__ __forceinline__ const T& param_sel(
 // small helpers
 // ---------------------------------------------------------------------------
 
+// A CG step past this solve's cycle length (a batch enqueues its longest
+// trajectory's cycle; the others skip the extra steps, as they would not run
+// them alone).
+__device__ __forceinline__ bool past_cycle(const Ctl* ctl, int step) {
+    return ctl->cycle_len > 0 && step >= ctl->cycle_len;
+}
+
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
     constexpr int NLH = NL / 2 + 1;
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
-    const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    const bool stop = (p.ctl->done | (MODE == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, flags >> LS_STEP_SHIFT)) : 0)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
@@ -341,6 +348,8 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
     if constexpr (MODE == LS_FULL) {
         const double* src_ep = has_epsp ? p.epsp : p.eps;
         const float* src_cr = want_corr ? p.corr : reinterpret_cast<const float*>(p.eps);
+        // the last CG residual r goes with the correction x (w4 holds it here)
+        const float* src_r = (want_corr && p.r32) ? p.r32 : reinterpret_cast<const float*>(p.eps);
 #pragma unroll
         for (int k = 0; k < VPT; ++k) {
 #pragma unroll
@@ -348,6 +357,7 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
                 e[k][c] = p.eps[c * g.N + vid[k]];
                 ep[k][c] = src_ep[c * g.N + vid[k]];
                 cr[k][c] = src_cr[c * g.N + vid[k]];
+                w4[k][c] = src_r[c * g.N + vid[k]];
             }
             mu64[k] = p.mu[vid[k]];
             lam64[k] = p.lam[vid[k]];
@@ -391,21 +401,33 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
             const int li = vox / NL, j = vox % NL;
             double ee[C], ek[C];
             double tre = 0.0, tr0 = 0.0;
-            double crc[C];
+            // a pending CG cycle's iterate: x, then its last residual r (the
+            // increment the basic scheme would add there), as the final pass
+            // absorbs them (shift is 0 whenever a correction is pending)
+            double crc[C], rrc[C];
 #pragma unroll
-            for (int c = 0; c < C; ++c) crc[c] = (double)cr[k][c];
+            for (int c = 0; c < C; ++c) {
+                crc[c] = (double)cr[k][c];
+                rrc[c] = p.r32 ? (double)w4[k][c] : 0.0;
+            }
             if constexpr (C == 6) {
-                if (add_corr && p.mandel) {   // the CG iterate x in Mandel components
+                if (add_corr && p.mandel) {   // the CG vectors in Mandel components
                     double t[6];
                     mandel_to_cart(crc, t);
 #pragma unroll
                     for (int c = 0; c < 6; ++c) crc[c] = t[c];
+                    mandel_to_cart(rrc, t);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) rrc[c] = t[c];
                 }
             }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 ek[c] = e[k][c] + shift[c];
-                if (add_corr) ek[c] += crc[c];
+                if (add_corr) {
+                    ek[c] += crc[c];
+                    ek[c] += rrc[c];
+                }
                 ee[c] = has_epsp ? ek[c] - ep[k][c] : ek[c];
             }
 #pragma unroll
@@ -487,13 +509,13 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
 // ---------------------------------------------------------------------------
 template <typename SR, int NM, int DIR, bool BATCH>
 __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
-                                                        int speculative) {
+                                                        int speculative, int step) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int T = FftPlan<NM>::threads;
     constexpr int L = mid_cols<SR, NM>();
     constexpr int NMP = padded<NM>();
     using cx = cplx<SR>;
-    const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
     if (speculative && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* work = tw + NM;
@@ -522,7 +544,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // ---------------------------------------------------------------------------
 template <typename SR, int NF, int C, bool BATCH, bool PLANE = false>
 __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
-                                                                int speculative) {
+                                                                int speculative, int step) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int D = (C == 3) ? 2 : 3;
     constexpr int T = FftPlan<NF>::threads;
@@ -530,7 +552,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     constexpr int L = G * C;
     constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
-    const bool stop = (p.ctl->done | (mode == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
     if (speculative && stop) return;   // a speculative step: skip the loads
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* work = tw + NF;
@@ -652,7 +674,7 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
     constexpr int SPT = (PAIRS * NLH * C + 255) / 256;
     constexpr int D = (C == 3) ? 2 : 3;
     using cx = cplx<SR>;
-    const bool stop = (p.ctl->done | (MODE == LS_CG ? p.ctl->cg_stop : 0)) != 0;
+    const bool stop = (p.ctl->done | (MODE == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, flags >> LS_STEP_SHIFT)) : 0)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
     cx* tw = reinterpret_cast<cx*>(smem_raw);
     cx* buf = tw + NL;
@@ -688,8 +710,9 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
     const bool want_corr = (MODE == LS_FULL) && (flags & LS_FLAG_CORR) && p.corr;
     using VR = typename std::conditional<MODE == LS_FULL, double, float>::type;
     VR old[VPT][C];
-    float cr[VPT][C];
+    float cr[VPT][C], rr[VPT][C];
     const float* src_cr = want_corr ? p.corr : reinterpret_cast<const float*>(p.eps);
+    const float* src_r = (want_corr && p.r32) ? p.r32 : reinterpret_cast<const float*>(p.eps);
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
 #pragma unroll
@@ -697,6 +720,7 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
             if constexpr (MODE == LS_FULL) {
                 old[k][c] = p.eps[c * g.N + vid[k]];
                 cr[k][c] = src_cr[c * g.N + vid[k]];
+                rr[k][c] = src_r[c * g.N + vid[k]];
             } else {
                 old[k][c] = p.r32[c * g.N + vid[k]];
             }
@@ -740,15 +764,21 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
             if (!ok[k]) continue;
             const int vox = threadIdx.x + 256 * k;
             const int li = vox / NL, j = vox % NL;
-            double crc[C], dd[C];
+            double crc[C], rrc[C], dd[C];
 #pragma unroll
-            for (int c = 0; c < C; ++c) crc[c] = (double)cr[k][c];
+            for (int c = 0; c < C; ++c) {
+                crc[c] = (double)cr[k][c];
+                rrc[c] = p.r32 ? (double)rr[k][c] : 0.0;
+            }
             if constexpr (C == 6) {
                 if (add_corr && p.mandel) {
                     double t[6];
                     mandel_to_cart(crc, t);
 #pragma unroll
                     for (int c = 0; c < 6; ++c) crc[c] = t[c];
+                    mandel_to_cart(rrc, t);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) rrc[c] = t[c];
                 }
             }
 #pragma unroll
@@ -756,7 +786,10 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
                 const SR et = bufr[2 * ((c * PAIRS + (li >> 1)) * NLP + j) + (li & 1)];
                 const double w = (c < D) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
                 double o = old[k][c] + shift[c];
-                if (add_corr) o += crc[c];
+                if (add_corr) {
+                    o += crc[c];
+                    o += rrc[c];
+                }
                 const double nw = bar[c] + (double)et;
                 const double d = nw - o;
                 acc[0] += w * d * d;
@@ -900,7 +933,10 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     constexpr int VOX = NM * NL;
     constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
     using cx = cplx<float>;
-    const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
+    // step == cycle_len: only the folded reduction of the cycle's last step
+    const int step = flags >> LS_STEP_SHIFT, cyc = p.ctl->cycle_len;
+    const bool at_end = cyc > 0 && step == cyc;
+    const bool stop = (p.ctl->done | p.ctl->cg_stop | (cyc > 0 && step > cyc)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
     cx* tw_l = reinterpret_cast<cx*>(smem_raw);
     cx* tw_m = tw_l + NL;
@@ -971,7 +1007,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
             s_halt = probe ? 0 : d.halt;
         }
         __syncthreads();
-        if (s_halt) return;
+        if (s_halt || at_end) return;
         al = (float)s_al;
         be = (float)s_be;
     }
@@ -1024,7 +1060,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
     constexpr int SPT = (NM * NLH + kPlaneThreads - 1) / kPlaneThreads;
     using cx = cplx<float>;
-    const bool stop = (p.ctl->done | p.ctl->cg_stop) != 0;
+    const bool stop = (p.ctl->done | p.ctl->cg_stop | past_cycle(p.ctl, flags >> LS_STEP_SHIFT)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
     cx* tw_l = reinterpret_cast<cx*>(smem_raw);
     cx* tw_m = tw_l + NL;
@@ -1112,7 +1148,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
 template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
-    if (p.ctl->done || (MODE == LS_CG && p.ctl->cg_stop)) return;
+    if (p.ctl->done || (MODE == LS_CG && (p.ctl->cg_stop || past_cycle(p.ctl, flags >> LS_STEP_SHIFT)))) return;
     __shared__ double red[8 * 4 * 2];
     double tot[3];
     reduce_partials<3>(p.partials, nblocks, tot, red);
@@ -1338,7 +1374,7 @@ static void launch_inv_last(const LsLaunch& p, int flags, hipStream_t s) {
 }
 
 template <typename SR, int DIR, bool BATCH>
-static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0) {
+static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0, int step = 0) {
     const int nm = p.g.nm;
     const long total = (long)p.g.nf * p.g.nlh * p.g.C;
 #define MGK_CASE(NN)                                                                                       \
@@ -1346,7 +1382,7 @@ static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0)
         constexpr int L = mid_cols<SR, NN>();                                                              \
         const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);                              \
         hipLaunchKernelGGL((k_mid<SR, NN, DIR, BATCH>), dim3(blocks_for(total, L), p.B), dim3(L * FftPlan<NN>::threads), \
-                           sm, s, p.h, p.P, mode, spec);                                                          \
+                           sm, s, p.h, p.P, mode, spec, step);                                                    \
         return;                                                                                            \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -1355,7 +1391,7 @@ static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0)
 }
 
 template <typename SR, int C, bool BATCH, bool PLANE = false>
-static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int spec = 0) {
+static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int spec = 0, int step = 0) {
     const int nf = p.g.nf;
     const long nxi = (long)p.g.nm * p.g.nlh;
 #define MGK_CASE(NN)                                                                                 \
@@ -1363,7 +1399,7 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int s
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
         hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH, PLANE>), dim3(blocks_for(nxi, G), p.B),              \
-                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode, spec);                \
+                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode, spec, step);          \
         return;                                                                                      \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -1439,20 +1475,21 @@ template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags,
 
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
     const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
+    const int step = flags >> LS_STEP_SHIFT;
     if constexpr (C == 6) {
         if (p.plane) {
             // the next step's forward plane pass folds in this step's
             // reduction; the cycle's last step keeps a reduction kernel
             launch_fwd_plane<BATCH>(p, flags, s);
-            launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG, spec);
+            launch_first_gamma<float, 6, BATCH, true>(p, s, LS_CG, spec, step);
             launch_inv_plane<BATCH>(p, flags | LS_FLAG_PLANE_SLOTS, s, (flags & LS_FLAG_LAST) != 0);
             return;
         }
     }
     launch_fwd_last<float, C, LS_CG, BATCH>(p, flags, s);
-    if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG, spec);
-    launch_first_gamma<float, C, BATCH>(p, s, LS_CG, spec);
-    if (C == 6 && p.g.nm > 1) launch_mid<float, +1, BATCH>(p, s, LS_CG, spec);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, -1, BATCH>(p, s, LS_CG, spec, step);
+    launch_first_gamma<float, C, BATCH>(p, s, LS_CG, spec, step);
+    if (C == 6 && p.g.nm > 1) launch_mid<float, +1, BATCH>(p, s, LS_CG, spec, step);
     launch_inv_last<float, C, LS_CG, BATCH>(p, flags, s);
 }
 

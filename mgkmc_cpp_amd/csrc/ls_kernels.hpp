@@ -19,6 +19,8 @@ enum {
     LS_FLAG_PROBE = 256,      // timing launch: a folded reduction is loaded and computed, not applied
     LS_FLAG_PLANE_SLOTS = 512 // set by the launcher: the CG reduction uses the parity slots (plane passes)
 };
+// a CG step's index within its cycle rides in the flags' high bits
+constexpr int LS_STEP_SHIFT = 16;
 
 // how a solve starts
 enum {

@@ -315,12 +315,12 @@ def test_device_event_without_rate_or_with_unstable_modes():
         assert np.array_equal(a.get_field(f), before[f]), f
 
 
-def test_batch_matches_separate_fp32_refine_plane_passes():
+def test_batch_equals_separate_fp32_refine_plane_passes():
     # FP32+refine on 16^3 (square planes: the CG steps run as plane passes
-    # with folded reductions).  A batch runs every trajectory's CG cycles with
-    # the batch's longest predicted cycle, so a solve can end in another cycle
-    # than when run alone: the same events, times and fields within the
-    # FP32+refine bar, not bit for bit (FP64 batches are, see below).
+    # with folded reductions).  Each trajectory of a batch keeps its own CG
+    # cycle length, and a pending CG iterate is absorbed the same way wherever
+    # the refinement rounds end, so a batch equals the same trajectories run
+    # one by one, bit for bit.
     from mgkmc_cpp_amd import SimulationBatch
     seeds, events = (42, 7, 1234), 8
     cfgs = [_cfg(seed=sd, dims=3, nx=16, ny=16, nz=16, M=12, precision=abi.MGK_FP32_REFINE) for sd in seeds]
@@ -334,15 +334,11 @@ def test_batch_matches_separate_fp32_refine_plane_passes():
     st_batch = batch.event_steps(events)
     for a, b, sa, sb in zip(single, batched, st_single, st_batch):
         assert sa.kmc_events == sb.kmc_events == events
-        assert sb.ls_iters_fp32 > 0
+        assert (sa.ls_iters, sa.ls_iters_fp32) == (sb.ls_iters, sb.ls_iters_fp32)
         ea, eb = a.events(), b.events()
         assert [e.flat_idx for e in ea] == [e.flat_idx for e in eb]
-        ta, tb = np.array([e.time for e in ea]), np.array([e.time for e in eb])
-        rt = np.abs(ta - tb) / np.abs(ta)
-        assert rt.max() < 1e-6, rt
-        for f in (abi.FIELD_EPS, abi.FIELD_SIG):
-            assert _rel(b.get_field(f), a.get_field(f)) < 1e-6, f
-        for f in (abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_Q0):
+        assert [e.time for e in ea] == [e.time for e in eb]
+        for f in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_Q0):
             assert np.array_equal(a.get_field(f), b.get_field(f)), f
     # against the FP64 oracle: the same events; the stress within 1e-5 of max
     # |sigma| (2.7e-6 measured).  Both solves stop at tol 1e-8 from warm
