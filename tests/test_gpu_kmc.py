@@ -315,16 +315,19 @@ def test_device_event_without_rate_or_with_unstable_modes():
         assert np.array_equal(a.get_field(f), before[f]), f
 
 
-def test_batch_equals_separate_fp32_refine_plane_passes():
-    # FP32+refine on 16^3 (square planes: the CG steps run as plane passes
-    # with folded reductions).  Each trajectory of a batch keeps its own CG
-    # cycle length, and a pending CG iterate is absorbed the same way wherever
-    # the refinement rounds end, so a batch equals the same trajectories run
-    # one by one, bit for bit.
+@pytest.mark.parametrize("shape", [(16, 16, 16), (16, 16, 32), (32, 32)])
+def test_batch_equals_separate_fp32_refine(shape):
+    # FP32+refine: 16^3 runs its CG steps as plane passes with folded
+    # reductions, 16x16x32 and 32^2 the line passes.  Each trajectory of a
+    # batch keeps its own CG cycle length, and a pending CG iterate is absorbed
+    # the same way wherever the refinement rounds end, so a batch equals the
+    # same trajectories run one by one, bit for bit.
     from mgkmc_cpp_amd import SimulationBatch
     seeds, events = (42, 7, 1234), 8
-    cfgs = [_cfg(seed=sd, dims=3, nx=16, ny=16, nz=16, M=12, precision=abi.MGK_FP32_REFINE) for sd in seeds]
-    mats = [_field(16 ** 3, seed=17 + i) for i in range(len(seeds))]
+    D = len(shape)
+    geo = dict(dims=D, nx=shape[0], ny=shape[1]) | (dict(nz=shape[2], M=12) if D == 3 else dict(M=6))
+    cfgs = [_cfg(seed=sd, precision=abi.MGK_FP32_REFINE, **geo) for sd in seeds]
+    mats = [_field(int(np.prod(shape)), seed=17 + i) for i in range(len(seeds))]
     single = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
     batched = [Simulation(c, E, nu) for c, (E, nu) in zip(cfgs, mats)]
     for s in single + batched:
