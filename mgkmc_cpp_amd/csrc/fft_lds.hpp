@@ -1,7 +1,8 @@
 // fft_lds.hpp -- LDS-resident mixed-radix (2/3/4/5/8) Stockham FFT for gfx950.
 //
 // A workgroup holds a batch of complex lines of one compile-time length N in
-// LDS.  Each line is served by T = plan.threads consecutive lanes; in every
+// LDS.  Each line is served by T = plan.threads lanes (lds_fft: the line
+// index runs fastest across lanes, the column passes alike); in every
 // stage a lane loads its V = N/T values into registers, the workgroup meets at
 // a barrier, and the lane writes its butterfly outputs back (single buffer,
 // read-all-then-write-all).  Twiddles come from a per-length table W_N^k that
@@ -225,15 +226,20 @@ __device__ __forceinline__ void run_stages(cplx<Real>* line, int es, int t, bool
 // Transform `nlines` lines in LDS: line l starts at base + l*ls, element j at
 // + j*es (all in complex units).  Every thread of the workgroup must call this
 // with the same arguments (it contains barriers).  tw: N-entry table in LDS.
+// Lanes take the line index fastest: one access of a wave reads element
+// j + r*N/R (or writes position k) of consecutive lines, which a line stride
+// of N + 1 spreads over distinct banks.  With the lane index fastest instead,
+// the T lanes of a line and the lines beside them meet on the same banks
+// (element t + 8r of line l sits on bank l + t for N = 64, T = 8).
 template <int N, int DIR, typename Real>
 __device__ __forceinline__ void lds_fft(cplx<Real>* base, int nlines, int ls, int es, const cplx<Real>* tw) {
     constexpr int T = FftPlan<N>::threads;
     const int nthr = blockDim.x;
-    const int lines_per_pass = nthr / T;
+    const int lines_per_pass = nlines < nthr / T ? nlines : nthr / T;
     for (int l0 = 0; l0 < nlines; l0 += lines_per_pass) {
-        const int l = l0 + (int)threadIdx.x / T;
-        const int t = (int)threadIdx.x % T;
-        const bool active = ((int)threadIdx.x < lines_per_pass * T) && (l < nlines);
+        const int l = l0 + (int)threadIdx.x % lines_per_pass;
+        const int t = (int)threadIdx.x / lines_per_pass;
+        const bool active = (t < T) && (l < nlines);
         cplx<Real>* line = base + (active ? l : 0) * ls;
         run_stages<N, 0, 1, DIR, Real>(line, es, t, active, tw);
     }

@@ -299,6 +299,16 @@ template <typename SR, int N, int C> constexpr int gamma_points() {
     return (C == 6 && most >= g_align) ? (most / g_align) * g_align : most;
 }
 
+// Lanes per block of the F pass: the FFT lanes (G*C lines of T lanes), or
+// more when that leaves the Green operator's N*G (point, frequency) items
+// short of one per lane; always a multiple of G.
+template <typename SR, int N, int C> constexpr int gamma_threads() {
+    constexpr int G = gamma_points<SR, N, C>();
+    constexpr int fft = G * C * FftPlan<N>::threads;
+    constexpr int green = (cmin(N * G, kColThreadsMax) / G) * G;
+    return cmax(fft, green);
+}
+
 // ---------------------------------------------------------------------------
 // Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW = 2*PAIRS lines per
 // block, VPT voxels per lane.
@@ -569,7 +579,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     constexpr int T8 = kPlaneTile;
     const int gl = PLANE ? (l / (C * T8)) * T8 + l % T8 : l / C;
     const int cl = PLANE ? (l % (C * T8)) / T8 : l % C;
-    const bool active = xi0 + gl < nxi;
+    const bool active = t < T && xi0 + gl < nxi;   // lanes past the FFT lanes serve the Green operator only
     const LineRef<SR> column{spec + (active ? xi0 * C + l : 0), nxi * C};
     const LineRef<SR> mine{work + (gl * C + cl) * NFP, 1};
     cx v[NF / T];
@@ -582,23 +592,29 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     const SR a = (SR)(0.5 / p.mu0);
     const SR B = (SR)((p.lam0 + p.mu0) / (p.mu0 * (p.lam0 + 2.0 * p.mu0)));
     const SR scale = (SR)(1.0 / (double)g.N);
-    for (int idx = threadIdx.x; idx < NF * G; idx += blockDim.x) {
-        const int gg = idx % G, f = idx / G;
-        const long xi = xi0 + gg;
-        if (xi >= nxi) continue;
-        const int km = (int)(xi / g.nlh), kl = (int)(xi % g.nlh);
-        // signed frequencies k/n as fftfreq (linear_elastic_solver.hpp:22-33)
-        auto sfreq = [](int k, int n, bool& nyq) -> SR {
-            nyq = (n % 2 == 0) && (k == n / 2) && (n > 1);
-            const int lim = (n + 1) / 2;
-            return (SR)(k < lim ? k : k - n) / (SR)n;
-        };
+    // signed frequencies k/n as fftfreq (linear_elastic_solver.hpp:22-33)
+    auto sfreq = [](int k, int n, bool& nyq) -> SR {
+        nyq = (n % 2 == 0) && (k == n / 2) && (n > 1);
+        const int lim = (n + 1) / 2;
+        return (SR)(k < lim ? k : k - n) / (SR)n;
+    };
+    // The block has a multiple of G lanes (gamma_threads), so a lane keeps
+    // its point gg for every f it visits: the point's M and L frequencies
+    // are computed once.
+    const int gg = (int)threadIdx.x % G;
+    const long xi = xi0 + gg;
+    const int km = xi < nxi ? (int)(xi / g.nlh) : 0, kl = xi < nxi ? (int)(xi % g.nlh) : 0;
+    bool pm_nyq, pl_nyq;
+    const SR pm = sfreq(km, g.nm, pm_nyq), pl = sfreq(kl, g.nl, pl_nyq);
+    for (int f = (int)threadIdx.x / G; f < NF && xi < nxi; f += (int)blockDim.x / G) {
         bool nf_, nm_, nl_;
         SR qa[3];
         qa[0] = sfreq(f, NF, nf_);
         if (D == 3) {
-            qa[1] = sfreq(km, g.nm, nm_);
-            qa[2] = sfreq(kl, g.nl, nl_);
+            qa[1] = pm;
+            nm_ = pm_nyq;
+            qa[2] = pl;
+            nl_ = pl_nyq;
             // FFT axes (F, M, L) to directions (x, y, z); M has length 1
             // (frequency 0, no Nyquist bin) when perm is not PERM_XYZ
             if (g.perm == PERM_XZY) {
@@ -610,7 +626,8 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
             }
         } else {
             nm_ = false;
-            qa[1] = sfreq(kl, g.nl, nl_);
+            qa[1] = pl;
+            nl_ = pl_nyq;
         }
         cx tt[C], o[C];
 #pragma unroll
@@ -1399,7 +1416,7 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int s
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
         hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH, PLANE>), dim3(blocks_for(nxi, G), p.B),              \
-                           dim3(G * C * FftPlan<NN>::threads), sm, s, p.h, p.P, mode, spec, step);          \
+                           dim3(gamma_threads<SR, NN, C>()), sm, s, p.h, p.P, mode, spec, step);           \
         return;                                                                                      \
     }
     MGK_LEN_CASES(MGK_CASE)
