@@ -939,6 +939,13 @@ __device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&to
 #endif
 constexpr int kPlaneThreads = MGK_PLANE_THREADS;   // lanes per plane block (a build-time tuning knob)
 
+// Four consecutive voxels of one row: the plane passes move the per-voxel
+// CG vectors in 16-byte accesses (planes hold a multiple of 64 voxels, and
+// every plane starts 16-byte aligned).
+typedef float vox4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ vox4 ld_vox4(const float* b, int q) { return *reinterpret_cast<const vox4*>(b + 4 * q); }
+__device__ __forceinline__ void st_vox4(float* b, int q, vox4 v) { *reinterpret_cast<vox4*>(b + 4 * q) = v; }
+
 template <int NP> constexpr size_t plane_lds_bytes() {
     return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<float>);
 }
@@ -948,7 +955,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
     constexpr int VOX = NM * NL;
-    constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int V4 = VOX / 4;                                  // groups of four voxels
+    constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;   // groups per lane
     using cx = cplx<float>;
     // step == cycle_len: only the folded reduction of the cycle's last step
     const int step = flags >> LS_STEP_SHIFT, cyc = p.ctl->cycle_len;
@@ -983,20 +991,21 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
         }
     }
 
-    float r[VPT], w[VPT], pv[VPT], sv[VPT], xv[VPT], dm[VPT], dl[VPT];
+    const vox4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    vox4 r[Q4], w[Q4], pv[Q4], sv[Q4], xv[Q4], dm[Q4], dl[Q4];
 #pragma unroll
-    for (int k = 0; k < VPT; ++k) {
-        const int vox = threadIdx.x + kPlaneThreads * k;
-        const int vv = vox < VOX ? vox : 0;
-        r[k] = p.r32[base + vv];
-        dm[k] = p.dmu32[(long)f * VOX + vv];
-        dl[k] = (c == 0) ? p.dlam32[(long)f * VOX + vv] : 0.f;
-        w[k] = pv[k] = sv[k] = xv[k] = 0.f;
+    for (int k = 0; k < Q4; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        const int qq = q < V4 ? q : 0;
+        r[k] = ld_vox4(p.r32 + base, qq);
+        dm[k] = ld_vox4(p.dmu32 + (long)f * VOX, qq);
+        dl[k] = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
+        w[k] = pv[k] = sv[k] = xv[k] = zero4;
         if (upd) {
-            w[k] = p.w32[base + vv];
-            pv[k] = p.p32[base + vv];
-            sv[k] = p.s32[base + vv];
-            xv[k] = p.corr[base + vv];
+            w[k] = ld_vox4(p.w32 + base, qq);
+            pv[k] = ld_vox4(p.p32 + base, qq);
+            sv[k] = ld_vox4(p.s32 + base, qq);
+            xv[k] = ld_vox4(p.corr + base, qq);
         }
     }
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
@@ -1030,25 +1039,33 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     }
     float* bufr = reinterpret_cast<float*>(buf);
 #pragma unroll
-    for (int k = 0; k < VPT; ++k) {
-        const int vox = threadIdx.x + kPlaneThreads * k;
-        if (vox >= VOX) continue;
-        float y = r[k];
+    for (int k = 0; k < Q4; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        if (q >= V4) continue;
+        vox4 y = r[k];
         if (upd) {
-            const float pp = upd1 ? r[k] : r[k] + be * pv[k];
-            const float ss = upd1 ? w[k] : w[k] + be * sv[k];
-            // x is the correction of eps_1 = eps_0 + r0 (see pass A)
-            const float xx = upd1 ? (al - 1.0f) * r[k] : xv[k] + al * pp;
-            y = r[k] - al * ss;
-            p.p32[base + vox] = pp;
-            p.s32[base + vox] = ss;
-            p.corr[base + vox] = xx;
-            p.r32[base + vox] = y;
+            vox4 pp, ss, xx;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                pp[e] = upd1 ? r[k][e] : r[k][e] + be * pv[k][e];
+                ss[e] = upd1 ? w[k][e] : w[k][e] + be * sv[k][e];
+                // x is the correction of eps_1 = eps_0 + r0 (see pass A)
+                xx[e] = upd1 ? (al - 1.0f) * r[k][e] : xv[k][e] + al * pp[e];
+                y[e] = r[k][e] - al * ss[e];
+            }
+            st_vox4(p.p32 + base, q, pp);
+            st_vox4(p.s32 + base, q, ss);
+            st_vox4(p.corr + base, q, xx);
+            st_vox4(p.r32 + base, q, y);
         }
-        // (C - C0) in Mandel components: 3 dlam + 2 dmu on e1, 2 dmu elsewhere
-        const float tau = (c == 0) ? (3.0f * dl[k] + 2.0f * dm[k]) * y : 2.0f * dm[k] * y;
-        const int m = vox / NL, j = vox % NL;
-        bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = tau;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // (C - C0) in Mandel components: 3 dlam + 2 dmu on e1, 2 dmu elsewhere
+            const float tau = (c == 0) ? (3.0f * dl[k][e] + 2.0f * dm[k][e]) * y[e] : 2.0f * dm[k][e] * y[e];
+            const int vox = 4 * q + e;
+            const int m = vox / NL, j = vox % NL;
+            bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = tau;
+        }
     }
     __syncthreads();
     lds_fft<NL, -1, float>(buf, NM / 2, NLP, 1, tw_l);
@@ -1074,7 +1091,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
     constexpr int VOX = NM * NL;
-    constexpr int VPT = (VOX + kPlaneThreads - 1) / kPlaneThreads;
+    constexpr int V4 = VOX / 4;
+    constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;
     constexpr int SPT = (NM * NLH + kPlaneThreads - 1) / kPlaneThreads;
     using cx = cplx<float>;
     const bool stop = (p.ctl->done | p.ctl->cg_stop | past_cycle(p.ctl, flags >> LS_STEP_SHIFT)) != 0;
@@ -1096,11 +1114,11 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         const int i = idx < NM * NLH ? idx : 0;
         sp[k] = spec[(i / kPlaneTile) * (6 * kPlaneTile) + i % kPlaneTile];
     }
-    float r[VPT];
+    vox4 r[Q4];
 #pragma unroll
-    for (int k = 0; k < VPT; ++k) {
-        const int vox = threadIdx.x + kPlaneThreads * k;
-        r[k] = p.r32[base + (vox < VOX ? vox : 0)];
+    for (int k = 0; k < Q4; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        r[k] = ld_vox4(p.r32 + base, q < V4 ? q : 0);
     }
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
@@ -1134,18 +1152,23 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     const float* bufr = reinterpret_cast<const float*>(buf);
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < VPT; ++k) {
-        const int vox = threadIdx.x + kPlaneThreads * k;
-        if (vox >= VOX) continue;
-        const int m = vox / NL, j = vox % NL;
-        const float et = bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
-        const float rv = r[k];
-        const float wv = rv - et;
-        p.w32[base + vox] = wv;
-        const double rd = (double)rv, wd = (double)wv;
-        acc[0] += rd * rd;
-        acc[1] += wd * rd;
-        acc[2] += rd * rd;
+    for (int k = 0; k < Q4; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        if (q >= V4) continue;
+        vox4 wv;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int vox = 4 * q + e;
+            const int m = vox / NL, j = vox % NL;
+            const float et = bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
+            const float rv = r[k][e];
+            wv[e] = rv - et;
+            const double rd = (double)rv, wd = (double)wv[e];
+            acc[0] += rd * rd;
+            acc[1] += wd * rd;
+            acc[2] += rd * rd;
+        }
+        st_vox4(p.w32 + base, q, wv);
     }
     acc[0] *= c0;
     acc[1] *= c0;
