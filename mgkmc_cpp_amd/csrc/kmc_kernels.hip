@@ -792,12 +792,113 @@ __device__ __forceinline__ void flip_body(const KmcParams& p, int flat, double t
     }
 }
 
+// flip_body spread over the lanes of a block (every lane calls it; it has
+// barriers).  When every neighbour is a distinct voxel other than v (all axes
+// of length >= 3), lane q < 3^D updates cell q of the neighbourhood, lanes
+// 32..32+C-1 the voxel's eps_p and prev_dir components, and after a barrier
+// (every lane has read the flipped mode's catalog row) the lanes write the
+// redraw.  Each address is written by one lane with the serial version's
+// arithmetic, so the memory state equals flip_body's.  Otherwise lane 0 runs
+// flip_body.
+template <int D, int C>
+__device__ __forceinline__ void flip_body_block(const KmcParams& p, int flat, double time, int first_mode,
+                                                int n_modes, int has_cat, int has_q0, int time_from_ctl,
+                                                const RedrawArgs& ra, int ra_on) {
+    __shared__ int s_flat, s_first;
+    __shared__ double s_time;
+    const Geom g = p.g;
+    const bool distinct = g.nx >= 3 && g.ny >= 3 && (D == 2 || g.nz >= 3) && blockDim.x >= 64;
+    if (!distinct) {
+        if (threadIdx.x == 0) flip_body<D, C>(p, flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl, ra, ra_on);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        if (flat < 0) flat = p.kctl->go ? p.kctl->sel : -1;
+        s_flat = flat;
+        s_time = (flat >= 0 && time_from_ctl) ? p.kctl->time_new : time;
+        s_first = (flat >= 0 && first_mode < 0) ? flat % p.ph.M : first_mode;
+    }
+    __syncthreads();
+    flat = s_flat;
+    if (flat < 0) return;   // uniform: no event
+    time = s_time;
+    first_mode = s_first;
+    const KmcPhys ph = p.ph;
+    const long nmodes = g.N * ph.M;
+    const long v = flat / ph.M;
+    const double* stage = ra_on ? ra.v : p.staging;
+    const int t = threadIdx.x;
+    constexpr int NCELL = (D == 2) ? 9 : 27;
+    constexpr int CENTRE = NCELL / 2;
+    // loads: the flipped mode's catalog row (every lane), the lane's cell,
+    // the lane's eps_p component
+    double c[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) c[k] = p.cat_shared ? p.cat[k * ph.M + flat % ph.M] : p.cat[k * nmodes + flat];
+    long nb = v;
+    double g0 = 0.0, g1 = 0.0;
+    if (t < NCELL) {
+        int x, y, z = 0;
+        if (D == 2) { y = (int)(v % g.ny); x = (int)(v / g.ny); }
+        else { z = (int)(v % g.nz); const long tt = v / g.nz; y = (int)(tt % g.ny); x = (int)(tt / g.ny); }
+        const int dx = (D == 2) ? t / 3 - 1 : t / 9 - 1;
+        const int dy = (D == 2) ? t % 3 - 1 : (t / 3) % 3 - 1;
+        const int dz = (D == 2) ? 0 : t % 3 - 1;
+        const int xn = (x + dx + g.nx) % g.nx, yn = (y + dy + g.ny) % g.ny;
+        const int zn = (D == 3) ? (z + dz + g.nz) % g.nz : 0;
+        nb = (D == 2) ? (long)xn * g.ny + yn : ((long)xn * g.ny + yn) * g.nz + zn;
+        g0 = p.soft[nb];
+        g1 = p.soft[g.N + nb];
+    }
+    const int kc = t - 32;
+    double ep = 0.0;
+    if (kc >= 0 && kc < C) ep = p.eps_p[kc * g.N + v];
+    double sum_sq;
+    if constexpr (D == 2) {
+        const double e11 = c[0], e22 = c[1], e12 = c[2];
+        sum_sq = (e12 * e12) + (e22 * e22 + e11 * e11 + (e11 - e22) * (e11 - e22)) / 6.0;
+    } else {
+        const double e11 = c[0], e22 = c[1], e33 = c[2], e12 = c[3], e13 = c[4], e23 = c[5];
+        sum_sq = (e12 * e12 + e23 * e23 + e13 * e13) +
+                 ((e22 - e33) * (e22 - e33) + (e33 - e11) * (e33 - e11) + (e11 - e22) * (e11 - e22)) / 6.0;
+    }
+    if (t == CENTRE) {
+        double gp = g0 + ph.jp * sum_sq;
+        if (ph.cap > 0.0 && gp > ph.cap) gp = ph.cap;
+        p.soft[v] = gp;
+        p.soft[g.N + v] = ph.jt * sum_sq;
+        p.last_t[v] = time;
+    } else if (t < NCELL && ph.nfrac > 0.0) {
+        double gn = g0 + ph.nfrac * ph.jp * sum_sq;
+        if (ph.cap > 0.0 && gn > ph.cap) gn = ph.cap;
+        p.soft[nb] = gn;
+        p.soft[g.N + nb] = g1 + ph.nfrac * ph.jt * sum_sq;
+    }
+    if (kc >= 0 && kc < C) {
+#pragma unroll
+        for (int k = 0; k < C; ++k)
+            if (k == kc) {
+                p.eps_p[k * g.N + v] = ep + c[k];
+                p.prev_dir[k * g.N + v] = c[k];
+                p.kctl->flip_c[k] = c[k];
+            }
+        if (kc == 0) p.kctl->flip_voxel = (int)v;
+    }
+    __syncthreads();   // the catalog row is read before the redraw may overwrite it
+    for (int j = t; j < n_modes; j += blockDim.x) {
+        const long mi = v * ph.M + first_mode + j;
+        if (has_cat && !p.cat_shared)
+            for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = stage[j * C + k];
+        if (has_q0) p.Q0[mi] = stage[n_modes * C + j];
+    }
+}
+
 template <int D, int C, bool BATCH>
 __global__ void k_flip(const KmcParams p_arg, const KmcParams* __restrict__ PK, int flat, double time, int first_mode,
                        int n_modes, int has_cat, int has_q0, int time_from_ctl, const RedrawArgs ra, int ra_on) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    flip_body<D, C>(param_sel<BATCH>(p_arg, PK), flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl, ra,
-                    ra_on);
+    if (blockIdx.x != 0) return;
+    flip_body_block<D, C>(param_sel<BATCH>(p_arg, PK), flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl,
+                          ra, ra_on);
 }
 
 // ---------------------------------------------------------------------------
@@ -935,7 +1036,7 @@ __global__ __launch_bounds__(1024) void k_event_chain(const KmcParams p_arg, con
     select_body_1024(p, ea.u2);
     if constexpr (FLIP) {
         __syncthreads();
-        if (threadIdx.x == 0) flip_body<D, C>(p, -1, ea.time, first_mode, n_modes, has_cat, has_q0, 1, ra, ra_on);
+        flip_body_block<D, C>(p, -1, ea.time, first_mode, n_modes, has_cat, has_q0, 1, ra, ra_on);
     }
 }
 
