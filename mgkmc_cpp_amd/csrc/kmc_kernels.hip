@@ -294,6 +294,8 @@ __device__ __forceinline__ void rates_scan_body(const KmcParams& p, int nb) {
     const int b0 = threadIdx.x * chunk;
     double csum = 0.0;
     int cun = 0, cinc = 0;
+    double vs0[G];   // the first group's sums and counts, kept for the prefix pass
+    int vu0[G];
     for (int g0 = 0; g0 < chunk; g0 += G) {
         double vs[G];
         int vu[G], vi[G];
@@ -305,6 +307,10 @@ __device__ __forceinline__ void rates_scan_body(const KmcParams& p, int nb) {
             vs[k] = p.bsum[ok[k] ? b : 0];
             vu[k] = p.bunst[ok[k] ? b : 0];
             vi[k] = p.binc[ok[k] ? b : 0];
+        }
+        if (g0 == 0) {
+#pragma unroll
+            for (int k = 0; k < G; ++k) { vs0[k] = vs[k]; vu0[k] = vu[k]; }
         }
 #pragma unroll
         for (int k = 0; k < G; ++k)
@@ -343,8 +349,8 @@ __device__ __forceinline__ void rates_scan_body(const KmcParams& p, int nb) {
         for (int k = 0; k < G; ++k) {
             const int b = b0 + g0 + k;
             ok[k] = (g0 + k < chunk) && (b < nb);
-            vs[k] = p.bsum[ok[k] ? b : 0];
-            vu[k] = p.bunst[ok[k] ? b : 0];
+            vs[k] = (g0 == 0) ? vs0[k] : p.bsum[ok[k] ? b : 0];
+            vu[k] = (g0 == 0) ? vu0[k] : p.bunst[ok[k] ? b : 0];
         }
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -493,7 +499,7 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     const double total = kc->total_rate;
     const double target = u2 * total;
     __shared__ int s_b1, s_first1;
-    __shared__ double s_scan1[kRateBlock];
+    __shared__ double s_scan1[kRateBlock], s_q1[kRateBlock];
     __shared__ double s_wave[16];
     if (threadIdx.x == 0) {
         s_b1 = nb - 1;   // no prefix reaches the target: the last block
@@ -501,13 +507,26 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     }
     __syncthreads();
     {
-        const int chunk = (nb + 1023) / 1024;   // rates_scan_body's chunks
+        // the first block of the lane's chunk (rates_scan_body's chunks) whose
+        // inclusive prefix reaches the target; a group's prefixes load together
+        constexpr int G = 8;
+        const int chunk = (nb + 1023) / 1024;
         const int b0 = threadIdx.x * chunk;
-        for (int k = 0; k < chunk; ++k) {
-            const int b = b0 + k;
-            if (b >= nb) break;
-            if (p.bincl[b] >= target) {
-                atomicMin(&s_b1, b);
+        for (int g0 = 0; g0 < chunk; g0 += G) {
+            double vb[G];
+            bool ok[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const int b = b0 + g0 + k;
+                ok[k] = (g0 + k < chunk) && (b < nb);
+                vb[k] = p.bincl[ok[k] ? b : 0];
+            }
+            int hit = -1;
+#pragma unroll
+            for (int k = G - 1; k >= 0; --k)
+                if (ok[k] && vb[k] >= target) hit = b0 + g0 + k;
+            if (hit >= 0) {
+                atomicMin(&s_b1, hit);
                 break;
             }
         }
@@ -518,7 +537,9 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     const double prev = (b > 0) ? p.bincl[b - 1] : 0.0;
     const long i = base + threadIdx.x;
     bool incl = false;
-    const double r = (i < nmodes) ? mode_rate(p.ph, p.Q[i], incl) : 0.0;
+    const double qi = (i < nmodes) ? p.Q[i] : 0.0;
+    s_q1[threadIdx.x] = qi;
+    const double r = (i < nmodes) ? mode_rate(p.ph, qi, incl) : 0.0;
     double v = incl ? r : 0.0;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -544,6 +565,7 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     if (threadIdx.x != 0) return;
     const double u = 1.1102230246251565e-16;   // 2^-53
     int sel = -1, amb = 1;
+    double qs = 0.0;
     if (s_first1 < kRateBlock) {
         const int li = s_first1;
         const long im = base + li;
@@ -553,12 +575,13 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
         const double mP = 2.0 * ((double)im + 2048.0) * u * Sp + 2.0 * (double)nmodes * u * total;
         sel = (int)im;
         amb = (Sm - target <= mS) || (target - Sp <= mP) ? 1 : 0;
+        qs = s_q1[li];
     }
     kc->u2 = u2;
     kc->target = target;
     kc->sel = sel;
     kc->ambiguous = amb;
-    kc->q_sel = (sel >= 0) ? p.Q[sel] : 0.0;
+    kc->q_sel = qs;
 }
 
 template <bool BATCH>
