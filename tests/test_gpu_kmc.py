@@ -146,6 +146,25 @@ def test_flip_and_redraw_match_oracle():
     assert np.array_equal(got_cat, cat) and np.array_equal(ctx.get_field(abi.FIELD_Q0)[3:6], q0)
 
 
+@pytest.mark.parametrize("shape", [(8, 1, 8), (4, 4, 1), (4, 4, 4)])
+def test_flip_on_thin_and_small_grids_matches_oracle(shape):
+    # an axis of length 1 makes neighbours alias (the flipped voxel is its own
+    # neighbour along that axis): the flip then runs on one lane in the
+    # reference's order; 4^3 takes the one-lane-per-cell path
+    c = _cfg(dims=3, nx=shape[0], ny=shape[1], nz=shape[2], M=3, neighbor_softening_fraction=0.3,
+             catalog_mode=abi.CATALOG_RANDOM)
+    N = int(np.prod(shape))
+    E, nu = _field(N)
+    osim = _evolved_oracle(c, E, nu, events=4)
+    ctx = _device_from_oracle(osim, c, E, nu)
+    for k, t in enumerate((1.25, 1.5, 2.0, 2.5)):
+        flat = (k * 7 + 1) % (N * c.M)
+        osim.apply_flip(flat, t)
+        ctx.apply_flip(flat, t)
+        for f in (abi.FIELD_EPS_PLASTIC, abi.FIELD_SOFT, abi.FIELD_LAST_EVENT_TIME, abi.FIELD_PREV_DIR):
+            assert np.array_equal(ctx.get_field(f), osim.get_field(f)), (shape, flat, f)
+
+
 def test_select_flip_chain_equals_select_then_flip():
     c = _cfg(nx=8, ny=8, M=4, instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM,
              neighbor_softening_fraction=0.2)
