@@ -1727,10 +1727,19 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
             }
             ++n_active;
         }
-        // LS control block, parameter blocks and event arguments
-        c.init_ctl(on ? e.max_iter : 1, on ? e.tol : 0.0);
+        // LS control block, parameter blocks and event arguments; an inactive
+        // trajectory's host state stays as it is (its row's control block is
+        // done before the first iteration)
         Ctl& h = b.h_ctl[y];
-        h = *c.hctl;
+        if (on) {
+            c.init_ctl(e.max_iter, e.tol);
+            h = *c.hctl;
+        } else {
+            std::memset(&h, 0, sizeof(h));
+            h.max_iter = 1;
+            h.done = 1;
+            h.skip_final = 1;
+        }
         h.cycle_len = c.inner_per_outer;   // this trajectory's own cycle, as when it runs alone
         const int keep = c.mean_valid ? 1 : 0;
         if (on) all_keep = all_keep && keep;
@@ -1741,6 +1750,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         KmcParams kp = c.K;
         kp.kctl = b.d_kctl + y;
         kp.staging = b.d_stage + (size_t)y * b.stage_per;
+        kp.skip = on ? 0 : 1;
         b.h_kp[y] = kp;
         EventArgs& ea = b.h_ea[y];
         std::memset(&ea, 0, sizeof(ea));
@@ -1844,10 +1854,15 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     for (int y = 0; y < B; ++y) {
         mgk_ctx& c = *b.c[y];
         const KmcCtl& k = b.h_kctl[y];
-        *c.hctl = b.h_ctl[y];
-        *c.hkctl = k;
         mgk_event_out& o = out[y];
         std::memset(&o, 0, sizeof(o));
+        if (active && !active[y]) {
+            o.outcome = MGK_EVENT_SKIPPED;
+            o.flat_idx = -1;
+            continue;
+        }
+        *c.hctl = b.h_ctl[y];
+        *c.hkctl = k;
         o.outcome = k.outcome;
         o.total_rate = k.total_rate;
         o.n_unstable = k.n_unstable;

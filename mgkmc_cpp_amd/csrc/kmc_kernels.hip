@@ -131,6 +131,7 @@ template <int D, int C, bool BATCH>
 __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, const KmcParams* __restrict__ PK,
                                                         double time, const double* __restrict__ times) {
     const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    if (BATCH && p.skip) return;   // inactive batch row: Q and the block sums stay as they are
     if (BATCH) time = times[blockIdx.y];
     constexpr int K = kRateBlock / kRateThreads;
     const Geom g = p.g;
@@ -190,6 +191,7 @@ template <int D, int C, bool BATCH>
 __global__ __launch_bounds__(kRateThreads) void k_rates_staged(const KmcParams p_arg, const KmcParams* __restrict__ PK,
                                                                double time, const double* __restrict__ times) {
     const KmcParams& p = param_sel<BATCH>(p_arg, PK);
+    if (BATCH && p.skip) return;   // inactive batch row: Q and the block sums stay as they are
     if (BATCH) time = times[blockIdx.y];
     constexpr int K = kRateBlock / kRateThreads;
     constexpr int NV = kRateBlock / kStageMinM + 2;
@@ -490,7 +492,13 @@ __device__ __forceinline__ void select_body(const KmcParams& p, double u2) {
 // writes), and the chosen block's 1024 modes get one lane each, with a wave
 // scan and a scan of the 16 wave totals.  The summation order differs from
 // select_body's; the rounding-bound test holds for any order of the 1024
-// terms.
+// terms.  It always names a mode when the total rate is positive:
+//  * no in-block prefix reaches a target that the block's inclusive prefix
+//    reaches (the two sums round differently): the block's last included
+//    mode, flagged ambiguous;
+//  * target 0 (u2 == 0) ahead of a block without included modes: the first
+//    included mode after it, which is the reference's answer (every prefix of
+//    an included mode is >= 0).
 __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) {
     const long nmodes = p.g.N * p.ph.M;
     KmcCtl* kc = p.kctl;
@@ -498,12 +506,14 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     const int nb = kc->nblocks;
     const double total = kc->total_rate;
     const double target = u2 * total;
-    __shared__ int s_b1, s_first1;
+    __shared__ int s_b1, s_first1, s_last1, s_fwd1;
     __shared__ double s_scan1[kRateBlock], s_q1[kRateBlock];
     __shared__ double s_wave[16];
     if (threadIdx.x == 0) {
         s_b1 = nb - 1;   // no prefix reaches the target: the last block
         s_first1 = kRateBlock;
+        s_last1 = -1;
+        s_fwd1 = -1;
     }
     __syncthreads();
     {
@@ -560,13 +570,39 @@ __device__ __forceinline__ void select_body_1024(const KmcParams& p, double u2) 
     __syncthreads();
     const double S = prev + (s_wave[wid] + v);
     s_scan1[threadIdx.x] = S;
-    if (i < nmodes && incl && S >= target) atomicMin(&s_first1, (int)threadIdx.x);
+    if (i < nmodes && incl) {
+        if (S >= target) atomicMin(&s_first1, (int)threadIdx.x);
+        atomicMax(&s_last1, (int)threadIdx.x);
+    }
     __syncthreads();
+    if (s_first1 == kRateBlock && s_last1 < 0) {
+        // block b holds no included mode (only a target of 0 can pick such a
+        // block): the first included mode of the blocks after it
+        for (int bb = b + 1; bb < nb; ++bb) {
+            const long j = (long)bb * kRateBlock + threadIdx.x;
+            bool inc2 = false;
+            if (j < nmodes) (void)mode_rate(p.ph, p.Q[j], inc2);
+            if (inc2) atomicMin(&s_first1, (int)threadIdx.x);
+            __syncthreads();
+            const bool found = s_first1 < kRateBlock;
+            if (found && threadIdx.x == 0) s_fwd1 = bb;
+            __syncthreads();
+            if (found) break;
+        }
+    }
     if (threadIdx.x != 0) return;
     const double u = 1.1102230246251565e-16;   // 2^-53
     int sel = -1, amb = 1;
     double qs = 0.0;
-    if (s_first1 < kRateBlock) {
+    if (s_fwd1 >= 0) {
+        sel = (int)((long)s_fwd1 * kRateBlock + s_first1);
+        amb = 0;
+        qs = p.Q[sel];
+    } else if (s_first1 == kRateBlock && s_last1 >= 0) {
+        sel = (int)(base + s_last1);
+        amb = 1;
+        qs = s_q1[s_last1];
+    } else if (s_first1 < kRateBlock) {
         const int li = s_first1;
         const long im = base + li;
         const double Sm = s_scan1[li];
@@ -1058,6 +1094,16 @@ __global__ __launch_bounds__(1024) void k_event_chain(const KmcParams p_arg, con
     __syncthreads();
     select_body_1024(p, ea.u2);
     if constexpr (FLIP) {
+        __syncthreads();
+        if (threadIdx.x == 0 && p.kctl->go && p.kctl->sel < 0) {
+            // unreachable with a positive total rate; never report an event
+            // without a mode: no solve, no final pass, outcome "no rate"
+            Ctl* ctl = BATCH ? P[blockIdx.y].ctl : ctl_arg;
+            p.kctl->go = 0;
+            p.kctl->outcome = 2;
+            ctl->done = 1;
+            ctl->skip_final = 1;
+        }
         __syncthreads();
         flip_body_block<D, C>(p, -1, ea.time, first_mode, n_modes, has_cat, has_q0, 1, ra, ra_on);
     }

@@ -106,6 +106,7 @@ struct KmcParams {
     const double* staging;   // redraw values: n_modes*C catalog then n_modes Q0
     int* unst_idx;           // compacted unstable indices
     double* unst_q;
+    int skip;                // batch row of an inactive trajectory: the rate pass leaves its state alone
 };
 
 constexpr int kRateBlock = 1024;   // modes per rate block

@@ -53,12 +53,15 @@ def _box_muller_sequential(seed, n):
     return out
 
 
-def correlated_field(shape, mean, std, corr, seed, clip_min=None, clip_max=None):
+def correlated_field(shape, mean, std, corr, seed, clip_min=None, clip_max=None, noise=None):
     """Gaussian random field on a periodic grid, flattened in the reference's
-    voxel order (x slowest)."""
+    voxel order (x slowest).  noise (N values) replaces the seeded draws."""
     shape = tuple(int(s) for s in shape)
     N = int(np.prod(shape))
-    field = box_muller_normals(seed, N).reshape(shape)
+    if noise is not None:
+        field = np.asarray(noise, dtype=np.float64).reshape(shape)
+    else:
+        field = box_muller_normals(seed, N).reshape(shape)
     hat = np.fft.fftn(field, norm="ortho")
     f2 = np.zeros(shape)
     for ax, n in enumerate(shape):

@@ -737,6 +737,11 @@ public:
             rng = d.rng_before;
             return out.outcome;
         }
+        // an event always names a mode of this grid; anything else is a device
+        // error, reported before any host array is indexed with it
+        if (out.flat_idx < 0 || (long)out.flat_idx >= (long)Q0.size())
+            throw std::runtime_error("mgk_kmc_event: event reported without a valid mode (flat index " +
+                                     std::to_string(out.flat_idx) + ")");
         const double t_wait = -std::log(d.r_val) / total_rate_;   // the reference's expression (= out.t_wait)
         time += t_wait;
         const bool solved = !fast_patching_enabled || flips_since_sync + 1 >= sync_interval;

@@ -376,13 +376,16 @@ static double box_muller_normal(std::mt19937& rng) {
 }
 
 // generate_correlated_field_cpp (:130-258)
+// noise != nullptr replaces step 1's N draws (for pinning the filter with
+// chosen inputs, e.g. a unit impulse)
 static void correlated_field(int nx, int ny, int nz, bool is3d, double mean_t, double std_t, double corr,
-                             bool has_min, double cmin, bool has_max, double cmax, int seed, double* out) {
+                             bool has_min, double cmin, bool has_max, double cmax, int seed, double* out,
+                             const double* noise = nullptr) {
     if (!is3d) nz = 1;
     long N = (long)nx * ny * nz;
     std::vector<double> field(N);
     std::mt19937 rng(seed);
-    for (long i = 0; i < N; ++i) field[i] = box_muller_normal(rng);
+    for (long i = 0; i < N; ++i) field[i] = noise ? noise[i] : box_muller_normal(rng);
     std::vector<int> shape = is3d ? std::vector<int>{nx, ny, nz} : std::vector<int>{nx, ny};
     std::vector<cd> tmp(N), hat(N);
     for (long i = 0; i < N; ++i) tmp[i] = cd(field[i], 0.0);
@@ -1134,6 +1137,11 @@ void orc_correlated_field(int nx, int ny, int nz, int is3d, double mean, double 
     correlated_field(nx, ny, nz, is3d != 0, mean, sd, corr, has_min != 0, cmin, has_max != 0, cmax, seed, out);
 }
 
+void orc_correlated_field_noise(int nx, int ny, int nz, int is3d, double mean, double sd, double corr,
+                                const double* noise, double* out) {
+    correlated_field(nx, ny, nz, is3d != 0, mean, sd, corr, false, 0.0, false, 0.0, 0, out, noise);
+}
+
 // libstdc++ stream probes: k draws of the given kind from mt19937(seed)
 // kind 0: raw 32-bit words, 1: uniform_real_distribution<double>(0,1),
 // 2: one normal_distribution<double>(0,1) object.
@@ -1178,6 +1186,16 @@ int orc_sim_event_steps(orc_sim* s, int count, int di, int dj, mgk_run_stats* st
 
 int orc_sim_elastic_run(orc_sim* s) { return ORC_DISPATCH(s, (S->elastic_run(), S->last_iters)); }
 int orc_sim_update_barriers(orc_sim* s) { return ORC_DISPATCH(s, (S->update_barriers(), 0)); }
+
+// run_cascade on the current state: {local_step, total_flips}
+int orc_sim_run_cascade(orc_sim* s, int* local_step, int* total_flips) {
+    return ORC_DISPATCH(s, ([&]() {
+        auto r = S->run_cascade();
+        *local_step = r.first;
+        *total_flips = r.second;
+        return 0;
+    })());
+}
 
 // rates + total + selection for a given u2 (used to pin the selection rule)
 int orc_sim_rates(orc_sim* s, double* total, int* n_included, double* rates_out, int* idx_out) {

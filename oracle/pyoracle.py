@@ -57,6 +57,8 @@ def lib(ref=False):
                                       C.c_int, C.c_double, DP])
     _sig(L, "orc_correlated_field", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                            C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, DP])
+    _sig(L, "orc_correlated_field_noise", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                 C.c_double, DP, DP])
     _sig(L, "orc_rng_stream", None, [C.c_int, C.c_int, C.c_int, DP])
     _sig(L, "orc_barrier_draws", None, [C.POINTER(abi.SimConfig), C.c_int, DP])
     _sig(L, "orc_sim_create", C.c_int, [C.POINTER(abi.SimConfig), DP, DP, C.POINTER(P)])
@@ -65,6 +67,7 @@ def lib(ref=False):
     _sig(L, "orc_sim_event_steps", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(abi.RunStats)])
     _sig(L, "orc_sim_elastic_run", C.c_int, [P])
     _sig(L, "orc_sim_update_barriers", C.c_int, [P])
+    _sig(L, "orc_sim_run_cascade", C.c_int, [P, IP, IP])
     _sig(L, "orc_sim_rates", C.c_int, [P, DP, IP, DP, IP])
     _sig(L, "orc_select_sequential", C.c_int, [DP, C.c_int, C.c_double])
     _sig(L, "orc_sim_apply_flip", C.c_int, [P, C.c_int, C.c_double])
@@ -159,6 +162,17 @@ def correlated_field(shape, mean, std, corr, seed, clip_min=None, clip_max=None)
     return out
 
 
+def correlated_field_noise(shape, mean, std, corr, noise):
+    """The same generator with the N noise values given instead of drawn."""
+    is3d = len(shape) == 3
+    nx, ny = shape[0], shape[1]
+    nz = shape[2] if is3d else 1
+    out = np.zeros(nx * ny * nz)
+    nz_ = np.ascontiguousarray(np.asarray(noise, dtype=np.float64).reshape(-1))
+    lib().orc_correlated_field_noise(nx, ny, nz, 1 if is3d else 0, mean, std, corr, _dp(nz_), _dp(out))
+    return out
+
+
 def rng_stream(seed, kind, k):
     out = np.zeros(k)
     lib().orc_rng_stream(seed, kind, k, _dp(out))
@@ -220,6 +234,12 @@ class OracleSim:
 
     def update_barriers(self):
         self._lib.orc_sim_update_barriers(self._h)
+
+    def run_cascade(self):
+        """run_cascade on the current state: (local_step, total_flips)."""
+        a, b = C.c_int(0), C.c_int(0)
+        self._lib.orc_sim_run_cascade(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def rates(self):
         nm = self.N * self.M

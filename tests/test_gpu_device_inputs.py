@@ -115,3 +115,15 @@ def test_simulation_with_device_draws_runs_and_repeats():
     assert len(a.events()) == 20
     assert np.array_equal(a.get_field(abi.FIELD_Q0), b.get_field(abi.FIELD_Q0))
     assert [e.flat_idx for e in a.events()] == [e.flat_idx for e in b.events()]
+
+
+@pytest.mark.parametrize("shape,at,corr", [((32, 32), (0, 0), 3.0), ((16, 12), (5, 7), 2.0),
+                                           ((8, 10, 6), (3, 0, 5), 1.5)])
+def test_field_filter_closed_form_impulse(shape, at, corr):
+    # a unit impulse through the device filter (three circular convolutions)
+    # equals the standardised Gaussian kernel summed mode by mode
+    from test_oracle_pinning import _impulse_field
+    noise = np.zeros(shape)
+    noise[at] = 1.0
+    got = correlated_field_device(shape, 0.0, 1.0, corr, 0, noise=noise)
+    assert np.abs(got - _impulse_field(shape, at, corr)).max() < 1e-12

@@ -244,3 +244,240 @@ def test_event_cap_and_instability_termination():
     st2 = sim2.run(abi.default_run_config(n_global_steps=10, step_size=1e4, max_kmc_steps_pct=0.05))
     assert st2.terminated == 1
     assert st2.kmc_events == int(0.05 * N)   # the terminating event is not counted
+
+
+# ---------------------------------------------------------------------------
+# run_cascade (kmc_solver.hpp:464-564) restated in numpy on a 4x4 grid: the
+# whole loop -- update_barriers, find_unstable_indices, the sort by Q, one
+# flip per voxel per round (the first, most negative, hit), the flips and the
+# redraw draws in sorted order, an elastic_run per round, the local_step > N
+# stop -- against the oracle's C++ restatement.  The LS solve here is a
+# numpy transcription of spectral_solver_2d (linear_elastic_solver.hpp:146-309)
+# with numpy.fft, independent of the oracle's own FFT.
+# ---------------------------------------------------------------------------
+def _np_green_2d(nx, ny, pixel, lam0, mu0):
+    xi = [2.0 * math.pi * np.fft.fftfreq(n, d=pixel) for n in (nx, ny)]   # Nyquist at -n/2, as fftfreq (:22-33)
+    A = 1.0 / (4.0 * mu0)
+    B = (lam0 + mu0) / (mu0 * (lam0 + 2.0 * mu0))
+    G = np.zeros((nx, ny, 2, 2, 2, 2))
+    d = np.eye(2)
+    for x in range(nx):
+        for y in range(ny):
+            q = np.array([xi[0][x], xi[1][y]])
+            n2 = q @ q
+            if n2 == 0.0:
+                continue
+            for k in range(2):
+                for h in range(2):
+                    for i in range(2):
+                        for j in range(2):
+                            G[x, y, k, h, i, j] = (A * (d[k, i] * q[h] * q[j] + d[h, i] * q[k] * q[j] + d[k, j] * q[h] * q[i]
+                                                        + d[h, j] * q[k] * q[i]) / n2 - B * q[k] * q[h] * q[i] * q[j] / n2 ** 2)
+    return G
+
+
+class _NpCascadeModel:
+    """KmcSimulation2D's state and the parts of it run_cascade touches."""
+
+    def __init__(self, osim, c, E, nu, stream):
+        self.c, self.s = c, stream
+        self.nx, self.ny, self.M = c.nx, c.ny, c.M
+        self.N = c.nx * c.ny
+        self.E, self.nu = np.asarray(E, float), np.asarray(nu, float)
+        self.eps = osim.get_field(abi.FIELD_EPS)
+        self.sig = osim.get_field(abi.FIELD_SIG)
+        self.eps_p = osim.get_field(abi.FIELD_EPS_PLASTIC)
+        self.Q0 = osim.get_field(abi.FIELD_Q0)
+        self.cat = osim.get_field(abi.FIELD_CATALOG)
+        self.soft = osim.get_field(abi.FIELD_SOFT)
+        self.last = osim.get_field(abi.FIELD_LAST_EVENT_TIME)
+        self.prev = osim.get_field(abi.FIELD_PREV_DIR)
+        self.eps_macro, _, self.time = osim.macro()
+        self.tau = osim.tau()
+        ea = sum(self.E) / self.N
+        na = sum(self.nu) / self.N
+        mu0 = ea / (2.0 * (1.0 + na))
+        lam0 = ea * na / ((1.0 + na) * (1.0 - 2.0 * na))
+        self.lam0, self.mu0 = lam0, mu0
+        self.mu = self.E / (2.0 * (1.0 + self.nu))
+        self.lam = self.E * self.nu / ((1.0 + self.nu) * (1.0 - 2.0 * self.nu))
+        self.G = _np_green_2d(c.nx, c.ny, c.pixel, lam0, mu0).reshape(self.N, 2, 2, 2, 2)
+
+    def _stress(self, eps):
+        e = eps - self.eps_p
+        tr = e[:, 0, 0] + e[:, 1, 1]
+        return 2.0 * self.mu[:, None, None] * e + (self.lam * tr)[:, None, None] * np.eye(2)
+
+    def elastic_run(self, tol=1e-6, max_iter=200):
+        # spectral_solver_2d with eps_bar = eps_macro, warm start (:180-191)
+        bar = self.eps_macro.copy()
+        eps = self.eps + (bar - self.eps.mean(axis=0))
+        for _ in range(max_iter):
+            sig = self._stress(eps)
+            tr = eps[:, 0, 0] + eps[:, 1, 1]
+            tau = sig - (2.0 * self.mu0 * eps + (self.lam0 * tr)[:, None, None] * np.eye(2))
+            th = np.fft.fft2(tau.reshape(self.nx, self.ny, 2, 2), axes=(0, 1), norm="ortho").reshape(self.N, 2, 2)
+            eh = -np.einsum("nkhrs,nrs->nkh", self.G, th)
+            et = np.fft.ifft2(eh.reshape(self.nx, self.ny, 2, 2), axes=(0, 1), norm="ortho").real.reshape(self.N, 2, 2)
+            new = bar + (et - et.mean(axis=0))
+            diff = math.sqrt(((new - eps) ** 2).sum()) / (math.sqrt((eps ** 2).sum()) + 1e-20)
+            eps = new
+            if diff < tol:
+                break
+        self.eps = eps
+        self.sig = self._stress(eps)
+        self.eps_macro = eps.mean(axis=0)
+
+    def Q(self):
+        c, M = self.c, self.M
+        Q = np.zeros(self.N * M)
+        for v in range(self.N):
+            g_t = self.soft[v, 1]
+            gtc = 0.0
+            if g_t > 0.0:
+                if math.isinf(self.tau):
+                    gtc = g_t
+                elif self.last[v] != -math.inf:
+                    gtc = g_t * math.exp(-max(self.time - self.last[v], 0.0) / self.tau)
+            gb = self.soft[v, 0] + gtc
+            if c.softening_cap > 0.0 and gb > c.softening_cap:
+                gb = c.softening_cap
+            for m in range(M):
+                i = v * M + m
+                w = sum(self.sig[v, a, b] * self.cat[i, a, b] for a in range(2) for b in range(2))
+                Q[i] = self.Q0[i] * math.exp(-gb) - 0.5 * c.pixel ** 3 * (w / 1e9) * 6.241509
+        return Q
+
+    def flip(self, v, m):
+        c, i = self.c, v * self.M + m
+        e = self.cat[i]
+        self.eps_p[v] = self.eps_p[v] + e
+        e11, e22, e12 = e[0, 0], e[1, 1], e[0, 1]
+        ss = (e12 * e12) + (e22 * e22 + e11 * e11 + (e11 - e22) ** 2) / 6.0
+        gp = self.soft[v, 0] + c.jp * ss
+        self.soft[v, 0] = min(gp, c.softening_cap) if c.softening_cap > 0.0 else gp
+        self.soft[v, 1] = c.jt * ss
+        self.last[v] = self.time
+        if c.neighbor_softening_fraction > 0.0:
+            x, y = divmod(v, self.ny)
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    if dx == 0 and dy == 0:
+                        continue
+                    nb = ((x + dx + self.nx) % self.nx) * self.ny + (y + dy + self.ny) % self.ny
+                    g = self.soft[nb, 0] + c.neighbor_softening_fraction * c.jp * ss
+                    self.soft[nb, 0] = min(g, c.softening_cap) if c.softening_cap > 0.0 else g
+                    self.soft[nb, 1] += c.neighbor_softening_fraction * c.jt * ss
+
+    def redraw(self, v):
+        # redraw_directions and redraw_barriers (:494-512): a fresh normal
+        # object per flip; per mode gxx, gxy, then a gaussian barrier
+        # (BarrierGenerator: fresh object, clipped at min_cutoff)
+        c = self.c
+        sn = Stream.Normal(self.s)
+        for mm in range(self.M):
+            i = v * self.M + mm
+            gxx = 0.5 * c.gamma0 * sn()
+            gxy = 0.5 * c.gamma0 * sn()
+            self.cat[i] = [[gxx, gxy], [gxy, -gxx]]
+            self.Q0[i] = max(Stream.Normal(self.s, c.b_mean, c.b_std)(), c.b_min_cutoff)
+
+    def run_cascade(self, rounds_log):
+        total, local, limit = 0, 0, self.N
+        while True:
+            Q = self.Q()
+            unstable = [i for i in range(self.N * self.M) if Q[i] < self.c.stability_threshold]   # ascending (:412-421)
+            if not unstable:
+                break
+            assert len({Q[i] for i in unstable}) == len(unstable), "ties would leave std::sort's order open"
+            unstable.sort(key=lambda i: Q[i])
+            done, flips = set(), []
+            for i in unstable:
+                v, m = divmod(i, self.M)
+                if v in done:
+                    continue
+                done.add(v)
+                flips.append(i)
+                C = self.cat[i].copy()
+                self.flip(v, m)
+                self.prev[v] = C
+                self.redraw(v)
+            rounds_log.append(dict(flips=flips, Q=Q, eps_p=self.eps_p.copy()))
+            self.elastic_run()
+            total += len(flips)
+            local += 1
+            if local > limit:
+                break
+        return local, total
+
+
+def _cascade_setup(**kw):
+    base = dict(nx=4, ny=4, M=3, instability_kmc=0, neighbor_softening_fraction=0.2, seed=5)
+    base.update(kw)
+    c = _cfg(**base)
+    N = 16
+    rng = np.random.default_rng(11)
+    E = 85e9 * (1.0 + 0.15 * rng.standard_normal(N))
+    nu = np.full(N, 0.35)
+    osim = po.OracleSim(c, E, nu)
+    s = Stream(c.seed)
+    std_normal = Stream.Normal(s)   # the constructor's draws (:223-237) put the stream where the oracle's is
+    for _ in range(N * c.M):
+        Stream.Normal(s, c.b_mean, c.b_std)()
+        std_normal()
+        std_normal()
+    return c, E, nu, osim, s
+
+
+def test_run_cascade_brute_force_4x4():
+    c, E, nu, osim, s = _cascade_setup()
+    # unstable modes with distinct barriers; voxel 2 holds two (the more
+    # negative, mode 1, must be the one flipped), voxel 11 two as well
+    Q0 = osim.get_field(abi.FIELD_Q0)
+    for i, q in ((2 * 3 + 0, -0.30), (2 * 3 + 1, -0.35), (5 * 3 + 2, -0.10), (11 * 3 + 0, -0.20), (11 * 3 + 2, -0.05)):
+        Q0[i] = q
+    osim.set_field(abi.FIELD_Q0, Q0)
+    osim.set_macro(np.array([[1e-3, 0.0], [0.0, 0.0]]), 0.5)
+    osim.elastic_run()
+    model = _NpCascadeModel(osim, c, E, nu, s)
+    cat_before = model.cat.copy()
+    log = []
+    want = model.run_cascade(log)
+    got = osim.run_cascade()
+    assert got == want, (got, want)
+    assert want[1] >= 4 and want[0] >= 1
+    # the first round flips in ascending Q, one mode per voxel: mode 1 of
+    # voxel 2 (-0.35) before mode 0 (-0.30, skipped), mode 0 of voxel 11
+    # (-0.20) and not mode 2 (-0.05)
+    r0 = log[0]
+    assert [r0["Q"][i] for i in r0["flips"]] == sorted(r0["Q"][i] for i in r0["flips"])
+    assert r0["flips"][0] == 7 and 6 not in r0["flips"]
+    assert 33 in r0["flips"] and 35 not in r0["flips"] and 17 in r0["flips"]
+    assert np.array_equal(r0["eps_p"][2], cat_before[7])
+    # every flip and redraw in the reference's order: the random draws land
+    # on the same voxels, so Q0 and the catalog are equal bit for bit
+    assert np.array_equal(osim.get_field(abi.FIELD_Q0), model.Q0)
+    assert np.array_equal(osim.get_field(abi.FIELD_CATALOG), model.cat)
+    assert np.array_equal(osim.get_field(abi.FIELD_EPS_PLASTIC), model.eps_p)
+    assert np.array_equal(osim.get_field(abi.FIELD_SOFT), model.soft)
+    assert np.array_equal(osim.get_field(abi.FIELD_PREV_DIR), model.prev)
+    assert np.array_equal(osim.get_field(abi.FIELD_LAST_EVENT_TIME), model.last)
+    sig = osim.get_field(abi.FIELD_SIG)
+    assert np.abs(sig - model.sig).max() < 1e-11 * np.abs(model.sig).max()
+
+
+def test_run_cascade_stops_after_n_plus_one_rounds():
+    # a threshold above every barrier keeps all modes unstable: each round
+    # flips every voxel once, and the loop stops when local_step > N
+    # (kmc_solver.hpp:556-557): N + 1 rounds of N flips
+    c, E, nu, osim, s = _cascade_setup(stability_threshold=1e3, neighbor_softening_fraction=0.0)
+    osim.set_macro(np.array([[1e-3, 0.0], [0.0, 0.0]]), 0.0)
+    osim.elastic_run()
+    model = _NpCascadeModel(osim, c, E, nu, s)
+    log = []
+    want = model.run_cascade(log)
+    assert want == (17, 17 * 16)
+    assert all(len(r["flips"]) == 16 for r in log)
+    assert osim.run_cascade() == want
+    assert np.array_equal(osim.get_field(abi.FIELD_Q0), model.Q0)
+    assert np.array_equal(osim.get_field(abi.FIELD_EPS_PLASTIC), model.eps_p)

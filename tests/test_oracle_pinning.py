@@ -249,3 +249,63 @@ def test_correlated_field_statistics():
     assert abs(f.std() - 12.75) < 1e-10
     g = po.correlated_field((32, 32), 85.0, 12.75, 3.0, 42, clip_min=60.0)
     assert g.min() == 60.0
+
+
+def _impulse_field(shape, at, corr, coeff=None):
+    """Closed form of the filter step of generate_correlated_field_cpp
+    (material_generator.hpp:139-219) for a unit impulse at voxel `at`:
+    g(x) = (1/N) sum_f exp(-2 pi^2 corr^2 |f|^2) cos(2 pi f.(x - at)) with
+    f = k/n in [-1/2, 1/2) per axis, then standardised with the population
+    deviation (:222-235).  Written as an explicit sum over modes (no FFT)."""
+    shape = tuple(shape)
+    N = int(np.prod(shape))
+    coeff = -2.0 * math.pi ** 2 * corr ** 2 if coeff is None else coeff
+    axes = []
+    for ax, n in enumerate(shape):
+        k = np.arange(n)
+        f = np.where(k < (n + 1) // 2, k, k - n) / n
+        x = (np.arange(n) - at[ax]) % n
+        # per-axis factor: exp(coeff f^2) and the phase 2 pi f x
+        axes.append((f, x))
+    g = np.zeros(shape)
+    grids = np.meshgrid(*[a[1] for a in axes], indexing="ij")
+    for idx in np.ndindex(*shape):
+        fs = [axes[a][0][idx[a]] for a in range(len(shape))]
+        h = math.exp(coeff * sum(f * f for f in fs))
+        ph = sum(2.0 * math.pi * fs[a] * grids[a] for a in range(len(shape)))
+        g += h * np.cos(ph)
+    g = (g / N).reshape(-1)
+    m = g.mean()
+    sd = math.sqrt(((g - m) ** 2).sum() / N)
+    return (g - m) / sd
+
+
+def test_correlated_field_filter_closed_form():
+    # a unit impulse through the generator's filter gives the standardised
+    # Gaussian kernel of the closed form; the standardisation (an affine map)
+    # cannot hide a wrong filter, because the kernel's shape is compared voxel
+    # by voxel.  A filter with the wrong constant (pi^2 instead of 2 pi^2) or
+    # with frequencies not divided by n is far off.
+    from mgkmc_cpp_amd import fields
+    for shape, at, corr in (((32, 32), (0, 0), 3.0), ((16, 12), (5, 7), 2.0), ((8, 10, 6), (3, 0, 5), 1.5)):
+        N = int(np.prod(shape))
+        noise = np.zeros(shape)
+        noise[at] = 1.0
+        want = _impulse_field(shape, at, corr)
+        got = po.correlated_field_noise(shape, 0.0, 1.0, corr, noise)
+        assert np.abs(got - want).max() < 1e-12, shape
+        host = fields.correlated_field(shape, 0.0, 1.0, corr, 0, noise=noise)
+        assert np.abs(host - want).max() < 1e-12, shape
+        # the mean and deviation targets are applied after standardising
+        got2 = po.correlated_field_noise(shape, 85.0, 12.75, corr, noise)
+        assert np.abs(got2 - (85.0 + 12.75 * want)).max() < 1e-10
+        wrong_const = _impulse_field(shape, at, corr, coeff=-math.pi ** 2 * corr ** 2)
+        assert np.abs(wrong_const - want).max() > 1e-2
+        # corr is the real-space deviation of the kernel: the filter
+        # exp(-2 pi^2 s^2 f^2) is the transform of exp(-x^2 / (2 s^2)), so the
+        # kernel (above its periodic floor) falls to exp(-1/2) at x = s
+        if len(shape) == 2 and shape[0] == 32:
+            prof = got.reshape(shape)[:, 0]
+            base = prof.min()
+            r = (prof[3] - base) / (prof[0] - base)
+            assert abs(r - math.exp(-0.5)) < 0.02, r
