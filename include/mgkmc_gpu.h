@@ -279,6 +279,13 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
 /* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
 
+/* Measurement helper (no counterpart in the reference, whose verbose mode
+ * prints the same quantity every 10 iterations, linear_elastic_solver.hpp:294):
+ * the relative increment of each iteration of the last solve in the order run
+ * (FP64 iterations and FP32 inner steps interleaved; at most 32), n_out the
+ * count; at most cap values are written. */
+int mgk_last_solve_history(mgk_ctx* ctx, double* diffs, int cap, int* n_out);
+
 /* Device memory (no counterpart in the reference): the bytes a context
  * holds, and the bytes a context of this shape and mode count will hold
  * after mgk_create + mgk_kmc_init (M = 0: no KMC state) and a catalog upload

@@ -79,6 +79,7 @@ def gpu():
         _sig(lib, "mgk_timer_start", C.c_int, [P])
         _sig(lib, "mgk_timer_stop", C.c_int, [P, DP])
         _sig(lib, "mgk_last_solve_stats", C.c_int, [P, IP, IP])
+        _sig(lib, "mgk_last_solve_history", C.c_int, [P, DP, C.c_int, IP])
         _sig(lib, "mgk_device_bytes", C.c_longlong, [P])
         _sig(lib, "mgk_estimate_bytes", C.c_longlong, [C.c_int] * 6)
         _sig(lib, "mgk_time_ls_pass", C.c_int, [P, C.c_int, C.c_int, C.c_int, DP])

@@ -1643,6 +1643,17 @@ int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters) {
     });
 }
 
+int mgk_last_solve_history(mgk_ctx* ctx, double* diffs, int cap, int* n_out) {
+    return guarded([&]() -> int {
+        if (!ctx || !n_out || (cap > 0 && !diffs)) throw ArgError("null argument");
+        const Ctl& h = *ctx->hctl;
+        const int n = std::min(32, h.iters + h.inner);
+        *n_out = n;
+        for (int k = 0; k < std::min(n, cap); ++k) diffs[k] = h.diff_hist[k];
+        return MGK_OK;
+    });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

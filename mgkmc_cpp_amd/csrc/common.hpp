@@ -85,6 +85,7 @@ struct Ctl {
     // and reads slot (j - 1) & 1, which no block of that kernel writes
     double cgs_gamma[2], cgs_alpha[2], cgs_beta[2];
     int cgs_inner[2];
+    double cycle_r0;       // relative FP64 increment that started the current refinement cycle
 };
 
 struct Status {

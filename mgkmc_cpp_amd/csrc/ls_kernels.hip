@@ -874,15 +874,31 @@ struct CgDecision {
     int halt;   // converged, at the iteration cap, or broken down
 };
 
+// The FP32 recursive residual of a CG cycle drifts from the true residual by
+// rounding in the FP32 operator and vector updates, by up to about kCgDrift
+// times the residual the cycle started from (the true residual measured by the
+// FP64 iteration after a cycle: at most 1.5e-7 of the cycle's start on 16^3
+// trajectories whose flips make up the whole field, 1.1e-7 at 64^3).  The
+// recursive residual must reach tol minus that margin, so that the true one is
+// below tol.  Where the margin would exceed half the tolerance, a converged
+// cycle does not end the solve: it ends the cycle, and the next FP64
+// iteration measures the true residual and makes the convergence decision,
+// as in the basic scheme.
+constexpr double kCgDrift = 3e-7;
+
 __device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&tot)[3], bool first, bool slots,
                                                      int slot_new, bool write) {
     const double gam = tot[0], del = tot[1];
     const int inner = (slots ? ctl->cgs_inner[slot_new ^ 1] : ctl->inner) + 1;
-    bool done = false;
+    bool done = false, verify = false;
     double diff = 0.0;
     if (!first) {
         diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
-        done = diff < ctl->tol || ctl->iters + inner >= ctl->max_iter;
+        const double margin = kCgDrift * ctl->cycle_r0;
+        const bool trusted = margin <= 0.5 * ctl->tol;
+        const bool conv = diff < (trusted ? ctl->tol - margin : ctl->tol);
+        done = (conv && trusted) || ctl->iters + inner >= ctl->max_iter;
+        verify = conv && !done;
     }
     double al, be;
     if (first) {
@@ -894,7 +910,8 @@ __device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&to
         be = gam / g_old;
         al = gam / (del - be * gam / a_old);
     }
-    const bool stop = !(al > 0.0) || !isfinite(al) || !isfinite(be);   // breakdown: back to FP64
+    // breakdown, or a converged cycle to verify: back to FP64
+    const bool stop = verify || !(al > 0.0) || !isfinite(al) || !isfinite(be);
     if (write) {
         ctl->inner = inner;
         if (!first) {
@@ -1200,6 +1217,7 @@ __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const L
             ctl->diff_sq = tot[0];
             ctl->norm_sq = tot[1];
             ctl->eps_norm = sqrt(tot[2] + ctl->norm_extra);
+            ctl->cycle_r0 = diff;
             if (ctl->iters + ctl->inner < 32) ctl->diff_hist[ctl->iters + ctl->inner] = diff;
             ctl->iters += 1;
             ctl->corr_valid = 0;   // a pending correction was absorbed into eps

@@ -282,6 +282,11 @@ class Context:
         check(gpu().mgk_last_solve_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def last_solve_history(self):
+        """Relative increment of every iteration of the last solve, in order
+        (FP64 iterations and FP32 inner steps interleaved)."""
+        return solve_history(self._h)
+
 
 def spectral_solver_2d(nx, ny, E, nu, eps_bar, plane_mode, eps=None, max_iter=200, tol=1e-6, eps_plastic=None):
     """spectral_solver_2d (linear_elastic_solver.hpp:146-309) on the device.
@@ -300,6 +305,13 @@ def spectral_solver_3d(nx, ny, nz, E, nu, eps_bar, eps=None, max_iter=200, tol=1
         return ctx.ls_solve(eps_bar, eps=eps, eps_plastic=eps_plastic, max_iter=max_iter, tol=tol)
     finally:
         ctx.close()
+
+
+def solve_history(ctx_handle):
+    buf = np.zeros(32)
+    n = C.c_int(0)
+    check(gpu().mgk_last_solve_history(ctx_handle, _dp(buf), 32, C.byref(n)))
+    return buf[: n.value].copy()
 
 
 class Simulation:
