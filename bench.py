@@ -64,6 +64,39 @@ def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
     return 2 * spec                  # B, C, D: read and write the spectrum
 
 
+# The algorithmic bytes of the survey's three-pass model of one LS iteration
+# (bytes per voxel, 3D, C = 6 strain components, s = 4 (FP32) or 8 (FP64)):
+#   pass A  tau and the forward transforms of the first two axes: reads the
+#           iterate (C s), eps_p (C, FP64), lam and mu (2 s), writes C s;
+#   pass B  the transform along the third axis, Gamma0 and its inverse:
+#           reads and writes C s;
+#   pass C  the inverse transforms and the update: reads C s and the old
+#           iterate (C s), writes C s.
+# b_iter = (7C + 2) s + 8C: 224 B (FP32), 400 B (FP64).  The pipeline's passes
+# map onto the model as: FP32 plane passes 6 -> A, 7 -> B, 8 -> C; the
+# five-pass pipeline 0 + 1 -> A, 2 -> B, 3 + 4 -> C (the model's bytes of a
+# split pass go to its first kernel and 0 to the second, so a per-kernel
+# figure is only quoted for 6, 7, 8, 2 and the whole-event figure below).
+def _model_bytes_per_voxel(pass_id, fp32):
+    Cc, sz = 6, (4 if fp32 else 8)
+    a = (2 * Cc + 2) * sz + 8 * Cc
+    b = 2 * Cc * sz
+    c = 3 * Cc * sz
+    return {0: a, 1: 0, 2: b, 3: 0, 4: c, 5: a, 6: a, 7: b, 8: c}[pass_id]
+
+
+def _model_event_bytes(info32, N, M=12):
+    """The survey's bytes of one event: K b_iter(FP32) N + R b_iter(FP64) N
+    + b_solve_end N + b_rate, with K the CG steps and R the FP64 iterations
+    per event, b_solve_end = 8C (the final FP64 stress), b_rate = N (8C + 8M
+    + 24) for a shared catalog (stress, Q0, softening state)."""
+    Cc = 6
+    K = info32["fp32_iters_per_event"]
+    R = info32["ls_iters_per_event"]
+    b = K * ((7 * Cc + 2) * 4 + 8 * Cc) + R * ((7 * Cc + 2) * 8 + 8 * Cc) + 8 * Cc + (8 * Cc + 8 * M + 24)
+    return b * N
+
+
 def _launches_per_event(pass_id, fp32, info):
     """Launches of a pass per event in the FP32+refine run."""
     cycles = info["ls_iters_per_event"]          # FP64 iterations = refinement cycles
@@ -105,7 +138,10 @@ def run_gpu(precision, steps, warmup, E, nu):
 def kernel_roofline(sim, info32):
     """Average launch time of every pipeline pass (HIP events on the
     context's stream), weighted by its launches per event in the FP32+refine
-    run; the pass with the largest time per event is reported."""
+    run; the pass with the largest time per event is reported.  `achieved`
+    uses the survey model's bytes of that pass (_model_bytes_per_voxel); the
+    bytes the kernel itself has to move (CG vectors included) are reported
+    beside it as kernel_bytes_per_launch / kernel_gbs."""
     nl = SHAPE[2]
     N = int(np.prod(SHAPE))
     rows = []
@@ -113,14 +149,23 @@ def kernel_roofline(sim, info32):
         for pass_id in _passes(fp32):
             ms = sim.time_ls_pass(pass_id, fp32=fp32, reps=20)
             b = _algorithmic_bytes_per_voxel(pass_id, fp32, nl) * N
+            mb = _model_bytes_per_voxel(pass_id, fp32) * N
             n = _launches_per_event(pass_id, fp32, info32)
             rows.append(dict(kernel=PASS_NAMES[pass_id] + ("<float>" if fp32 else "<double>"), pass_id=pass_id,
-                             fp32=fp32, ms=round(ms, 5), bytes=int(b), launches_per_event=round(n, 2),
-                             ms_per_event=round(ms * n, 5), gbs=round(b / (ms * 1e-3) / 1e9, 1)))
+                             fp32=fp32, ms=round(ms, 5), model_bytes=int(mb), bytes=int(b),
+                             launches_per_event=round(n, 2), ms_per_event=round(ms * n, 5),
+                             model_gbs=round(mb / (ms * 1e-3) / 1e9, 1), gbs=round(b / (ms * 1e-3) / 1e9, 1)))
     top = max(rows, key=lambda r: r["ms_per_event"])
-    achieved = top["bytes"] / (top["ms"] * 1e-3) / 1e9
-    return top, rows, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_measured_traffic(top["kernel"]))
+    achieved = top["model_bytes"] / (top["ms"] * 1e-3) / 1e9
+    own = top["bytes"] / (top["ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_measured_traffic(top["kernel"]),
+                traffic_source="profiles/traffic_64cube.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same "
+                               "pass, calibrated on a device copy (not measured in this run)",
+                bytes_model="survey three-pass model, bytes per voxel " + str(_model_bytes_per_voxel(top["pass_id"], top["fp32"])),
+                kernel_bytes_per_launch=int(top["bytes"]), kernel_gbs=round(own, 1),
+                kernel_frac=round(own / HBM_PEAK_GBS, 4))
+    return top, rows, roof
 
 
 def _measured_traffic(kernel):
@@ -135,22 +180,35 @@ def _measured_traffic(kernel):
         return None
 
 
-def fp64_rates(shapes, events=8):
-    """Plain FP64 single-trajectory events/s per grid shape (the sequential
-    baseline of the serving and batch workloads), from a short run of each."""
+def fp64_rates(shapes, events=200, min_s=0.0):
+    """Plain FP64 single-trajectory events/s per grid shape: the sequential
+    baseline of the serving and batch workloads.  Each shape runs the served
+    requests' own workload (instability mode "kmc", inputs drawn on the
+    device) for `events` events after 2 of warm-up (longer if min_s asks);
+    the cascades and exact-selection passes of the sample are reported."""
     from mgkmc_cpp_amd import abi
     from mgkmc_cpp_amd.serving import Request, request_simulation
-    rates = {}
+    rates, detail = {}, {}
     for shape in shapes:
-        sim = request_simulation(Request(rid=-1, shape=shape, events=events, seed=99, precision=abi.MGK_FP64))
+        sim = request_simulation(Request(rid=-1, shape=shape, events=events, seed=99, precision=abi.MGK_FP64,
+                                         instability_kmc=1, device_inputs=True))
         sim.elastic_run()
         sim.event_steps(2)
+        n = casc = exact = 0
         t0 = time.perf_counter()
-        st = sim.event_steps(events)
+        while n < events or time.perf_counter() - t0 < min_s:
+            st = sim.event_steps(min(events, 50))
+            n += int(st.kmc_events)
+            casc += int(st.cascades)
+            exact += int(st.exact_selections)
+            if st.kmc_events == 0:
+                break
         dt = time.perf_counter() - t0
-        rates[shape] = st.kmc_events / dt if dt > 0 else 0.0
+        rates[shape] = n / dt if dt > 0 else 0.0
+        detail["x".join(map(str, shape))] = dict(events=n, seconds=round(dt, 3), events_per_s=round(rates[shape], 1),
+                                                 cascades=casc, exact_selections=exact)
         sim.close()
-    return rates
+    return rates, detail
 
 
 def serve(requests, budget_gb, label):
@@ -165,11 +223,11 @@ def serve(requests, budget_gb, label):
     srv.run(requests)
     wall = time.perf_counter() - t0
     summ = serving.summary(requests)
-    rates = fp64_rates(sorted({r.shape for r in requests}, key=lambda x: (len(x), x)))
+    rates, detail = fp64_rates(sorted({r.shape for r in requests}, key=lambda x: (len(x), x)))
     seq = sum(r.events_done / rates[r.shape] for r in requests if rates[r.shape] > 0)
     summ.update(wall_s=round(wall, 3), sequential_fp64_s=round(seq, 3),
                 speedup_vs_sequential_fp64=round(seq / summ["span_s"], 2) if summ["span_s"] else None,
-                fp64_single_events_per_s={"x".join(map(str, k)): round(v, 1) for k, v in rates.items()})
+                fp64_single=detail)
     for k in ("p50_latency_s", "p99_latency_s", "events_per_s", "span_s"):
         if summ.get(k) is not None:
             summ[k] = round(summ[k], 4)
@@ -336,6 +394,7 @@ def main():
                     help="c2: the headline line (with a bounded c4 sample); c3 / c4: the full batch / serving runs")
     ap.add_argument("--serving-requests", type=int, default=48, help="requests of the bounded c4 sample")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="leave out the full c3 batch run of the default line")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="HBM admission budget of the serving loop")
     ap.add_argument("--cpu-worker", type=int, default=None, help=argparse.SUPPRESS)
     a = ap.parse_args()
@@ -364,8 +423,14 @@ def main():
     top, rows, roof = kernel_roofline(sim32, i32)
     print(f"[bench] pass timings {json.dumps(rows)}", file=sys.stderr, flush=True)
     roof["kernel"] = top["kernel"]
-    roof["bytes_per_launch"] = int(top["bytes"])
+    roof["bytes_per_launch"] = int(top["model_bytes"])
     roof["avg_launch_ms"] = round(top["ms"], 5)
+    N = int(np.prod(SHAPE))
+    ev_bytes = _model_event_bytes(i32, N)
+    ev_gbs = ev_bytes / (i32["seconds"] / max(i32["events"], 1)) / 1e9
+    roof["per_event"] = dict(model_bytes=int(ev_bytes), achieved=round(ev_gbs, 1), frac=round(ev_gbs / HBM_PEAK_GBS, 4),
+                             note="survey bytes of a whole event (CG steps, FP64 iterations, final stress, rate "
+                                  "pass) over the measured time per event")
     sim32.close()
     print("[bench] FP32+refine against FP64: per-field errors", file=sys.stderr, flush=True)
     errs = field_errors(E, nu)
@@ -380,6 +445,12 @@ def main():
         print(f"[bench] CPU baseline {cpu}", file=sys.stderr, flush=True)
         cpu_all = cpu_aggregate(a.cpu_budget)
         print(f"[bench] CPU aggregate {cpu_all}", file=sys.stderr, flush=True)
+    c3 = None
+    if not a.no_serving and not a.no_c3:
+        reqs = configs.c3_requests(256, events=2000)
+        c3 = serve(reqs, a.budget_gb, "c3")
+        c3["workload"] = ("c3: 256 trajectories on grids drawn from 32^2, 64^2, 128^2, 32^3, 48^3 (seed 7), "
+                          "2000 events each, FP32+refine, batched by grid")
     serving_sample = None
     if not a.no_serving:
         # bounded c4 sample: Poisson arrivals at 50/s, grids up to 64^3, events
@@ -417,6 +488,7 @@ def main():
         "fp32_refine_errors_vs_fp64": errs,
         "whole_run": whole,
         "serving": serving_sample,
+        "c3": c3,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -229,7 +229,8 @@ typedef struct mgk_event_out {
     int flat_idx;                /* selected flat mode, -1 without an event */
     double total_rate, t_wait, q_sel;
     int iters;                   /* LS iterations of the solve */
-    int pad;
+    int exact_selection;         /* 1: the selection was within the rounding window and the
+                                    left-to-right pass (mgk_select_exact's) decided it (FP64 only) */
     double diff;
 } mgk_event_out;
 int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, double* sig_macro,

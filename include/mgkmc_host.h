@@ -95,6 +95,8 @@ typedef struct mgk_run_stats {
     int    terminated;           /* instability termination (kmc_solver.hpp:925-929) */
     int    event_cap_reached;
     double time;
+    long long exact_selections;  /* events whose selection the left-to-right pass decided
+                                    (targets in the rounding window of the parallel prefix) */
 } mgk_run_stats;
 
 /* One record per accepted KMC event (kmc_solver.hpp:811-937). */

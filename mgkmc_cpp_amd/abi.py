@@ -77,7 +77,7 @@ class EventOut(C.Structure):
         ("n_unstable", C.c_int), ("n_included", C.c_int),
         ("flat_idx", C.c_int),
         ("total_rate", C.c_double), ("t_wait", C.c_double), ("q_sel", C.c_double),
-        ("iters", C.c_int), ("pad", C.c_int),
+        ("iters", C.c_int), ("exact_selection", C.c_int),
         ("diff", C.c_double),
     ]
 
@@ -163,6 +163,7 @@ class RunStats(C.Structure):
         ("terminated", C.c_int),
         ("event_cap_reached", C.c_int),
         ("time", C.c_double),
+        ("exact_selections", C.c_longlong),
     ]
 
 

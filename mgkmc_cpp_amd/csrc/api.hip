@@ -1155,6 +1155,7 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
             out->t_wait = k.t_wait;
             out->flat_idx = k.go ? k.sel : -1;
             out->q_sel = k.go ? k.q_sel : 0.0;
+            out->exact_selection = (k.go && k.ambiguous && c.precision == MGK_FP64) ? 1 : 0;
             if (k.outcome == MGK_EVENT_DONE) {
                 const double x = in->strain_rate * k.t_wait;
                 const int drv = in->drv_i + Dd * in->drv_j;
@@ -1175,6 +1176,7 @@ int mgk_kmc_event(mgk_ctx* ctx, const mgk_event_in* in, double* eps_macro, doubl
         out->t_wait = k.t_wait;
         out->flat_idx = k.go ? k.sel : -1;
         out->q_sel = k.go ? k.q_sel : 0.0;
+        out->exact_selection = (k.go && k.ambiguous && c.precision == MGK_FP64) ? 1 : 0;
         if (k.outcome != MGK_EVENT_DONE) return MGK_OK;
         out->iters = it;
         out->diff = diff;
@@ -1881,6 +1883,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         o.t_wait = k.t_wait;
         o.flat_idx = k.go ? k.sel : -1;
         o.q_sel = k.go ? k.q_sel : 0.0;
+        o.exact_selection = (k.go && k.ambiguous && !refine) ? 1 : 0;
         if (k.outcome != MGK_EVENT_DONE) continue;
         c.absorb_solve(tol);
         o.iters = c.hctl->iters;

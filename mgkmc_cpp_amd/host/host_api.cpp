@@ -176,6 +176,7 @@ static void fill_stats(const mgk_host::RunCounters& k, double time, mgk_run_stat
     st->ls_solves = k.ls_solves;
     st->ls_iters = k.ls_iters;
     st->ls_iters_fp32 = k.ls_iters_fp32;
+    st->exact_selections = k.exact_selections;
     st->terminated = k.terminated ? 1 : 0;
     st->event_cap_reached = k.event_cap_reached ? 1 : 0;
     st->time = time;

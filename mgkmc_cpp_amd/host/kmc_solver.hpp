@@ -132,6 +132,7 @@ template <int D> inline Mat<D> fixed_shear(int m, int M, double g) {
 struct RunCounters {
     long long kmc_events = 0, cascades = 0, cascade_flips = 0, elastic_steps = 0;
     long long ls_solves = 0, ls_iters = 0, ls_iters_fp32 = 0;
+    long long exact_selections = 0;   // not part of checkpoints (a diagnostic)
     bool terminated = false, event_cap_reached = false;
 };
 
@@ -757,6 +758,7 @@ public:
             counters.ls_iters_fp32 += inner;
         }
         last_iters_ = out.iters;
+        if (out.exact_selection) counters.exact_selections++;
         const long flat = out.flat_idx;
         const long v = flat / M;
         const int inst = (out.q_sel <= stability_threshold) ? 1 : 0;
