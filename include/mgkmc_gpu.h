@@ -339,6 +339,22 @@ int mgk_correlated_field(int dims, int nx, int ny, int nz, double mean, double s
                          double clip_min, int has_max, double clip_max, unsigned long long seed, const double* noise,
                          double* out);
 
+/* mgk_correlated_field without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that admit requests without hipMalloc).  noise_dev
+ * (N device doubles, or NULL for device normals keyed by seed) and out_dev
+ * (N device doubles) are the caller's; the work is enqueued on `stream` (a
+ * hipStream_t, NULL for the null stream) and the call returns without waiting.
+ * Every other buffer (the filter taps, the ping-pong field, the moments) lives
+ * in one static device array of about 16 MiB per GPU, held until the process
+ * ends.  Limits: N <= 2^21 voxels (128^3) and every axis <= 2048; larger grids
+ * return MGK_E_ARG.  Calls on one GPU share that array: calls that overlap on
+ * one GPU race.  The taps are computed on the device, so the result equals
+ * mgk_correlated_field's to about 1e-13 relative. */
+int mgk_correlated_field_noalloc(int dims, int nx, int ny, int nz, double mean, double std, double corr,
+                                 int has_min, double clip_min, int has_max, double clip_max, unsigned long long seed,
+                                 const double* noise_dev, double* out_dev, void* stream);
+
 /* Trajectory state snapshot (no counterpart in the reference, whose
  * output.checkpoint_interval is parsed and ignored, main.cpp:252-258): the
  * strain, stress and plastic-strain fields, the KMC fields (Q, Q0, catalog,

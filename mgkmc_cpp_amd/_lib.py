@@ -59,6 +59,9 @@ def gpu():
         _sig(lib, "mgk_correlated_field", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                                     C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
                                                     C.c_ulonglong, DP, DP])
+        _sig(lib, "mgk_correlated_field_noalloc", C.c_int,
+             [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+              C.c_double, C.c_ulonglong, C.c_void_p, C.c_void_p, C.c_void_p])
         _sig(lib, "mgk_patch_unit", C.c_int, [P, DP])
         _sig(lib, "mgk_patch_apply", C.c_int, [P, C.c_int, DP, C.c_double])
         _sig(lib, "mgk_state_save", C.c_int, [P, C.c_void_p, C.c_longlong])

@@ -1319,6 +1319,19 @@ int mgk_correlated_field(int dims, int nx, int ny, int nz, double mean, double s
     });
 }
 
+int mgk_correlated_field_noalloc(int dims, int nx, int ny, int nz, double mean, double std, double corr,
+                                 int has_min, double clip_min, int has_max, double clip_max, unsigned long long seed,
+                                 const double* noise_dev, double* out_dev, void* stream) {
+    return guarded([&]() -> int {
+        if (!out_dev) throw ArgError("null output");
+        if ((dims != 2 && dims != 3) || nx < 1 || ny < 1 || (dims == 3 && nz < 1)) throw ArgError("bad grid");
+        const int n[3] = {nx, ny, dims == 3 ? nz : 1};
+        FieldParams f{mean, std, corr, has_min ? 1 : 0, has_max ? 1 : 0, clip_min, clip_max};
+        gen_correlated_field_noalloc(n, dims, f, seed, noise_dev, out_dev, static_cast<hipStream_t>(stream));
+        return MGK_OK;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Fast patching (kmc_solver.hpp:276-332, :1256-1331, :867-878)
 // ---------------------------------------------------------------------------

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace mgk {
@@ -313,6 +314,111 @@ void gen_correlated_field(const int* n, int dims, const FieldParams& f, unsigned
         throw;
     }
     cleanup();
+}
+
+// ---------------------------------------------------------------------------
+// The correlated field without allocation: the same kernels, with every
+// buffer beyond the caller's noise and output in `scratch`, a file-scope
+// device array (one copy per GPU, held until the process ends):
+//   [0, kNoAllocMaxVoxels)            the ping-pong field buffer
+//   then 3 x kNoAllocMaxAxis          the filter taps of each axis
+//   then kNoAllocParts + 2            the moment partials, mean and deviation
+// The taps are computed on the device (k_filter_taps) instead of on the host.
+// Calls on one GPU share the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocParts = 256;
+constexpr size_t kScratchDoubles = (size_t)kNoAllocMaxVoxels + 3 * (size_t)kNoAllocMaxAxis + kNoAllocParts + 2;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+// h[x] = (1/n) sum_k exp(c f_k^2) cos(2 pi ((k x) mod n) / n), f = fftfreq(n):
+// filter_kernel's taps, summed with compensation in FP64
+__global__ __launch_bounds__(256) void k_filter_taps(double* __restrict__ h, int n, double corr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const double pi = 3.141592653589793;
+    const double c = -2.0 * pi * pi * corr * corr;
+    const int lim = (n + 1) / 2;
+    double acc = 0.0, comp = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double f = (double)(k < lim ? k : k - n) / (double)n;
+        const double term = exp(c * f * f) * cospi(2.0 * (double)(((long)k * x) % n) / (double)n) - comp;
+        const double t = acc + term;
+        comp = (t - acc) - term;
+        acc = t;
+    }
+    h[x] = acc / (double)n;
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on the current GPU, looked up once per GPU
+// (the array's host-side name is not a device address).
+double* scratch_address() {
+    int dev = 0;
+    MGK_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw ArgError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        MGK_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw HipError("hipGetSymbolAddress(scratch) returned no address");
+        g_scratch[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch[dev];
+}
+
+void gen_correlated_field_noalloc(const int* n, int dims, const FieldParams& f, unsigned long long seed,
+                                  const double* noise, double* out, hipStream_t s) {
+    long N = 1;
+    for (int a = 0; a < dims; ++a) {
+        if (n[a] > kNoAllocMaxAxis)
+            throw ArgError("axis of length " + std::to_string(n[a]) + " over the no-allocation limit of " +
+                           std::to_string(kNoAllocMaxAxis));
+        N *= n[a];
+    }
+    if (N > kNoAllocMaxVoxels)
+        throw ArgError("grid of " + std::to_string(N) + " voxels over the no-allocation limit of " +
+                       std::to_string(kNoAllocMaxVoxels));
+    double* sc = scratch_address();
+    double* y = sc;                                   // ping-pong partner of `out`
+    double* taps = sc + kNoAllocMaxVoxels;            // 3 x kNoAllocMaxAxis
+    double* part = taps + 3 * (size_t)kNoAllocMaxAxis;
+    double* stat = part + kNoAllocParts;
+    for (int a = 0; a < dims; ++a)
+        hipLaunchKernelGGL(k_filter_taps, dim3((unsigned)((n[a] + 255) / 256)), dim3(256), 0, s,
+                           taps + (size_t)a * kNoAllocMaxAxis, n[a], f.corr);
+    // dims passes alternate between `out` and y and must end in `out`
+    const double* src;
+    double* dst;
+    if (noise) {
+        src = noise;
+        dst = (dims % 2) ? out : y;
+    } else {
+        double* start = (dims % 2) ? y : out;
+        hipLaunchKernelGGL(k_normals, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, start, N, seed);
+        src = start;
+        dst = (start == out) ? y : out;
+    }
+    long inner = 1;
+    for (int ax = dims - 1; ax >= 0; --ax) {
+        const long nlines = N / n[ax];
+        hipLaunchKernelGGL(k_conv_axis, dim3((unsigned)nlines), dim3(std::min(256, ((n[ax] + 63) / 64) * 64)),
+                           2 * n[ax] * sizeof(double), s, src, dst, taps + (size_t)ax * kNoAllocMaxAxis, n[ax], inner,
+                           inner, nlines);
+        src = dst;
+        dst = (dst == out) ? y : out;
+        inner *= n[ax];
+    }
+    hipLaunchKernelGGL(k_moments, dim3(kNoAllocParts), dim3(256), 0, s, out, N, stat, part, 0);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, s, part, kNoAllocParts, N, stat, 0);
+    hipLaunchKernelGGL(k_moments, dim3(kNoAllocParts), dim3(256), 0, s, out, N, stat, part, 1);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, s, part, kNoAllocParts, N, stat, 1);
+    hipLaunchKernelGGL(k_standardize, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, N, stat, f);
+    MGK_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace mgk

@@ -33,4 +33,15 @@ void gen_enqueue_modes(double* Q0, double* cat, long nmodes, int C, const DrawPa
 void gen_correlated_field(const int* n, int dims, const FieldParams& f, unsigned long long seed, const double* noise,
                           double* out);
 
+// Limits of the variant without allocation (its scratch array is about 16 MiB).
+constexpr long kNoAllocMaxVoxels = 1L << 21;   // 128^3
+constexpr int kNoAllocMaxAxis = 2048;
+
+// The same field enqueued on stream s with no allocation: noise (N device
+// normals, or null for device normals from `seed`) and out (N device values)
+// are the caller's device buffers; every other buffer is in the file-scope
+// scratch array.  Throws ArgError beyond the limits above.
+void gen_correlated_field_noalloc(const int* n, int dims, const FieldParams& f, unsigned long long seed,
+                                  const double* noise, double* out, hipStream_t s);
+
 }  // namespace mgk

@@ -37,6 +37,20 @@ def test_abi_version_and_error_path_without_device():
     assert rc == 1 and b"null" in g.mgk_last_error()
 
 
+def test_noalloc_field_rejects_grids_over_its_limit_without_device():
+    from mgkmc_cpp_amd import _lib
+    g = _lib.gpu()
+    # 160^3 voxels exceed the 2^21-voxel scratch array; an axis over 2048 too
+    rc = g.mgk_correlated_field_noalloc(3, 160, 160, 160, 85.0, 12.75, 4.0, 0, 0.0, 0, 0.0, 1, None,
+                                        ctypes.c_void_p(1), None)
+    assert rc == 1 and b"no-allocation limit" in g.mgk_last_error()
+    rc = g.mgk_correlated_field_noalloc(2, 4096, 2, 1, 85.0, 12.75, 4.0, 0, 0.0, 0, 0.0, 1, None,
+                                        ctypes.c_void_p(1), None)
+    assert rc == 1 and b"no-allocation limit" in g.mgk_last_error()
+    rc = g.mgk_correlated_field_noalloc(2, 8, 8, 1, 85.0, 12.75, 4.0, 0, 0.0, 0, 0.0, 1, None, None, None)
+    assert rc == 1 and b"null" in g.mgk_last_error()
+
+
 def test_footprint_estimate_without_device():
     # pure arithmetic: per-voxel catalogs add C*N*M doubles over a shared one,
     # the KMC state grows with M, and bad shapes are rejected
