@@ -57,12 +57,37 @@ def build(verbose=False):
     host_so = os.path.join(LIB, "libmgkmc_host.so")
     tmp = host_so + ".tmp"
     _run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-Wno-unused-function",
+          "-fvisibility=hidden", "-fvisibility-inlines-hidden",
           os.path.join(HOST, "host_api.cpp"), "-o", tmp,
           "-L" + LIB, "-lmgkmc_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, host_so)
+    callers = build_dropin_callers()
     if verbose:
-        print("built", gpu_so, host_so)
+        print("built", gpu_so, host_so, *callers)
     return gpu_so, host_so
+
+
+DROPIN = os.path.join(os.path.dirname(HERE), "tests", "dropin")
+
+
+def build_dropin_callers():
+    """The reference-style caller of the drop-in C++ headers (tests/dropin),
+    on the built-in tensor type and on a second layout-identical type, linked
+    against the device library: test programs, built with the rest so that
+    GPU runs only execute them."""
+    src = os.path.join(DROPIN, "reference_style_caller.cpp")
+    if not os.path.exists(src):
+        return []
+    out_dir = os.path.join(DROPIN, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for name, extra in (("caller_builtin", []), ("caller_layout", ["-DCALLER_LAYOUT_TYPE"])):
+        exe = os.path.join(out_dir, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall"] + extra + ["-I" + HOST, "-I" + DROPIN, src, "-o", exe + ".tmp",
+              "-L" + LIB, "-lmgkmc_gpu", "-Wl,-rpath,$ORIGIN/../../../mgkmc_cpp_amd/lib"])
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
 
 
 if __name__ == "__main__":

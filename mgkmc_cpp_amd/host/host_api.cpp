@@ -10,8 +10,13 @@
 #include <type_traits>
 #include <vector>
 
+// only the C ABI is exported (the library is built with hidden visibility):
+// the header-only C++ classes it instantiates stay private to it, so a caller
+// that instantiates them with its own tensor types gets no clash
+#pragma GCC visibility push(default)
 #include "../../include/mgkmc_gpu.h"
 #include "../../include/mgkmc_host.h"
+#pragma GCC visibility pop
 #include "kmc_solver.hpp"
 
 // get_current_timestamp (declared by kmc_solver.hpp:22, defined in main.cpp:18-24)
@@ -330,7 +335,7 @@ int mgk_sim_get_macro(mgk_sim* s, double* eps_macro, double* sig_macro, double* 
     return guard([&]() -> int {
         return s->with([&](auto& sim) {
             const int F = (int)(sizeof(sim.eps_macro) / sizeof(double));
-            for (int k = 0; k < F; ++k) { eps_macro[k] = sim.eps_macro.a[k]; sig_macro[k] = sim.sig_macro.a[k]; }
+            for (int k = 0; k < F; ++k) { eps_macro[k] = sim.eps_macro.data()[k]; sig_macro[k] = sim.sig_macro.data()[k]; }
             *time = sim.time;
             return MGK_OK;
         });
@@ -341,7 +346,7 @@ int mgk_sim_set_macro(mgk_sim* s, const double* eps_macro, double time) {
     return guard([&]() -> int {
         return s->with([&](auto& sim) {
             const int F = (int)(sizeof(sim.eps_macro) / sizeof(double));
-            for (int k = 0; k < F; ++k) sim.eps_macro.a[k] = eps_macro[k];
+            for (int k = 0; k < F; ++k) sim.eps_macro.data()[k] = eps_macro[k];
             sim.time = time;
             return MGK_OK;
         });

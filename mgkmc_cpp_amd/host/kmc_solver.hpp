@@ -775,7 +775,7 @@ public:
             e.ls_iters = last_iters_;
             e.time = time;
             e.total_rate = total_rate_;
-            for (int k = 0; k < D * D; ++k) { e.eps_macro[k] = eps_macro.a[k]; e.sig_macro[k] = sig_macro.a[k]; }
+            for (int k = 0; k < D * D; ++k) { e.eps_macro[k] = eps_macro.data()[k]; e.sig_macro[k] = sig_macro.data()[k]; }
             events.push_back(e);
         }
         if (is_instab) *is_instab = inst;
@@ -1165,7 +1165,7 @@ void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& 
         for (int b = 0; b < B; ++b) {
             if (!active[b]) continue;
             sims[b]->prepare_event(forever, component, uniform_dist, draws[b], in[b]);
-            for (int k = 0; k < D * D; ++k) em[(size_t)b * D * D + k] = sims[b]->eps_macro.a[k];
+            for (int k = 0; k < D * D; ++k) em[(size_t)b * D * D + k] = sims[b]->eps_macro.data()[k];
             any = true;
         }
         if (!any) break;
@@ -1176,8 +1176,8 @@ void batch_event_steps(mgk_batch* batch, const std::vector<KmcSimulationT<D>*>& 
             KmcSimulationT<D>& sim = *sims[b];
             if (out[b].outcome == MGK_EVENT_DONE)
                 for (int k = 0; k < D * D; ++k) {
-                    sim.eps_macro.a[k] = em[(size_t)b * D * D + k];
-                    sim.sig_macro.a[k] = sm[(size_t)b * D * D + k];
+                    sim.eps_macro.data()[k] = em[(size_t)b * D * D + k];
+                    sim.sig_macro.data()[k] = sm[(size_t)b * D * D + k];
                 }
             const int outcome = sim.complete_event(draws[b], out[b], nullptr, nullptr);
             if (outcome == MGK_EVENT_UNSTABLE) {

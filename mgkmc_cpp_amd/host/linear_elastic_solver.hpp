@@ -4,9 +4,10 @@
 //
 // Same free functions and signatures as the reference (fftfreq,
 // compute_lame_2d/3d, stress_from_strain_2d/3d, green_operator_2d/3d,
-// spectral_solver_2d/3d: linear_elastic_solver.hpp:22-593); the tensor types
-// are mgkt::Matrix2d/3d, layout-identical to Eigen's.  fft2d/ifft2d/fft3d/
-// ifft3d are not provided: the device path has no host FFT.
+// fft2d/ifft2d/fft3d/ifft3d, spectral_solver_2d/3d:
+// linear_elastic_solver.hpp:22-593).  The tensor types are mgkt::Matrix2d/3d:
+// Eigen's own when Eigen is included first (mgk_tensor.hpp).  The FFT helpers
+// run on the host (the device solve has its own transforms inside its passes).
 //
 // The Gamma argument is accepted and not read: the device evaluates the Green
 // operator on the fly from the mean medium (mean E, mean nu), which is what
@@ -15,6 +16,7 @@
 #pragma once
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -158,6 +160,78 @@ inline std::vector<GreenTensor3D> green_operator_3d(int nx, int ny, int nz, cons
 }
 
 namespace mgk_host {
+
+// Host DFT of any length for the FFT helpers: recursive mixed-radix
+// Cooley-Tukey (decimation in time by the smallest prime factor), O(n sum p)
+// per line; twiddles evaluated in long double.
+inline void dft_rec(const std::complex<double>* in, long is, std::complex<double>* out, int n, int sign) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = n;
+    for (int f = 2; (long)f * f <= n; ++f)
+        if (n % f == 0) { p = f; break; }
+    const int m = n / p;
+    for (int r = 0; r < p; ++r) dft_rec(in + r * is, is * p, out + (long)r * m, m, sign);
+    std::vector<std::complex<double>> tw(n), tmp(n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int e = 0; e < n; ++e) {
+        const long double a = two_pi * (long double)e / (long double)n;
+        tw[e] = std::complex<double>((double)cosl(a), (double)(sign * sinl(a)));
+    }
+    for (int k = 0; k < m; ++k)
+        for (int q = 0; q < p; ++q) {
+            std::complex<double> acc = 0.0;
+            for (int r = 0; r < p; ++r) acc += tw[(long)r * (k + (long)m * q) % n] * out[(long)r * m + k];
+            tmp[k + (long)m * q] = acc;
+        }
+    std::copy(tmp.begin(), tmp.end(), out);
+}
+
+// c2c over every axis of a row-major array (last axis contiguous), scaled by
+// 1/sqrt(N) as the reference's ortho pocketfft calls (:130, :139, :411, :424).
+inline void c2c_axes(std::vector<std::complex<double>>& a, const std::vector<int>& shape, bool forward) {
+    long N = 1;
+    for (int n : shape) N *= n;
+    const int sign = forward ? -1 : 1;
+    std::vector<std::complex<double>> line, res;
+    long stride = N;
+    for (size_t ax = 0; ax < shape.size(); ++ax) {
+        const int n = shape[ax];
+        stride /= n;   // elements between neighbours along this axis
+        line.resize(n);
+        res.resize(n);
+        for (long outer = 0; outer < N / ((long)n * stride); ++outer)
+            for (long in = 0; in < stride; ++in) {
+                const long base = outer * n * stride + in;
+                for (int j = 0; j < n; ++j) line[j] = a[base + (long)j * stride];
+                dft_rec(line.data(), 1, res.data(), n, sign);
+                for (int j = 0; j < n; ++j) a[base + (long)j * stride] = res[j];
+            }
+    }
+    const double f = 1.0 / std::sqrt((double)N);
+    for (auto& v : a) v *= f;
+}
+
+inline void fft_real_nd(const std::vector<int>& shape, const std::vector<double>& real_field,
+                        std::vector<std::complex<double>>& complex_hat) {
+    const size_t N = real_field.size();
+    std::vector<std::complex<double>> t(N);
+    for (size_t i = 0; i < N; ++i) t[i] = std::complex<double>(real_field[i], 0.0);
+    c2c_axes(t, shape, true);
+    complex_hat.resize(N);
+    std::copy(t.begin(), t.end(), complex_hat.begin());
+}
+
+inline void ifft_real_nd(const std::vector<int>& shape, const std::vector<std::complex<double>>& complex_hat,
+                         std::vector<double>& real_field) {
+    std::vector<std::complex<double>> t(complex_hat.begin(), complex_hat.end());
+    c2c_axes(t, shape, false);
+    real_field.resize(t.size());
+    for (size_t i = 0; i < t.size(); ++i) real_field[i] = t[i].real();
+}
+
 template <int D>
 inline void spectral_solve(int nx, int ny, int nz, const std::vector<double>& E, const std::vector<double>& nu,
                            const mgkt::Matrix<D, D>& eps_bar, int plane, std::vector<mgkt::Matrix<D, D>>& eps,
@@ -180,6 +254,23 @@ inline void spectral_solve(int nx, int ny, int nz, const std::vector<double>& E,
     if (diff) *diff = df;
 }
 }  // namespace mgk_host
+
+// fft2d / ifft2d / fft3d / ifft3d (:122-143, :399-428): complex DFT of real
+// data over every axis with ortho scaling; the inverse keeps the real part.
+inline void fft2d(int nx, int ny, const std::vector<double>& real_field, std::vector<std::complex<double>>& complex_hat) {
+    mgk_host::fft_real_nd({nx, ny}, real_field, complex_hat);
+}
+inline void ifft2d(int nx, int ny, const std::vector<std::complex<double>>& complex_hat, std::vector<double>& real_field) {
+    mgk_host::ifft_real_nd({nx, ny}, complex_hat, real_field);
+}
+inline void fft3d(int nx, int ny, int nz, const std::vector<double>& real_field,
+                  std::vector<std::complex<double>>& complex_hat) {
+    mgk_host::fft_real_nd({nx, ny, nz}, real_field, complex_hat);
+}
+inline void ifft3d(int nx, int ny, int nz, const std::vector<std::complex<double>>& complex_hat,
+                   std::vector<double>& real_field) {
+    mgk_host::ifft_real_nd({nx, ny, nz}, complex_hat, real_field);
+}
 
 // spectral_solver_2d (:146-309) on the device
 inline void spectral_solver_2d(int nx, int ny, const std::vector<double>& E, const std::vector<double>& nu,

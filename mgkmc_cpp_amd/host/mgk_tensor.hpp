@@ -1,12 +1,36 @@
-// mgk_tensor.hpp -- fixed-size 2x2 / 3x3 FP64 tensors for the drop-in host
-// API.  Column-major storage with the same memory layout as
-// Eigen::Matrix2d / Matrix3d / Vector2d (the reference's member types,
-// kmc_solver.hpp:136-149), so vectors of either can be handed to the C ABI
-// through data().  Only the operations the reference's KMC and solver code
-// use are provided (Zero, Identity, (i,j), trace, squaredNorm, norm,
-// transpose, + - * /).
+// mgk_tensor.hpp -- the 2x2 / 3x3 FP64 tensor types of the drop-in host API.
+//
+// The reference's members and free functions take Eigen::Matrix2d, Matrix3d and
+// Vector2d (kmc_solver.hpp:136-149, linear_elastic_solver.hpp:146-160).  The
+// drop-in headers name them mgkt::Matrix2d, mgkt::Matrix3d, mgkt::Vector2d,
+// which are:
+//   * Eigen's own types when Eigen is included before the drop-in headers
+//     (EIGEN_WORLD_VERSION defined): a reference caller's
+//     std::vector<Eigen::Matrix3d> is passed as is;
+//   * MGK_TENSOR_TEMPLATE<R, C> when a caller defines that macro to a
+//     column-major template of its own with the same interface;
+//   * otherwise the small built-in class below.
+// Whatever the choice, the layout is Eigen's: R*C contiguous doubles in
+// column-major order, so vectors of them go to the C ABI through data().
+// The drop-in code uses only this part of Eigen's interface: Zero, Identity,
+// (i, j), (i), data, trace, squaredNorm, norm, transpose, + - += -=, scalar
+// * and /, /=.
 #pragma once
 #include <cmath>
+
+#if defined(MGK_TENSOR_TEMPLATE)
+
+namespace mgkt {
+template <int R, int C> using Matrix = MGK_TENSOR_TEMPLATE<R, C>;
+}
+
+#elif defined(EIGEN_WORLD_VERSION)
+
+namespace mgkt {
+template <int R, int C> using Matrix = Eigen::Matrix<double, R, C>;
+}
+
+#else
 
 namespace mgkt {
 
@@ -55,14 +79,20 @@ template <int R, int C> struct Matrix {
     friend Matrix operator*(double s, const Matrix& m) { return m * s; }
 };
 
+}  // namespace mgkt
+
+#endif
+
+namespace mgkt {
+
 using Matrix2d = Matrix<2, 2>;
 using Matrix3d = Matrix<3, 3>;
 using Vector2d = Matrix<2, 1>;
 using Vector3d = Matrix<3, 1>;
 
-static_assert(sizeof(Matrix2d) == 4 * sizeof(double), "layout");
-static_assert(sizeof(Matrix3d) == 9 * sizeof(double), "layout");
-static_assert(sizeof(Vector2d) == 2 * sizeof(double), "layout");
-static_assert(sizeof(Vector3d) == 3 * sizeof(double), "layout");
+static_assert(sizeof(Matrix2d) == 4 * sizeof(double), "Matrix2d: 4 contiguous doubles");
+static_assert(sizeof(Matrix3d) == 9 * sizeof(double), "Matrix3d: 9 contiguous doubles");
+static_assert(sizeof(Vector2d) == 2 * sizeof(double), "Vector2d: 2 contiguous doubles");
+static_assert(sizeof(Vector3d) == 3 * sizeof(double), "Vector3d: 3 contiguous doubles");
 
 }  // namespace mgkt
