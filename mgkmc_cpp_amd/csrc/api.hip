@@ -126,9 +126,12 @@ struct mgk_ctx {
 
     // statistics
     int last_iters = 0, last_inner = 0, predicted = 4;
-    int inner_per_outer = 12;
+    int inner_per_outer = 12;          // CG steps enqueued for a solve's first refinement cycle
     int inner_hist[4] = {0, 0, 0, 0};
     unsigned inner_pos = 0;
+    int inner_per_outer2 = 4;          // ... and for each later cycle
+    int inner_hist2[4] = {0, 0, 0, 0};
+    unsigned inner_pos2 = 0;
     bool mean_valid = false;
     double mean_eps[6] = {0, 0, 0, 0, 0, 0};
 
@@ -385,7 +388,6 @@ struct mgk_ctx {
     int run_batches(bool warm, int max_iter, double tol, double* diff_out, bool event) {
         const LsLaunch p = lsl();
         const bool refine = (precision == MGK_FP32_REFINE);
-        const int J = inner_per_outer;
         int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
         bool final_done = false;
         if (max_iter > 0) {
@@ -396,7 +398,8 @@ struct mgk_ctx {
                     const int flags = ((k == 0 && warm) ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
                     ls_enqueue_iteration(p, LS_FULL, flags, stream);
                     if (refine) {
-                        const int sp = spec_from();
+                        const int J = (k == 0) ? inner_per_outer : inner_per_outer2;
+                        const int sp = (k == 0) ? spec_from() : spec_from2();
                         for (int j = 0; j < J; ++j)
                             ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp, J), stream);
                     }
@@ -469,26 +472,47 @@ struct mgk_ctx {
             if (h > 0) mn = std::min(mn, h);
         return mn;
     }
+    int spec_from2() const {
+        int mn = 1 << 30;
+        for (int h : inner_hist2)
+            if (h > 0) mn = std::min(mn, h);
+        return mn;
+    }
     static int cg_flags(int j, int spec_from, int J) {
         const int f = j == 0 ? LS_FLAG_CG_FIRST : (j == 1 ? LS_FLAG_CG_UPDATE1 : LS_FLAG_CG_UPDATE);
         return f | (j >= spec_from ? LS_FLAG_SPEC : 0) | ((j & 1) ? LS_FLAG_ODD : 0) | (j == J - 1 ? LS_FLAG_LAST : 0) |
                (j << LS_STEP_SHIFT);
     }
 
-    // CG steps per refinement cycle for the next solve: one more than the
-    // most any of the last few single-cycle solves used (steps past
-    // convergence return at once, but each still costs a launch).
+    // CG steps per refinement cycle for the next solve, the first cycle and
+    // the later ones apart: one more than the most any of the last few cycles
+    // of that kind used (steps past convergence return at once, but each still
+    // costs a launch); a cycle that ran out of steps without a decision makes
+    // the next ones of its kind longer.
     void tune_inner(double tol) {
         (void)tol;
-        const int cycles = std::max(1, hctl->iters);
-        const int per = (hctl->inner + cycles - 1) / cycles;
-        if (hctl->done && hctl->iters <= 1) {
-            inner_hist[inner_pos++ % 4] = per;
-            int mx = 0;
-            for (int h : inner_hist) mx = std::max(mx, h);
-            inner_per_outer = std::max(4, std::min(40, mx + 1));
-        } else {
+        const Ctl& h = *hctl;
+        int steps[2] = {h.cyc_steps[0], h.cyc_steps[1]};
+        const int last = h.inner - h.cycle_start;   // the cycle the solve ended in
+        if (last > 0 && h.iters >= 1) {
+            const int k = (h.iters <= 1) ? 0 : 1;
+            steps[k] = std::max(steps[k], last);
+        }
+        if (h.cyc_exhausted[0] || (!h.done && h.iters <= 1)) {
             inner_per_outer = std::min(40, inner_per_outer + 4);
+        } else if (steps[0] > 0) {
+            inner_hist[inner_pos++ % 4] = steps[0];
+            int mx = 0;
+            for (int v : inner_hist) mx = std::max(mx, v);
+            inner_per_outer = std::max(4, std::min(40, mx + 1));
+        }
+        if (h.cyc_exhausted[1] || (!h.done && h.iters > 1)) {
+            inner_per_outer2 = std::min(40, inner_per_outer2 + 2);
+        } else if (steps[1] > 0) {
+            inner_hist2[inner_pos2++ % 4] = steps[1];
+            int mx = 0;
+            for (int v : inner_hist2) mx = std::max(mx, v);
+            inner_per_outer2 = std::max(2, std::min(40, mx + 1));
         }
     }
 };
@@ -1500,6 +1524,8 @@ struct StateHeader {
     int predicted, inner_per_outer, inner_hist[4];
     unsigned inner_pos;
     int last_iters, last_inner;
+    int inner_per_outer2, inner_hist2[4];
+    unsigned inner_pos2;
 };
 const char kStateMagic[8] = {'M', 'G', 'K', 'S', 'T', 'A', 'T', 0};
 
@@ -1545,7 +1571,7 @@ int mgk_state_save(mgk_ctx* ctx, void* buf, long long bytes) {
         MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
         StateHeader h{};
         std::memcpy(h.magic, kStateMagic, sizeof(h.magic));
-        h.version = 1;
+        h.version = 2;
         h.dims = c.g.dims; h.nx = c.g.nx; h.ny = c.g.ny; h.nz = c.g.nz; h.C = c.g.C;
         h.M = c.kmc ? c.kp.M : 0;
         h.cat_shared = c.kmc ? c.K.cat_shared : 0;
@@ -1558,6 +1584,8 @@ int mgk_state_save(mgk_ctx* ctx, void* buf, long long bytes) {
         h.predicted = c.predicted; h.inner_per_outer = c.inner_per_outer;
         for (int k = 0; k < 4; ++k) h.inner_hist[k] = c.inner_hist[k];
         h.inner_pos = c.inner_pos; h.last_iters = c.last_iters; h.last_inner = c.last_inner;
+        h.inner_per_outer2 = c.inner_per_outer2; h.inner_pos2 = c.inner_pos2;
+        for (int k = 0; k < 4; ++k) h.inner_hist2[k] = c.inner_hist2[k];
         char* out = static_cast<char*>(buf);
         std::memcpy(out, &h, sizeof(h));
         out += sizeof(h);
@@ -1578,7 +1606,7 @@ int mgk_state_load(mgk_ctx* ctx, const void* buf, long long bytes) {
         StateHeader h;
         if (bytes < (long long)sizeof(h)) throw ArgError("state buffer too small");
         std::memcpy(&h, buf, sizeof(h));
-        if (std::memcmp(h.magic, kStateMagic, sizeof(h.magic)) != 0 || h.version != 1)
+        if (std::memcmp(h.magic, kStateMagic, sizeof(h.magic)) != 0 || h.version != 2)
             throw ArgError("not a trajectory state (bad magic or version)");
         if (h.dims != c.g.dims || h.nx != c.g.nx || h.ny != c.g.ny || h.nz != c.g.nz || h.C != c.g.C)
             throw ArgError("state is for another grid");
@@ -1615,6 +1643,8 @@ int mgk_state_load(mgk_ctx* ctx, const void* buf, long long bytes) {
         c.predicted = h.predicted; c.inner_per_outer = h.inner_per_outer;
         for (int k = 0; k < 4; ++k) c.inner_hist[k] = h.inner_hist[k];
         c.inner_pos = h.inner_pos; c.last_iters = h.last_iters; c.last_inner = h.last_inner;
+        c.inner_per_outer2 = h.inner_per_outer2; c.inner_pos2 = h.inner_pos2;
+        for (int k = 0; k < 4; ++k) c.inner_hist2[k] = h.inner_hist2[k];
         c.kctl_pending = false;
         MGK_HIP_CHECK(hipDeviceSynchronize());
         return MGK_OK;
@@ -1725,7 +1755,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     const Geom g = c0.g;
     const int Dd = c0.D(), F = Dd * Dd, C = g.C;
     const bool refine = (c0.precision == MGK_FP32_REFINE);
-    int max_iter = 0, J = 1, predicted = 1, sp = 1 << 30;
+    int max_iter = 0, J = 1, J2 = 1, predicted = 1, sp = 1 << 30, sp2 = 1 << 30;
     double tol = 0.0;
     int first_mode = 0, n_modes = -1, has_cat = 0, has_q0 = 0;
     bool all_keep = true;
@@ -1766,7 +1796,8 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
             h.done = 1;
             h.skip_final = 1;
         }
-        h.cycle_len = c.inner_per_outer;   // this trajectory's own cycle, as when it runs alone
+        h.cycle_len = c.inner_per_outer;   // this trajectory's own cycles, as when it runs alone
+        h.cycle_len2 = c.inner_per_outer2;
         const int keep = c.mean_valid ? 1 : 0;
         if (on) all_keep = all_keep && keep;
         h.keep_shift = keep;
@@ -1799,7 +1830,9 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         ea.active = on ? 1 : 0;
         b.h_times[y] = on ? e.time : 0.0;
         J = std::max(J, c.inner_per_outer);
+        J2 = std::max(J2, c.inner_per_outer2);
         sp = std::min(sp, c.spec_from());
+        sp2 = std::min(sp2, c.spec_from2());
         predicted = std::max(predicted, c.predicted);
     }
     if (n_active == 0) throw ArgError("no active trajectory in the batch call");
@@ -1812,8 +1845,10 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
             const int k = from + q;
             const int flags = (k == 0 ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
             ls_enqueue_iteration(L, LS_FULL, flags, st);
-            if (refine)
-                for (int j = 0; j < J; ++j) ls_enqueue_iteration(L, LS_CG, mgk_ctx::cg_flags(j, sp, J), st);
+            if (refine) {
+                const int Jk = (k == 0) ? J : J2, spk = (k == 0) ? sp : sp2;
+                for (int j = 0; j < Jk; ++j) ls_enqueue_iteration(L, LS_CG, mgk_ctx::cg_flags(j, spk, Jk), st);
+            }
         }
         ls_enqueue_final(L, st);
         MGK_HIP_CHECK(hipMemcpyAsync(b.h_ctl, b.d_ctl, sizeof(Ctl) * B, hipMemcpyDeviceToHost, st));
@@ -1838,7 +1873,8 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         iterations(0, nb0, st);
     };
     if (b.use_graphs) {
-        const std::vector<int> key{J, nb0, all_keep ? 1 : 0, n_modes, first_mode, has_cat, has_q0, refine ? 1 : 0, std::min(sp, J)};
+        const std::vector<int> key{J,       nb0,         all_keep ? 1 : 0, n_modes, first_mode, has_cat, has_q0,
+                                   refine ? 1 : 0, std::min(sp, J), J2, std::min(sp2, J2)};
         auto it = b.graphs.find(key);
         if (it == b.graphs.end()) {
             std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);

@@ -85,6 +85,11 @@ struct Ctl {
     // and reads slot (j - 1) & 1, which no block of that kernel writes
     double cgs_gamma[2], cgs_alpha[2], cgs_beta[2];
     int cgs_inner[2];
+    int cycle_len2;        // CG steps of the later cycles (after the second FP64 iteration; 0: as enqueued)
+    int cycle_start;       // inner count at the start of the current cycle
+    int cyc_steps[2];      // most CG steps of a finished first cycle, of a finished later cycle
+    int cyc_exhausted[2];  // a first / later cycle ran out of steps without a decision
+    int pad_;
     double cycle_r0;       // relative FP64 increment that started the current refinement cycle
 };
 

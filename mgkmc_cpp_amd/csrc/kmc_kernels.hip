@@ -1020,6 +1020,7 @@ __device__ __forceinline__ void event_prep_body(const KmcParams& p, Ctl* ctl, co
         for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(double)); ++k) w[k] = 0.0;
         ctl->max_iter = ea.max_iter;
         ctl->cycle_len = ea.cycle_len;
+        ctl->cycle_len2 = ea.cycle_len2;
         ctl->tol = ea.tol;
         ctl->keep_shift = ea.keep_shift;
         for (int c = 0; c < 6; ++c) ctl->mean_eps[c] = ea.mean_eps[c];

@@ -61,7 +61,7 @@ struct EventArgs {
     int init_ctl;          // 1: the attempt resets the LS control block itself (max_iter, tol, mean)
     int max_iter;
     int cycle_len;         // CG steps per refinement cycle of this trajectory (0: all enqueued)
-    int pad_;
+    int cycle_len2;        // the same for its later cycles
     double tol;
     double mean_eps[6];    // field mean of the last solve (used when keep_shift)
 };

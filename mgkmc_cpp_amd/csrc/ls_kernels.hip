@@ -76,8 +76,14 @@ template <bool BATCH, typename T> __device__ __forceinline__ const T& param_sel(
 // A CG step past this solve's cycle length (a batch enqueues its longest
 // trajectory's cycle; the others skip the extra steps, as they would not run
 // them alone).
+// The CG steps of the current refinement cycle: the first cycle (after the
+// first FP64 iteration) and the later ones have lengths of their own.
+__device__ __forceinline__ int cycle_len_now(const Ctl* ctl) {
+    return ctl->iters <= 1 ? ctl->cycle_len : ctl->cycle_len2;
+}
 __device__ __forceinline__ bool past_cycle(const Ctl* ctl, int step) {
-    return ctl->cycle_len > 0 && step >= ctl->cycle_len;
+    const int len = cycle_len_now(ctl);
+    return len > 0 && step >= len;
 }
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
@@ -976,7 +982,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;   // groups per lane
     using cx = cplx<float>;
     // step == cycle_len: only the folded reduction of the cycle's last step
-    const int step = flags >> LS_STEP_SHIFT, cyc = p.ctl->cycle_len;
+    const int step = flags >> LS_STEP_SHIFT, cyc = cycle_len_now(p.ctl);
     const bool at_end = cyc > 0 && step == cyc;
     const bool stop = (p.ctl->done | p.ctl->cg_stop | (cyc > 0 && step > cyc)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
@@ -1218,6 +1224,14 @@ __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const L
             ctl->norm_sq = tot[1];
             ctl->eps_norm = sqrt(tot[2] + ctl->norm_extra);
             ctl->cycle_r0 = diff;
+            if (ctl->iters >= 1 && ctl->inner > ctl->cycle_start) {
+                // the CG cycle before this iteration: its length, and whether it
+                // ran out of steps (no convergence, verification or breakdown stop)
+                const int k = (ctl->iters == 1) ? 0 : 1;
+                ctl->cyc_steps[k] = max(ctl->cyc_steps[k], ctl->inner - ctl->cycle_start);
+                if (!ctl->cg_stop) ctl->cyc_exhausted[k] = 1;
+            }
+            ctl->cycle_start = ctl->inner;
             if (ctl->iters + ctl->inner < 32) ctl->diff_hist[ctl->iters + ctl->inner] = diff;
             ctl->iters += 1;
             ctl->corr_valid = 0;   // a pending correction was absorbed into eps
