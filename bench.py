@@ -135,7 +135,40 @@ def run_gpu(precision, steps, warmup, E, nu):
     return sim, info
 
 
-def kernel_roofline(sim, info32):
+# Bytes per voxel the persistent CG cycle moves: per step the spectrum out
+# (F), read and written by pass C (G) and read back (I): 4 x 6 components x
+# (nl/2+1)/nl bins x 8 B; per launch r, the tau factor's dlam and dmu (read
+# by the plane blocks of their components) in, r and x out.
+def _cycle_kernel_bytes(steps_per_launch, nl):
+    h = (nl // 2 + 1) / nl
+    return steps_per_launch * 4 * 6 * h * 8 + (6 * 4 + 6 * 4 + 4) + 2 * 6 * 4
+
+
+def cycle_timing(E, nu, events=40, warmup=5):
+    """The persistent CG cycle in situ: `events` FP32+refine events of the
+    same workload with every cycle launch timed by HIP events on the context's
+    stream (mgk_cycle_timer); None when the grid does not run it."""
+    from mgkmc_cpp_amd import Simulation, abi, configs
+    from mgkmc_cpp_amd.solver import cycle_supported, cycle_timer, cycle_timer_read
+    cfg = configs.sim_config(SHAPE, precision=abi.MGK_FP32_REFINE, seed=42)
+    sim = Simulation(cfg, E, nu)
+    h = sim.device_context_handle()
+    if not cycle_supported(h):
+        sim.close()
+        return None
+    _preload(sim, 3)
+    sim.event_steps(warmup)
+    cycle_timer(h, True)
+    st = sim.event_steps(events)
+    ms, n, steps = cycle_timer_read(h)
+    cycle_timer(h, False)
+    sim.close()
+    if n == 0:
+        return None
+    return dict(ms=ms / n, launches=n, steps=steps, events=int(st.kmc_events))
+
+
+def kernel_roofline(sim, info32, cyc=None):
     """Average launch time of every pipeline pass (HIP events on the
     context's stream), weighted by its launches per event in the FP32+refine
     run; the pass with the largest time per event is reported.  `achieved`
@@ -155,6 +188,19 @@ def kernel_roofline(sim, info32):
                              fp32=fp32, ms=round(ms, 5), model_bytes=int(mb), bytes=int(b),
                              launches_per_event=round(n, 2), ms_per_event=round(ms * n, 5),
                              model_gbs=round(mb / (ms * 1e-3) / 1e9, 1), gbs=round(b / (ms * 1e-3) / 1e9, 1)))
+    if cyc is not None:
+        # the persistent cycle replaces the plane passes' launches: survey
+        # bytes of its CG steps (b_iter per step) per launch
+        spl = cyc["steps"] / cyc["launches"]
+        mb = spl * ((7 * 6 + 2) * 4 + 8 * 6) * N
+        b = _cycle_kernel_bytes(spl, nl) * N
+        n = cyc["launches"] / max(cyc["events"], 1)
+        ms = cyc["ms"]
+        rows = [r for r in rows if r["pass_id"] not in (6, 7, 8)]
+        rows.append(dict(kernel="k_cg_cycle<64>", pass_id="cycle", fp32=True, ms=round(ms, 5), model_bytes=int(mb),
+                         bytes=int(b), launches_per_event=round(n, 3), ms_per_event=round(ms * n, 5),
+                         steps_per_launch=round(spl, 3), model_gbs=round(mb / (ms * 1e-3) / 1e9, 1),
+                         gbs=round(b / (ms * 1e-3) / 1e9, 1)))
     top = max(rows, key=lambda r: r["ms_per_event"])
     achieved = top["model_bytes"] / (top["ms"] * 1e-3) / 1e9
     own = top["bytes"] / (top["ms"] * 1e-3) / 1e9
@@ -162,7 +208,9 @@ def kernel_roofline(sim, info32):
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_measured_traffic(top["kernel"]),
                 traffic_source="profiles/traffic_64cube.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same "
                                "pass, calibrated on a device copy (not measured in this run)",
-                bytes_model="survey three-pass model, bytes per voxel " + str(_model_bytes_per_voxel(top["pass_id"], top["fp32"])),
+                bytes_model=("survey three-pass model: b_iter 224 B per voxel per CG step, "
+                             f"{top['steps_per_launch']} steps per launch" if top["pass_id"] == "cycle" else
+                             "survey three-pass model, bytes per voxel " + str(_model_bytes_per_voxel(top["pass_id"], top["fp32"]))),
                 kernel_bytes_per_launch=int(top["bytes"]), kernel_gbs=round(own, 1),
                 kernel_frac=round(own / HBM_PEAK_GBS, 4))
     return top, rows, roof
@@ -253,6 +301,13 @@ def probe():
             ctx.time_ls_pass(pass_id, fp32=fp32, reps=4)
             print(f"[probe] pass {pass_id} {'fp32' if fp32 else 'fp64'}: 5 launches", flush=True)
     ctx.close()
+    # the persistent CG cycle runs inside events: 10 FP32+refine events
+    from mgkmc_cpp_amd import Simulation
+    sim = Simulation(configs.sim_config(SHAPE, precision=abi.MGK_FP32_REFINE, seed=42), E, nu)
+    _preload(sim, 3)
+    sim.event_steps(10)
+    sim.close()
+    print("[probe] 10 FP32+refine events", flush=True)
 
 
 def cpu_baseline(E, nu, budget_s=15.0):
@@ -420,7 +475,9 @@ def main():
     print(f"[bench] FP32+refine run", file=sys.stderr, flush=True)
     sim32, i32 = run_gpu(abi.MGK_FP32_REFINE, a.steps, a.warmup, E, nu)
     print(f"[bench] FP32+refine {i32}", file=sys.stderr, flush=True)
-    top, rows, roof = kernel_roofline(sim32, i32)
+    cyc = cycle_timing(E, nu)
+    print(f"[bench] persistent CG cycle {cyc}", file=sys.stderr, flush=True)
+    top, rows, roof = kernel_roofline(sim32, i32, cyc)
     print(f"[bench] pass timings {json.dumps(rows)}", file=sys.stderr, flush=True)
     roof["kernel"] = top["kernel"]
     roof["bytes_per_launch"] = int(top["model_bytes"])

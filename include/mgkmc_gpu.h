@@ -277,6 +277,22 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms);
  * strain field undefined: a measurement context, not a simulation one. */
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms);
 
+/* Persistent CG cycles (no counterpart in the reference, whose
+ * spectral_solver_3d runs the basic scheme, linear_elastic_solver.hpp:494-581):
+ * on an FP32+refine context whose grid is cubic with 64-point planes, each
+ * refinement cycle's CG steps run as one cooperative launch that keeps the CG
+ * vectors on chip (iterates equal to the separately launched plane passes,
+ * bit for bit; the environment variable MGK_LS_CYCLE=0 turns it off).
+ * mgk_cycle_supported: 1 when this context's solves use it.
+ * mgk_cycle_timer(on): measurement helper; resets the counters and, when on,
+ * times every cycle launch of the context's solves with HIP events on the
+ * context's stream (a solve whose refinement batch holds one cycle);
+ * mgk_cycle_timer_read: the summed duration in ms, the timed launches and the
+ * CG steps they ran. */
+int mgk_cycle_supported(mgk_ctx* ctx, int* on);
+int mgk_cycle_timer(mgk_ctx* ctx, int on);
+int mgk_cycle_timer_read(mgk_ctx* ctx, double* total_ms, long long* launches, long long* steps);
+
 /* Statistics of the last LS solve: FP64 iterations, FP32 inner iterations. */
 int mgk_last_solve_stats(mgk_ctx* ctx, int* fp64_iters, int* fp32_iters);
 

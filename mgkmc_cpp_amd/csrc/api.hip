@@ -96,6 +96,11 @@ struct mgk_ctx {
     int precision = MGK_FP64;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // in-situ timing of the persistent CG cycle launches (mgk_cycle_timer)
+    hipEvent_t cyc_ev0 = nullptr, cyc_ev1 = nullptr;
+    bool cyc_timing = false;
+    double cyc_ms = 0.0;
+    long long cyc_launches = 0, cyc_steps = 0;
     int device = 0;
 
     // material
@@ -206,10 +211,13 @@ struct mgk_ctx {
         fh(h_lsp); fh(h_kp); fh(h_ea);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (cyc_ev0) (void)hipEventDestroy(cyc_ev0);
+        if (cyc_ev1) (void)hipEventDestroy(cyc_ev1);
         if (stream) (void)hipStreamDestroy(stream);
         E = nullptr;
         stream = nullptr;
         ev0 = ev1 = nullptr;
+        cyc_ev0 = cyc_ev1 = nullptr;
     }
 
     int D() const { return g.dims; }
@@ -263,6 +271,8 @@ struct mgk_ctx {
         }
         p.ctl = ctl;
         p.partials = partials;
+        p.gbar = (precision == MGK_FP32_REFINE && ls_cycle_supported(g))
+                     ? reinterpret_cast<unsigned*>(partials + ls_gbar_offset(g)) : nullptr;
         return p;
     }
 
@@ -390,18 +400,27 @@ struct mgk_ctx {
         const bool refine = (precision == MGK_FP32_REFINE);
         int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
         bool final_done = false;
+        int inner_before = 0;   // CG steps counted before this batch
         if (max_iter > 0) {
             int batch = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
             while (true) {
+                // a batch with a single cycle launch is timed when asked
+                const bool timed = cyc_timing && refine && p.h.gbar && batch == 1;
                 for (int b = 0; b < batch; ++b) {
                     const int k = enqueued + b;
                     const int flags = ((k == 0 && warm) ? LS_FLAG_SHIFT : 0) | ((refine && k > 0) ? LS_FLAG_CORR : 0);
                     ls_enqueue_iteration(p, LS_FULL, flags, stream);
                     if (refine) {
                         const int J = (k == 0) ? inner_per_outer : inner_per_outer2;
-                        const int sp = (k == 0) ? spec_from() : spec_from2();
-                        for (int j = 0; j < J; ++j)
-                            ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp, J), stream);
+                        if (p.h.gbar) {
+                            if (timed) MGK_HIP_CHECK(hipEventRecord(cyc_ev0, stream));
+                            ls_enqueue_cycle(p, J, stream);   // the whole cycle in one launch
+                            if (timed) MGK_HIP_CHECK(hipEventRecord(cyc_ev1, stream));
+                        } else {
+                            const int sp = (k == 0) ? spec_from() : spec_from2();
+                            for (int j = 0; j < J; ++j)
+                                ls_enqueue_iteration(p, LS_CG, cg_flags(j, sp, J), stream);
+                        }
                     }
                 }
                 enqueued += batch;
@@ -413,6 +432,15 @@ struct mgk_ctx {
                                              hipMemcpyDeviceToHost, stream));
                 kctl_pending = false;
                 MGK_HIP_CHECK(hipStreamSynchronize(stream));
+                if (hctl->cg_err) throw HipError("persistent CG cycle: grid barrier timed out");
+                if (timed && hctl->inner > inner_before) {   // the cycle ran (not converged before it)
+                    float ms = 0.f;
+                    MGK_HIP_CHECK(hipEventElapsedTime(&ms, cyc_ev0, cyc_ev1));
+                    cyc_ms += ms;
+                    cyc_launches += 1;
+                    cyc_steps += hctl->inner - inner_before;
+                }
+                inner_before = hctl->inner;
                 if (hctl->done || enqueued >= max_iter) { final_done = true; break; }
                 batch = std::min(max_iter - enqueued, std::max(refine ? 1 : 2, batch / 2));
             }
@@ -551,6 +579,8 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
             MGK_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             MGK_HIP_CHECK(hipEventCreate(&c->ev0));
             MGK_HIP_CHECK(hipEventCreate(&c->ev1));
+            MGK_HIP_CHECK(hipEventCreate(&c->cyc_ev0));
+            MGK_HIP_CHECK(hipEventCreate(&c->cyc_ev1));
             const long N = g.N;
             // material: mean medium and per-voxel Lame constants
             // (linear_elastic_solver.hpp:163-178, :447-462)
@@ -1272,6 +1302,35 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
     });
 }
 
+int mgk_cycle_timer(mgk_ctx* ctx, int on) {
+    return guarded([&]() -> int {
+        if (!ctx) throw ArgError("null context");
+        ctx->cyc_timing = on != 0;
+        ctx->cyc_ms = 0.0;
+        ctx->cyc_launches = 0;
+        ctx->cyc_steps = 0;
+        return MGK_OK;
+    });
+}
+
+int mgk_cycle_timer_read(mgk_ctx* ctx, double* total_ms, long long* launches, long long* steps) {
+    return guarded([&]() -> int {
+        if (!ctx || !total_ms || !launches || !steps) throw ArgError("null argument");
+        *total_ms = ctx->cyc_ms;
+        *launches = ctx->cyc_launches;
+        *steps = ctx->cyc_steps;
+        return MGK_OK;
+    });
+}
+
+int mgk_cycle_supported(mgk_ctx* ctx, int* on) {
+    return guarded([&]() -> int {
+        if (!ctx || !on) throw ArgError("null argument");
+        *on = (ctx->precision == MGK_FP32_REFINE && ls_cycle_supported(ctx->g)) ? 1 : 0;
+        return MGK_OK;
+    });
+}
+
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
     return guarded([&]() -> int {
         if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
@@ -1803,6 +1862,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
         h.keep_shift = keep;
         LsParams lp = c.ls_params();
         lp.ctl = b.d_ctl + y;
+        lp.gbar = nullptr;   // batches run their CG steps as plane passes
         b.h_lsp[y] = lp;
         KmcParams kp = c.K;
         kp.kctl = b.d_kctl + y;

@@ -89,7 +89,7 @@ struct Ctl {
     int cycle_start;       // inner count at the start of the current cycle
     int cyc_steps[2];      // most CG steps of a finished first cycle, of a finished later cycle
     int cyc_exhausted[2];  // a first / later cycle ran out of steps without a decision
-    int pad_;
+    int cg_err;            // a persistent CG cycle's grid barrier timed out
     double cycle_r0;       // relative FP64 increment that started the current refinement cycle
 };
 

@@ -231,18 +231,25 @@ __device__ __forceinline__ void run_stages(cplx<Real>* line, int es, int t, bool
 // of N + 1 spreads over distinct banks.  With the lane index fastest instead,
 // the T lanes of a line and the lines beside them meet on the same banks
 // (element t + 8r of line l sits on bank l + t for N = 64, T = 8).
+// lds_fft_lane: the same with the lane index `tid` of `nthr` lanes given by
+// the caller.
 template <int N, int DIR, typename Real>
-__device__ __forceinline__ void lds_fft(cplx<Real>* base, int nlines, int ls, int es, const cplx<Real>* tw) {
+__device__ __forceinline__ void lds_fft_lane(cplx<Real>* base, int nlines, int ls, int es, const cplx<Real>* tw, int tid,
+                                             int nthr) {
     constexpr int T = FftPlan<N>::threads;
-    const int nthr = blockDim.x;
     const int lines_per_pass = nlines < nthr / T ? nlines : nthr / T;
     for (int l0 = 0; l0 < nlines; l0 += lines_per_pass) {
-        const int l = l0 + (int)threadIdx.x % lines_per_pass;
-        const int t = (int)threadIdx.x / lines_per_pass;
+        const int l = l0 + tid % lines_per_pass;
+        const int t = tid / lines_per_pass;
         const bool active = (t < T) && (l < nlines);
         cplx<Real>* line = base + (active ? l : 0) * ls;
         run_stages<N, 0, 1, DIR, Real>(line, es, t, active, tw);
     }
+}
+
+template <int N, int DIR, typename Real>
+__device__ __forceinline__ void lds_fft(cplx<Real>* base, int nlines, int ls, int es, const cplx<Real>* tw) {
+    lds_fft_lane<N, DIR, Real>(base, nlines, ls, es, tw, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -55,6 +55,7 @@
 #include "fft_lds.hpp"
 #include "ls_kernels.hpp"
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <type_traits>
 
@@ -113,32 +114,129 @@ template <int K> __device__ __forceinline__ void block_sum(double (&v)[K], doubl
     __syncthreads();
 }
 
-// Fixed-order reduction of part[b*K + k] over blocks (one block, plain loads:
-// the partials were written by an earlier kernel on the same stream).
-template <int K> __device__ __forceinline__ void reduce_partials(const double* part, int nblocks, double (&out)[K],
-                                                                 double* scratch) {
+// Fixed-order reduction of part[b*K + k] over blocks by the first VT lanes of
+// one block (the others add zeros), loads through ld(i).  With VT = 256 a
+// wider block sums in exactly the order of a 256-lane one.
+template <int K, class Ld>
+__device__ __forceinline__ void reduce_partials_ld(const Ld& ld, int nblocks, int VT, double (&out)[K], double* scratch) {
     constexpr int U = 4;   // partial rows per lane loaded together
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int b0 = threadIdx.x; b0 < nblocks; b0 += U * blockDim.x) {
+    const int b_first = ((int)threadIdx.x < VT) ? (int)threadIdx.x : nblocks;
+    for (int b0 = b_first; b0 < nblocks; b0 += U * VT) {
         double t[U][K];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int b = b0 + u * (int)blockDim.x;
+            const int b = b0 + u * VT;
             const long bb = (b < nblocks) ? b : 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) t[u][k] = part[bb * K + k];
+            for (int k = 0; k < K; ++k) t[u][k] = ld(bb * K + k);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (b0 + u * (int)blockDim.x < nblocks)
+            if (b0 + u * VT < nblocks)
 #pragma unroll
                 for (int k = 0; k < K; ++k) v[k] += t[u][k];
     }
     block_sum<K>(v, scratch);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = v[k];
+}
+
+// The same with plain loads over the whole block (the partials were written
+// by an earlier kernel on the same stream).
+template <int K> __device__ __forceinline__ void reduce_partials(const double* part, int nblocks, double (&out)[K],
+                                                                 double* scratch) {
+    reduce_partials_ld<K>([part](long i) { return part[i]; }, nblocks, (int)blockDim.x, out, scratch);
+}
+
+// ---------------------------------------------------------------------------
+// Device-coherent accesses and the grid barrier of a cooperative launch.
+// The blocks of one persistent launch exchange the spectrum and the reduction
+// partials through buffer loads and stores with the sc1 cache-policy bit
+// (device scope): they bypass the XCD-private L2 caches, so a grid barrier
+// needs no release/acquire fence, which on MI355X writes back and invalidates
+// the L2 and costs several microseconds per barrier.  Buffer accesses are
+// also bounds-checked by the hardware against the resource's size.
+// ---------------------------------------------------------------------------
+constexpr int kCohPolicy = 16;                 // the sc1 bit of the buffer instructions' cache policy
+constexpr int kBufferWord3 = 0x00020000;       // raw (untyped, unswizzled) buffer resource on gfx9
+
+struct CohBuf {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ CohBuf(const void* base, unsigned bytes)
+        : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufferWord3)) {}
+    __device__ __forceinline__ cplx<float> ldc(long i) const {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(i * 8), 0, kCohPolicy);
+        return __builtin_bit_cast(cplx<float>, v);
+    }
+    __device__ __forceinline__ void stc(long i, cplx<float> x) const {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0);   // only for its type
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(v), x), rs, (int)(i * 8), 0, kCohPolicy);
+    }
+    __device__ __forceinline__ double ldd(long i) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(i * 8), 0, kCohPolicy));
+    }
+    __device__ __forceinline__ void std_(long i, double x) const {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0);   // only for its type
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(v), x), rs, (int)(i * 8), 0, kCohPolicy);
+    }
+};
+
+// A strided line of complex FP32 values through a CohBuf (the column
+// accessor of the F pass inside the persistent launch).
+struct CohLine {
+    CohBuf b;
+    long off, es;
+    __device__ __forceinline__ cplx<float> operator()(int j) const { return b.ldc(off + j * es); }
+    __device__ __forceinline__ void operator()(int j, cplx<float> v) const { b.stc(off + j * es, v); }
+};
+
+// Grid barrier words (unsigned, one 128-byte line each): [0] groups arrived,
+// [32 (1 + x)] blocks of group x arrived, [32 (9 + x)] release epoch of group
+// x, [32 * 17] error flag.  Group x holds the blocks with blockIdx.x % 8 == x
+// (the XCD the block runs on), so at most 8 counters and 8 flags are shared
+// across XCDs.  The words are zeroed before every launch that uses them.
+constexpr int kBarWords = 32 * 18;
+constexpr unsigned kSpinLimit = 1u << 21;      // polls before a barrier gives up (seconds)
+
+// Every block of the launch calls this with the same epoch (1, 2, ...).  A
+// lane's stores are performed before its block arrives (s_waitcnt vmcnt(0)),
+// and the data exchanged across the barrier moves through CohBuf accesses.
+// Returns false on a timeout (the error flag is set and every later barrier
+// returns false at once), so every block leaves a failed launch.
+__device__ __forceinline__ bool grid_sync(unsigned* w, unsigned epoch, int* s_flag) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nb = gridDim.x;
+        const unsigned ng = nb < 8 ? nb : 8u;
+        const unsigned x = blockIdx.x % 8;
+        const unsigned nx = nb / 8 + (x < nb % 8 ? 1u : 0u);
+        int err = __hip_atomic_load(w + 32 * 17, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        if (!err) {
+            const unsigned old = __hip_atomic_fetch_add(w + 32 * (1 + x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1 == epoch * nx) {
+                const unsigned top = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (top + 1 == epoch * ng)
+                    for (unsigned q = 0; q < ng; ++q)
+                        __hip_atomic_store(w + 32 * (9 + q), epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            unsigned spins = 0;
+            while (__hip_atomic_load(w + 32 * (9 + x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                if (++spins > kSpinLimit || __hip_atomic_load(w + 32 * 17, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(w + 32 * 17, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    err = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        *s_flag = err;
+    }
+    __syncthreads();
+    return *s_flag == 0;
 }
 
 template <typename SR> __device__ __forceinline__ void stage_twiddles(cplx<SR>* dst, const cplx<SR>* src, int n) {
@@ -558,45 +656,49 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // f).  A block owns G frequency points (m, l-bin) with all C components, i.e.
 // L = G*C consecutive columns; lane (l, t) = (tid % L, tid / L).
 // ---------------------------------------------------------------------------
-template <typename SR, int NF, int C, bool BATCH, bool PLANE = false>
-__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
-                                                                int speculative, int step) {
-    const LsParams& p = param_sel<BATCH>(p_arg, P);
+// Lane roles of the F pass for a block owning the G points from xi0 on.
+// Both spectrum layouts keep the C components of a run of points
+// contiguous, so lane l reads column xi0*C + l either way: interleaved, lane l
+// is (point l / C, component l % C); plane-tiled (tiles of kPlaneTile points
+// per component), lane l is (point (l / (C*T8))*T8 + l % T8, component
+// (l % (C*T8)) / T8).
+template <typename SR, int NF, int C, bool PLANE> struct GammaLanes {
+    static constexpr int T = FftPlan<NF>::threads;
+    static constexpr int G = gamma_points<SR, NF, C>();
+    static constexpr int L = G * C;
+    static constexpr int T8 = kPlaneTile;
+    int l, t, gl, cl;
+    bool active;   // lanes past the FFT lanes serve the Green operator only
+    __device__ __forceinline__ GammaLanes(long xi0, long nxi, int tid) {
+        l = tid % L;
+        t = tid / L;
+        gl = PLANE ? (l / (C * T8)) * T8 + l % T8 : l / C;
+        cl = PLANE ? (l % (C * T8)) / T8 : l % C;
+        active = t < T && xi0 + gl < nxi;
+    }
+};
+
+// The F pass of one block after its column gather: forward c2c along F into
+// the block's LDS lines, eps_hat = -Gamma0_eff(xi):tau_hat / N per point, the
+// inverse c2c along F, stored through `column` (global memory or CohLine).
+// `tw`: the F twiddles in LDS; `work`: G*C padded lines of LDS.
+template <typename SR, int NF, int C, bool PLANE, class Col>
+__device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threads], const GammaLanes<SR, NF, C, PLANE>& ln,
+                                           const Col& column, const cplx<SR>* tw, cplx<SR>* work, long xi0,
+                                           const Geom& g, double lam0, double mu0, int tid, int nthr) {
     constexpr int D = (C == 3) ? 2 : 3;
-    constexpr int T = FftPlan<NF>::threads;
     constexpr int G = gamma_points<SR, NF, C>();
-    constexpr int L = G * C;
     constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
-    const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
-    if (speculative && stop) return;   // a speculative step: skip the loads
-    cx* tw = reinterpret_cast<cx*>(smem_raw);
-    cx* work = tw + NF;
-    const Geom g = p.g;
     const long nxi = (long)g.nm * g.nlh;
-    const int l = threadIdx.x % L, t = threadIdx.x / L;
-    const long xi0 = (long)blockIdx.x * G;
-    cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
-    // Both layouts keep the C components of a run of points contiguous, so
-    // lane l reads column xi0*C + l either way: interleaved, lane l is
-    // (point l / C, component l % C); plane-tiled (tiles of kPlaneTile points
-    // per component), lane l is (point (l / (C*T8))*T8 + l % T8, component
-    // (l % (C*T8)) / T8).
-    constexpr int T8 = kPlaneTile;
-    const int gl = PLANE ? (l / (C * T8)) * T8 + l % T8 : l / C;
-    const int cl = PLANE ? (l % (C * T8)) / T8 : l % C;
-    const bool active = t < T && xi0 + gl < nxi;   // lanes past the FFT lanes serve the Green operator only
-    const LineRef<SR> column{spec + (active ? xi0 * C + l : 0), nxi * C};
-    const LineRef<SR> mine{work + (gl * C + cl) * NFP, 1};
-    cx v[NF / T];
-    line_gather<NF, SR>(column, t, active, v);
-    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
-    if (stop) return;
+    const int t = ln.t;
+    const bool active = ln.active;
+    const LineRef<SR> mine{work + (ln.gl * C + ln.cl) * NFP, 1};
     line_finish<NF, -1, SR, false, true>(v, t, active, mine, mine, tw);
     __syncthreads();
 
-    const SR a = (SR)(0.5 / p.mu0);
-    const SR B = (SR)((p.lam0 + p.mu0) / (p.mu0 * (p.lam0 + 2.0 * p.mu0)));
+    const SR a = (SR)(0.5 / mu0);
+    const SR B = (SR)((lam0 + mu0) / (mu0 * (lam0 + 2.0 * mu0)));
     const SR scale = (SR)(1.0 / (double)g.N);
     // signed frequencies k/n as fftfreq (linear_elastic_solver.hpp:22-33)
     auto sfreq = [](int k, int n, bool& nyq) -> SR {
@@ -607,12 +709,12 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     // The block has a multiple of G lanes (gamma_threads), so a lane keeps
     // its point gg for every f it visits: the point's M and L frequencies
     // are computed once.
-    const int gg = (int)threadIdx.x % G;
+    const int gg = tid % G;
     const long xi = xi0 + gg;
     const int km = xi < nxi ? (int)(xi / g.nlh) : 0, kl = xi < nxi ? (int)(xi % g.nlh) : 0;
     bool pm_nyq, pl_nyq;
     const SR pm = sfreq(km, g.nm, pm_nyq), pl = sfreq(kl, g.nl, pl_nyq);
-    for (int f = (int)threadIdx.x / G; f < NF && xi < nxi; f += (int)blockDim.x / G) {
+    for (int f = tid / G; f < NF && xi < nxi; f += nthr / G) {
         bool nf_, nm_, nl_;
         SR qa[3];
         qa[0] = sfreq(f, NF, nf_);
@@ -679,6 +781,29 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     __syncthreads();
     line_gather<NF, SR>(mine, t, active, v);
     line_finish<NF, +1, SR, true, false>(v, t, active, column, mine, tw);
+}
+
+template <typename SR, int NF, int C, bool BATCH, bool PLANE = false>
+__global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
+                                                                int speculative, int step) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    constexpr int G = gamma_points<SR, NF, C>();
+    using cx = cplx<SR>;
+    const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
+    if (speculative && stop) return;   // a speculative step: skip the loads
+    cx* tw = reinterpret_cast<cx*>(smem_raw);
+    cx* work = tw + NF;
+    const Geom g = p.g;
+    const long nxi = (long)g.nm * g.nlh;
+    const long xi0 = (long)blockIdx.x * G;
+    cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
+    const GammaLanes<SR, NF, C, PLANE> ln(xi0, nxi, (int)threadIdx.x);
+    const LineRef<SR> column{spec + (ln.active ? xi0 * C + ln.l : 0), nxi * C};
+    cx v[NF / FftPlan<NF>::threads];
+    line_gather<NF, SR>(column, ln.t, ln.active, v);
+    stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
+    if (stop) return;
+    gamma_core<SR, NF, C, PLANE>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -892,18 +1017,44 @@ struct CgDecision {
 // as in the basic scheme.
 constexpr double kCgDrift = 3e-7;
 
-__device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&tot)[3], bool first, bool slots,
-                                                     int slot_new, bool write) {
+// The control-block values a CG reduction reads, the decision it takes.
+struct CgIn {
+    int inner_prev;        // CG steps counted before this one
+    double g_old, a_old;   // the previous step's gamma and alpha (not read by a cycle's first step)
+    int iters, max_iter;
+    double tol, eps_norm, cycle_r0;
+};
+struct CgOut {
+    double al, be, gam, diff;
+    int inner;
+    bool done, stop;
+};
+
+__device__ __forceinline__ CgIn cg_in(const Ctl* ctl, bool slots, int slot_new) {
+    CgIn s;
+    s.inner_prev = slots ? ctl->cgs_inner[slot_new ^ 1] : ctl->inner;
+    s.g_old = slots ? ctl->cgs_gamma[slot_new ^ 1] : ctl->cg_gamma;
+    s.a_old = slots ? ctl->cgs_alpha[slot_new ^ 1] : ctl->cg_alpha;
+    s.iters = ctl->iters;
+    s.max_iter = ctl->max_iter;
+    s.tol = ctl->tol;
+    s.eps_norm = ctl->eps_norm;
+    s.cycle_r0 = ctl->cycle_r0;
+    return s;
+}
+
+__device__ __forceinline__ CgOut cg_core(const CgIn& s, const double (&tot)[3], bool first) {
+    CgOut o;
     const double gam = tot[0], del = tot[1];
-    const int inner = (slots ? ctl->cgs_inner[slot_new ^ 1] : ctl->inner) + 1;
+    const int inner = s.inner_prev + 1;
     bool done = false, verify = false;
     double diff = 0.0;
     if (!first) {
-        diff = sqrt(tot[2]) / (ctl->eps_norm + 1e-20);
-        const double margin = kCgDrift * ctl->cycle_r0;
-        const bool trusted = margin <= 0.5 * ctl->tol;
-        const bool conv = diff < (trusted ? ctl->tol - margin : ctl->tol);
-        done = (conv && trusted) || ctl->iters + inner >= ctl->max_iter;
+        diff = sqrt(tot[2]) / (s.eps_norm + 1e-20);
+        const double margin = kCgDrift * s.cycle_r0;
+        const bool trusted = margin <= 0.5 * s.tol;
+        const bool conv = diff < (trusted ? s.tol - margin : s.tol);
+        done = (conv && trusted) || s.iters + inner >= s.max_iter;
         verify = conv && !done;
     }
     double al, be;
@@ -911,33 +1062,47 @@ __device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&to
         be = 0.0;
         al = gam / del;
     } else {
-        const double g_old = slots ? ctl->cgs_gamma[slot_new ^ 1] : ctl->cg_gamma;
-        const double a_old = slots ? ctl->cgs_alpha[slot_new ^ 1] : ctl->cg_alpha;
-        be = gam / g_old;
-        al = gam / (del - be * gam / a_old);
+        be = gam / s.g_old;
+        al = gam / (del - be * gam / s.a_old);
     }
     // breakdown, or a converged cycle to verify: back to FP64
     const bool stop = verify || !(al > 0.0) || !isfinite(al) || !isfinite(be);
-    if (write) {
-        ctl->inner = inner;
-        if (!first) {
-            ctl->corr_valid = 1;
-            ctl->diff = diff;
-            if (ctl->iters + inner < 32) ctl->diff_hist[ctl->iters + inner] = diff;
-            if (done) ctl->done = 1;
-        }
-        if (stop) ctl->cg_stop = 1;
-        ctl->cg_alpha = al;
-        ctl->cg_beta = be;
-        ctl->cg_gamma = gam;
-        if (slots) {
-            ctl->cgs_inner[slot_new] = inner;
-            ctl->cgs_alpha[slot_new] = al;
-            ctl->cgs_beta[slot_new] = be;
-            ctl->cgs_gamma[slot_new] = gam;
-        }
+    o.al = al;
+    o.be = be;
+    o.gam = gam;
+    o.diff = diff;
+    o.inner = inner;
+    o.done = done;
+    o.stop = stop;
+    return o;
+}
+
+__device__ __forceinline__ void cg_write(Ctl* ctl, const CgOut& o, bool first, bool slots, int slot_new, int iters) {
+    ctl->inner = o.inner;
+    if (!first) {
+        ctl->corr_valid = 1;
+        ctl->diff = o.diff;
+        if (iters + o.inner < 32) ctl->diff_hist[iters + o.inner] = o.diff;
+        if (o.done) ctl->done = 1;
     }
-    return {al, be, (done || stop) ? 1 : 0};
+    if (o.stop) ctl->cg_stop = 1;
+    ctl->cg_alpha = o.al;
+    ctl->cg_beta = o.be;
+    ctl->cg_gamma = o.gam;
+    if (slots) {
+        ctl->cgs_inner[slot_new] = o.inner;
+        ctl->cgs_alpha[slot_new] = o.al;
+        ctl->cgs_beta[slot_new] = o.be;
+        ctl->cgs_gamma[slot_new] = o.gam;
+    }
+}
+
+__device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&tot)[3], bool first, bool slots,
+                                                     int slot_new, bool write) {
+    const CgIn s = cg_in(ctl, slots, slot_new);
+    const CgOut o = cg_core(s, tot, first);
+    if (write) cg_write(ctl, o, first, slots, slot_new, s.iters);
+    return {o.al, o.be, (o.done || o.stop) ? 1 : 0};
 }
 
 // ---------------------------------------------------------------------------
@@ -973,13 +1138,175 @@ template <int NP> constexpr size_t plane_lds_bytes() {
     return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<float>);
 }
 
+// Sizes of one (component, plane) block's work.
+template <int NP> struct PlaneDims {
+    static constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
+    static constexpr int VOX = NM * NL;
+    static constexpr int V4 = VOX / 4;                                          // groups of four voxels
+    static constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;         // groups per lane
+    static constexpr int NS = NM * NLH;                                         // half-spectrum points
+    static constexpr int SPT = (NS + kPlaneThreads - 1) / kPlaneThreads;        // points per lane
+    static constexpr int PU = (6 * 256 + kPlaneThreads - 1) / kPlaneThreads;    // partial rows per lane (nf <= 256)
+};
+
+// Offset of plane point idx = m*NLH + kb within its component's runs of the
+// tiled spectrum (plane f, component c start at f*NM*NLH*6 + c*kPlaneTile).
+__device__ __forceinline__ long tile_off(int idx) {
+    return (long)(idx / kPlaneTile) * (6 * kPlaneTile) + idx % kPlaneTile;
+}
+
+// (C - C0) of component c in Mandel components: 3 dlam + 2 dmu on e1, 2 dmu
+// elsewhere (four voxels).
+__device__ __forceinline__ vox4 tau_factor(int c, const vox4& dl, const vox4& dm) {
+    vox4 t;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) t[e] = (c == 0) ? (3.0f * dl[e] + 2.0f * dm[e]) : 2.0f * dm[e];
+    return t;
+}
+
+// CG update of four voxels of component c (in registers), then
+// tau_c = (C - C0)_c r_c (tf: tau_factor) into the packed-row LDS layout (row
+// m of the plane is the real or imaginary part of complex line m/2).
+template <int NP>
+__device__ __forceinline__ void plane_update_tau(int q, bool upd, bool upd1, float al, float be, vox4& r,
+                                                 const vox4& w, vox4& pv, vox4& sv, vox4& xv, const vox4& tf,
+                                                 float* bufr) {
+    using PD = PlaneDims<NP>;
+    vox4 y = r;
+    if (upd) {
+        vox4 pp, ss, xx;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            pp[e] = upd1 ? r[e] : r[e] + be * pv[e];
+            ss[e] = upd1 ? w[e] : w[e] + be * sv[e];
+            // x is the correction of eps_1 = eps_0 + r0 (see pass A)
+            xx[e] = upd1 ? (al - 1.0f) * r[e] : xv[e] + al * pp[e];
+            y[e] = r[e] - al * ss[e];
+        }
+        pv = pp;
+        sv = ss;
+        xv = xx;
+        r = y;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float tau = tf[e] * y[e];
+        const int vox = 4 * q + e;
+        const int m = vox / PD::NL, j = vox % PD::NL;
+        bufr[2 * ((m >> 1) * PD::NLP + j) + (m & 1)] = tau;
+    }
+}
+
+// Forward in-plane transforms of the packed rows in LDS: r2c along L (two
+// rows per complex line, split into the two half spectra), c2c along M; the
+// plane's half spectrum leaves through st(idx, value), idx = m*NLH + kb.
+template <int NP, class St>
+__device__ __forceinline__ void plane_fwd_fft(cplx<float>* buf, cplx<float>* pl, const cplx<float>* tw_l,
+                                              const cplx<float>* tw_m, const St& st, int tid) {
+    using PD = PlaneDims<NP>;
+    using cx = cplx<float>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
+    __syncthreads();
+    lds_fft_lane<NL, -1, float>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+    // split each packed row pair into the two half spectra, transposed
+    for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+        const int pr = idx / NLH, kb = idx % NLH;
+        const cx z = buf[pr * NLP + kb];
+        const cx zm = buf[pr * NLP + (NL - kb) % NL];
+        pl[kb * NMP + 2 * pr] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
+        pl[kb * NMP + 2 * pr + 1] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
+    }
+    __syncthreads();
+    lds_fft_lane<NM, -1, float>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
+    for (int idx = tid; idx < NM * NLH; idx += kPlaneThreads) {
+        const int m = idx / NLH, kb = idx % NLH;
+        st(idx, pl[kb * NMP + m]);
+    }
+}
+
+// Inverse in-plane transforms: the plane's half spectrum (point
+// idx = threadIdx.x + kPlaneThreads*k in sp[k]) into LDS, c2c along M, c2r
+// along L; eps_tilde_c is left in buf's packed-row layout.
+template <int NP>
+__device__ __forceinline__ void plane_inv_fft(const cplx<float> (&sp)[PlaneDims<NP>::SPT], cplx<float>* buf,
+                                              cplx<float>* pl, const cplx<float>* tw_l, const cplx<float>* tw_m, int tid) {
+    using PD = PlaneDims<NP>;
+    using cx = cplx<float>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
+#pragma unroll
+    for (int k = 0; k < PD::SPT; ++k) {
+        const int idx = tid + kPlaneThreads * k;
+        if (idx < NM * NLH) pl[(idx % NLH) * NMP + idx / NLH] = sp[k];
+    }
+    __syncthreads();
+    lds_fft_lane<NM, +1, float>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
+    // pack row pairs into complex lines for the c2r along L
+    for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+        const int pr = idx / NLH, kb = idx % NLH;
+        const cx A = pl[kb * NMP + 2 * pr], Bv = pl[kb * NMP + 2 * pr + 1];
+        cx* line = buf + pr * NLP;
+        if (kb == 0 || 2 * kb == NL) {
+            line[kb] = {A.x, Bv.x};   // self-conjugate bins: real parts only (c2r)
+        } else {
+            line[kb] = {A.x - Bv.y, A.y + Bv.x};
+            line[NL - kb] = {A.x + Bv.y, Bv.x - A.y};
+        }
+    }
+    __syncthreads();
+    lds_fft_lane<NL, +1, float>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+}
+
+// w_c = r_c - eps_tilde_c (eps_tilde in buf, as plane_inv_fft leaves it) and
+// this lane's share of the C0-energy products (r, r)_C0, (w, r)_C0 (C0 =
+// 3 lam0 + 2 mu0 on e1, 2 mu0 elsewhere) and ||r||^2.
+template <int NP>
+__device__ __forceinline__ void plane_w_partials(const float* bufr, int c, double lam0, double mu0,
+                                                 const vox4 (&r)[PlaneDims<NP>::Q4], vox4 (&w)[PlaneDims<NP>::Q4],
+                                                 double (&acc)[3], int tid) {
+    using PD = PlaneDims<NP>;
+    const double c0 = (c == 0) ? 3.0 * lam0 + 2.0 * mu0 : 2.0 * mu0;
+    acc[0] = acc[1] = acc[2] = 0.0;
+#pragma unroll
+    for (int k = 0; k < PD::Q4; ++k) {
+        const int q = tid + kPlaneThreads * k;
+        if (q >= PD::V4) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int vox = 4 * q + e;
+            const int m = vox / PD::NL, j = vox % PD::NL;
+            const float et = bufr[2 * ((m >> 1) * PD::NLP + j) + (m & 1)];
+            const float rv = r[k][e];
+            w[k][e] = rv - et;
+            const double rd = (double)rv, wd = (double)w[k][e];
+            acc[0] += rd * rd;
+            acc[1] += wd * rd;
+            acc[2] += rd * rd;
+        }
+    }
+    acc[0] *= c0;
+    acc[1] *= c0;
+}
+
+// The folded reduction's per-lane sums of the previous step's partial rows
+// (row b = threadIdx.x + kPlaneThreads*u, loaded into part[.][u]); a
+// block_sum of tot follows.
+template <int PU>
+__device__ __forceinline__ void plane_fold_sum(const double (&part)[3][PU], int nbp, double (&tot)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < PU; ++u)
+            if ((int)threadIdx.x + kPlaneThreads * u < nbp) v += part[k][u];
+        tot[k] = v;
+    }
+}
+
 template <int NP, bool BATCH>
 __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
-    constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
-    constexpr int VOX = NM * NL;
-    constexpr int V4 = VOX / 4;                                  // groups of four voxels
-    constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;   // groups per lane
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, PU = PD::PU;
     using cx = cplx<float>;
     // step == cycle_len: only the folded reduction of the cycle's last step
     const int step = flags >> LS_STEP_SHIFT, cyc = cycle_len_now(p.ctl);
@@ -1002,7 +1329,6 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     __shared__ double red[16 * 3];
     __shared__ double s_al, s_be;
     __shared__ int s_halt;
-    constexpr int PU = (6 * 256 + kPlaneThreads - 1) / kPlaneThreads;   // partial rows per lane (nf <= 256)
     double part[3][PU];
     const int nbp = 6 * g.nf;   // blocks of the previous step's inverse plane pass
     if (upd) {
@@ -1038,14 +1364,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
     if (upd) {
         double tot[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            double v = 0.0;
-#pragma unroll
-            for (int u = 0; u < PU; ++u)
-                if ((int)threadIdx.x + kPlaneThreads * u < nbp) v += part[k][u];
-            tot[k] = v;
-        }
+        plane_fold_sum<PU>(part, nbp, tot);
         block_sum<3>(tot, red);
         if (threadIdx.x == 0) {
             const bool probe = flags & LS_FLAG_PROBE;
@@ -1065,58 +1384,23 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     for (int k = 0; k < Q4; ++k) {
         const int q = threadIdx.x + kPlaneThreads * k;
         if (q >= V4) continue;
-        vox4 y = r[k];
+        plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv[k], xv[k], tau_factor(c, dl[k], dm[k]), bufr);
         if (upd) {
-            vox4 pp, ss, xx;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                pp[e] = upd1 ? r[k][e] : r[k][e] + be * pv[k][e];
-                ss[e] = upd1 ? w[k][e] : w[k][e] + be * sv[k][e];
-                // x is the correction of eps_1 = eps_0 + r0 (see pass A)
-                xx[e] = upd1 ? (al - 1.0f) * r[k][e] : xv[k][e] + al * pp[e];
-                y[e] = r[k][e] - al * ss[e];
-            }
-            st_vox4(p.p32 + base, q, pp);
-            st_vox4(p.s32 + base, q, ss);
-            st_vox4(p.corr + base, q, xx);
-            st_vox4(p.r32 + base, q, y);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            // (C - C0) in Mandel components: 3 dlam + 2 dmu on e1, 2 dmu elsewhere
-            const float tau = (c == 0) ? (3.0f * dl[k][e] + 2.0f * dm[k][e]) * y[e] : 2.0f * dm[k][e] * y[e];
-            const int vox = 4 * q + e;
-            const int m = vox / NL, j = vox % NL;
-            bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = tau;
+            st_vox4(p.p32 + base, q, pv[k]);
+            st_vox4(p.s32 + base, q, sv[k]);
+            st_vox4(p.corr + base, q, xv[k]);
+            st_vox4(p.r32 + base, q, r[k]);
         }
     }
-    __syncthreads();
-    lds_fft<NL, -1, float>(buf, NM / 2, NLP, 1, tw_l);
-    // split each packed row pair into the two half spectra, transposed
-    for (int idx = threadIdx.x; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
-        const int pr = idx / NLH, kb = idx % NLH;
-        const cx z = buf[pr * NLP + kb];
-        const cx zm = buf[pr * NLP + (NL - kb) % NL];
-        pl[kb * NMP + 2 * pr] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
-        pl[kb * NMP + 2 * pr + 1] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
-    }
-    __syncthreads();
-    lds_fft<NM, -1, float>(pl, NLH, NMP, 1, tw_m);
-    cx* spec = reinterpret_cast<cx*>(p.spec32) + (long)f * NM * NLH * 6 + c * kPlaneTile;
-    for (int idx = threadIdx.x; idx < NM * NLH; idx += kPlaneThreads) {
-        const int m = idx / NLH, kb = idx % NLH;
-        spec[(idx / kPlaneTile) * (6 * kPlaneTile) + idx % kPlaneTile] = pl[kb * NMP + m];
-    }
+    cx* spec = reinterpret_cast<cx*>(p.spec32) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
+    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { spec[tile_off(idx)] = v; }, (int)threadIdx.x);
 }
 
 template <int NP, bool BATCH>
 __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
-    constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
-    constexpr int VOX = NM * NL;
-    constexpr int V4 = VOX / 4;
-    constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;
-    constexpr int SPT = (NM * NLH + kPlaneThreads - 1) / kPlaneThreads;
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, SPT = PD::SPT;
     using cx = cplx<float>;
     const bool stop = (p.ctl->done | p.ctl->cg_stop | past_cycle(p.ctl, flags >> LS_STEP_SHIFT)) != 0;
     if ((flags & LS_FLAG_SPEC) && stop) return;
@@ -1129,15 +1413,14 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
     const long base = (long)c * g.N + (long)f * VOX;
 
-    const cx* spec = reinterpret_cast<const cx*>(p.spec32) + (long)f * NM * NLH * 6 + c * kPlaneTile;
+    const cx* spec = reinterpret_cast<const cx*>(p.spec32) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
     cx sp[SPT];
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
         const int idx = threadIdx.x + kPlaneThreads * k;
-        const int i = idx < NM * NLH ? idx : 0;
-        sp[k] = spec[(i / kPlaneTile) * (6 * kPlaneTile) + i % kPlaneTile];
+        sp[k] = spec[tile_off(idx < PD::NS ? idx : 0)];
     }
-    vox4 r[Q4];
+    vox4 r[Q4], w[Q4];
 #pragma unroll
     for (int k = 0; k < Q4; ++k) {
         const int q = threadIdx.x + kPlaneThreads * k;
@@ -1147,59 +1430,214 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
     if (stop) return;
 
-#pragma unroll
-    for (int k = 0; k < SPT; ++k) {
-        const int idx = threadIdx.x + kPlaneThreads * k;
-        if (idx < NM * NLH) pl[(idx % NLH) * NMP + idx / NLH] = sp[k];
-    }
-    __syncthreads();
-    lds_fft<NM, +1, float>(pl, NLH, NMP, 1, tw_m);
-    // pack row pairs into complex lines for the c2r along L
-    for (int idx = threadIdx.x; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
-        const int pr = idx / NLH, kb = idx % NLH;
-        const cx A = pl[kb * NMP + 2 * pr], Bv = pl[kb * NMP + 2 * pr + 1];
-        cx* line = buf + pr * NLP;
-        if (kb == 0 || 2 * kb == NL) {
-            line[kb] = {A.x, Bv.x};   // self-conjugate bins: real parts only (c2r)
-        } else {
-            line[kb] = {A.x - Bv.y, A.y + Bv.x};
-            line[NL - kb] = {A.x + Bv.y, Bv.x - A.y};
-        }
-    }
-    __syncthreads();
-    lds_fft<NL, +1, float>(buf, NM / 2, NLP, 1, tw_l);
-
-    // w_c = r_c - eps_tilde_c and this component's share of (r, r)_C0,
-    // (w, r)_C0 (C0 = 3 lam0 + 2 mu0 on e1, 2 mu0 elsewhere) and ||r||^2
-    const double c0 = (c == 0) ? 3.0 * p.lam0 + 2.0 * p.mu0 : 2.0 * p.mu0;
-    const float* bufr = reinterpret_cast<const float*>(buf);
-    double acc[3] = {0.0, 0.0, 0.0};
+    plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
+    double acc[3];
+    plane_w_partials<NP>(reinterpret_cast<const float*>(buf), c, p.lam0, p.mu0, r, w, acc, (int)threadIdx.x);
 #pragma unroll
     for (int k = 0; k < Q4; ++k) {
         const int q = threadIdx.x + kPlaneThreads * k;
-        if (q >= V4) continue;
-        vox4 wv;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int vox = 4 * q + e;
-            const int m = vox / NL, j = vox % NL;
-            const float et = bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
-            const float rv = r[k][e];
-            wv[e] = rv - et;
-            const double rd = (double)rv, wd = (double)wv[e];
-            acc[0] += rd * rd;
-            acc[1] += wd * rd;
-            acc[2] += rd * rd;
-        }
-        st_vox4(p.w32 + base, q, wv);
+        if (q < V4) st_vox4(p.w32 + base, q, w[k]);
     }
-    acc[0] *= c0;
-    acc[1] *= c0;
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
         p.partials[3 * blockIdx.x] = acc[0];
         p.partials[3 * blockIdx.x + 1] = acc[1];
         p.partials[3 * blockIdx.x + 2] = acc[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent CG cycle (one context on a cubic plane grid).  One cooperative
+// launch runs all CG steps of a refinement cycle.  Block x is (component,
+// plane) as in the plane passes; its CG vectors r, w, p, s and the iterate x
+// stay in registers for the whole cycle, so a step moves only the spectrum
+// (out and back in once per phase) and the reduction partials, through
+// device-coherent accesses.  A step's phases, between grid barriers:
+//   F: CG update, tau_c, forward in-plane transforms, spectrum out;
+//   G: blocks below the tile count run pass C on one tile each (F transform,
+//      Green operator, inverse F transform);
+//   I: inverse in-plane transforms, w_c = r_c - eps_tilde_c, partials out;
+// then every block reduces all partials in the same order and takes the same
+// decision.  The arithmetic and the reduction orders are the plane passes'
+// (the folded reduction; k_ls_reduce's for the step enqueued last), so the
+// iterates and the control block equal theirs bit for bit.  At the end the
+// blocks write r and x back for the final pass or the next FP64 iteration.
+// The grid barrier's words (p.gbar) are zeroed by the FP64 iteration's
+// reduction that precedes every cycle.
+// ---------------------------------------------------------------------------
+// LDS of a cycle block: twiddles of the three axes, the plane passes' or
+// pass C's buffers (one at a time), then the block's s and x (NP*NP floats
+// each, 16-byte aligned).
+template <int NP> constexpr size_t cycle_work_cplx() {
+    constexpr int G = gamma_points<float, NP, 6>();
+    constexpr size_t plane = (size_t)(NP / 2) * padded<NP>() + (size_t)(NP / 2 + 1) * padded<NP>();
+    constexpr size_t gamma = (size_t)G * 6 * padded<NP>();
+    return ((3 * NP + (plane > gamma ? plane : gamma)) + 1) / 2 * 2;
+}
+// The lane index as a value the compiler cannot move out of the step loop:
+// addresses derived from it are recomputed in each phase instead of being
+// held (and spilled) across the whole cycle.
+__device__ __forceinline__ int lane_opaque() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+template <int NP> constexpr size_t cycle_lds_bytes() {
+    return cycle_work_cplx<NP>() * sizeof(cplx<float>) + 2 * (size_t)NP * NP * sizeof(float);
+}
+
+template <int NP>
+__global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_cycle(const LsParams p, int J) {
+    using PD = PlaneDims<NP>;
+    constexpr int NF = NP;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, SPT = PD::SPT,
+                  PU = PD::PU;
+    constexpr int G = gamma_points<float, NF, 6>();
+    static_assert(gamma_threads<float, NF, 6>() == kPlaneThreads, "pass C runs on the plane blocks' lanes");
+    using cx = cplx<float>;
+    Ctl* ctl = p.ctl;
+    // Every block reads the control block before its first barrier arrival;
+    // block 0 writes it only after the first barrier.
+    if ((ctl->done | ctl->cg_stop) != 0) return;
+    const int cyc = cycle_len_now(ctl);
+    const int steps = (cyc > 0 && cyc < J) ? cyc : J;
+    CgIn st = cg_in(ctl, true, 0);   // step 0 counts on from slot 1, the cycle's start
+    cx* tw_l = reinterpret_cast<cx*>(smem_raw);
+    cx* tw_m = tw_l + NL;
+    cx* tw_f = tw_m + NM;
+    cx* buf = tw_f + NF;               // plane passes: NM/2 packed row pairs of NLP ...
+    cx* pl = buf + (NM / 2) * NLP;     // ... and NLH columns of NMP; pass C: its lines from buf on
+    float* s_lds = reinterpret_cast<float*>(reinterpret_cast<cx*>(smem_raw) + cycle_work_cplx<NP>());   // s, then x
+    float* x_lds = s_lds + VOX;
+    __shared__ double red[16 * 3];
+    __shared__ double s_al, s_be;
+    __shared__ int s_halt, s_flag;
+    const Geom g = p.g;
+    const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
+    const long base = (long)c * g.N + (long)f * VOX;
+
+    // r, w, p and the tau factor in registers; s and x in LDS (they are
+    // touched once per step only)
+    const vox4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    vox4 r[Q4], w[Q4], pv[Q4], tf[Q4];
+#pragma unroll
+    for (int k = 0; k < Q4; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        const int qq = q < V4 ? q : 0;
+        r[k] = ld_vox4(p.r32 + base, qq);
+        const vox4 dm = ld_vox4(p.dmu32 + (long)f * VOX, qq);
+        const vox4 dl = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
+        tf[k] = tau_factor(c, dl, dm);
+        w[k] = pv[k] = zero4;
+        if (q < V4) {
+            st_vox4(s_lds, q, zero4);
+            st_vox4(x_lds, q, zero4);
+        }
+    }
+    stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
+    stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
+    stage_twiddles<float>(tw_f, reinterpret_cast<const cx*>(p.tw_f32), NF);
+
+    const long nxi = (long)g.nm * g.nlh;
+    const CohBuf spec(p.spec32, (unsigned)((long)g.nf * nxi * 6 * sizeof(cx)));
+    const CohBuf parts(p.partials, (unsigned)(3 * 6 * g.nf * sizeof(double)));
+    const long plane_off = (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
+    const int ntiles = (int)(nxi / G);   // nxi is a multiple of G on plane grids
+    const int nbp = 6 * g.nf;
+    unsigned epoch = 0;
+    float al = 0.f, be = 0.f;
+    bool ok = true, updated = false;
+    for (int j = 0; j < steps; ++j) {
+        // F: update (from the previous step's decision), tau, forward in-plane transforms
+        const bool upd = j >= 1, upd1 = j == 1;
+        float* bufr = reinterpret_cast<float*>(buf);
+        int tid = lane_opaque();
+#pragma unroll
+        for (int k = 0; k < Q4; ++k) {
+            const int q = tid + kPlaneThreads * k;
+            if (q >= V4) continue;
+            vox4 sv = ld_vox4(s_lds, q), xv = ld_vox4(x_lds, q);
+            plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv, xv, tf[k], bufr);
+            st_vox4(s_lds, q, sv);
+            st_vox4(x_lds, q, xv);
+        }
+        updated = updated || upd;
+        plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [&](int idx, cx v) { spec.stc(plane_off + tile_off(idx), v); }, tid);
+        if (!(ok = grid_sync(p.gbar, ++epoch, &s_flag))) break;
+
+        // G: pass C on tile blockIdx.x
+        tid = lane_opaque();
+        if ((int)blockIdx.x < ntiles) {
+            const long xi0 = (long)blockIdx.x * G;
+            const GammaLanes<float, NF, 6, true> ln(xi0, nxi, tid);
+            const CohLine column{spec, ln.active ? xi0 * 6 + ln.l : 0, nxi * 6};
+            cx v[NF / FftPlan<NF>::threads];
+            line_gather<NF, float>(column, ln.t, ln.active, v);
+            gamma_core<float, NF, 6, true>(v, ln, column, tw_f, buf, xi0, g, p.lam0, p.mu0, tid, kPlaneThreads);
+        }
+        if (!(ok = grid_sync(p.gbar, ++epoch, &s_flag))) break;
+
+        // I: inverse in-plane transforms, w, partials
+        tid = lane_opaque();
+        cx sp[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int idx = tid + kPlaneThreads * k;
+            sp[k] = spec.ldc(plane_off + tile_off(idx < PD::NS ? idx : 0));
+        }
+        plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, tid);
+        double acc[3];
+        plane_w_partials<NP>(reinterpret_cast<const float*>(buf), c, p.lam0, p.mu0, r, w, acc, tid);
+        block_sum<3>(acc, red);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) parts.std_(3 * blockIdx.x + k, acc[k]);
+        if (!(ok = grid_sync(p.gbar, ++epoch, &s_flag))) break;
+
+        // the step's reduction and decision, the same in every block
+        tid = lane_opaque();
+        double tot[3];
+        if (j == J - 1) {
+            reduce_partials_ld<3>([&](long i) { return parts.ldd(i); }, nbp, 256, tot, red);
+        } else {
+            double part[3][PU];
+#pragma unroll
+            for (int u = 0; u < PU; ++u) {
+                const int b = tid + kPlaneThreads * u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) part[k][u] = parts.ldd(3 * (b < nbp ? b : 0) + k);
+            }
+            plane_fold_sum<PU>(part, nbp, tot);
+            block_sum<3>(tot, red);
+        }
+        if (threadIdx.x == 0) {
+            const CgOut o = cg_core(st, tot, j == 0);
+            if (blockIdx.x == 0) cg_write(ctl, o, j == 0, true, j & 1, st.iters);
+            s_al = o.al;
+            s_be = o.be;
+            s_halt = (o.done || o.stop) ? 1 : 0;
+            st.inner_prev = o.inner;
+            st.g_old = o.gam;
+            st.a_old = o.al;
+        }
+        __syncthreads();
+        al = (float)s_al;
+        be = (float)s_be;
+        if (s_halt) break;
+    }
+    if (!ok) {
+        if (threadIdx.x == 0) ctl->cg_err = 1;
+        return;
+    }
+    if (updated) {
+#pragma unroll
+        for (int k = 0; k < Q4; ++k) {
+            const int q = threadIdx.x + kPlaneThreads * k;
+            if (q >= V4) continue;
+            st_vox4(p.r32 + base, q, r[k]);
+            st_vox4(p.corr + base, q, ld_vox4(x_lds, q));
+        }
     }
 }
 
@@ -1212,6 +1650,11 @@ template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void k_ls_reduce(const LsParams p_arg, const LsParams* __restrict__ P, int nblocks, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     if (p.ctl->done || (MODE == LS_CG && (p.ctl->cg_stop || past_cycle(p.ctl, flags >> LS_STEP_SHIFT)))) return;
+    if constexpr (MODE == LS_FULL) {
+        // a persistent CG cycle may follow: its grid barrier starts from zero
+        if (p.gbar)
+            for (int i = threadIdx.x; i < kBarWords; i += blockDim.x) p.gbar[i] = 0u;
+    }
     __shared__ double red[8 * 4 * 2];
     double tot[3];
     reduce_partials<3>(p.partials, nblocks, tot, red);
@@ -1479,13 +1922,18 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int s
     throw UnsupportedError("FFT length not supported on the first axis");
 }
 
-int ls_partial_slots(const Geom& g) {
-    // largest grid of any reducing kernel times its partial width: pass E
-    // runs at least two lines per block, the final pass at most 1024 blocks
+// Doubles of block partials: the largest grid of any reducing kernel times
+// its partial width (pass E runs at least two lines per block, the final pass
+// at most 1024 blocks), rounded to whole 128-byte lines.
+static long partial_rows(const Geom& g) {
     long lines = g.nlines();
     long nb = (lines + 1) / 2;
-    return (int)std::max(nb * 3, 1024L * 12);
+    return (std::max(nb * 3, 1024L * 12) + 15) / 16 * 16;
 }
+
+// The partials buffer ends with the persistent CG cycle's grid barrier words.
+int ls_partial_slots(const Geom& g) { return (int)(partial_rows(g) + kBarWords / 2); }
+long ls_gbar_offset(const Geom& g) { return partial_rows(g); }
 
 template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int flags, hipStream_t s) {
     launch_fwd_last<double, C, LS_FULL, BATCH>(p, flags, s);
@@ -1543,6 +1991,66 @@ template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags,
     MGK_PLANE_CASES(MGK_CASE)
 #undef MGK_CASE
     throw UnsupportedError("plane length not supported");
+}
+
+// Persistent CG cycles: cubic plane grids of these lengths, when the
+// device holds all 6 n blocks at once (decided once per length).  Measured
+// per CG step on MI355X: 28.6 us at 64^3 against 31.4 us for the three
+// plane-pass launches (FP32+refine 2460 against 2298 events/s); at 32^3 and
+// 48^3 the two are within 2% (the launched passes slightly ahead), so only
+// 64^3 takes the persistent cycle.
+#define MGK_CYCLE_CASES(X) X(64)
+
+template <int NP> static bool cycle_fits() {
+    static std::atomic<int> state{0};   // 0 unknown, 1 fits, 2 does not
+    if (state.load() == 0) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t pr;
+        bool ok = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess &&
+                  pr.cooperativeLaunch &&
+                  hipFuncSetAttribute((const void*)k_cg_cycle<NP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)cycle_lds_bytes<NP>()) == hipSuccess &&
+                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_cg_cycle<NP>, kPlaneThreads,
+                                                               cycle_lds_bytes<NP>()) == hipSuccess;
+        ok = ok && (long)per_cu * pr.multiProcessorCount >= 6L * NP;
+        state.store(ok ? 1 : 2);
+    }
+    return state.load() == 1;
+}
+
+bool ls_cycle_supported(const Geom& g) {
+    const char* e = std::getenv("MGK_LS_CYCLE");   // MGK_LS_CYCLE=0: CG steps as separate plane passes
+    if ((e && e[0] == '0') || !ls_plane_path(g) || g.nf != g.nm) return false;
+#define MGK_CASE(NN) if (g.nm == NN) return cycle_fits<NN>();
+    MGK_CYCLE_CASES(MGK_CASE)
+#undef MGK_CASE
+    return false;
+}
+
+void ls_enqueue_cycle(const LsLaunch& p, int J, hipStream_t s) {
+    if (p.batch || !p.h.gbar) throw ArgError("persistent CG cycles run on a single context");
+    LsParams h = p.h;
+    void* args[] = {&h, &J};
+#define MGK_CASE(NN)                                                                                           \
+    if (p.g.nm == NN) {                                                                                        \
+        MGK_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_cg_cycle<NN>, dim3(6 * NN), dim3(kPlaneThreads), \
+                                                 args, cycle_lds_bytes<NN>(), s));                             \
+        return;                                                                                                \
+    }
+    MGK_CYCLE_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("persistent CG cycle: grid not supported");
+}
+
+// Zeroes the grid barrier words (a cycle launched without the FP64
+// iteration's reduction before it: kernel timing).
+__global__ void k_gbar_reset(unsigned* w) {
+    for (int i = threadIdx.x; i < kBarWords; i += blockDim.x) w[i] = 0u;
+}
+
+void ls_enqueue_gbar_reset(const LsLaunch& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_gbar_reset, dim3(1), dim3(256), 0, s, p.h.gbar);
+    MGK_HIP_CHECK(hipGetLastError());
 }
 
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {

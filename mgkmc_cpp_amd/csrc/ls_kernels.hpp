@@ -55,6 +55,7 @@ struct LsParams {
     const void* tw_f32; const void* tw_m32; const void* tw_l32;
     Ctl* ctl;
     double* partials;
+    unsigned* gbar;         // grid barrier words of the persistent CG cycle (inside the partials buffer), or null
 };
 
 // One launch of a pipeline step over B trajectories of one shape: row
@@ -76,6 +77,15 @@ bool ls_length_supported(int n);
 bool ls_plane_supported(const Geom& g);
 bool ls_plane_path(const Geom& g);
 int ls_partial_slots(const Geom& g);
+// offset (doubles) of the grid barrier words inside the partials buffer
+long ls_gbar_offset(const Geom& g);
+// Persistent CG cycles (k_cg_cycle): a single context on a cubic plane grid
+// whose 6 n blocks the device holds at once (MGK_LS_CYCLE=0 turns them off).
+bool ls_cycle_supported(const Geom& g);
+// All CG steps of one refinement cycle (at most J, as J enqueued plane-pass
+// steps would run) in one cooperative launch.
+void ls_enqueue_cycle(const LsLaunch& l, int J, hipStream_t s);
+void ls_enqueue_gbar_reset(const LsLaunch& l, hipStream_t s);
 void ls_enqueue_start(const LsLaunch& l, int start, hipStream_t s);
 void ls_enqueue_iteration(const LsLaunch& l, int mode, int flags, hipStream_t s);
 void ls_enqueue_final(const LsLaunch& l, hipStream_t s);

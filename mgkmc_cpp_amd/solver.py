@@ -307,6 +307,29 @@ def spectral_solver_3d(nx, ny, nz, E, nu, eps_bar, eps=None, max_iter=200, tol=1
         ctx.close()
 
 
+def cycle_supported(ctx_handle):
+    """True when the context's FP32+refine solves run each CG cycle as one
+    persistent launch (mgk_cycle_supported)."""
+    on = C.c_int(0)
+    check(gpu().mgk_cycle_supported(ctx_handle, C.byref(on)))
+    return bool(on.value)
+
+
+def cycle_timer(ctx_handle, on=True):
+    """Reset the persistent-cycle counters and time (on) every cycle launch
+    of the context's solves with HIP events on its stream."""
+    check(gpu().mgk_cycle_timer(ctx_handle, 1 if on else 0))
+
+
+def cycle_timer_read(ctx_handle):
+    """(summed ms, timed launches, CG steps they ran) since cycle_timer."""
+    ms = C.c_double(0.0)
+    n = C.c_longlong(0)
+    st = C.c_longlong(0)
+    check(gpu().mgk_cycle_timer_read(ctx_handle, C.byref(ms), C.byref(n), C.byref(st)))
+    return ms.value, int(n.value), int(st.value)
+
+
 def solve_history(ctx_handle):
     buf = np.zeros(32)
     n = C.c_int(0)
