@@ -255,6 +255,13 @@ int mgk_batch_size(const mgk_batch* b);
  * on its outcome are captured once per launch structure into a HIP graph and
  * replayed; 0: launched one by one (same results). */
 int mgk_batch_set_graphs(mgk_batch* b, int on);
+/* Replaces the context of one row (no counterpart in the reference: the
+ * serving loop's continuous batching).  The new context has the batch's
+ * shape, precision and mode count and is not in another row; the old one is
+ * no longer used by the batch and may be destroyed.  Captured graphs stay
+ * valid: a row's parameter blocks are rebuilt from its context on every
+ * event call. */
+int mgk_batch_set_member(mgk_batch* b, int slot, mgk_ctx* ctx);
 int mgk_batch_kmc_event(mgk_batch* b, const mgk_event_in* in, const int* active, double* eps_macro,
                         double* sig_macro, mgk_event_out* out);
 

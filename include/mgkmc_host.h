@@ -179,6 +179,11 @@ int  mgk_sim_batch_create(mgk_sim* const* sims, int B, mgk_sim_batch** out);
 int  mgk_sim_batch_destroy(mgk_sim_batch* b);
 int  mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, const long long* caps,
                                mgk_run_stats* stats);
+/* Puts simulation `sim` (same dims, shape, precision and mode count, not in
+ * another row) into row `slot` in place of the one there, which the batch no
+ * longer uses (no counterpart in the reference: the serving loop's
+ * continuous batching; mgk_batch_set_member). */
+int  mgk_sim_batch_set_member(mgk_sim_batch* b, int slot, mgk_sim* sim);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,7 @@ def host():
         _sig(lib, "mgk_sim_batch_destroy", C.c_int, [P])
         _sig(lib, "mgk_sim_batch_event_steps", C.c_int,
              [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(abi.RunStats)])
+        _sig(lib, "mgk_sim_batch_set_member", C.c_int, [P, C.c_int, P])
         _host = lib
     return _host
 

@@ -2055,6 +2055,26 @@ int mgk_batch_destroy(mgk_batch* b) {
 
 int mgk_batch_size(const mgk_batch* b) { return b ? b->B : 0; }
 
+int mgk_batch_set_member(mgk_batch* b, int slot, mgk_ctx* ctx) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!b || !ctx) throw ArgError("null argument");
+        if (slot < 0 || slot >= b->B) throw ArgError("slot out of range");
+        if (!ctx->kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const mgk_ctx* a = b->c[slot == 0 && b->B > 1 ? 1 : 0];
+        if (ctx->g.dims != a->g.dims || ctx->g.nx != a->g.nx || ctx->g.ny != a->g.ny || ctx->g.nz != a->g.nz ||
+            ctx->precision != a->precision || ctx->kp.M != a->kp.M)
+            throw ArgError("the trajectories of a batch share shape, precision and mode count");
+        for (int y = 0; y < b->B; ++y)
+            if (y != slot && b->c[y] == ctx) throw ArgError("a context appears twice in a batch");
+        // the rows' parameter blocks are rebuilt from the members on every
+        // event call (and copied by the captured graphs): no graph changes
+        MGK_HIP_CHECK(hipStreamSynchronize(b->stream));
+        b->c[slot] = ctx;
+        return MGK_OK;
+    });
+}
+
 int mgk_batch_set_graphs(mgk_batch* b, int on) {
     return guarded([&]() -> int {
         std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);

@@ -391,12 +391,13 @@ template <typename SR, int N> constexpr int mid_cols() {
     return pow2floor(cmax(1, cmin(cmin(col_threads<SR, 6>() / T, by_lds), MGK_MID_COLS_CAP)));
 }
 
-// Frequency points (all C components each) per block of the F pass.
-template <typename SR, int N, int C> constexpr int gamma_points() {
+// Frequency points (all C components each) per block of the F pass with
+// LANES lanes per block.
+template <typename SR, int N, int C, int LANES = col_threads<SR, C>()> constexpr int gamma_points() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_lds =
         (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
-    constexpr int by_thr = col_threads<SR, C>() / (T * C);
+    constexpr int by_thr = LANES / (T * C);
     constexpr int most = cmax(1, cmin(cmin(by_thr, by_lds), MGK_GAMMA_POINTS_CAP));
     constexpr int point_bytes = C * (int)sizeof(cplx<SR>);
     constexpr int g_align = 128 / cgcd(128, point_bytes);   // points per whole number of lines
@@ -406,12 +407,17 @@ template <typename SR, int N, int C> constexpr int gamma_points() {
 // Lanes per block of the F pass: the FFT lanes (G*C lines of T lanes), or
 // more when that leaves the Green operator's N*G (point, frequency) items
 // short of one per lane; always a multiple of G.
-template <typename SR, int N, int C> constexpr int gamma_threads() {
-    constexpr int G = gamma_points<SR, N, C>();
+template <typename SR, int N, int C, int LANES = col_threads<SR, C>()> constexpr int gamma_threads() {
+    constexpr int G = gamma_points<SR, N, C, LANES>();
     constexpr int fft = G * C * FftPlan<N>::threads;
-    constexpr int green = (cmin(N * G, kColThreadsMax) / G) * G;
+    constexpr int green = (cmin(N * G, cmax(LANES, kColThreadsMax)) / G) * G;
     return cmax(fft, green);
 }
+
+// Lanes of pass C on the plane passes' tiled spectrum: tiles of kPlaneTile
+// points need whole tiles per block, which 128-point lines reach with 1024
+// lanes (8 points) and not with 512 (5).
+template <int NF> constexpr int plane_gamma_lanes() { return NF > 64 ? 1024 : col_threads<float, 6>(); }
 
 // ---------------------------------------------------------------------------
 // Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW = 2*PAIRS lines per
@@ -662,9 +668,9 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // is (point l / C, component l % C); plane-tiled (tiles of kPlaneTile points
 // per component), lane l is (point (l / (C*T8))*T8 + l % T8, component
 // (l % (C*T8)) / T8).
-template <typename SR, int NF, int C, bool PLANE> struct GammaLanes {
+template <typename SR, int NF, int C, bool PLANE, int LANES = col_threads<SR, C>()> struct GammaLanes {
     static constexpr int T = FftPlan<NF>::threads;
-    static constexpr int G = gamma_points<SR, NF, C>();
+    static constexpr int G = gamma_points<SR, NF, C, LANES>();
     static constexpr int L = G * C;
     static constexpr int T8 = kPlaneTile;
     int l, t, gl, cl;
@@ -682,12 +688,13 @@ template <typename SR, int NF, int C, bool PLANE> struct GammaLanes {
 // the block's LDS lines, eps_hat = -Gamma0_eff(xi):tau_hat / N per point, the
 // inverse c2c along F, stored through `column` (global memory or CohLine).
 // `tw`: the F twiddles in LDS; `work`: G*C padded lines of LDS.
-template <typename SR, int NF, int C, bool PLANE, class Col>
-__device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threads], const GammaLanes<SR, NF, C, PLANE>& ln,
+template <typename SR, int NF, int C, bool PLANE, class Col, int LANES = col_threads<SR, C>()>
+__device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threads],
+                                           const GammaLanes<SR, NF, C, PLANE, LANES>& ln,
                                            const Col& column, const cplx<SR>* tw, cplx<SR>* work, long xi0,
                                            const Geom& g, double lam0, double mu0, int tid, int nthr) {
     constexpr int D = (C == 3) ? 2 : 3;
-    constexpr int G = gamma_points<SR, NF, C>();
+    constexpr int G = gamma_points<SR, NF, C, LANES>();
     constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
     const long nxi = (long)g.nm * g.nlh;
@@ -804,6 +811,33 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
     if (stop) return;
     gamma_core<SR, NF, C, PLANE>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x, (int)blockDim.x);
+}
+
+// Pass C on the tiled spectrum of 128-point plane grids: 1024 lanes, so a
+// block owns whole tiles (FP32 CG steps only).
+template <int NF, bool BATCH>
+__global__ __launch_bounds__(1024) void k_first_gamma_wide(const LsParams p_arg, const LsParams* __restrict__ P,
+                                                           int speculative, int step) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    constexpr int LANES = plane_gamma_lanes<NF>();
+    constexpr int G = gamma_points<float, NF, 6, LANES>();
+    using cx = cplx<float>;
+    const bool stop = (p.ctl->done | p.ctl->cg_stop | past_cycle(p.ctl, step)) != 0;
+    if (speculative && stop) return;
+    cx* tw = reinterpret_cast<cx*>(smem_raw);
+    cx* work = tw + NF;
+    const Geom g = p.g;
+    const long nxi = (long)g.nm * g.nlh;
+    const long xi0 = (long)blockIdx.x * G;
+    cx* spec = reinterpret_cast<cx*>(p.spec32);
+    const GammaLanes<float, NF, 6, true, LANES> ln(xi0, nxi, (int)threadIdx.x);
+    const LineRef<float> column{spec + (ln.active ? xi0 * 6 + ln.l : 0), nxi * 6};
+    cx v[NF / FftPlan<NF>::threads];
+    line_gather<NF, float>(column, ln.t, ln.active, v);
+    stage_twiddles<float>(tw, reinterpret_cast<const cx*>(p.tw_f32), NF);
+    if (stop) return;
+    gamma_core<float, NF, 6, true, LineRef<float>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0,
+                                                          (int)threadIdx.x, (int)blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1147,6 +1181,7 @@ template <int NP> struct PlaneDims {
     static constexpr int NS = NM * NLH;                                         // half-spectrum points
     static constexpr int SPT = (NS + kPlaneThreads - 1) / kPlaneThreads;        // points per lane
     static constexpr int PU = (6 * 256 + kPlaneThreads - 1) / kPlaneThreads;    // partial rows per lane (nf <= 256)
+    static constexpr int QC = Q4 > 2 ? 2 : Q4;                                  // groups a lane holds at once
 };
 
 // Offset of plane point idx = m*NLH + kb within its component's runs of the
@@ -1287,6 +1322,47 @@ __device__ __forceinline__ void plane_w_partials(const float* bufr, int c, doubl
     acc[1] *= c0;
 }
 
+// plane_w_partials for planes whose vector groups a lane cannot hold at once:
+// r read and w written in chunks of PlaneDims::QC groups, the partial sums in
+// the same order.
+template <int NP>
+__device__ __forceinline__ void plane_w_partials_stream(const float* bufr, int c, double lam0, double mu0,
+                                                        const float* rg, float* wg, double (&acc)[3], int tid) {
+    using PD = PlaneDims<NP>;
+    constexpr int QC = PD::QC;
+    const double c0 = (c == 0) ? 3.0 * lam0 + 2.0 * mu0 : 2.0 * mu0;
+    acc[0] = acc[1] = acc[2] = 0.0;
+    for (int ch = 0; ch < PD::Q4 / QC; ++ch) {
+        vox4 r[QC];
+#pragma unroll
+        for (int k = 0; k < QC; ++k) {
+            const int q = tid + kPlaneThreads * (ch * QC + k);
+            r[k] = ld_vox4(rg, q < PD::V4 ? q : 0);
+        }
+#pragma unroll
+        for (int k = 0; k < QC; ++k) {
+            const int q = tid + kPlaneThreads * (ch * QC + k);
+            if (q >= PD::V4) continue;
+            vox4 w;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int vox = 4 * q + e;
+                const int m = vox / PD::NL, j = vox % PD::NL;
+                const float et = bufr[2 * ((m >> 1) * PD::NLP + j) + (m & 1)];
+                const float rv = r[k][e];
+                w[e] = rv - et;
+                const double rd = (double)rv, wd = (double)w[e];
+                acc[0] += rd * rd;
+                acc[1] += wd * rd;
+                acc[2] += rd * rd;
+            }
+            st_vox4(wg, q, w);
+        }
+    }
+    acc[0] *= c0;
+    acc[1] *= c0;
+}
+
 // The folded reduction's per-lane sums of the previous step's partial rows
 // (row b = threadIdx.x + kPlaneThreads*u, loaded into part[.][u]); a
 // block_sum of tot follows.
@@ -1340,23 +1416,30 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
         }
     }
 
+    // The vector groups of a lane are held QC at a time (all of them up to
+    // 64-point planes, loaded before the folded reduction; 128-point planes
+    // stream them in chunks after it).
+    constexpr int QC = PD::QC, NCH = Q4 / QC;
     const vox4 zero4 = {0.f, 0.f, 0.f, 0.f};
-    vox4 r[Q4], w[Q4], pv[Q4], sv[Q4], xv[Q4], dm[Q4], dl[Q4];
+    vox4 r[QC], w[QC], pv[QC], sv[QC], xv[QC], dm[QC], dl[QC];
+    auto load_chunk = [&](int ch) {
 #pragma unroll
-    for (int k = 0; k < Q4; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        const int qq = q < V4 ? q : 0;
-        r[k] = ld_vox4(p.r32 + base, qq);
-        dm[k] = ld_vox4(p.dmu32 + (long)f * VOX, qq);
-        dl[k] = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
-        w[k] = pv[k] = sv[k] = xv[k] = zero4;
-        if (upd) {
-            w[k] = ld_vox4(p.w32 + base, qq);
-            pv[k] = ld_vox4(p.p32 + base, qq);
-            sv[k] = ld_vox4(p.s32 + base, qq);
-            xv[k] = ld_vox4(p.corr + base, qq);
+        for (int k = 0; k < QC; ++k) {
+            const int q = threadIdx.x + kPlaneThreads * (ch * QC + k);
+            const int qq = q < V4 ? q : 0;
+            r[k] = ld_vox4(p.r32 + base, qq);
+            dm[k] = ld_vox4(p.dmu32 + (long)f * VOX, qq);
+            dl[k] = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
+            w[k] = pv[k] = sv[k] = xv[k] = zero4;
+            if (upd) {
+                w[k] = ld_vox4(p.w32 + base, qq);
+                pv[k] = ld_vox4(p.p32 + base, qq);
+                sv[k] = ld_vox4(p.s32 + base, qq);
+                xv[k] = ld_vox4(p.corr + base, qq);
+            }
         }
-    }
+    };
+    load_chunk(0);
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
     stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
     if (stop) return;
@@ -1380,16 +1463,20 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
         be = (float)s_be;
     }
     float* bufr = reinterpret_cast<float*>(buf);
+    for (int ch = 0; ch < NCH; ++ch) {
+        if (ch > 0) load_chunk(ch);
 #pragma unroll
-    for (int k = 0; k < Q4; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        if (q >= V4) continue;
-        plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv[k], xv[k], tau_factor(c, dl[k], dm[k]), bufr);
-        if (upd) {
-            st_vox4(p.p32 + base, q, pv[k]);
-            st_vox4(p.s32 + base, q, sv[k]);
-            st_vox4(p.corr + base, q, xv[k]);
-            st_vox4(p.r32 + base, q, r[k]);
+        for (int k = 0; k < QC; ++k) {
+            const int q = threadIdx.x + kPlaneThreads * (ch * QC + k);
+            if (q >= V4) continue;
+            plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv[k], xv[k], tau_factor(c, dl[k], dm[k]),
+                                 bufr);
+            if (upd) {
+                st_vox4(p.p32 + base, q, pv[k]);
+                st_vox4(p.s32 + base, q, sv[k]);
+                st_vox4(p.corr + base, q, xv[k]);
+                st_vox4(p.r32 + base, q, r[k]);
+            }
         }
     }
     cx* spec = reinterpret_cast<cx*>(p.spec32) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
@@ -1420,23 +1507,33 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         const int idx = threadIdx.x + kPlaneThreads * k;
         sp[k] = spec[tile_off(idx < PD::NS ? idx : 0)];
     }
-    vox4 r[Q4], w[Q4];
-#pragma unroll
-    for (int k = 0; k < Q4; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        r[k] = ld_vox4(p.r32 + base, q < V4 ? q : 0);
-    }
-    stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
-    stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
-    if (stop) return;
-
-    plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
     double acc[3];
-    plane_w_partials<NP>(reinterpret_cast<const float*>(buf), c, p.lam0, p.mu0, r, w, acc, (int)threadIdx.x);
+    if constexpr (PD::QC == Q4) {
+        // every vector group in registers, loaded before the transforms
+        vox4 r[Q4], w[Q4];
 #pragma unroll
-    for (int k = 0; k < Q4; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        if (q < V4) st_vox4(p.w32 + base, q, w[k]);
+        for (int k = 0; k < Q4; ++k) {
+            const int q = threadIdx.x + kPlaneThreads * k;
+            r[k] = ld_vox4(p.r32 + base, q < V4 ? q : 0);
+        }
+        stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
+        stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
+        if (stop) return;
+        plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
+        plane_w_partials<NP>(reinterpret_cast<const float*>(buf), c, p.lam0, p.mu0, r, w, acc, (int)threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < Q4; ++k) {
+            const int q = threadIdx.x + kPlaneThreads * k;
+            if (q < V4) st_vox4(p.w32 + base, q, w[k]);
+        }
+    } else {
+        // 128-point planes: r streamed in chunks after the transforms
+        stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
+        stage_twiddles<float>(tw_m, reinterpret_cast<const cx*>(p.tw_m32), NM);
+        if (stop) return;
+        plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
+        const float* bufr = reinterpret_cast<const float*>(buf);
+        plane_w_partials_stream<NP>(bufr, c, p.lam0, p.mu0, p.r32 + base, p.w32 + base, acc, (int)threadIdx.x);
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
@@ -1911,6 +2008,14 @@ static void launch_first_gamma(const LsLaunch& p, hipStream_t s, int mode, int s
     const long nxi = (long)p.g.nm * p.g.nlh;
 #define MGK_CASE(NN)                                                                                 \
     if (nf == NN) {                                                                                  \
+        if constexpr (PLANE && std::is_same<SR, float>::value && C == 6 && NN == 128) {              \
+            constexpr int LW = plane_gamma_lanes<NN>();                                              \
+            constexpr int G = gamma_points<float, NN, 6, LW>();                                      \
+            const size_t sm = (size_t)(NN + G * 6 * padded<NN>()) * sizeof(cplx<float>);             \
+            hipLaunchKernelGGL((k_first_gamma_wide<NN, BATCH>), dim3(blocks_for(nxi, G), p.B),         \
+                               dim3(gamma_threads<float, NN, 6, LW>()), sm, s, p.h, p.P, spec, step);   \
+            return;                                                                                  \
+        }                                                                                            \
         constexpr int G = gamma_points<SR, NN, C>();                                                 \
         const size_t sm = (size_t)(NN + G * C * padded<NN>()) * sizeof(cplx<SR>);                    \
         hipLaunchKernelGGL((k_first_gamma<SR, NN, C, BATCH, PLANE>), dim3(blocks_for(nxi, G), p.B),              \
@@ -1943,7 +2048,7 @@ template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int f
     launch_inv_last<double, C, LS_FULL, BATCH>(p, flags, s);
 }
 
-#define MGK_PLANE_CASES(X) X(8) X(16) X(32) X(48) X(64)
+#define MGK_PLANE_CASES(X) X(8) X(16) X(32) X(48) X(64) X(128)
 
 bool ls_plane_path(const Geom& g) {
     const char* e = std::getenv("MGK_LS_PLANE");   // read per launch set-up: a test can compare both paths
@@ -1951,7 +2056,7 @@ bool ls_plane_path(const Geom& g) {
 }
 
 static bool plane_gamma_tiles(int nf) {
-#define MGK_CASE(NN) if (nf == NN) return gamma_points<float, NN, 6>() % kPlaneTile == 0;
+#define MGK_CASE(NN) if (nf == NN) return gamma_points<float, NN, 6, plane_gamma_lanes<NN>()>() % kPlaneTile == 0;
     MGK_LEN_CASES(MGK_CASE)
 #undef MGK_CASE
     return false;
@@ -1966,9 +2071,17 @@ bool ls_plane_supported(const Geom& g) {
     return false;
 }
 
+// Kernels whose dynamic LDS exceeds the default 64 KiB limit are allowed
+// their size once (128-point planes: 132 KiB).
+template <class K> static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024) MGK_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 template <bool BATCH> static void launch_fwd_plane(const LsLaunch& p, int flags, hipStream_t s) {
 #define MGK_CASE(NN)                                                                                  \
     if (p.g.nm == NN) {                                                                               \
+        static const bool lds_ok = (allow_lds(k_fwd_plane<NN, BATCH>, plane_lds_bytes<NN>()), true);  \
+        (void)lds_ok;                                                                                 \
         hipLaunchKernelGGL((k_fwd_plane<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads),       \
                            plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
         return;                                                                                       \
@@ -1982,6 +2095,8 @@ template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags,
     const int nb = 6 * p.g.nf;
 #define MGK_CASE(NN)                                                                                  \
     if (p.g.nm == NN) {                                                                               \
+        static const bool lds_ok = (allow_lds(k_inv_plane<NN, BATCH>, plane_lds_bytes<NN>()), true);  \
+        (void)lds_ok;                                                                                 \
         hipLaunchKernelGGL((k_inv_plane<NN, BATCH>), dim3(nb, p.B), dim3(kPlaneThreads),              \
                            plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
         if (reduce)                                                                                   \

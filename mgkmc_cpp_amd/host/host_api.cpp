@@ -416,6 +416,18 @@ int mgk_sim_batch_destroy(mgk_sim_batch* b) {
     return MGK_OK;
 }
 
+int mgk_sim_batch_set_member(mgk_sim_batch* b, int slot, mgk_sim* sim) {
+    return guard([&]() -> int {
+        if (!b || !sim) throw std::runtime_error("null argument");
+        if (slot < 0 || slot >= (int)b->sims.size()) throw std::runtime_error("slot out of range");
+        if (sim->dims != b->dims) throw std::runtime_error("the simulations of a batch share dims");
+        mgk_ctx* ctx = sim->with([&](auto& s) { return s.ctx; });
+        if (mgk_batch_set_member(b->dev, slot, ctx) != MGK_OK) throw std::runtime_error(mgk_last_error());
+        b->sims[slot] = sim;
+        return MGK_OK;
+    });
+}
+
 int mgk_sim_batch_event_steps(mgk_sim_batch* b, int n, int drv_i, int drv_j, const long long* caps,
                               mgk_run_stats* stats) {
     return guard([&]() -> int {

@@ -1,5 +1,5 @@
 """Many independent KMC trajectories on one GPU: batched execution and a
-request-serving loop.
+request-serving loop with continuous batching.
 
 The reference runs one trajectory per process (main.cpp:380-412).  Here
 independent trajectories of one grid shape advance together through a device
@@ -8,9 +8,15 @@ and a server admits arriving requests by device-memory footprint:
 
 * requests wait in arrival order until their estimated footprint
   (mgk_estimate_bytes) fits the HBM budget;
-* an admitted request joins the worker of its grid shape; each worker owns
-  its trajectories and their batch, rebuilt when the membership changes, and
-  advances them a few events per step of the loop;
+* an admitted request goes to the worker of its grid shape and precision; a
+  set-up thread of that worker builds its simulation and runs the first
+  (cold) LS solve on the simulation's own stream, beside the batch;
+* the worker's batch has rows (slots).  A finished request's row is parked:
+  inactive (no event cap), its simulation kept until a joining request takes
+  the row (mgk_sim_batch_set_member), so membership changes keep the batch
+  and its captured HIP graphs.  The batch is rebuilt only to grow, to drop
+  parked rows when the admission loop waits for device memory, or when parked
+  rows outnumber the running ones;
 * workers of different shapes run in their own threads (the native calls
   release the interpreter lock), so their launches overlap on the GPU;
 * a request completes when its trajectory has made its events (or has no
@@ -66,6 +72,15 @@ def request_footprint(req):
     return estimate_bytes(req.shape, M, per_voxel_catalog=False)
 
 
+class _Slot:
+    """One row of a worker's batch: a running request, or a parked one whose
+    simulation stays until the row is reused or dropped."""
+    __slots__ = ("req", "sim", "parked")
+
+    def __init__(self, req, sim):
+        self.req, self.sim, self.parked = req, sim, False
+
+
 class _Worker(threading.Thread):
     """The trajectories of one grid shape and precision."""
 
@@ -73,16 +88,54 @@ class _Worker(threading.Thread):
         super().__init__(daemon=True)
         self.server = server
         self.key = key
-        self.inbox: List[Request] = []
-        self.members: List[Tuple[Request, Simulation]] = []
+        self.inbox: List[Request] = []                        # admitted, not yet set up
+        self.ready: List[Tuple[Request, Simulation]] = []     # set up, waiting for a row
+        self.slots: List[_Slot] = []
         self.batch: Optional[SimulationBatch] = None
         self.cv = threading.Condition()
         self.error: Optional[BaseException] = None
+        self.setting_up = 0
+        self.rebuilds = 0       # batches built (growth, compaction)
+        self.reuses = 0         # joins into a parked row
+        self._setup = threading.Thread(target=self._setup_run, daemon=True)
 
     def submit(self, req):
         with self.cv:
             self.inbox.append(req)
-            self.cv.notify()
+            self.cv.notify_all()
+
+    def start(self):
+        super().start()
+        self._setup.start()
+
+    # -- set-up thread: simulation and first solve, off the batch's path ----
+    def _setup_run(self):
+        try:
+            srv = self.server
+            while True:
+                with self.cv:
+                    while not self.inbox and not srv._closing:
+                        self.cv.wait(0.05)
+                    if not self.inbox:
+                        return
+                    req = self.inbox.pop(0)
+                    self.setting_up += 1
+                try:
+                    sim = request_simulation(req)
+                    sim.elastic_run()
+                except BaseException:
+                    with self.cv:
+                        self.setting_up -= 1
+                    raise
+                # one critical section: the worker never sees the request in
+                # neither list
+                with self.cv:
+                    self.setting_up -= 1
+                    self.ready.append((req, sim))
+                    self.cv.notify_all()
+        except BaseException as e:   # reported by Server.run
+            self.error = e
+            self.server._wake()
 
     def run(self):
         try:
@@ -91,49 +144,81 @@ class _Worker(threading.Thread):
             self.error = e
             self.server._wake()
 
+    def _active(self):
+        return [sl for sl in self.slots if not sl.parked]
+
+    def _drop(self, slot):
+        """Releases a parked row's simulation and its footprint."""
+        slot.sim.close()
+        self.server._release(slot.req)
+
+    def _rebuild(self, joining):
+        """A batch of the running rows plus `joining`; parked rows go."""
+        for sl in self.slots:
+            if sl.parked:
+                self._drop(sl)
+        self.slots = [sl for sl in self.slots if not sl.parked] + [_Slot(r, s) for r, s in joining]
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        if self.slots:
+            self.batch = SimulationBatch([sl.sim for sl in self.slots])
+            self.rebuilds += 1
+
     def _loop(self):
         srv = self.server
         while True:
             with self.cv:
-                while not self.inbox and not self.members and not srv._closing:
+                while (not self.ready and not self._active() and not (srv._closing and not self.inbox
+                                                                        and not self.setting_up)):
+                    if self.slots and srv.pressure:
+                        break
                     self.cv.wait(0.05)
-                new, self.inbox = self.inbox, []
-                if not new and not self.members and srv._closing:
+                new, self.ready = self.ready, []
+                finished = srv._closing and not self.inbox and not self.setting_up and not new
+            t = srv.now()
+            for req, _ in new:
+                req.started = t
+            # joining requests take parked rows first; the rest grow the batch
+            grow = []
+            for req, sim in new:
+                park = next((sl for sl in self.slots if sl.parked), None)
+                if park is not None and self.batch is not None:
+                    self.batch.set_member(self.slots.index(park), sim)
+                    self._drop(park)
+                    park.req, park.sim, park.parked = req, sim, False
+                    self.reuses += 1
+                else:
+                    grow.append((req, sim))
+            # rebuild to grow, to give parked rows' memory to a waiting
+            # admission, or when parked rows outnumber the running ones (every
+            # row's blocks are launched each step, inactive ones included)
+            n_parked = sum(1 for sl in self.slots if sl.parked)
+            if grow or (n_parked and (srv.pressure or n_parked > len(self.slots) - n_parked)):
+                self._rebuild(grow)
+            if not self._active():
+                # nothing runs: parked rows give their memory back
+                if self.slots:
+                    self._rebuild([])
+                if finished:
                     return
-            for req in new:
-                sim = request_simulation(req)
-                sim.elastic_run()
-                req.started = srv.now()
-                self.members.append((req, sim))
-            if new or self.batch is None:
-                if self.batch is not None:
-                    self.batch.close()
-                self.batch = SimulationBatch([s for _, s in self.members]) if self.members else None
-            if not self.members:
                 continue
-            caps = [r.events - r.events_done for r, _ in self.members]
+            caps = [0 if sl.parked else sl.req.events - sl.req.events_done for sl in self.slots]
             stats = self.batch.event_steps(srv.step_events, caps=caps)
-            done = []
-            for (req, sim), st in zip(self.members, stats):
+            t = srv.now()
+            for sl, st in zip(self.slots, stats):
+                if sl.parked:
+                    continue
                 made = int(st.kmc_events)
-                req.events_done += made
+                sl.req.events_done += made
                 # every attempt makes an event, runs a cascade or finds no
                 # rate: a step with neither events nor cascades has stopped
                 stopped = made == 0 and int(st.cascades) == 0
-                if req.events_done >= req.events or stopped:
-                    done.append((req, sim))
-            if done:
-                t = srv.now()
-                self.batch.close()
-                self.batch = None
-                for req, sim in done:
-                    req.finished = t
+                if sl.req.events_done >= sl.req.events or stopped:
+                    sl.req.finished = t
                     if srv.on_complete is not None:
-                        srv.on_complete(req, sim)
-                    sim.close()
-                    srv._release(req)
-                ids = {id(r) for r, _ in done}
-                self.members = [(r, s) for r, s in self.members if id(r) not in ids]
+                        srv.on_complete(sl.req, sl.sim)
+                    sl.parked = True
 
 
 class Server:
@@ -158,6 +243,7 @@ class Server:
         self._closing = False
         self._t0 = None
         self.workers = {}
+        self.pressure = False   # the admission loop waits for device memory: parked rows go
 
     def now(self):
         return time.perf_counter() - self._t0
@@ -201,8 +287,10 @@ class Server:
                 continue
             with self._wake_cv:
                 while self.used + r.footprint > self.budget:
+                    self.pressure = True
                     self._wake_cv.wait(0.01)
                     self._check()
+                self.pressure = False
                 self.used += r.footprint
                 self.peak_used = max(self.peak_used, self.used)
             r.admitted = self.now()
@@ -213,11 +301,16 @@ class Server:
             with w.cv:
                 w.cv.notify()
         for w in self.workers.values():
-            while w.is_alive():
+            while w.is_alive() or w._setup.is_alive():
                 w.join(timeout=0.2)
                 self._check()
         self._check()
         return requests
+
+    def stats(self):
+        """Batches built and joins into parked rows, over all workers."""
+        return dict(rebuilds=sum(w.rebuilds for w in self.workers.values()),
+                    reuses=sum(w.reuses for w in self.workers.values()))
 
     def _report(self):
         if self.progress is None or self.now() - self._last_progress < self.progress_s:

@@ -490,6 +490,13 @@ class SimulationBatch:
         except Exception:
             pass
 
+    def set_member(self, slot, sim):
+        """Puts `sim` (same shape, precision and mode count) into row `slot`
+        in place of the simulation there, which the batch no longer uses
+        (mgk_sim_batch_set_member): the batch and its captured graphs stay."""
+        check_host(host().mgk_sim_batch_set_member(self._h, int(slot), sim._h))
+        self.sims[slot] = sim
+
     def event_steps(self, n, drv=(0, 0), caps=None):
         """Up to n attempts for every simulation, simulation b stopping after
         caps[b] events (caps=None: no cap); one RunStats per simulation."""

@@ -67,3 +67,32 @@ def test_served_requests_equal_standalone_runs(device_inputs):
         assert g_made == made == r.events_done
         assert g_idx == idx
         assert np.array_equal(g_sig, sig) and np.array_equal(g_epsp, epsp)
+
+
+def test_fp32_refine_serving_reuses_parked_rows_and_equals_standalone_runs():
+    # A long request keeps its shape's batch running while short ones arrive
+    # one after another: each short request's row is parked when it finishes
+    # and taken by the next (the batch and its graphs stay), and the set-up
+    # solves run beside the batch.  Every request's trajectory equals its
+    # standalone FP32+refine run bit for bit.
+    specs = [((16, 16), 300)] + [((16, 16), 12 + 2 * i) for i in range(6)] + [((8, 8, 8), 10), ((8, 8, 8), 14)]
+    reqs = [Request(rid=i, shape=s, events=n, seed=80 + i, arrival=0.0 if i == 0 else 0.05 + 0.06 * i,
+                    precision=abi.MGK_FP32_REFINE, instability_kmc=1) for i, (s, n) in enumerate(specs)]
+    got = {}
+
+    def keep(req, sim):
+        got[req.rid] = (req.events_done, sim.get_field(abi.FIELD_SIG), sim.get_field(abi.FIELD_EPS_PLASTIC),
+                        [e.flat_idx for e in sim.events()])
+
+    budget = sum(request_footprint(r) for r in reqs)
+    srv = Server(budget, step_events=4, on_complete=keep)
+    srv.run(reqs)
+    print("serving stats", srv.stats(), summary(reqs))
+    assert srv.stats()["reuses"] >= 1
+    assert srv.peak_used <= budget and srv.used == 0
+    for r in reqs:
+        made, sig, epsp, idx = _alone(r)
+        g_made, g_sig, g_epsp, g_idx = got[r.rid]
+        assert g_made == made == r.events_done
+        assert g_idx == idx
+        assert np.array_equal(g_sig, sig) and np.array_equal(g_epsp, epsp)
