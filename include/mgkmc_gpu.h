@@ -290,13 +290,32 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
  * refinement cycle's CG steps run as one cooperative launch that keeps the CG
  * vectors on chip (iterates equal to the separately launched plane passes,
  * bit for bit; the environment variable MGK_LS_CYCLE=0 turns it off).
- * mgk_cycle_supported: 1 when this context's solves use it.
+ * mgk_cycle_supported: 1 when this context's solves use it (the grid runs
+ * it and the autotuner, or mgk_set_tuning, kept it on).
  * mgk_cycle_timer(on): measurement helper; resets the counters and, when on,
  * times every cycle launch of the context's solves with HIP events on the
  * context's stream (a solve whose refinement batch holds one cycle);
  * mgk_cycle_timer_read: the summed duration in ms, the timed launches and the
  * CG steps they ran. */
 int mgk_cycle_supported(mgk_ctx* ctx, int* on);
+
+/* Autotuner (no counterpart in the reference): the first context of a grid
+ * and precision in a process times the block-shape variants of its LS passes
+ * (pass A's line pairs and the M passes' columns, for the FP64 iterations and
+ * the five-pass FP32 CG steps, on 3D lines of 32 points or more) and, where
+ * the persistent CG cycle applies, a cold solve with it against one with the
+ * launched plane passes; later contexts of that grid reuse the choice.  No
+ * choice changes a result.  MGK_AUTOTUNE=0 keeps the default shapes.
+ * choices (6 ints): pass A variant for FP64 iterations, for FP32 CG steps,
+ * M-pass variant for FP64, for FP32 (0 default, 1 half-size blocks), the
+ * persistent cycle (1 on, 0 off, -1 not applicable), 1 if the grid was timed
+ * in this process.  ms (10 doubles, nullable): the timings behind them (the
+ * two variants of each of the four passes, then the cold solve with and
+ * without the cycle; 0 where not timed). */
+int mgk_tuning(mgk_ctx* ctx, int* choices, double* ms);
+/* Sets this context's variants (choices as mgk_tuning's first five; the
+ * cycle entry 0 or 1), for measurements and tests. */
+int mgk_set_tuning(mgk_ctx* ctx, const int* choices);
 int mgk_cycle_timer(mgk_ctx* ctx, int on);
 int mgk_cycle_timer_read(mgk_ctx* ctx, double* total_ms, long long* launches, long long* steps);
 

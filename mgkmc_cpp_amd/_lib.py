@@ -87,6 +87,8 @@ def gpu():
         _sig(lib, "mgk_estimate_bytes", C.c_longlong, [C.c_int] * 6)
         _sig(lib, "mgk_time_ls_pass", C.c_int, [P, C.c_int, C.c_int, C.c_int, DP])
         _sig(lib, "mgk_cycle_supported", C.c_int, [P, IP])
+        _sig(lib, "mgk_tuning", C.c_int, [P, IP, DP])
+        _sig(lib, "mgk_set_tuning", C.c_int, [P, IP])
         _sig(lib, "mgk_cycle_timer", C.c_int, [P, C.c_int])
         _sig(lib, "mgk_cycle_timer_read", C.c_int, [P, DP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)])
         _gpu = lib

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -96,6 +97,9 @@ struct mgk_ctx {
     int precision = MGK_FP64;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // block-shape variants and CG pipeline chosen by the autotuner
+    LsTune tune{};
+    int use_cycle = 1;       // the persistent CG cycle where the grid supports it (else the plane passes)
     // in-situ timing of the persistent CG cycle launches (mgk_cycle_timer)
     hipEvent_t cyc_ev0 = nullptr, cyc_ev1 = nullptr;
     bool cyc_timing = false;
@@ -271,7 +275,7 @@ struct mgk_ctx {
         }
         p.ctl = ctl;
         p.partials = partials;
-        p.gbar = (precision == MGK_FP32_REFINE && ls_cycle_supported(g))
+        p.gbar = (precision == MGK_FP32_REFINE && use_cycle && ls_cycle_supported(g))
                      ? reinterpret_cast<unsigned*>(partials + ls_gbar_offset(g)) : nullptr;
         return p;
     }
@@ -286,7 +290,9 @@ struct mgk_ctx {
             MGK_HIP_CHECK(hipMemcpyAsync(d_lsp + slot, h_lsp + slot, sizeof(LsParams), hipMemcpyHostToDevice, stream));
             lsp_up[slot] = true;
         }
-        return LsLaunch{g, d_lsp + slot, 1, h_lsp[slot], false, h_lsp[slot].mandel != 0};
+        LsLaunch L{g, d_lsp + slot, 1, h_lsp[slot], false, h_lsp[slot].mandel != 0};
+        L.tune = tune;
+        return L;
     }
     KmcLaunch kml() {
         if (!kp_up || std::memcmp(h_kp, &K, sizeof(KmcParams)) != 0) {
@@ -545,6 +551,149 @@ struct mgk_ctx {
     }
 };
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// Autotuner.  On the first context of a (grid, precision) the block-shape
+// variants of the passes that have them (pass A's line pairs, the M passes'
+// columns, for FP64 iterations and for five-pass FP32 CG steps on 3D lines of
+// 32 points or more) are timed on the new context's buffers, and on grids
+// that run the persistent CG cycle a cold solve with it is timed against one
+// with the launched plane passes.  A variant replaces the default only when
+// it is more than 3% faster.  The choice is kept for the process; no variant
+// changes a result (a line's or a column's arithmetic does not depend on the
+// block shape, and the cycle equals the plane passes bit for bit).
+// MGK_AUTOTUNE=0 keeps the defaults.
+// ---------------------------------------------------------------------------
+struct TuneRecord {
+    LsTune tune{};
+    int use_cycle = 1;
+    int tuned = 0;             // 1: timed in this process (0: defaults)
+    double ms[10] = {0};       // A FP64 v0 v1, B/D FP64 v0 v1, A FP32 v0 v1, B/D FP32 v0 v1, solve cycle, solve plane
+};
+std::map<std::vector<int>, TuneRecord> g_tunes;
+long g_tune_runs = 0;
+
+// Average ms of `reps` launches of one pass variant on the context's stream.
+double time_variant(mgk_ctx& c, int pass, int mode, const LsTune& t, int reps) {
+    std::memset(c.hctl, 0, sizeof(Ctl));
+    c.hctl->iters = 1;
+    c.hctl->max_iter = 1 << 30;
+    MGK_HIP_CHECK(hipMemcpyAsync(c.ctl, c.hctl, sizeof(Ctl), hipMemcpyHostToDevice, c.stream));
+    LsParams q = c.ls_params();
+    q.corr = (mode == LS_CG) ? c.corr : nullptr;
+    q.r32 = (mode == LS_CG) ? c.cg : nullptr;
+    LsLaunch p = c.lsl(1, &q);
+    p.tune = t;
+    ls_enqueue_pass(p, pass, mode, c.stream);   // warm-up launch
+    MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    for (int r = 0; r < reps; ++r) ls_enqueue_pass(p, pass, mode, c.stream);
+    MGK_HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    MGK_HIP_CHECK(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    return (double)ms / reps;
+}
+
+// Wall ms of one cold LS solve (unit uniaxial strain) on the context.
+double time_cold_solve(mgk_ctx& c) {
+    const double bar[6] = {1e-3, 0, 0, 0, 0, 0};
+    MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+    MGK_HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    c.solve(bar, 0.0, false, 200, 1e-8, nullptr);
+    MGK_HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    MGK_HIP_CHECK(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    MGK_HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    return (double)ms;
+}
+
+void autotune(mgk_ctx& c) {
+    const Geom& g = c.g;
+    const std::vector<int> key{g.dims, g.nx, g.ny, g.nz, c.precision};
+    auto it = g_tunes.find(key);
+    if (it == g_tunes.end()) {
+        TuneRecord rec;
+        const char* e = std::getenv("MGK_AUTOTUNE");
+        if (!(e && e[0] == '0')) {
+            const bool refine = (c.precision == MGK_FP32_REFINE);
+            const bool shapes_a = (g.C == 6 && g.nl >= 32);
+            const bool shapes_m = (g.C == 6 && g.nm >= 32);
+            const bool cg5 = refine && !ls_plane_path(g);   // FP32 CG steps on the five-pass pipeline
+            auto pick = [](double d, double v) { return v < 0.97 * d ? 1 : 0; };
+            for (int m = 0; m < 2; ++m) {
+                if (m == 1 && !cg5) continue;
+                const int mode = (m == 0) ? LS_FULL : LS_CG;
+                if (shapes_a) {
+                    LsTune t = rec.tune;
+                    t.fwd[m] = 0;
+                    const double d = time_variant(c, (m == 0) ? 0 : 5, mode, t, 10);
+                    t.fwd[m] = 1;
+                    const double v = time_variant(c, (m == 0) ? 0 : 5, mode, t, 10);
+                    rec.ms[4 * m] = d;
+                    rec.ms[4 * m + 1] = v;
+                    rec.tune.fwd[m] = pick(d, v);
+                }
+                if (shapes_m) {
+                    LsTune t = rec.tune;
+                    t.mid[m] = 0;
+                    const double d = time_variant(c, 1, mode, t, 10);
+                    t.mid[m] = 1;
+                    const double v = time_variant(c, 1, mode, t, 10);
+                    rec.ms[4 * m + 2] = d;
+                    rec.ms[4 * m + 3] = v;
+                    rec.tune.mid[m] = pick(d, v);
+                }
+            }
+            c.tune = rec.tune;
+            if (refine && ls_cycle_supported(g)) {
+                // each pipeline once for warm-up, then timed
+                double best[2] = {0, 0};
+                for (int round = 0; round < 2; ++round)
+                    for (int uc = 1; uc >= 0; --uc) {
+                        c.use_cycle = uc;
+                        best[uc] = time_cold_solve(c);
+                    }
+                rec.ms[8] = best[1];
+                rec.ms[9] = best[0];
+                rec.use_cycle = (best[0] < 0.97 * best[1]) ? 0 : 1;
+            }
+            rec.tuned = 1;
+            ++g_tune_runs;
+        }
+        it = g_tunes.emplace(key, rec).first;
+    }
+    c.tune = it->second.tune;
+    c.use_cycle = it->second.use_cycle;
+}
+
+// Back to a fresh context after the autotuner's launches: fields, CG
+// buffers, control block and the solver statistics.
+void reset_after_tuning(mgk_ctx& c) {
+    const Geom& g = c.g;
+    const long N = g.N;
+    const int C = g.C;
+    MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+    MGK_HIP_CHECK(hipMemset(c.eps, 0, sizeof(double) * C * N));
+    MGK_HIP_CHECK(hipMemset(c.sig, 0, sizeof(double) * C * N));
+    MGK_HIP_CHECK(hipMemset(c.spec, 0, sizeof(double) * 2 * C * g.NS));
+    MGK_HIP_CHECK(hipMemset(c.cg, 0, sizeof(float) * 4 * C * N));
+    MGK_HIP_CHECK(hipMemset(c.corr, 0, sizeof(float) * C * N));
+    MGK_HIP_CHECK(hipMemset(c.ctl, 0, sizeof(Ctl)));
+    std::memset(c.hctl, 0, sizeof(Ctl));
+    c.last_iters = c.last_inner = 0;
+    c.predicted = 4;
+    c.inner_per_outer = 12;
+    c.inner_per_outer2 = 4;
+    for (int k = 0; k < 4; ++k) c.inner_hist[k] = c.inner_hist2[k] = 0;
+    c.inner_pos = c.inner_pos2 = 0;
+    c.mean_valid = false;
+    for (int k = 0; k < 6; ++k) c.mean_eps[k] = 0.0;
+    c.lsp_up[0] = c.lsp_up[1] = false;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -648,6 +797,9 @@ int mgk_create(int dims, int nx, int ny, int nz, double pixel, int plane_mode, i
                 MGK_HIP_CHECK(hipMemcpy(c->tw32[a], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
             }
             MGK_HIP_CHECK(hipDeviceSynchronize());
+            const long runs = g_tune_runs;
+            autotune(*c);
+            if (g_tune_runs != runs) reset_after_tuning(*c);
         } catch (...) {
             delete c;
             throw;
@@ -1302,6 +1454,38 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms) {
     });
 }
 
+int mgk_tuning(mgk_ctx* ctx, int* choices, double* ms) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || !choices) throw ArgError("null argument");
+        const Geom& g = ctx->g;
+        const auto it = g_tunes.find(std::vector<int>{g.dims, g.nx, g.ny, g.nz, ctx->precision});
+        choices[0] = ctx->tune.fwd[0];
+        choices[1] = ctx->tune.fwd[1];
+        choices[2] = ctx->tune.mid[0];
+        choices[3] = ctx->tune.mid[1];
+        choices[4] = (ctx->precision == MGK_FP32_REFINE && ls_cycle_supported(g)) ? ctx->use_cycle : -1;
+        choices[5] = (it != g_tunes.end()) ? it->second.tuned : 0;
+        if (ms)
+            for (int k = 0; k < 10; ++k) ms[k] = (it != g_tunes.end()) ? it->second.ms[k] : 0.0;
+        return MGK_OK;
+    });
+}
+
+int mgk_set_tuning(mgk_ctx* ctx, const int* choices) {
+    return guarded([&]() -> int {
+        if (!ctx || !choices) throw ArgError("null argument");
+        for (int k = 0; k < 4; ++k)
+            if (choices[k] != 0 && choices[k] != 1) throw ArgError("variant must be 0 or 1");
+        ctx->tune.fwd[0] = choices[0];
+        ctx->tune.fwd[1] = choices[1];
+        ctx->tune.mid[0] = choices[2];
+        ctx->tune.mid[1] = choices[3];
+        ctx->use_cycle = choices[4] != 0 ? 1 : 0;
+        return MGK_OK;
+    });
+}
+
 int mgk_cycle_timer(mgk_ctx* ctx, int on) {
     return guarded([&]() -> int {
         if (!ctx) throw ArgError("null context");
@@ -1326,7 +1510,7 @@ int mgk_cycle_timer_read(mgk_ctx* ctx, double* total_ms, long long* launches, lo
 int mgk_cycle_supported(mgk_ctx* ctx, int* on) {
     return guarded([&]() -> int {
         if (!ctx || !on) throw ArgError("null argument");
-        *on = (ctx->precision == MGK_FP32_REFINE && ls_cycle_supported(ctx->g)) ? 1 : 0;
+        *on = (ctx->precision == MGK_FP32_REFINE && ctx->use_cycle && ls_cycle_supported(ctx->g)) ? 1 : 0;
         return MGK_OK;
     });
 }
@@ -1898,7 +2082,8 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     if (n_active == 0) throw ArgError("no active trajectory in the batch call");
     hipStream_t s = b.stream;
     const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0], true};
-    const LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true, ls_plane_path(g)};
+    LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true, ls_plane_path(g)};
+    L.tune = c0.tune;   // every member has the batch's shape, hence its tuning
     const int nb0 = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
     auto iterations = [&](int from, int count, hipStream_t st) {
         for (int q = 0; q < count; ++q) {

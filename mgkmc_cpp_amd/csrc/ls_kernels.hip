@@ -380,6 +380,12 @@ template <typename SR, int N, int C, int CAP> constexpr int last_pairs() {
     return pow2floor(cmax(1, cmin(cmin(by_thr, by_lds), CAP)));
 }
 template <typename SR, int N, int C> constexpr int fwd_pairs() { return last_pairs<SR, N, C, MGK_FWD_PAIRS_CAP>(); }
+// Block-shape variants the autotuner times: HALF = 1 halves pass A's line
+// pairs and the M passes' columns per block (more, smaller blocks).  The
+// arithmetic of a line or column does not depend on them.
+template <typename SR, int N, int C, int HALF> constexpr int fwd_pairs_v() {
+    return HALF ? cmax(1, fwd_pairs<SR, N, C>() / 2) : fwd_pairs<SR, N, C>();
+}
 template <typename SR, int N, int C> constexpr int inv_pairs() {
     return last_pairs<SR, N, C, (C == 6) ? MGK_INV_PAIRS_CAP_3D : MGK_INV_PAIRS_CAP_2D>();
 }
@@ -389,6 +395,9 @@ template <typename SR, int N> constexpr int mid_cols() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_lds = (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (padded<N>() * (int)sizeof(cplx<SR>)));
     return pow2floor(cmax(1, cmin(cmin(col_threads<SR, 6>() / T, by_lds), MGK_MID_COLS_CAP)));
+}
+template <typename SR, int N, int HALF> constexpr int mid_cols_v() {
+    return HALF ? cmax(1, mid_cols<SR, N>() / 2) : mid_cols<SR, N>();
 }
 
 // Frequency points (all C components each) per block of the F pass with
@@ -423,10 +432,10 @@ template <int NF> constexpr int plane_gamma_lanes() { return NF > 64 ? 1024 : co
 // Pass A: tau + r2c along L.  Lines l = f*nm + m, LPW = 2*PAIRS lines per
 // block, VPT voxels per lane.
 // ---------------------------------------------------------------------------
-template <typename SR, int NL, int C, int MODE, bool BATCH>
+template <typename SR, int NL, int C, int MODE, bool BATCH, int HALF = 0>
 __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
-    constexpr int PAIRS = fwd_pairs<SR, NL, C>();
+    constexpr int PAIRS = fwd_pairs_v<SR, NL, C, HALF>();
     constexpr int LPW = 2 * PAIRS;
     constexpr int VPT = (LPW * NL + 255) / 256;
     constexpr int NLP = padded<NL>();
@@ -627,12 +636,12 @@ __global__ __launch_bounds__(256) void k_fwd_last(const LsParams p_arg, const Ls
 // owns L consecutive columns; lane (l, t) = (tid % L, tid / L), so a wave's
 // gather and store touch consecutive columns of one row.
 // ---------------------------------------------------------------------------
-template <typename SR, int NM, int DIR, bool BATCH>
+template <typename SR, int NM, int DIR, bool BATCH, int HALF = 0>
 __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
                                                         int speculative, int step) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     constexpr int T = FftPlan<NM>::threads;
-    constexpr int L = mid_cols<SR, NM>();
+    constexpr int L = mid_cols_v<SR, NM, HALF>();
     constexpr int NMP = padded<NM>();
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
@@ -1952,15 +1961,27 @@ bool ls_length_supported(int n) {
 
 static int blocks_for(long n, int per) { return (int)((n + per - 1) / per); }
 
+template <typename SR, int NN, int C, int MODE, bool BATCH, int HALF>
+static void launch_fwd_last_v(const LsLaunch& p, int flags, hipStream_t s) {
+    constexpr int pairs = fwd_pairs_v<SR, NN, C, HALF>();
+    const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);
+    hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE, BATCH, HALF>), dim3(blocks_for(p.g.nlines(), 2 * pairs), p.B),
+                       dim3(256), sm, s, p.h, p.P, flags);
+}
+
+// Variant 1 exists for 3D lines of 32 points or more (the tuned grids).
+template <int C, int NN> constexpr bool tuned_len() { return C == 6 && NN >= 32; }
+
 template <typename SR, int C, int MODE, bool BATCH>
 static void launch_fwd_last(const LsLaunch& p, int flags, hipStream_t s) {
     const int nl = p.g.nl;
+    const int var = p.tune.fwd[MODE == LS_FULL ? 0 : 1];
 #define MGK_CASE(NN)                                                                                          \
     if (nl == NN) {                                                                                           \
-        constexpr int pairs = fwd_pairs<SR, NN, C>();                                                         \
-        const size_t sm = (size_t)(NN + C * pairs * padded<NN>()) * sizeof(cplx<SR>);                        \
-        hipLaunchKernelGGL((k_fwd_last<SR, NN, C, MODE, BATCH>), dim3(blocks_for(p.g.nlines(), 2 * pairs), p.B), dim3(256), \
-                           sm, s, p.h, p.P, flags);                                                                  \
+        if constexpr (tuned_len<C, NN>()) {                                                                   \
+            if (var) { launch_fwd_last_v<SR, NN, C, MODE, BATCH, 1>(p, flags, s); return; }                  \
+        }                                                                                                     \
+        launch_fwd_last_v<SR, NN, C, MODE, BATCH, 0>(p, flags, s);                                            \
         return;                                                                                               \
     }
     MGK_LEN_CASES(MGK_CASE)
@@ -1985,16 +2006,25 @@ static void launch_inv_last(const LsLaunch& p, int flags, hipStream_t s) {
     throw UnsupportedError("FFT length not supported on the last axis");
 }
 
+template <typename SR, int NN, int DIR, bool BATCH, int HALF>
+static void launch_mid_v(const LsLaunch& p, hipStream_t s, int mode, int spec, int step) {
+    const long total = (long)p.g.nf * p.g.nlh * p.g.C;
+    constexpr int L = mid_cols_v<SR, NN, HALF>();
+    const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);
+    hipLaunchKernelGGL((k_mid<SR, NN, DIR, BATCH, HALF>), dim3(blocks_for(total, L), p.B), dim3(L * FftPlan<NN>::threads),
+                       sm, s, p.h, p.P, mode, spec, step);
+}
+
 template <typename SR, int DIR, bool BATCH>
 static void launch_mid(const LsLaunch& p, hipStream_t s, int mode, int spec = 0, int step = 0) {
     const int nm = p.g.nm;
-    const long total = (long)p.g.nf * p.g.nlh * p.g.C;
+    const int var = p.tune.mid[mode == LS_FULL ? 0 : 1];
 #define MGK_CASE(NN)                                                                                       \
     if (nm == NN) {                                                                                        \
-        constexpr int L = mid_cols<SR, NN>();                                                              \
-        const size_t sm = (size_t)(NN + L * padded<NN>()) * sizeof(cplx<SR>);                              \
-        hipLaunchKernelGGL((k_mid<SR, NN, DIR, BATCH>), dim3(blocks_for(total, L), p.B), dim3(L * FftPlan<NN>::threads), \
-                           sm, s, p.h, p.P, mode, spec, step);                                                    \
+        if constexpr (tuned_len<6, NN>()) {                                                                \
+            if (var) { launch_mid_v<SR, NN, DIR, BATCH, 1>(p, s, mode, spec, step); return; }             \
+        }                                                                                                  \
+        launch_mid_v<SR, NN, DIR, BATCH, 0>(p, s, mode, spec, step);                                       \
         return;                                                                                            \
     }
     MGK_LEN_CASES(MGK_CASE)

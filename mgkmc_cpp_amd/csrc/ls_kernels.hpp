@@ -62,6 +62,13 @@ struct LsParams {
 // blockIdx.y of every grid works on the trajectory whose parameter block is
 // P[blockIdx.y] (device memory).  A single context (batch = false, B = 1)
 // passes its block by value instead (kernel arguments are preloaded).
+// Block-shape variants of a shape's passes, chosen by the autotuner
+// (index 0: FP64 iterations, 1: FP32 CG steps); 0 is the default shape.
+struct LsTune {
+    int fwd[2] = {0, 0};   // pass A: 1 = half the line pairs per block
+    int mid[2] = {0, 0};   // passes B, D: 1 = half the columns per block
+};
+
 struct LsLaunch {
     Geom g;
     const LsParams* P;
@@ -69,6 +76,7 @@ struct LsLaunch {
     LsParams h;   // host copy of a single context's parameter block (passed by value)
     bool batch;   // a batch (any B, also 1): parameter blocks from P
     bool plane = false;   // CG steps run the plane passes (ls_plane_path)
+    LsTune tune{};        // block-shape variants of this shape (autotuner)
 };
 
 bool ls_length_supported(int n);

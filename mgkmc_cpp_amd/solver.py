@@ -315,6 +315,24 @@ def cycle_supported(ctx_handle):
     return bool(on.value)
 
 
+def tuning(ctx_handle):
+    """The autotuner's choices for the context's grid (mgk_tuning): a dict of
+    the pass variants, the CG pipeline, whether the grid was timed, and the
+    timings in ms."""
+    ch = (C.c_int * 6)()
+    ms = np.zeros(10)
+    check(gpu().mgk_tuning(ctx_handle, ch, _dp(ms)))
+    return dict(fwd_fp64=ch[0], fwd_fp32=ch[1], mid_fp64=ch[2], mid_fp32=ch[3], cycle=ch[4], tuned=bool(ch[5]),
+                ms=dict(fwd_fp64=ms[0:2].tolist(), mid_fp64=ms[2:4].tolist(), fwd_fp32=ms[4:6].tolist(),
+                        mid_fp32=ms[6:8].tolist(), solve_cycle=float(ms[8]), solve_plane=float(ms[9])))
+
+
+def set_tuning(ctx_handle, fwd_fp64=0, fwd_fp32=0, mid_fp64=0, mid_fp32=0, cycle=1):
+    """Forces the context's block-shape variants and CG pipeline (mgk_set_tuning)."""
+    ch = (C.c_int * 5)(fwd_fp64, fwd_fp32, mid_fp64, mid_fp32, cycle)
+    check(gpu().mgk_set_tuning(ctx_handle, ch))
+
+
 def cycle_timer(ctx_handle, on=True):
     """Reset the persistent-cycle counters and time (on) every cycle launch
     of the context's solves with HIP events on its stream."""
