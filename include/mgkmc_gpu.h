@@ -160,6 +160,16 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time,
                    const double* new_catalog, const double* new_Q0,
                    int first_mode, int n_modes);
 
+/* The flips of one cascade round in their order (run_cascade's loop over the
+ * sorted unstable modes, kmc_solver.hpp:488-523, 3D :1500-1533): flip i is
+ * flat_idx[i], with new_catalog (n * n_modes * D*D, AoS) and new_Q0
+ * (n * n_modes) holding its redraw, n_modes = M (all_modes: modes 0..M-1 of
+ * the voxel) or 1 (the flipped mode).  Each flip sees the softening of the
+ * flips before it, as mgk_apply_flip called n times; one upload and one
+ * launch per 64 flips. */
+int mgk_apply_flips(mgk_ctx* ctx, int n, const int* flat_idx, double time, const double* new_catalog,
+                    const double* new_Q0, int all_modes);
+
 /* find_unstable_indices (kmc_solver.hpp:412-421): flat indices with
  * Q < stability_threshold in ascending order, and their Q values.
  * Writes at most cap entries; n_out is the full count. */

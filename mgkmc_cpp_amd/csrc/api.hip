@@ -37,6 +37,9 @@ int fail(int code, const std::string& msg) {
 // that make them, and every graph capture, hold this lock.
 std::recursive_mutex g_device_mu;
 
+// Flips of a cascade round uploaded and applied per launch (mgk_apply_flips).
+constexpr int kFlipChunk = 64;
+
 template <typename F> int guarded(F&& f) {
     try {
         return f();
@@ -152,6 +155,11 @@ struct mgk_ctx {
     KmcCtl* hkctl = nullptr;
     double* kstage = nullptr;
     double* hkstage = nullptr;
+    // a cascade round's flips, uploaded kFlipChunk at a time (mgk_apply_flips)
+    double* fstage = nullptr;
+    double* hfstage = nullptr;
+    int* fidx = nullptr;
+    int* hfidx = nullptr;
     int* unst_idx = nullptr;
     double* unst_q = nullptr;
     std::vector<double> shared_cat;   // [C][M] host copy of a shared catalog
@@ -210,7 +218,8 @@ struct mgk_ctx {
         fh(hctl); fh(hstage);
         f(K.Q); f(K.Q0); f(K.cat); f(K.soft); f(K.last_t); f(K.prev_dir); f(K.bsum); f(K.bincl);
         f(K.bunst); f(K.bunst_off); f(K.binc); f(kstage); f(unst_idx); f(unst_q);   // kctl: inside ctl's block
-        fh(hkstage);
+        f(fstage); f(fidx);
+        fh(hkstage); fh(hfstage); fh(hfidx);
         f(d_lsp); f(d_kp); f(d_ea); f(pk); f(pkm); f(psu);
         fh(h_lsp); fh(h_kp); fh(h_ea);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1050,7 +1059,10 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
             auto f = [](void* p) { if (p) (void)hipFree(p); };
             f(c.K.Q); f(c.K.Q0); f(c.K.cat); f(c.K.soft); f(c.K.last_t); f(c.K.prev_dir); f(c.K.bsum);
             f(c.K.bincl); f(c.K.bunst); f(c.K.bunst_off); f(c.K.binc); f(c.kstage); f(c.unst_idx); f(c.unst_q);
+            f(c.fstage); f(c.fidx);
             if (c.hkstage) (void)hipHostFree(c.hkstage);
+            if (c.hfstage) (void)hipHostFree(c.hfstage);
+            if (c.hfidx) (void)hipHostFree(c.hfidx);
             c.dev_bytes -= c.kmc_bytes;
             c.kmc_bytes = 0;
             c.cat_cap = 0;
@@ -1093,6 +1105,10 @@ int mgk_kmc_init(mgk_ctx* ctx, const mgk_kmc_params* kp) {
         c.kstage = c.dal<double>((size_t)kp->M * (C + 1), true);
         c.hkstage = halloc<double>((size_t)kp->M * (C + 1));
         K.staging = c.kstage;
+        c.fstage = c.dal<double>((size_t)kFlipChunk * kp->M * (C + 1), true);
+        c.hfstage = halloc<double>((size_t)kFlipChunk * kp->M * (C + 1));
+        c.fidx = c.dal<int>(kFlipChunk, true);
+        c.hfidx = halloc<int>(kFlipChunk);
         c.unst_idx = c.dal<int>(nm, true);
         c.unst_q = c.dal<double>(nm, true);
         K.unst_idx = c.unst_idx;
@@ -1242,6 +1258,74 @@ int mgk_apply_flip(mgk_ctx* ctx, int flat_idx, double time, const double* new_ca
 // Redraw values of a selected voxel into the pinned staging buffer, checked
 // against the mode range and the shared-catalog mode.  Returns the number of
 // modes staged; *has_cat / *has_q0 say which halves the flip applies.
+int mgk_apply_flips(mgk_ctx* ctx, int n, const int* flat_idx, double time, const double* new_catalog,
+                    const double* new_Q0, int all_modes) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::recursive_mutex> device_lock(g_device_mu);
+        if (!ctx || (n > 0 && !flat_idx)) throw ArgError("null argument");
+        if (n < 0) throw ArgError("negative flip count");
+        mgk_ctx& c = *ctx;
+        if (!c.kmc) throw ArgError("KMC state not initialised (mgk_kmc_init)");
+        const long nm = c.g.N * c.kp.M;
+        const int C = c.g.C, F = c.full(), M = c.kp.M;
+        const int nmo = all_modes ? M : 1;
+        for (int i = 0; i < n; ++i)
+            if (flat_idx[i] < 0 || flat_idx[i] >= nm) throw ArgError("flat index out of range");
+        if (n == 0) return MGK_OK;
+        double skw[3];
+        if (new_catalog && c.K.cat_shared) {
+            // as mgk_apply_flip: redraws that all reproduce the shared strains
+            // keep the shared copy, any other makes the catalog per-(voxel, mode)
+            const std::vector<double>& shared = c.shared_cat;
+            bool same = true;
+            double h[6];
+            for (int i = 0; i < n && same; ++i) {
+                const int first = all_modes ? 0 : flat_idx[i] % M;
+                for (int j = 0; j < nmo && same; ++j) {
+                    c.split(new_catalog + ((size_t)i * nmo + j) * F, h, skw);
+                    for (int k = 0; k < C; ++k)
+                        if (h[k] != shared[(size_t)k * M + first + j]) { same = false; break; }
+                }
+            }
+            if (same) {
+                new_catalog = nullptr;
+            } else {
+                std::vector<double> full((size_t)C * nm);
+                for (int k = 0; k < C; ++k)
+                    for (long i = 0; i < nm; ++i) full[(size_t)k * nm + i] = shared[(size_t)k * M + i % M];
+                MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+                c.ensure_cat(full.size());
+                MGK_HIP_CHECK(hipMemcpyAsync(c.K.cat, full.data(), full.size() * sizeof(double),
+                                             hipMemcpyHostToDevice, c.stream));
+                MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+                c.K.cat_shared = 0;
+            }
+        }
+        const int has_cat = new_catalog ? 1 : 0, has_q0 = new_Q0 ? 1 : 0;
+        const size_t per = (size_t)nmo * (C + 1);
+        const KmcLaunch KL = c.kml();
+        for (int i0 = 0; i0 < n; i0 += kFlipChunk) {
+            const int cnt = std::min(kFlipChunk, n - i0);
+            MGK_HIP_CHECK(hipStreamSynchronize(c.stream));   // the pinned buffers may still feed the last chunk
+            for (int i = 0; i < cnt; ++i) {
+                c.hfidx[i] = flat_idx[i0 + i];
+                double* h = c.hfstage + (size_t)i * per;
+                for (int j = 0; j < nmo; ++j) {
+                    if (has_cat) c.split(new_catalog + ((size_t)(i0 + i) * nmo + j) * F, h + (size_t)j * C, skw);
+                    if (has_q0) h[(size_t)nmo * C + j] = new_Q0[(size_t)(i0 + i) * nmo + j];
+                }
+            }
+            MGK_HIP_CHECK(hipMemcpyAsync(c.fidx, c.hfidx, sizeof(int) * cnt, hipMemcpyHostToDevice, c.stream));
+            if (has_cat || has_q0)
+                MGK_HIP_CHECK(hipMemcpyAsync(c.fstage, c.hfstage, sizeof(double) * per * cnt, hipMemcpyHostToDevice,
+                                             c.stream));
+            kmc_enqueue_flip_list(KL, c.fidx, c.fstage, cnt, time, all_modes ? 1 : 0, has_cat, has_q0, c.stream);
+        }
+        MGK_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return MGK_OK;
+    });
+}
+
 static int stage_redraw(mgk_ctx& c, const double* new_catalog, const double* new_Q0, int first_mode, int n_modes,
                         int* has_cat, int* has_q0, double* dest = nullptr) {
     const int C = c.g.C, F = c.full(), M = c.kp.M;
@@ -1917,6 +2001,7 @@ long long mgk_estimate_bytes(int dims, int nx, int ny, int nz, int M, int per_vo
         b += r(2 * N, 8) + r(N, 8) + r(C * N, 8);         // softening, last event time, previous direction
         b += 2 * r(nb, 8) + 3 * r(nb, 4);                 // block sums and counts
         b += r((long long)M * (C + 1), 8);
+        b += r((long long)kFlipChunk * M * (C + 1), 8) + r(kFlipChunk, 4);   // cascade flip lists
         b += r(nm, 4) + r(nm, 8);                         // unstable list
     }
     return b;

@@ -752,7 +752,8 @@ __global__ __launch_bounds__(256) void k_select_seq(const KmcParams p_arg, const
 // ---------------------------------------------------------------------------
 template <int D, int C>
 __device__ __forceinline__ void flip_body(const KmcParams& p, int flat, double time, int first_mode, int n_modes,
-                                          int has_cat, int has_q0, int time_from_ctl, const RedrawArgs& ra, int ra_on) {
+                                          int has_cat, int has_q0, int time_from_ctl, const RedrawArgs& ra, int ra_on,
+                                          const double* stage_ptr = nullptr) {
     // flat < 0: the mode the last selection chose, if that attempt produced
     // an event; first_mode < 0: redraw only that mode (both redraw switches
     // off, kmc_solver.hpp:856-864)
@@ -767,7 +768,7 @@ __device__ __forceinline__ void flip_body(const KmcParams& p, int flat, double t
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long v = flat / ph.M;
-    const double* stage = ra_on ? ra.v : p.staging;
+    const double* stage = stage_ptr ? stage_ptr : (ra_on ? ra.v : p.staging);
     double c[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) c[k] = p.cat_shared ? p.cat[k * ph.M + flat % ph.M] : p.cat[k * nmodes + flat];
@@ -862,13 +863,14 @@ __device__ __forceinline__ void flip_body(const KmcParams& p, int flat, double t
 template <int D, int C>
 __device__ __forceinline__ void flip_body_block(const KmcParams& p, int flat, double time, int first_mode,
                                                 int n_modes, int has_cat, int has_q0, int time_from_ctl,
-                                                const RedrawArgs& ra, int ra_on) {
+                                                const RedrawArgs& ra, int ra_on, const double* stage_ptr = nullptr) {
     __shared__ int s_flat, s_first;
     __shared__ double s_time;
     const Geom g = p.g;
     const bool distinct = g.nx >= 3 && g.ny >= 3 && (D == 2 || g.nz >= 3) && blockDim.x >= 64;
     if (!distinct) {
-        if (threadIdx.x == 0) flip_body<D, C>(p, flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl, ra, ra_on);
+        if (threadIdx.x == 0)
+            flip_body<D, C>(p, flat, time, first_mode, n_modes, has_cat, has_q0, time_from_ctl, ra, ra_on, stage_ptr);
         return;
     }
     if (threadIdx.x == 0) {
@@ -885,7 +887,7 @@ __device__ __forceinline__ void flip_body_block(const KmcParams& p, int flat, do
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
     const long v = flat / ph.M;
-    const double* stage = ra_on ? ra.v : p.staging;
+    const double* stage = stage_ptr ? stage_ptr : (ra_on ? ra.v : p.staging);
     const int t = threadIdx.x;
     constexpr int NCELL = (D == 2) ? 9 : 27;
     constexpr int CENTRE = NCELL / 2;
@@ -949,6 +951,24 @@ __device__ __forceinline__ void flip_body_block(const KmcParams& p, int flat, do
         if (has_cat && !p.cat_shared)
             for (int k = 0; k < C; ++k) p.cat[k * nmodes + mi] = stage[j * C + k];
         if (has_q0) p.Q0[mi] = stage[n_modes * C + j];
+    }
+}
+
+// A cascade round's flips in their order (run_cascade, kmc_solver.hpp:488-523,
+// :1500-1533): one block applies them one after another, each flip's
+// neighbourhood updates complete before the next flip reads it.  Flip i's
+// redraw values sit at stage + i * n_modes * (C + 1) (catalog then Q0); every
+// flip redraws all M modes (all_modes) or only its own mode.
+template <int D, int C>
+__global__ void k_flip_list(const KmcParams p, const int* __restrict__ flats, const double* __restrict__ stage, int n,
+                            double time, int all_modes, int has_cat, int has_q0) {
+    if (blockIdx.x != 0) return;
+    const int n_modes = all_modes ? p.ph.M : 1;
+    const RedrawArgs ra{};
+    for (int i = 0; i < n; ++i) {
+        flip_body_block<D, C>(p, flats[i], time, all_modes ? 0 : -1, n_modes, has_cat, has_q0, 0, ra, 0,
+                              stage + (size_t)i * n_modes * (C + 1));
+        __syncthreads();
     }
 }
 
@@ -1167,6 +1187,18 @@ static void enqueue_flip_t(const KmcLaunch& l, int flat, double time, int first_
 }
 
 bool kmc_redraw_fits_args(int n_modes, int C) { return n_modes * (C + 1) <= kRedrawArgs; }
+
+void kmc_enqueue_flip_list(const KmcLaunch& l, const int* flats, const double* stage, int n, double time,
+                           int all_modes, int has_cat, int has_q0, hipStream_t s) {
+    if (l.batch) throw ArgError("flip lists run on a single trajectory");
+    if (l.g.dims == 2)
+        hipLaunchKernelGGL((k_flip_list<2, 3>), dim3(1), dim3(64), 0, s, l.h, flats, stage, n, time, all_modes, has_cat,
+                           has_q0);
+    else
+        hipLaunchKernelGGL((k_flip_list<3, 6>), dim3(1), dim3(64), 0, s, l.h, flats, stage, n, time, all_modes, has_cat,
+                           has_q0);
+    MGK_HIP_CHECK(hipGetLastError());
+}
 
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s, int time_from_ctl, const double* host_redraw) {

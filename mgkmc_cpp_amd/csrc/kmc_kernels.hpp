@@ -140,6 +140,10 @@ void kmc_enqueue_select_fallback(const KmcLaunch& l, int only_if_ambiguous, hipS
 void kmc_enqueue_flip(const KmcLaunch& l, int flat, double time, int first_mode, int n_modes, int has_cat,
                       int has_q0, hipStream_t s, int time_from_ctl = 0, const double* host_redraw = nullptr);
 bool kmc_redraw_fits_args(int n_modes, int C);
+// A cascade round's flips (device flat indices and redraw values, n_modes =
+// M (all_modes) or 1 per flip) applied in order by one block.
+void kmc_enqueue_flip_list(const KmcLaunch& l, const int* flats, const double* stage, int n, double time,
+                           int all_modes, int has_cat, int has_q0, hipStream_t s);
 void kmc_enqueue_unstable(const KmcLaunch& l, hipStream_t s);
 // Fast-patching stress update: sig += sunit * pred (pred != 0), the kernel
 // superposition around voxel for the flipped strain c (symmetric

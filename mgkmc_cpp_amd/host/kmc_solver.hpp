@@ -443,43 +443,75 @@ public:
     }
 
     // redraw after a flip (kmc_solver.hpp:839-864, cascade :499-523) + flip
-    void flip_and_redraw(long voxel, long flat) {
-        const Mat<D> flipped = catalog[flat];   // the strain before its redraw (:488, :1501)
+    // The redraw of flipping mode `flat` of `voxel` (:839-864, cascade :499-523),
+    // drawn from the host stream in the reference's order into cat[0..n) and
+    // q0[0..n): all M modes of the voxel when a redraw switch is on, else the
+    // flipped mode alone (first = its mode).
+    void draw_redraw(long voxel, long flat, Mat<D>* cat, double* q0, int& first, int& n, bool& has_cat,
+                     bool& has_q0) {
         std::normal_distribution<double> standard_normal(0.0, 1.0);   // fresh per event (:840, :499)
         std::normal_distribution<double>* sn = (D == 2) ? &standard_normal : nullptr;
-        redraw_cat_.resize((size_t)M);
-        redraw_q0_.resize((size_t)M);
-        int first = 0, n = 0;
-        bool has_cat = false, has_q0 = false;
         if (redraw_directions || redraw_barriers) {
             for (int mm = 0; mm < M; ++mm) {
                 const long mi = voxel * M + mm;
-                redraw_cat_[mm] = catalog[mi];
-                redraw_q0_[mm] = Q0[mi];
-                if (redraw_directions) redraw_cat_[mm] = draw_mode(sn, mm);
-                if (redraw_barriers) redraw_q0_[mm] = barrier_gen.generate(rng);
+                cat[mm] = catalog[mi];
+                q0[mm] = Q0[mi];
+                if (redraw_directions) cat[mm] = draw_mode(sn, mm);
+                if (redraw_barriers) q0[mm] = barrier_gen.generate(rng);
             }
             first = 0;
             n = M;
             has_cat = redraw_directions;
             has_q0 = redraw_barriers;
         } else {
-            redraw_cat_[0] = draw_mode(sn, (int)(flat % M));
-            redraw_q0_[0] = barrier_gen.generate(rng);
+            cat[0] = draw_mode(sn, (int)(flat % M));
+            q0[0] = barrier_gen.generate(rng);
             first = (int)(flat % M);
             n = 1;
             has_cat = has_q0 = true;
         }
+    }
+
+    // The host copies of a redrawn voxel's entries.
+    void keep_redraw(long voxel, const Mat<D>* cat, const double* q0, int first, int n, bool has_cat, bool has_q0) {
+        for (int j = 0; j < n; ++j) {
+            const long mi = voxel * M + first + j;
+            if (has_cat) catalog[mi] = cat[j];
+            if (has_q0) Q0[mi] = q0[j];
+        }
+    }
+
+    void flip_and_redraw(long voxel, long flat) {
+        const Mat<D> flipped = catalog[flat];   // the strain before its redraw (:488, :1501)
+        redraw_cat_.resize((size_t)M);
+        redraw_q0_.resize((size_t)M);
+        int first = 0, n = 0;
+        bool has_cat = false, has_q0 = false;
+        draw_redraw(voxel, flat, redraw_cat_.data(), redraw_q0_.data(), first, n, has_cat, has_q0);
         check(mgk_apply_flip(ctx, (int)flat, time, has_cat ? redraw_cat_[0].data() : nullptr,
                              has_q0 ? redraw_q0_.data() : nullptr, first, n),
               "mgk_apply_flip");
-        // keep the host copies of the redrawn entries current
-        for (int j = 0; j < n; ++j) {
-            const long mi = voxel * M + first + j;
-            if (has_cat) catalog[mi] = redraw_cat_[j];
-            if (has_q0) Q0[mi] = redraw_q0_[j];
-        }
+        keep_redraw(voxel, redraw_cat_.data(), redraw_q0_.data(), first, n, has_cat, has_q0);
         if (fast_patching_enabled) check(mgk_patch_apply(ctx, (int)voxel, flipped.data(), 0.0), "mgk_patch_apply");
+    }
+
+    // A cascade round's flips (distinct voxels, in order): every redraw drawn
+    // first, in the flips' order, then one device call for all of them.
+    void flip_round(const std::vector<int>& flips) {
+        const bool all = redraw_directions || redraw_barriers;
+        const size_t per = all ? (size_t)M : 1;
+        round_cat_.resize(flips.size() * per);
+        round_q0_.resize(flips.size() * per);
+        int first = 0, n = 0;
+        bool has_cat = false, has_q0 = false;
+        for (size_t i = 0; i < flips.size(); ++i)
+            draw_redraw(flips[i] / M, flips[i], &round_cat_[i * per], &round_q0_[i * per], first, n, has_cat, has_q0);
+        check(mgk_apply_flips(ctx, (int)flips.size(), flips.data(), time, has_cat ? round_cat_[0].data() : nullptr,
+                              has_q0 ? round_q0_.data() : nullptr, all ? 1 : 0),
+              "mgk_apply_flips");
+        for (size_t i = 0; i < flips.size(); ++i)
+            keep_redraw(flips[i] / M, &round_cat_[i * per], &round_q0_[i * per], all ? 0 : flips[i] % M, (int)per,
+                        has_cat, has_q0);
     }
 
     // run_cascade (kmc_solver.hpp:464-564, :1473-1590)
@@ -497,13 +529,19 @@ public:
             for (size_t k = 0; k < unstable.size(); ++k) qv[unstable[k]] = unstable_q_[k];
             std::sort(unstable.begin(), unstable.end(), [&](int a, int b) { return qv[a] < qv[b]; });
             std::vector<bool> flipped(N(), false);
-            int batch = 0;
+            std::vector<int> flips;
             for (int mi : unstable) {
                 const long v = mi / M;
                 if (flipped[v]) continue;
                 flipped[v] = true;
-                batch++;
-                flip_and_redraw(v, mi);
+                flips.push_back(mi);
+            }
+            const int batch = (int)flips.size();
+            if (fast_patching_enabled) {
+                // each flip patches the stress before the next one
+                for (int mi : flips) flip_and_redraw(mi / M, mi);
+            } else if (batch > 0) {
+                flip_round(flips);
             }
             if (!fast_patching_enabled) elastic_run();
             total_flips += batch;
@@ -1136,6 +1174,8 @@ protected:
     int last_iters_ = 0;
     std::vector<double> unstable_q_;
     std::vector<Mat<D>> redraw_cat_;
+    std::vector<Mat<D>> round_cat_;   // a cascade round's redraws
+    std::vector<double> round_q0_;
     std::vector<double> redraw_q0_;
 };
 
