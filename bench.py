@@ -33,7 +33,8 @@ PRELOAD_STRAIN = 0.01
 
 
 PASS_NAMES = {0: "k_fwd_last", 1: "k_mid<fwd>", 2: "k_first_gamma", 3: "k_mid<inv>", 4: "k_inv_last",
-              5: "k_fwd_last+cg_update", 6: "k_fwd_plane+cg_update", 7: "k_first_gamma<plane>", 8: "k_inv_plane"}
+              5: "k_fwd_last+cg_update", 6: "k_fwd_plane+cg_update", 7: "k_first_gamma<plane>", 8: "k_inv_plane",
+              9: "k_fwd_plane64", 10: "k_first_gamma<plane>", 11: "k_inv_plane64"}
 
 
 def plane_passes():
@@ -42,10 +43,16 @@ def plane_passes():
     return os.environ.get("MGK_LS_PLANE", "1") != "0"
 
 
+def plane64_passes():
+    """The FP64 iterations run as plane passes on the 64^3 grid unless
+    MGK_LS_PLANE64=0."""
+    return os.environ.get("MGK_LS_PLANE64", "1") != "0"
+
+
 def _passes(fp32):
     """Pipeline passes of one precision as the event loop runs them."""
     if not fp32:
-        return [0, 1, 2, 3, 4]
+        return [9, 10, 11] if plane64_passes() else [0, 1, 2, 3, 4]
     return [6, 7, 8] if plane_passes() else [0, 1, 2, 3, 4, 5]
 
 
@@ -55,11 +62,11 @@ def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
     h = (nl // 2 + 1) / nl
     cs = 8 if fp32 else 16           # complex bytes
     spec = 6 * h * cs                # one half-spectrum tensor field per voxel
-    if pass_id == 0:                 # y, dlam, dmu | eps, eps_p, lam, mu  -> spectrum
+    if pass_id in (0, 9):            # y, dlam, dmu | eps, eps_p, lam, mu  -> spectrum
         return ((6 * 4 + 2 * 4) if fp32 else (6 * 8 + 6 * 8 + 2 * 8)) + spec
     if pass_id in (5, 6):            # r w p s x read, p s x r written, dlam dmu -> spectrum
         return 5 * 6 * 4 + 4 * 6 * 4 + 2 * 4 + spec
-    if pass_id in (4, 8):            # spectrum -> w (reads r) | eps_new (reads eps, writes the FP32 increment)
+    if pass_id in (4, 8, 11):        # spectrum -> w (reads r) | eps_new (reads eps, writes the FP32 increment)
         return spec + (2 * 6 * 4 if fp32 else 2 * 6 * 8 + 6 * 4)
     return 2 * spec                  # B, C, D: read and write the spectrum
 
@@ -73,16 +80,17 @@ def _algorithmic_bytes_per_voxel(pass_id, fp32, nl):
 #   pass C  the inverse transforms and the update: reads C s and the old
 #           iterate (C s), writes C s.
 # b_iter = (7C + 2) s + 8C: 224 B (FP32), 400 B (FP64).  The pipeline's passes
-# map onto the model as: FP32 plane passes 6 -> A, 7 -> B, 8 -> C; the
-# five-pass pipeline 0 + 1 -> A, 2 -> B, 3 + 4 -> C (the model's bytes of a
-# split pass go to its first kernel and 0 to the second, so a per-kernel
-# figure is only quoted for 6, 7, 8, 2 and the whole-event figure below).
+# map onto the model as: FP32 plane passes 6 -> A, 7 -> B, 8 -> C, FP64
+# plane passes 9 -> A, 10 -> B, 11 -> C; the five-pass pipeline 0 + 1 -> A,
+# 2 -> B, 3 + 4 -> C (the model's bytes of a split pass go to its first
+# kernel and 0 to the second, so a per-kernel figure is only quoted for the
+# plane passes, 2 and the whole-event figure below).
 def _model_bytes_per_voxel(pass_id, fp32):
     Cc, sz = 6, (4 if fp32 else 8)
     a = (2 * Cc + 2) * sz + 8 * Cc
     b = 2 * Cc * sz
     c = 3 * Cc * sz
-    return {0: a, 1: 0, 2: b, 3: 0, 4: c, 5: a, 6: a, 7: b, 8: c}[pass_id]
+    return {0: a, 1: 0, 2: b, 3: 0, 4: c, 5: a, 6: a, 7: b, 8: c, 9: a, 10: b, 11: c}[pass_id]
 
 
 def _model_event_bytes(info32, N, M=12):

@@ -288,7 +288,9 @@ int mgk_timer_stop(mgk_ctx* ctx, double* ms);
  * update, FP32+refine contexts only; the plane passes of 3D grids with square
  * nm = nl planes, FP32 only: 6 forward plane with the CG vector update
  * (FP32+refine contexts), 7 pass 2 on the plane layout, 8 inverse plane
- * without its reduction), FP64 (fp32 == 0) or FP32, launched
+ * without its reduction; the FP64 plane passes of such grids with planes of
+ * up to 64 points, FP64 only: 9 forward plane, 10 pass 2 on the tiled FP64
+ * spectrum, 11 inverse plane without its reduction), FP64 (fp32 == 0) or FP32, launched
  * reps times back to back on the context's stream between two HIP events.
  * Leaves the spectral buffer, the FP32 CG vectors and (pass 4 in FP64) the
  * strain field undefined: a measurement context, not a simulation one. */

@@ -286,6 +286,7 @@ struct mgk_ctx {
         p.partials = partials;
         p.gbar = (precision == MGK_FP32_REFINE && use_cycle && ls_cycle_supported(g))
                      ? reinterpret_cast<unsigned*>(partials + ls_gbar_offset(g)) : nullptr;
+        p.r0_cart = (precision == MGK_FP32_REFINE && ls_plane64_path(g)) ? 1 : 0;
         return p;
     }
 
@@ -300,6 +301,7 @@ struct mgk_ctx {
             lsp_up[slot] = true;
         }
         LsLaunch L{g, d_lsp + slot, 1, h_lsp[slot], false, h_lsp[slot].mandel != 0};
+        L.plane64 = ls_plane64_path(g);
         L.tune = tune;
         return L;
     }
@@ -626,12 +628,14 @@ void autotune(mgk_ctx& c) {
         const char* e = std::getenv("MGK_AUTOTUNE");
         if (!(e && e[0] == '0')) {
             const bool refine = (c.precision == MGK_FP32_REFINE);
+            const bool p64 = ls_plane64_path(g);   // FP64 iterations on the plane passes: no pass A or M pass
             const bool shapes_a = (g.C == 6 && g.nl >= 32);
             const bool shapes_m = (g.C == 6 && g.nm >= 32);
             const bool cg5 = refine && !ls_plane_path(g);   // FP32 CG steps on the five-pass pipeline
             auto pick = [](double d, double v) { return v < 0.97 * d ? 1 : 0; };
             for (int m = 0; m < 2; ++m) {
                 if (m == 1 && !cg5) continue;
+                if (m == 0 && p64) continue;
                 const int mode = (m == 0) ? LS_FULL : LS_CG;
                 if (shapes_a) {
                     LsTune t = rec.tune;
@@ -1602,9 +1606,11 @@ int mgk_cycle_supported(mgk_ctx* ctx, int* on) {
 int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms) {
     return guarded([&]() -> int {
         if (!ctx || !avg_ms || reps < 1) throw ArgError("bad argument");
-        if (pass < 0 || pass > 8 || (ctx->g.nm == 1 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
+        if (pass < 0 || pass > 11 || (ctx->g.nm == 1 && (pass == 1 || pass == 3))) throw ArgError("bad pass");
         if ((pass == 5 || pass == 6) && (!fp32 || !ctx->corr)) throw ArgError("passes 5 and 6 need an FP32+refine context");
-        if (pass >= 6 && (!fp32 || !ls_plane_supported(ctx->g))) throw ArgError("plane passes: FP32 on a supported 3D grid");
+        if (pass >= 6 && pass <= 8 && (!fp32 || !ls_plane_supported(ctx->g)))
+            throw ArgError("plane passes: FP32 on a supported 3D grid");
+        if (pass >= 9 && (fp32 || !ls_plane64_path(ctx->g))) throw ArgError("FP64 plane passes: FP64 on a supported 3D grid");
         mgk_ctx& c = *ctx;
         // a control block that lets every pass run (not done, shift off)
         std::memset(c.hctl, 0, sizeof(Ctl));
@@ -2168,6 +2174,7 @@ void batch_event(mgk_batch& b, const mgk_event_in* in, const int* active, double
     hipStream_t s = b.stream;
     const KmcLaunch KL{g, c0.kp.M, b.d_kp, B, b.h_kp[0], true};
     LsLaunch L{g, b.d_lsp, B, b.h_lsp[0], true, ls_plane_path(g)};
+    L.plane64 = ls_plane64_path(g);
     L.tune = c0.tune;   // every member has the batch's shape, hence its tuning
     const int nb0 = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
     auto iterations = [&](int from, int count, hipStream_t st) {

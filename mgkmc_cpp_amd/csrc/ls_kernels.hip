@@ -402,22 +402,24 @@ template <typename SR, int N, int HALF> constexpr int mid_cols_v() {
 
 // Frequency points (all C components each) per block of the F pass with
 // LANES lanes per block.
-template <typename SR, int N, int C, int LANES = col_threads<SR, C>()> constexpr int gamma_points() {
+// TILED: the plane passes' spectrum, read in whole tiles of kPlaneTile points.
+template <typename SR, int N, int C, int LANES = col_threads<SR, C>(), bool TILED = false> constexpr int gamma_points() {
     constexpr int T = FftPlan<N>::threads;
     constexpr int by_lds =
         (int)((kColLdsBudget - N * (int)sizeof(cplx<SR>)) / (C * padded<N>() * (int)sizeof(cplx<SR>)));
     constexpr int by_thr = LANES / (T * C);
     constexpr int most = cmax(1, cmin(cmin(by_thr, by_lds), MGK_GAMMA_POINTS_CAP));
     constexpr int point_bytes = C * (int)sizeof(cplx<SR>);
-    constexpr int g_align = 128 / cgcd(128, point_bytes);   // points per whole number of lines
+    constexpr int g_line = 128 / cgcd(128, point_bytes);   // points per whole number of lines
+    constexpr int g_align = TILED ? g_line * kPlaneTile / cgcd(g_line, kPlaneTile) : g_line;
     return (C == 6 && most >= g_align) ? (most / g_align) * g_align : most;
 }
 
 // Lanes per block of the F pass: the FFT lanes (G*C lines of T lanes), or
 // more when that leaves the Green operator's N*G (point, frequency) items
 // short of one per lane; always a multiple of G.
-template <typename SR, int N, int C, int LANES = col_threads<SR, C>()> constexpr int gamma_threads() {
-    constexpr int G = gamma_points<SR, N, C, LANES>();
+template <typename SR, int N, int C, int LANES = col_threads<SR, C>(), bool TILED = false> constexpr int gamma_threads() {
+    constexpr int G = gamma_points<SR, N, C, LANES, TILED>();
     constexpr int fft = G * C * FftPlan<N>::threads;
     constexpr int green = (cmin(N * G, cmax(LANES, kColThreadsMax)) / G) * G;
     return cmax(fft, green);
@@ -679,7 +681,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
 // (l % (C*T8)) / T8).
 template <typename SR, int NF, int C, bool PLANE, int LANES = col_threads<SR, C>()> struct GammaLanes {
     static constexpr int T = FftPlan<NF>::threads;
-    static constexpr int G = gamma_points<SR, NF, C, LANES>();
+    static constexpr int G = gamma_points<SR, NF, C, LANES, PLANE>();
     static constexpr int L = G * C;
     static constexpr int T8 = kPlaneTile;
     int l, t, gl, cl;
@@ -703,7 +705,9 @@ __device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threa
                                            const Col& column, const cplx<SR>* tw, cplx<SR>* work, long xi0,
                                            const Geom& g, double lam0, double mu0, int tid, int nthr) {
     constexpr int D = (C == 3) ? 2 : 3;
-    constexpr int G = gamma_points<SR, NF, C, LANES>();
+    constexpr int G = gamma_points<SR, NF, C, LANES, PLANE>();
+    // the FP32 plane spectra hold Mandel components, the FP64 ones Cartesian
+    constexpr bool MANDEL = PLANE && C == 6 && std::is_same<SR, float>::value;
     constexpr int NFP = padded<NF>();
     using cx = cplx<SR>;
     const long nxi = (long)g.nm * g.nlh;
@@ -756,7 +760,7 @@ __device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threa
         cx tt[C], o[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) tt[c] = work[(gg * C + c) * NFP + f];
-        if constexpr (PLANE && C == 6) {   // tau_hat in Mandel components
+        if constexpr (MANDEL) {   // tau_hat in Mandel components
             cx t6[6];
             mandel_to_cart_c(tt, t6);
 #pragma unroll
@@ -785,7 +789,7 @@ __device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threa
                 for (int c = 0; c < C; ++c) o[c] = {(SR)0.5 * (o[c].x + o2[c].x), (SR)0.5 * (o[c].y + o2[c].y)};
             }
         }
-        if constexpr (PLANE && C == 6) {
+        if constexpr (MANDEL) {
             cx m6[6];
             cart_to_mandel_c(o, m6);
 #pragma unroll
@@ -799,11 +803,11 @@ __device__ __forceinline__ void gamma_core(cplx<SR> (&v)[NF / FftPlan<NF>::threa
     line_finish<NF, +1, SR, true, false>(v, t, active, column, mine, tw);
 }
 
-template <typename SR, int NF, int C, bool BATCH, bool PLANE = false>
+template <typename SR, int NF, int C, bool BATCH, bool PLANE = false, int LANES = col_threads<SR, C>()>
 __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p_arg, const LsParams* __restrict__ P, int mode,
                                                                 int speculative, int step) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
-    constexpr int G = gamma_points<SR, NF, C>();
+    constexpr int G = gamma_points<SR, NF, C, LANES, PLANE>();
     using cx = cplx<SR>;
     const bool stop = (p.ctl->done | (mode == LS_CG ? (p.ctl->cg_stop | past_cycle(p.ctl, step)) : 0)) != 0;
     if (speculative && stop) return;   // a speculative step: skip the loads
@@ -813,13 +817,14 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     const long nxi = (long)g.nm * g.nlh;
     const long xi0 = (long)blockIdx.x * G;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
-    const GammaLanes<SR, NF, C, PLANE> ln(xi0, nxi, (int)threadIdx.x);
+    const GammaLanes<SR, NF, C, PLANE, LANES> ln(xi0, nxi, (int)threadIdx.x);
     const LineRef<SR> column{spec + (ln.active ? xi0 * C + ln.l : 0), nxi * C};
     cx v[NF / FftPlan<NF>::threads];
     line_gather<NF, SR>(column, ln.t, ln.active, v);
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
     if (stop) return;
-    gamma_core<SR, NF, C, PLANE>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x, (int)blockDim.x);
+    gamma_core<SR, NF, C, PLANE, LineRef<SR>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x,
+                                                     (int)blockDim.x);
 }
 
 // Pass C on the tiled spectrum of 128-point plane grids: 1024 lanes, so a
@@ -1244,24 +1249,24 @@ __device__ __forceinline__ void plane_update_tau(int q, bool upd, bool upd1, flo
 // Forward in-plane transforms of the packed rows in LDS: r2c along L (two
 // rows per complex line, split into the two half spectra), c2c along M; the
 // plane's half spectrum leaves through st(idx, value), idx = m*NLH + kb.
-template <int NP, class St>
-__device__ __forceinline__ void plane_fwd_fft(cplx<float>* buf, cplx<float>* pl, const cplx<float>* tw_l,
-                                              const cplx<float>* tw_m, const St& st, int tid) {
+template <int NP, class St, typename SR = float>
+__device__ __forceinline__ void plane_fwd_fft(cplx<SR>* buf, cplx<SR>* pl, const cplx<SR>* tw_l,
+                                              const cplx<SR>* tw_m, const St& st, int tid) {
     using PD = PlaneDims<NP>;
-    using cx = cplx<float>;
+    using cx = cplx<SR>;
     constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
     __syncthreads();
-    lds_fft_lane<NL, -1, float>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+    lds_fft_lane<NL, -1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
     // split each packed row pair into the two half spectra, transposed
     for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
         const int pr = idx / NLH, kb = idx % NLH;
         const cx z = buf[pr * NLP + kb];
         const cx zm = buf[pr * NLP + (NL - kb) % NL];
-        pl[kb * NMP + 2 * pr] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
-        pl[kb * NMP + 2 * pr + 1] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
+        pl[kb * NMP + 2 * pr] = {(SR)0.5 * (z.x + zm.x), (SR)0.5 * (z.y - zm.y)};
+        pl[kb * NMP + 2 * pr + 1] = {(SR)0.5 * (z.y + zm.y), (SR)-0.5 * (z.x - zm.x)};
     }
     __syncthreads();
-    lds_fft_lane<NM, -1, float>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
+    lds_fft_lane<NM, -1, SR>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
     for (int idx = tid; idx < NM * NLH; idx += kPlaneThreads) {
         const int m = idx / NLH, kb = idx % NLH;
         st(idx, pl[kb * NMP + m]);
@@ -1271,19 +1276,34 @@ __device__ __forceinline__ void plane_fwd_fft(cplx<float>* buf, cplx<float>* pl,
 // Inverse in-plane transforms: the plane's half spectrum (point
 // idx = threadIdx.x + kPlaneThreads*k in sp[k]) into LDS, c2c along M, c2r
 // along L; eps_tilde_c is left in buf's packed-row layout.
-template <int NP>
-__device__ __forceinline__ void plane_inv_fft(const cplx<float> (&sp)[PlaneDims<NP>::SPT], cplx<float>* buf,
-                                              cplx<float>* pl, const cplx<float>* tw_l, const cplx<float>* tw_m, int tid) {
+template <int NP, typename SR>
+__device__ __forceinline__ void plane_inv_fft_staged(cplx<SR>* buf, cplx<SR>* pl, const cplx<SR>* tw_l,
+                                                     const cplx<SR>* tw_m, int tid);
+
+template <int NP, typename SR = float>
+__device__ __forceinline__ void plane_inv_fft(const cplx<SR> (&sp)[PlaneDims<NP>::SPT], cplx<SR>* buf,
+                                              cplx<SR>* pl, const cplx<SR>* tw_l, const cplx<SR>* tw_m, int tid) {
     using PD = PlaneDims<NP>;
-    using cx = cplx<float>;
+    using cx = cplx<SR>;
     constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
 #pragma unroll
     for (int k = 0; k < PD::SPT; ++k) {
         const int idx = tid + kPlaneThreads * k;
         if (idx < NM * NLH) pl[(idx % NLH) * NMP + idx / NLH] = sp[k];
     }
+    plane_inv_fft_staged<NP, SR>(buf, pl, tw_l, tw_m, tid);
+}
+
+// plane_inv_fft after the spectrum's scatter: bin (m, kb) is in
+// pl[kb * NMP + m], not yet behind a barrier.
+template <int NP, typename SR>
+__device__ __forceinline__ void plane_inv_fft_staged(cplx<SR>* buf, cplx<SR>* pl, const cplx<SR>* tw_l,
+                                                     const cplx<SR>* tw_m, int tid) {
+    using PD = PlaneDims<NP>;
+    using cx = cplx<SR>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
     __syncthreads();
-    lds_fft_lane<NM, +1, float>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
+    lds_fft_lane<NM, +1, SR>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
     // pack row pairs into complex lines for the c2r along L
     for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
         const int pr = idx / NLH, kb = idx % NLH;
@@ -1297,7 +1317,7 @@ __device__ __forceinline__ void plane_inv_fft(const cplx<float> (&sp)[PlaneDims<
         }
     }
     __syncthreads();
-    lds_fft_lane<NL, +1, float>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+    lds_fft_lane<NL, +1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
 }
 
 // w_c = r_c - eps_tilde_c (eps_tilde in buf, as plane_inv_fft leaves it) and
@@ -1387,6 +1407,33 @@ __device__ __forceinline__ void plane_fold_sum(const double (&part)[3][PU], int 
     }
 }
 
+// Mandel component m of four voxels' Cartesian increment (the FP64 plane
+// passes' d in the p storage), as the five-pass pass E converts it.
+__device__ __forceinline__ vox4 r0_from_cart(const LsParams& p, int m, long base_plane, int q) {
+    const long N = p.g.N;
+    vox4 out;
+    if (m < 3) {
+        const vox4 a = ld_vox4(p.p32 + base_plane, q), b = ld_vox4(p.p32 + N + base_plane, q),
+                   cz = ld_vox4(p.p32 + 2 * N + base_plane, q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double t[6] = {(double)a[e], (double)b[e], (double)cz[e], 0.0, 0.0, 0.0}, mm[6];
+            cart_to_mandel(t, mm);
+            out[e] = (float)mm[m];
+        }
+    } else {
+        const vox4 a = ld_vox4(p.p32 + (long)m * N + base_plane, q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, mm[6];
+            t[m] = (double)a[e];
+            cart_to_mandel(t, mm);
+            out[e] = (float)mm[m];
+        }
+    }
+    return out;
+}
+
 template <int NP, bool BATCH>
 __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
@@ -1431,12 +1478,15 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
     constexpr int QC = PD::QC, NCH = Q4 / QC;
     const vox4 zero4 = {0.f, 0.f, 0.f, 0.f};
     vox4 r[QC], w[QC], pv[QC], sv[QC], xv[QC], dm[QC], dl[QC];
+    // the first step of a cycle after an FP64 plane iteration starts from the
+    // Cartesian increment (and stores its Mandel components for the steps after)
+    const bool r_cart = p.r0_cart && (flags & LS_FLAG_CG_FIRST);
     auto load_chunk = [&](int ch) {
 #pragma unroll
         for (int k = 0; k < QC; ++k) {
             const int q = threadIdx.x + kPlaneThreads * (ch * QC + k);
             const int qq = q < V4 ? q : 0;
-            r[k] = ld_vox4(p.r32 + base, qq);
+            r[k] = r_cart ? r0_from_cart(p, c, (long)f * VOX, qq) : ld_vox4(p.r32 + base, qq);
             dm[k] = ld_vox4(p.dmu32 + (long)f * VOX, qq);
             dl[k] = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
             w[k] = pv[k] = sv[k] = xv[k] = zero4;
@@ -1480,6 +1530,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
             if (q >= V4) continue;
             plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv[k], xv[k], tau_factor(c, dl[k], dm[k]),
                                  bufr);
+            if (r_cart) st_vox4(p.r32 + base, q, r[k]);
             if (upd) {
                 st_vox4(p.p32 + base, q, pv[k]);
                 st_vox4(p.s32 + base, q, sv[k]);
@@ -1543,6 +1594,206 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
         const float* bufr = reinterpret_cast<const float*>(buf);
         plane_w_partials_stream<NP>(bufr, c, p.lam0, p.mu0, p.r32 + base, p.w32 + base, acc, (int)threadIdx.x);
+    }
+    block_sum<3>(acc, red);
+    if (threadIdx.x == 0) {
+        p.partials[3 * blockIdx.x] = acc[0];
+        p.partials[3 * blockIdx.x + 1] = acc[1];
+        p.partials[3 * blockIdx.x + 2] = acc[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 plane passes (the basic scheme's iteration on square planes of up to
+// 64 points): one block per (Cartesian component c, plane f), three launches
+// per iteration instead of five.
+//   k_fwd_plane64: tau_c = (C - C0):eps' - C:eps_p of component c (a normal
+//                  component's block also reads the other two normal
+//                  components for the traces), both in-plane transforms;
+//   pass C on the tiled FP64 spectrum (Cartesian components);
+//   k_inv_plane64: both inverse transforms, eps_c = eps_bar_c + eps_tilde_c,
+//                  the block's share of the relative-increment norms, and in
+//                  FP32+refine contexts the increment d_c, written Cartesian
+//                  into the p vector's storage: the CG cycle that follows
+//                  converts it to Mandel components when it starts
+//                  (LsParams::r0_cart).
+// eps' = eps + warm-start shift + a pending CG iterate (x, then its last
+// residual, Mandel components converted as pass A converts them).
+// ---------------------------------------------------------------------------
+template <int NP> constexpr size_t plane64_lds_bytes() {
+    return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<double>);
+}
+
+// eps' of Cartesian component c at voxel v (and, for a normal component, the
+// traces of eps' and eps' - eps_p over the normal components).
+__device__ __forceinline__ void eps_prime64(const LsParams& p, int c, long v, bool add_shift, bool add_corr,
+                                            double& ek_c, double& ee_c, double& tr0, double& tre) {
+    const long N = p.g.N;
+    if (c < 3) {
+        double crc[6] = {0, 0, 0, 0, 0, 0}, rrc[6] = {0, 0, 0, 0, 0, 0}, tc[6], tr[6];
+        if (add_corr) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                crc[m] = (double)p.corr[m * N + v];
+                rrc[m] = p.r32 ? (double)p.r32[m * N + v] : 0.0;
+            }
+        }
+        mandel_to_cart(crc, tc);
+        mandel_to_cart(rrc, tr);
+        double ek[3], ee[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            ek[j] = p.eps[j * N + v] + (add_shift ? p.ctl->shift[j] : 0.0);
+            if (add_corr) {
+                ek[j] += tc[j];
+                ek[j] += tr[j];
+            }
+            ee[j] = p.epsp ? ek[j] - p.epsp[j * N + v] : ek[j];
+        }
+        tr0 = 0.0;
+        tre = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) { tre += ee[j]; tr0 += ek[j]; }
+        ek_c = ek[c];
+        ee_c = ee[c];
+    } else {
+        double e = p.eps[c * N + v] + (add_shift ? p.ctl->shift[c] : 0.0);
+        if (add_corr) {
+            double m6[6] = {0, 0, 0, 0, 0, 0}, t6[6];
+            m6[c] = (double)p.corr[c * N + v];
+            mandel_to_cart(m6, t6);
+            e += t6[c];
+            m6[c] = p.r32 ? (double)p.r32[c * N + v] : 0.0;
+            mandel_to_cart(m6, t6);
+            e += t6[c];
+        }
+        ek_c = e;
+        ee_c = p.epsp ? e - p.epsp[c * N + v] : e;
+        tr0 = tre = 0.0;
+    }
+}
+
+// eps' of Cartesian component c alone (the update of pass E).
+__device__ __forceinline__ double eps_prime64_c(const LsParams& p, int c, long v, bool add_shift, bool add_corr) {
+    const long N = p.g.N;
+    double e = p.eps[c * N + v] + (add_shift ? p.ctl->shift[c] : 0.0);
+    if (add_corr) {
+        double xm[6] = {0, 0, 0, 0, 0, 0}, rm[6] = {0, 0, 0, 0, 0, 0}, tx[6], trr[6];
+        if (c < 3) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                xm[m] = (double)p.corr[m * N + v];
+                rm[m] = p.r32 ? (double)p.r32[m * N + v] : 0.0;
+            }
+        } else {
+            const double xc = (double)p.corr[c * N + v], rc = p.r32 ? (double)p.r32[c * N + v] : 0.0;
+#pragma unroll
+            for (int m = 3; m < 6; ++m)   // selects instead of a dynamic index keep the arrays in registers
+                if (m == c) { xm[m] = xc; rm[m] = rc; }
+        }
+        mandel_to_cart(xm, tx);
+        mandel_to_cart(rm, trr);
+        double txc = 0.0, trc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+            if (m == c) { txc = tx[m]; trc = trr[m]; }
+        e += txc;
+        e += trc;
+    }
+    return e;
+}
+
+template <int NP, bool BATCH>
+__global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX;
+    using cx = cplx<double>;
+    if (p.ctl->done) return;
+    cx* tw_l = reinterpret_cast<cx*>(smem_raw);
+    cx* tw_m = tw_l + NL;
+    cx* buf = tw_m + NM;
+    cx* pl = buf + (NM / 2) * NLP;
+    const Geom g = p.g;
+    const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
+    stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
+    stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
+    const bool add_shift = flags & LS_FLAG_SHIFT;
+    const bool add_corr = (flags & LS_FLAG_CORR) && p.corr && p.ctl->corr_valid;
+    const double tm0 = 2.0 * p.mu0;
+    double* bufr = reinterpret_cast<double*>(buf);
+#pragma unroll 2
+    for (int k = 0; k < (VOX + kPlaneThreads - 1) / kPlaneThreads; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        if (VOX % kPlaneThreads != 0 && q >= VOX) break;   // planes of 16 and 48 points
+        const long v = (long)f * VOX + q;
+        double ek_c, ee_c, tr0, tre;
+        eps_prime64(p, c, v, add_shift, add_corr, ek_c, ee_c, tr0, tre);
+        const double tm = 2.0 * p.mu[v], lt = p.lam[v] * tre, lt0 = p.lam0 * tr0;
+        const double sv = tm * ee_c + (c < 3 ? lt : 0.0);
+        const double s0 = tm0 * ek_c + (c < 3 ? lt0 : 0.0);
+        const int m = q / NL, j = q % NL;
+        bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = sv - s0;
+    }
+    cx* spec = reinterpret_cast<cx*>(p.spec64) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
+    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { spec[tile_off(idx)] = v; }, (int)threadIdx.x);
+}
+
+template <int NP, bool BATCH>
+__global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+    const LsParams& p = param_sel<BATCH>(p_arg, P);
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, SPT = PD::SPT;
+    using cx = cplx<double>;
+    if (p.ctl->done) return;
+    cx* tw_l = reinterpret_cast<cx*>(smem_raw);
+    cx* tw_m = tw_l + NL;
+    cx* buf = tw_m + NM;
+    cx* pl = buf + (NM / 2) * NLP;
+    __shared__ double red[16 * 3];
+    const Geom g = p.g;
+    const int c = blockIdx.x / g.nf, f = blockIdx.x % g.nf;
+    const cx* spec = reinterpret_cast<const cx*>(p.spec64) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
+    if constexpr (!BATCH) {
+        cx sp[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int idx = threadIdx.x + kPlaneThreads * k;
+            sp[k] = spec[tile_off(idx < PD::NS ? idx : 0)];
+        }
+        stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
+        stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
+        plane_inv_fft<NP, double>(sp, buf, pl, tw_l, tw_m, (int)threadIdx.x);
+    } else {
+        // Batched launches read the twiddle pointers through the parameter
+        // block, and a register copy of the spectrum ends up in scratch
+        // there: the bins go straight to LDS instead (same placement).
+        stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
+        stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
+        for (int idx = threadIdx.x; idx < PD::NS; idx += kPlaneThreads)
+            pl[(idx % PD::NLH) * PD::NMP + idx / PD::NLH] = spec[tile_off(idx)];
+        plane_inv_fft_staged<NP, double>(buf, pl, tw_l, tw_m, (int)threadIdx.x);
+    }
+    const bool add_shift = flags & LS_FLAG_SHIFT;
+    const bool add_corr = (flags & LS_FLAG_CORR) && p.corr && p.ctl->corr_valid;
+    const double bar = p.ctl->eps_bar[c];
+    const double wgt = (c < 3) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
+    const double* bufr = reinterpret_cast<const double*>(buf);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int k = 0; k < (VOX + kPlaneThreads - 1) / kPlaneThreads; ++k) {
+        const int q = threadIdx.x + kPlaneThreads * k;
+        if (VOX % kPlaneThreads != 0 && q >= VOX) break;   // planes of 16 and 48 points
+        const long v = (long)f * VOX + q;
+        const double o = eps_prime64_c(p, c, v, add_shift, add_corr);
+        const int m = q / NL, j = q % NL;
+        const double nw = bar + bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
+        const double d = nw - o;
+        acc[0] += wgt * d * d;
+        acc[1] += wgt * o * o;
+        acc[2] += wgt * nw * nw;
+        p.eps[(long)c * g.N + v] = nw;
+        if (p.r32) p.p32[(long)c * g.N + v] = (float)d;   // the increment that starts a CG cycle (Cartesian)
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
@@ -1631,7 +1882,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
     for (int k = 0; k < Q4; ++k) {
         const int q = threadIdx.x + kPlaneThreads * k;
         const int qq = q < V4 ? q : 0;
-        r[k] = ld_vox4(p.r32 + base, qq);
+        r[k] = p.r0_cart ? r0_from_cart(p, c, (long)f * VOX, qq) : ld_vox4(p.r32 + base, qq);
         const vox4 dm = ld_vox4(p.dmu32 + (long)f * VOX, qq);
         const vox4 dl = (c == 0) ? ld_vox4(p.dlam32 + (long)f * VOX, qq) : zero4;
         tf[k] = tau_factor(c, dl, dm);
@@ -2070,7 +2321,19 @@ static long partial_rows(const Geom& g) {
 int ls_partial_slots(const Geom& g) { return (int)(partial_rows(g) + kBarWords / 2); }
 long ls_gbar_offset(const Geom& g) { return partial_rows(g); }
 
+template <bool BATCH> static void launch_fwd_plane64(const LsLaunch& p, int flags, hipStream_t s);
+template <bool BATCH> static void launch_first_gamma64_tiled(const LsLaunch& p, hipStream_t s);
+template <bool BATCH> static void launch_inv_plane64(const LsLaunch& p, int flags, hipStream_t s, bool reduce);
+
 template <int C, bool BATCH> static void iteration_full(const LsLaunch& p, int flags, hipStream_t s) {
+    if constexpr (C == 6) {
+        if (p.plane64) {
+            launch_fwd_plane64<BATCH>(p, flags, s);
+            launch_first_gamma64_tiled<BATCH>(p, s);
+            launch_inv_plane64<BATCH>(p, flags, s, true);
+            return;
+        }
+    }
     launch_fwd_last<double, C, LS_FULL, BATCH>(p, flags, s);
     if (C == 6 && p.g.nm > 1) launch_mid<double, -1, BATCH>(p, s, LS_FULL);
     launch_first_gamma<double, C, BATCH>(p, s, LS_FULL);
@@ -2198,6 +2461,74 @@ void ls_enqueue_gbar_reset(const LsLaunch& p, hipStream_t s) {
     MGK_HIP_CHECK(hipGetLastError());
 }
 
+#define MGK_PLANE64_CASES(X) X(16) X(32) X(48) X(64)
+
+// FP64 pass C on the tiled spectrum needs whole tiles per block.
+static bool plane64_gamma_tiles(int nf) {
+#define MGK_CASE(NN) if (nf == NN) return gamma_points<double, NN, 6, 512, true>() % kPlaneTile == 0;
+    MGK_LEN_CASES(MGK_CASE)
+#undef MGK_CASE
+    return false;
+}
+
+bool ls_plane64_path(const Geom& g) {
+    const char* e = std::getenv("MGK_LS_PLANE64");
+    if ((e && e[0] == '0') || !ls_plane_path(g) || !plane64_gamma_tiles(g.nf)) return false;
+#define MGK_CASE(NN) if (g.nm == NN) return true;
+    MGK_PLANE64_CASES(MGK_CASE)
+#undef MGK_CASE
+    return false;
+}
+
+// FP64 pass C on the tiled spectrum: 512 lanes, whole tiles per block.
+template <bool BATCH> static void launch_first_gamma64_tiled(const LsLaunch& p, hipStream_t s) {
+    const long nxi = (long)p.g.nm * p.g.nlh;
+#define MGK_CASE(NN)                                                                                          \
+    if (p.g.nf == NN) {                                                                                       \
+        constexpr int G = gamma_points<double, NN, 6, 512, true>();                                           \
+        if constexpr (G % kPlaneTile == 0) {                                                                  \
+            const size_t sm = (size_t)(NN + G * 6 * padded<NN>()) * sizeof(cplx<double>);                     \
+            hipLaunchKernelGGL((k_first_gamma<double, NN, 6, BATCH, true, 512>), dim3(blocks_for(nxi, G), p.B), \
+                               dim3(gamma_threads<double, NN, 6, 512, true>()), sm, s, p.h, p.P, LS_FULL, 0, 0);  \
+            return;                                                                                           \
+        }                                                                                                     \
+    }
+    MGK_LEN_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("FP64 plane pass C: first-axis length without whole tiles");
+}
+
+template <bool BATCH> static void launch_fwd_plane64(const LsLaunch& p, int flags, hipStream_t s) {
+#define MGK_CASE(NN)                                                                                  \
+    if (p.g.nm == NN) {                                                                               \
+        static const bool lds_ok = (allow_lds(k_fwd_plane64<NN, BATCH>, plane64_lds_bytes<NN>()), true); \
+        (void)lds_ok;                                                                                 \
+        hipLaunchKernelGGL((k_fwd_plane64<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads),     \
+                           plane64_lds_bytes<NN>(), s, p.h, p.P, flags);                             \
+        return;                                                                                       \
+    }
+    MGK_PLANE64_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("FP64 plane length not supported");
+}
+
+template <bool BATCH> static void launch_inv_plane64(const LsLaunch& p, int flags, hipStream_t s, bool reduce) {
+    const int nb = 6 * p.g.nf;
+#define MGK_CASE(NN)                                                                                  \
+    if (p.g.nm == NN) {                                                                               \
+        static const bool lds_ok = (allow_lds(k_inv_plane64<NN, BATCH>, plane64_lds_bytes<NN>()), true); \
+        (void)lds_ok;                                                                                 \
+        hipLaunchKernelGGL((k_inv_plane64<NN, BATCH>), dim3(nb, p.B), dim3(kPlaneThreads),            \
+                           plane64_lds_bytes<NN>(), s, p.h, p.P, flags);                             \
+        if (reduce)                                                                                   \
+            hipLaunchKernelGGL((k_ls_reduce<LS_FULL, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb, flags); \
+        return;                                                                                       \
+    }
+    MGK_PLANE64_CASES(MGK_CASE)
+#undef MGK_CASE
+    throw UnsupportedError("FP64 plane length not supported");
+}
+
 template <int C, bool BATCH> static void iteration_delta(const LsLaunch& p, int flags, hipStream_t s) {
     const int spec = (flags & LS_FLAG_SPEC) ? 1 : 0;
     const int step = flags >> LS_STEP_SHIFT;
@@ -2272,6 +2603,18 @@ template <int C, bool BATCH> static void enqueue_pass_c(const LsLaunch& p, int p
         case 5:
             if (full) throw ArgError("the CG update pass runs in FP32 only");
             launch_fwd_last<float, C, LS_CG, BATCH>(p, LS_FLAG_CG_UPDATE, s);
+            break;
+        case 9:
+        case 10:
+        case 11:
+            if constexpr (C == 6) {
+                if (!full || !ls_plane64_path(p.g)) throw ArgError("FP64 plane passes run in FP64 on supported 3D grids");
+                if (pass == 9) launch_fwd_plane64<BATCH>(p, 0, s);
+                else if (pass == 10) launch_first_gamma64_tiled<BATCH>(p, s);
+                else launch_inv_plane64<BATCH>(p, 0, s, false);
+            } else {
+                throw ArgError("plane passes are 3D only");
+            }
             break;
         case 6:
         case 7:

@@ -56,6 +56,7 @@ struct LsParams {
     Ctl* ctl;
     double* partials;
     unsigned* gbar;         // grid barrier words of the persistent CG cycle (inside the partials buffer), or null
+    int r0_cart;            // FP64 plane passes: a cycle's starting increment is Cartesian, in p32's storage
 };
 
 // One launch of a pipeline step over B trajectories of one shape: row
@@ -76,6 +77,7 @@ struct LsLaunch {
     LsParams h;   // host copy of a single context's parameter block (passed by value)
     bool batch;   // a batch (any B, also 1): parameter blocks from P
     bool plane = false;   // CG steps run the plane passes (ls_plane_path)
+    bool plane64 = false; // FP64 iterations run the FP64 plane passes (ls_plane64_path)
     LsTune tune{};        // block-shape variants of this shape (autotuner)
 };
 
@@ -84,6 +86,9 @@ bool ls_length_supported(int n);
 // supported length; the environment variable MGK_LS_PLANE=0 turns them off)
 bool ls_plane_supported(const Geom& g);
 bool ls_plane_path(const Geom& g);
+// 3D grids whose FP64 iterations run as plane passes: the plane grids of up
+// to 64-point planes whose FP64 pass C tiles fit (MGK_LS_PLANE64=0 turns them off)
+bool ls_plane64_path(const Geom& g);
 int ls_partial_slots(const Geom& g);
 // offset (doubles) of the grid barrier words inside the partials buffer
 long ls_gbar_offset(const Geom& g);
@@ -100,7 +105,8 @@ void ls_enqueue_final(const LsLaunch& l, hipStream_t s);
 // one pass of the pipeline alone, for kernel timing: 0 A, 1 B, 2 C, 3 D,
 // 4 E (without its reduction), 5 A with the CG vector update (FP32); plane
 // passes (FP32): 6 forward plane with the CG update, 7 C on the plane
-// layout, 8 inverse plane (without its reduction)
+// layout, 8 inverse plane (without its reduction); FP64 plane passes: 9
+// forward plane, 10 C on the tiled FP64 spectrum, 11 inverse plane
 void ls_enqueue_pass(const LsLaunch& l, int pass, int mode, hipStream_t s);
 
 }  // namespace mgk
