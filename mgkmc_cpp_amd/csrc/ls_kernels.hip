@@ -1624,83 +1624,165 @@ template <int NP> constexpr size_t plane64_lds_bytes() {
     return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<double>);
 }
 
-// eps' of Cartesian component c at voxel v (and, for a normal component, the
-// traces of eps' and eps' - eps_p over the normal components).
-__device__ __forceinline__ void eps_prime64(const LsParams& p, int c, long v, bool add_shift, bool add_corr,
-                                            double& ek_c, double& ee_c, double& tr0, double& tre) {
+// The FP64 plane passes' voxel loops.  A lane takes its voxels U at a time
+// and issues every load of a chunk before it uses any value, so a chunk
+// costs one memory round trip; the loop bodies are specialised on the
+// block's kind (NORMAL: a normal component, whose stress needs the traces
+// over the three normal strains) and on a pending CG iterate (CORR: x, then
+// its last residual r, Mandel components, added to eps as pass A adds them).
+// A missing eps_p or r reads a valid stand-in buffer and is masked, so the
+// loads stay unconditional.  eps' = eps + warm-start shift (+ x + r).
+template <int NP> struct Plane64Loop {
+    static constexpr int VOX = PlaneDims<NP>::VOX;
+    static constexpr int K = (VOX + kPlaneThreads - 1) / kPlaneThreads;   // voxels per lane
+    static constexpr int U = K < 4 ? K : 4;                               // voxels per chunk
+    __device__ static int voxel(int k) { return (int)threadIdx.x + kPlaneThreads * k; }
+};
+
+constexpr double kRsqrt2 = 0.70710678118654752440;
+
+// tau_c = sigma_c(eps' - eps_p) - sigma0_c(eps') of this lane's voxels of
+// plane f into the packed-row plane buffer.
+template <int NP, bool NORMAL, bool CORR>
+__device__ __forceinline__ void plane64_tau(const LsParams& p, int c, int f, const double (&sh)[3], double sh_c,
+                                            double* bufr) {
+    using L = Plane64Loop<NP>;
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NLP = PD::NLP, VOX = PD::VOX, K = L::K, U = L::U;
+    constexpr int NC = NORMAL ? 3 : 1;
     const long N = p.g.N;
-    if (c < 3) {
-        double crc[6] = {0, 0, 0, 0, 0, 0}, rrc[6] = {0, 0, 0, 0, 0, 0}, tc[6], tr[6];
-        if (add_corr) {
+    const bool has_ep = p.epsp != nullptr, has_r = p.r32 != nullptr;
+    const double* ep = has_ep ? p.epsp : p.eps;
+    const float* rv = has_r ? p.r32 : p.corr;
+    const double tm0 = 2.0 * p.mu0, lam0 = p.lam0;
+    for (int k0 = 0; k0 < K; k0 += U) {
+        double e[NC][U], pe[NC][U], la[U], mu[U];
+        float x[NC][U], r[NC][U];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                crc[m] = (double)p.corr[m * N + v];
-                rrc[m] = p.r32 ? (double)p.r32[m * N + v] : 0.0;
+        for (int u = 0; u < U; ++u) {
+            const int q = L::voxel(k0 + u);
+            const long v = (long)f * VOX + (q < VOX ? q : VOX - 1);
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                const long o = (long)(NORMAL ? j : c) * N + v;
+                e[j][u] = p.eps[o];
+                pe[j][u] = ep[o];
+                if constexpr (CORR) {
+                    x[j][u] = p.corr[o];
+                    r[j][u] = rv[o];
+                }
             }
+            la[u] = p.lam[v];
+            mu[u] = p.mu[v];
         }
-        mandel_to_cart(crc, tc);
-        mandel_to_cart(rrc, tr);
-        double ek[3], ee[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            ek[j] = p.eps[j * N + v] + (add_shift ? p.ctl->shift[j] : 0.0);
-            if (add_corr) {
-                ek[j] += tc[j];
-                ek[j] += tr[j];
+        for (int u = 0; u < U; ++u) {
+            const int q = L::voxel(k0 + u);
+            if (k0 + u >= K || q >= VOX) continue;
+            double ek_c, ee_c, tr0 = 0.0, tre = 0.0;
+            if constexpr (NORMAL) {
+                double tc[6] = {0, 0, 0, 0, 0, 0}, tr[6] = {0, 0, 0, 0, 0, 0};
+                if constexpr (CORR) {
+                    const double crc[6] = {(double)x[0][u], (double)x[1][u], (double)x[2][u], 0.0, 0.0, 0.0};
+                    const double rrc[6] = {has_r ? (double)r[0][u] : 0.0, has_r ? (double)r[1][u] : 0.0,
+                                           has_r ? (double)r[2][u] : 0.0, 0.0, 0.0, 0.0};
+                    mandel_to_cart(crc, tc);
+                    mandel_to_cart(rrc, tr);
+                }
+                double ek[3], ee[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    ek[j] = e[j][u] + sh[j];
+                    if constexpr (CORR) {
+                        ek[j] += tc[j];
+                        ek[j] += tr[j];
+                    }
+                    ee[j] = has_ep ? ek[j] - pe[j][u] : ek[j];
+                }
+#pragma unroll
+                for (int j = 0; j < 3; ++j) { tre += ee[j]; tr0 += ek[j]; }
+                ek_c = (c == 0) ? ek[0] : (c == 1) ? ek[1] : ek[2];
+                ee_c = (c == 0) ? ee[0] : (c == 1) ? ee[1] : ee[2];
+            } else {
+                double ev = e[0][u] + sh_c;
+                if constexpr (CORR) {
+                    ev += (double)x[0][u] * kRsqrt2;
+                    ev += (has_r ? (double)r[0][u] : 0.0) * kRsqrt2;
+                }
+                ek_c = ev;
+                ee_c = has_ep ? ev - pe[0][u] : ev;
             }
-            ee[j] = p.epsp ? ek[j] - p.epsp[j * N + v] : ek[j];
+            const double tm = 2.0 * mu[u], lt = la[u] * tre, lt0 = lam0 * tr0;
+            const double sv = tm * ee_c + (NORMAL ? lt : 0.0);
+            const double s0 = tm0 * ek_c + (NORMAL ? lt0 : 0.0);
+            const int m = q / NL, jl = q % NL;
+            bufr[2 * ((m >> 1) * NLP + jl) + (m & 1)] = sv - s0;
         }
-        tr0 = 0.0;
-        tre = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) { tre += ee[j]; tr0 += ek[j]; }
-        ek_c = ek[c];
-        ee_c = ee[c];
-    } else {
-        double e = p.eps[c * N + v] + (add_shift ? p.ctl->shift[c] : 0.0);
-        if (add_corr) {
-            double m6[6] = {0, 0, 0, 0, 0, 0}, t6[6];
-            m6[c] = (double)p.corr[c * N + v];
-            mandel_to_cart(m6, t6);
-            e += t6[c];
-            m6[c] = p.r32 ? (double)p.r32[c * N + v] : 0.0;
-            mandel_to_cart(m6, t6);
-            e += t6[c];
-        }
-        ek_c = e;
-        ee_c = p.epsp ? e - p.epsp[c * N + v] : e;
-        tr0 = tre = 0.0;
     }
 }
 
-// eps' of Cartesian component c alone (the update of pass E).
-__device__ __forceinline__ double eps_prime64_c(const LsParams& p, int c, long v, bool add_shift, bool add_corr) {
+// The update of pass E for component c of this lane's voxels of plane f:
+// eps_c = eps_bar_c + eps_tilde_c (eps_tilde in the packed-row buffer), its
+// shares of ||eps_new - eps'||^2, ||eps'||^2, ||eps_new||^2 (weight wgt), and
+// in FP32+refine contexts the increment in FP32 (the next CG cycle's start).
+template <int NP, bool NORMAL, bool CORR>
+__device__ __forceinline__ void plane64_update(const LsParams& p, int c, int f, double sh_c, double bar, double wgt,
+                                               const double* bufr, double (&acc)[3]) {
+    using L = Plane64Loop<NP>;
+    using PD = PlaneDims<NP>;
+    constexpr int NL = PD::NL, NLP = PD::NLP, VOX = PD::VOX, K = L::K, U = L::U;
+    constexpr int NC = NORMAL ? 3 : 1;
     const long N = p.g.N;
-    double e = p.eps[c * N + v] + (add_shift ? p.ctl->shift[c] : 0.0);
-    if (add_corr) {
-        double xm[6] = {0, 0, 0, 0, 0, 0}, rm[6] = {0, 0, 0, 0, 0, 0}, tx[6], trr[6];
-        if (c < 3) {
+    const bool has_r = p.r32 != nullptr;
+    const float* rv = has_r ? p.r32 : p.corr;
+    for (int k0 = 0; k0 < K; k0 += U) {
+        double e[U];
+        float x[NC][U], r[NC][U];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                xm[m] = (double)p.corr[m * N + v];
-                rm[m] = p.r32 ? (double)p.r32[m * N + v] : 0.0;
+        for (int u = 0; u < U; ++u) {
+            const int q = L::voxel(k0 + u);
+            const long v = (long)f * VOX + (q < VOX ? q : VOX - 1);
+            e[u] = p.eps[(long)c * N + v];
+            if constexpr (CORR) {
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    const long o = (long)(NORMAL ? j : c) * N + v;
+                    x[j][u] = p.corr[o];
+                    r[j][u] = rv[o];
+                }
             }
-        } else {
-            const double xc = (double)p.corr[c * N + v], rc = p.r32 ? (double)p.r32[c * N + v] : 0.0;
-#pragma unroll
-            for (int m = 3; m < 6; ++m)   // selects instead of a dynamic index keep the arrays in registers
-                if (m == c) { xm[m] = xc; rm[m] = rc; }
         }
-        mandel_to_cart(xm, tx);
-        mandel_to_cart(rm, trr);
-        double txc = 0.0, trc = 0.0;
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
-            if (m == c) { txc = tx[m]; trc = trr[m]; }
-        e += txc;
-        e += trc;
+        for (int u = 0; u < U; ++u) {
+            const int q = L::voxel(k0 + u);
+            if (k0 + u >= K || q >= VOX) continue;
+            double o = e[u] + sh_c;
+            if constexpr (CORR) {
+                if constexpr (NORMAL) {
+                    const double xm[6] = {(double)x[0][u], (double)x[1][u], (double)x[2][u], 0.0, 0.0, 0.0};
+                    const double rm[6] = {has_r ? (double)r[0][u] : 0.0, has_r ? (double)r[1][u] : 0.0,
+                                          has_r ? (double)r[2][u] : 0.0, 0.0, 0.0, 0.0};
+                    double tx[6], trr[6];
+                    mandel_to_cart(xm, tx);
+                    mandel_to_cart(rm, trr);
+                    o += (c == 0) ? tx[0] : (c == 1) ? tx[1] : tx[2];
+                    o += (c == 0) ? trr[0] : (c == 1) ? trr[1] : trr[2];
+                } else {
+                    o += (double)x[0][u] * kRsqrt2;
+                    o += (has_r ? (double)r[0][u] : 0.0) * kRsqrt2;
+                }
+            }
+            const long v = (long)f * VOX + q;
+            const int m = q / NL, jl = q % NL;
+            const double nw = bar + bufr[2 * ((m >> 1) * NLP + jl) + (m & 1)];
+            const double d = nw - o;
+            acc[0] += wgt * d * d;
+            acc[1] += wgt * o * o;
+            acc[2] += wgt * nw * nw;
+            p.eps[(long)c * N + v] = nw;
+            if (has_r) p.p32[(long)c * N + v] = (float)d;   // the increment that starts a CG cycle (Cartesian)
+        }
     }
-    return e;
 }
 
 template <int NP, bool BATCH>
@@ -1720,20 +1802,17 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane64(const LsParams p_
     stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
     const bool add_shift = flags & LS_FLAG_SHIFT;
     const bool add_corr = (flags & LS_FLAG_CORR) && p.corr && p.ctl->corr_valid;
-    const double tm0 = 2.0 * p.mu0;
+    double sh[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sh[j] = add_shift ? p.ctl->shift[j] : 0.0;
+    const double sh_c = add_shift ? p.ctl->shift[c] : 0.0;
     double* bufr = reinterpret_cast<double*>(buf);
-#pragma unroll 2
-    for (int k = 0; k < (VOX + kPlaneThreads - 1) / kPlaneThreads; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        if (VOX % kPlaneThreads != 0 && q >= VOX) break;   // planes of 16 and 48 points
-        const long v = (long)f * VOX + q;
-        double ek_c, ee_c, tr0, tre;
-        eps_prime64(p, c, v, add_shift, add_corr, ek_c, ee_c, tr0, tre);
-        const double tm = 2.0 * p.mu[v], lt = p.lam[v] * tre, lt0 = p.lam0 * tr0;
-        const double sv = tm * ee_c + (c < 3 ? lt : 0.0);
-        const double s0 = tm0 * ek_c + (c < 3 ? lt0 : 0.0);
-        const int m = q / NL, j = q % NL;
-        bufr[2 * ((m >> 1) * NLP + j) + (m & 1)] = sv - s0;
+    if (c < 3) {
+        if (add_corr) plane64_tau<NP, true, true>(p, c, f, sh, sh_c, bufr);
+        else plane64_tau<NP, true, false>(p, c, f, sh, sh_c, bufr);
+    } else {
+        if (add_corr) plane64_tau<NP, false, true>(p, c, f, sh, sh_c, bufr);
+        else plane64_tau<NP, false, false>(p, c, f, sh, sh_c, bufr);
     }
     cx* spec = reinterpret_cast<cx*>(p.spec64) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
     plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { spec[tile_off(idx)] = v; }, (int)threadIdx.x);
@@ -1778,22 +1857,15 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
     const bool add_corr = (flags & LS_FLAG_CORR) && p.corr && p.ctl->corr_valid;
     const double bar = p.ctl->eps_bar[c];
     const double wgt = (c < 3) ? 1.0 : 2.0;   // off-diagonal entries appear twice in the full tensor
+    const double sh_c = add_shift ? p.ctl->shift[c] : 0.0;
     const double* bufr = reinterpret_cast<const double*>(buf);
     double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll 2
-    for (int k = 0; k < (VOX + kPlaneThreads - 1) / kPlaneThreads; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
-        if (VOX % kPlaneThreads != 0 && q >= VOX) break;   // planes of 16 and 48 points
-        const long v = (long)f * VOX + q;
-        const double o = eps_prime64_c(p, c, v, add_shift, add_corr);
-        const int m = q / NL, j = q % NL;
-        const double nw = bar + bufr[2 * ((m >> 1) * NLP + j) + (m & 1)];
-        const double d = nw - o;
-        acc[0] += wgt * d * d;
-        acc[1] += wgt * o * o;
-        acc[2] += wgt * nw * nw;
-        p.eps[(long)c * g.N + v] = nw;
-        if (p.r32) p.p32[(long)c * g.N + v] = (float)d;   // the increment that starts a CG cycle (Cartesian)
+    if (c < 3) {
+        if (add_corr) plane64_update<NP, true, true>(p, c, f, sh_c, bar, wgt, bufr, acc);
+        else plane64_update<NP, true, false>(p, c, f, sh_c, bar, wgt, bufr, acc);
+    } else {
+        if (add_corr) plane64_update<NP, false, true>(p, c, f, sh_c, bar, wgt, bufr, acc);
+        else plane64_update<NP, false, false>(p, c, f, sh_c, bar, wgt, bufr, acc);
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
