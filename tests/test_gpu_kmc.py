@@ -386,6 +386,9 @@ def test_batch_equals_separate_fp32_refine(shape):
     (dict(), 25, (42, 7, 1234)),                                           # 2D fixed catalog, cascade mode
     (dict(instability_kmc=1, catalog_mode=abi.CATALOG_RANDOM, neighbor_softening_fraction=0.2), 25, (42, 7, 1234)),
     (dict(dims=3, nx=8, ny=8, nz=8, M=12), 10, (42, 7, 1234)),
+    # FP64 iterations on the plane passes: the batched and single-context
+    # instantiations of a pass must round alike
+    (dict(dims=3, nx=16, ny=16, nz=16, M=12, instability_kmc=1), 6, (42, 7)),
     (dict(), 15, (42,)),                                                   # a batch of one
 ])
 def test_batch_equals_separate_trajectories(kw, events, seeds):

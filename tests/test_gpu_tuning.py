@@ -18,7 +18,10 @@ def _field(shape, seed, cov=0.15, mean=85e9):
     return mean * (1.0 + cov * rng.standard_normal(N)), np.full(N, 0.35)
 
 
-@pytest.mark.parametrize("shape,precision", [((48, 48, 48), abi.MGK_FP64), ((32, 48, 32), abi.MGK_FP32_REFINE),
+# FP64 on 48x32x48: planes that are not square keep the five-pass pipeline
+# (a square-plane grid runs its FP64 iterations as plane passes, which have no
+# block-shape variants)
+@pytest.mark.parametrize("shape,precision", [((48, 32, 48), abi.MGK_FP64), ((32, 48, 32), abi.MGK_FP32_REFINE),
                                              ((64, 64, 64), abi.MGK_FP32_REFINE)])
 def test_variants_do_not_change_results(shape, precision):
     E, nu = _field(shape, 41)
@@ -26,7 +29,9 @@ def test_variants_do_not_change_results(shape, precision):
     ctx = Context(shape, E, nu, precision=precision)
     info = tuning(ctx.handle)
     print(shape, precision, info)
-    assert info["tuned"] and info["ms"]["fwd_fp64"][0] > 0.0
+    assert info["tuned"]
+    if shape[1] != shape[2]:   # FP64 iterations on the five-pass pipeline: pass A's variants were timed
+        assert info["ms"]["fwd_fp64"][0] > 0.0
     if shape == (64, 64, 64):
         assert info["cycle"] in (0, 1) and info["ms"]["solve_cycle"] > 0.0 and info["ms"]["solve_plane"] > 0.0
     ctx.close()
