@@ -2059,7 +2059,9 @@ struct mgk_batch {
     // kernels, the predicted iterations, final pass, read-back) as HIP
     // graphs, one per launch structure.  Every per-call value is read from
     // device memory the graph's own copies fill, so a graph replays as is.
-    bool use_graphs = true;
+    // MGK_BATCH_GRAPHS=0 starts batches with eager launches (the same kernels
+    // in the same order), for tools that cannot follow stream capture.
+    bool use_graphs = !(std::getenv("MGK_BATCH_GRAPHS") && std::getenv("MGK_BATCH_GRAPHS")[0] == '0');
     std::map<std::vector<int>, hipGraphExec_t> graphs;
 
     void drop_graphs() {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 namespace mgk {
 
@@ -45,6 +46,29 @@ inline Geom make_geom(int dims, int nx, int ny, int nz) {
     g.NS = (long)g.nf * g.nm * g.nlh;
     g.C = (dims == 2) ? 3 : 6;
     return g;
+}
+
+// Every buffer a kernel reads or writes through its parameter block is
+// device (global) memory.  A pointer the compiler cannot trace to a kernel
+// argument -- a batched launch reads its parameter block from memory -- would
+// otherwise be accessed as a flat address, which counts on both the vector
+// memory and the LDS counters, so every LDS wait of the block also waits for
+// its outstanding HBM accesses.  These helpers access global memory directly.
+// Scalars and vector types are accessed as they are; a small struct (a complex
+// value) as the unsigned vector of its size, naturally aligned in its array.
+template <typename R> struct cplx;
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+typedef unsigned gwords2 __attribute__((ext_vector_type(2)));
+typedef unsigned gwords4 __attribute__((ext_vector_type(4)));
+template <typename T> struct GWords { using type = T; };
+template <typename R> struct GWords<cplx<R>> { using type = std::conditional_t<sizeof(R) == 8, gwords4, gwords2>; };
+template <typename T> __device__ __forceinline__ T gld(const T* p) {
+    using W = typename GWords<T>::type;
+    return __builtin_bit_cast(T, *(const gmem<W>*)p);
+}
+template <typename T> __device__ __forceinline__ void gst(T* p, const T& v) {
+    using W = typename GWords<T>::type;
+    *(gmem<W>*)p = __builtin_bit_cast(W, v);
 }
 
 // Symmetric component order on the device.

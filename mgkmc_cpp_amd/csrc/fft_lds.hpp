@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
+
 namespace mgk {
 
 template <typename R> struct cplx { R x, y; };
@@ -155,6 +157,14 @@ template <typename R> struct LineRef {
     long es;
     __device__ __forceinline__ cplx<R> operator()(int j) const { return p[j * es]; }
     __device__ __forceinline__ void operator()(int j, cplx<R> v) const { p[j * es] = v; }
+};
+// A line in device (global) memory: global, not flat, accesses also where the
+// base pointer comes from a parameter block in memory.
+template <typename R> struct GLineRef {
+    cplx<R>* p;
+    long es;
+    __device__ __forceinline__ cplx<R> operator()(int j) const { return gld(p + j * es); }
+    __device__ __forceinline__ void operator()(int j, cplx<R> v) const { gst(p + j * es, v); }
 };
 
 // ---------------------------------------------------------------------------

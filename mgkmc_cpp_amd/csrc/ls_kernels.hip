@@ -240,7 +240,7 @@ __device__ __forceinline__ bool grid_sync(unsigned* w, unsigned epoch, int* s_fl
 }
 
 template <typename SR> __device__ __forceinline__ void stage_twiddles(cplx<SR>* dst, const cplx<SR>* src, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = gld(src + i);
 }
 
 // Padded LDS line length (complex elements): keeps lines of neighbouring
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(kColThreadsMax) void k_mid(const LsParams p_arg, co
     const long f = active ? F / ncol : 0;
     const long col = active ? F % ncol : 0;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
-    const LineRef<SR> column{spec + f * (long)NM * ncol + col, ncol};
+    const GLineRef<SR> column{spec + f * (long)NM * ncol + col, ncol};
     cx v[NM / T];
     line_gather<NM, SR>(column, t, active, v);
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_m64 : p.tw_m32), NM);
@@ -818,12 +818,12 @@ __global__ __launch_bounds__(kColThreadsMax) void k_first_gamma(const LsParams p
     const long xi0 = (long)blockIdx.x * G;
     cx* spec = reinterpret_cast<cx*>(mode == LS_FULL ? p.spec64 : p.spec32);
     const GammaLanes<SR, NF, C, PLANE, LANES> ln(xi0, nxi, (int)threadIdx.x);
-    const LineRef<SR> column{spec + (ln.active ? xi0 * C + ln.l : 0), nxi * C};
+    const GLineRef<SR> column{spec + (ln.active ? xi0 * C + ln.l : 0), nxi * C};
     cx v[NF / FftPlan<NF>::threads];
     line_gather<NF, SR>(column, ln.t, ln.active, v);
     stage_twiddles<SR>(tw, reinterpret_cast<const cx*>(mode == LS_FULL ? p.tw_f64 : p.tw_f32), NF);
     if (stop) return;
-    gamma_core<SR, NF, C, PLANE, LineRef<SR>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x,
+    gamma_core<SR, NF, C, PLANE, GLineRef<SR>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0, (int)threadIdx.x,
                                                      (int)blockDim.x);
 }
 
@@ -845,12 +845,12 @@ __global__ __launch_bounds__(1024) void k_first_gamma_wide(const LsParams p_arg,
     const long xi0 = (long)blockIdx.x * G;
     cx* spec = reinterpret_cast<cx*>(p.spec32);
     const GammaLanes<float, NF, 6, true, LANES> ln(xi0, nxi, (int)threadIdx.x);
-    const LineRef<float> column{spec + (ln.active ? xi0 * 6 + ln.l : 0), nxi * 6};
+    const GLineRef<float> column{spec + (ln.active ? xi0 * 6 + ln.l : 0), nxi * 6};
     cx v[NF / FftPlan<NF>::threads];
     line_gather<NF, float>(column, ln.t, ln.active, v);
     stage_twiddles<float>(tw, reinterpret_cast<const cx*>(p.tw_f32), NF);
     if (stop) return;
-    gamma_core<float, NF, 6, true, LineRef<float>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0,
+    gamma_core<float, NF, 6, true, GLineRef<float>, LANES>(v, ln, column, tw, work, xi0, g, p.lam0, p.mu0,
                                                           (int)threadIdx.x, (int)blockDim.x);
 }
 
@@ -1033,9 +1033,9 @@ __global__ __launch_bounds__(256) void k_inv_last(const LsParams p_arg, const Ls
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
-        p.partials[3 * blockIdx.x] = acc[0];
-        p.partials[3 * blockIdx.x + 1] = acc[1];
-        p.partials[3 * blockIdx.x + 2] = acc[2];
+        gst(p.partials + 3 * blockIdx.x, acc[0]);
+        gst(p.partials + 3 * blockIdx.x + 1, acc[1]);
+        gst(p.partials + 3 * blockIdx.x + 2, acc[2]);
     }
 }
 
@@ -1179,8 +1179,11 @@ constexpr int kPlaneThreads = MGK_PLANE_THREADS;   // lanes per plane block (a b
 // CG vectors in 16-byte accesses (planes hold a multiple of 64 voxels, and
 // every plane starts 16-byte aligned).
 typedef float vox4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ vox4 ld_vox4(const float* b, int q) { return *reinterpret_cast<const vox4*>(b + 4 * q); }
-__device__ __forceinline__ void st_vox4(float* b, int q, vox4 v) { *reinterpret_cast<vox4*>(b + 4 * q) = v; }
+__device__ __forceinline__ vox4 ld_vox4(const float* b, int q) { return gld(reinterpret_cast<const vox4*>(b + 4 * q)); }
+__device__ __forceinline__ void st_vox4(float* b, int q, vox4 v) { gst(reinterpret_cast<vox4*>(b + 4 * q), v); }
+// the same for vectors a block keeps in LDS
+__device__ __forceinline__ vox4 ld_vox4_lds(const float* b, int q) { return *reinterpret_cast<const vox4*>(b + 4 * q); }
+__device__ __forceinline__ void st_vox4_lds(float* b, int q, vox4 v) { *reinterpret_cast<vox4*>(b + 4 * q) = v; }
 
 template <int NP> constexpr size_t plane_lds_bytes() {
     return (size_t)(2 * NP + (NP / 2) * padded<NP>() + (NP / 2 + 1) * padded<NP>()) * sizeof(cplx<float>);
@@ -1434,11 +1437,18 @@ __device__ __forceinline__ vox4 r0_from_cart(const LsParams& p, int m, long base
     return out;
 }
 
-template <int NP, bool BATCH>
-__global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+// PU: partial rows per lane of the folded reduction, 1 where the previous
+// step's 6*nf partials fit the block's lanes (the fold then adds the same
+// terms in the same order).  Up to 64-point planes at most 128 VGPRs, so two
+// blocks share a CU: the batched instantiation took 134 with three rows and
+// ran one block per CU.  A 128-point plane's LDS holds one block per CU.
+template <int NP, bool BATCH, int PU>
+__global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(NP > 64 ? 1 : 4))) void k_fwd_plane(
+    const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     using PD = PlaneDims<NP>;
-    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, PU = PD::PU;
+    constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4;
+    static_assert(PU >= 1 && PU <= PD::PU, "partial rows per lane");
     using cx = cplx<float>;
     // step == cycle_len: only the folded reduction of the cycle's last step
     const int step = flags >> LS_STEP_SHIFT, cyc = cycle_len_now(p.ctl);
@@ -1468,7 +1478,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
         for (int u = 0; u < PU; ++u) {
             const int b = threadIdx.x + kPlaneThreads * u;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) part[k][u] = p.partials[3 * (b < nbp ? b : 0) + k];
+            for (int k = 0; k < 3; ++k) part[k][u] = gld(p.partials + 3 * (b < nbp ? b : 0) + k);
         }
     }
 
@@ -1540,7 +1550,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane(const LsParams p_ar
         }
     }
     cx* spec = reinterpret_cast<cx*>(p.spec32) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
-    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { spec[tile_off(idx)] = v; }, (int)threadIdx.x);
+    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { gst(spec + tile_off(idx), v); }, (int)threadIdx.x);
 }
 
 template <int NP, bool BATCH>
@@ -1565,7 +1575,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
         const int idx = threadIdx.x + kPlaneThreads * k;
-        sp[k] = spec[tile_off(idx < PD::NS ? idx : 0)];
+        sp[k] = gld(spec + tile_off(idx < PD::NS ? idx : 0));
     }
     double acc[3];
     if constexpr (PD::QC == Q4) {
@@ -1597,9 +1607,9 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
-        p.partials[3 * blockIdx.x] = acc[0];
-        p.partials[3 * blockIdx.x + 1] = acc[1];
-        p.partials[3 * blockIdx.x + 2] = acc[2];
+        gst(p.partials + 3 * blockIdx.x, acc[0]);
+        gst(p.partials + 3 * blockIdx.x + 1, acc[1]);
+        gst(p.partials + 3 * blockIdx.x + 2, acc[2]);
     }
 }
 
@@ -1665,15 +1675,15 @@ __device__ __forceinline__ void plane64_tau(const LsParams& p, int c, int f, con
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
                 const long o = (long)(NORMAL ? j : c) * N + v;
-                e[j][u] = p.eps[o];
-                pe[j][u] = ep[o];
+                e[j][u] = gld(p.eps + o);
+                pe[j][u] = gld(ep + o);
                 if constexpr (CORR) {
-                    x[j][u] = p.corr[o];
-                    r[j][u] = rv[o];
+                    x[j][u] = gld(p.corr + o);
+                    r[j][u] = gld(rv + o);
                 }
             }
-            la[u] = p.lam[v];
-            mu[u] = p.mu[v];
+            la[u] = gld(p.lam + v);
+            mu[u] = gld(p.mu + v);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1742,13 +1752,13 @@ __device__ __forceinline__ void plane64_update(const LsParams& p, int c, int f, 
         for (int u = 0; u < U; ++u) {
             const int q = L::voxel(k0 + u);
             const long v = (long)f * VOX + (q < VOX ? q : VOX - 1);
-            e[u] = p.eps[(long)c * N + v];
+            e[u] = gld(p.eps + (long)c * N + v);
             if constexpr (CORR) {
 #pragma unroll
                 for (int j = 0; j < NC; ++j) {
                     const long o = (long)(NORMAL ? j : c) * N + v;
-                    x[j][u] = p.corr[o];
-                    r[j][u] = rv[o];
+                    x[j][u] = gld(p.corr + o);
+                    r[j][u] = gld(rv + o);
                 }
             }
         }
@@ -1779,8 +1789,8 @@ __device__ __forceinline__ void plane64_update(const LsParams& p, int c, int f, 
             acc[0] += wgt * d * d;
             acc[1] += wgt * o * o;
             acc[2] += wgt * nw * nw;
-            p.eps[(long)c * N + v] = nw;
-            if (has_r) p.p32[(long)c * N + v] = (float)d;   // the increment that starts a CG cycle (Cartesian)
+            gst(p.eps + (long)c * N + v, nw);
+            if (has_r) gst(p.p32 + (long)c * N + v, (float)d);   // the increment that starts a CG cycle (Cartesian)
         }
     }
 }
@@ -1815,7 +1825,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane64(const LsParams p_
         else plane64_tau<NP, false, false>(p, c, f, sh, sh_c, bufr);
     }
     cx* spec = reinterpret_cast<cx*>(p.spec64) + (long)f * NM * PD::NLH * 6 + c * kPlaneTile;
-    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { spec[tile_off(idx)] = v; }, (int)threadIdx.x);
+    plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [spec](int idx, cx v) { gst(spec + tile_off(idx), v); }, (int)threadIdx.x);
 }
 
 template <int NP, bool BATCH>
@@ -1838,7 +1848,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int idx = threadIdx.x + kPlaneThreads * k;
-            sp[k] = spec[tile_off(idx < PD::NS ? idx : 0)];
+            sp[k] = gld(spec + tile_off(idx < PD::NS ? idx : 0));
         }
         stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
         stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
@@ -1850,7 +1860,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
         stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
         stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
         for (int idx = threadIdx.x; idx < PD::NS; idx += kPlaneThreads)
-            pl[(idx % PD::NLH) * PD::NMP + idx / PD::NLH] = spec[tile_off(idx)];
+            pl[(idx % PD::NLH) * PD::NMP + idx / PD::NLH] = gld(spec + tile_off(idx));
         plane_inv_fft_staged<NP, double>(buf, pl, tw_l, tw_m, (int)threadIdx.x);
     }
     const bool add_shift = flags & LS_FLAG_SHIFT;
@@ -1869,9 +1879,9 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
-        p.partials[3 * blockIdx.x] = acc[0];
-        p.partials[3 * blockIdx.x + 1] = acc[1];
-        p.partials[3 * blockIdx.x + 2] = acc[2];
+        gst(p.partials + 3 * blockIdx.x, acc[0]);
+        gst(p.partials + 3 * blockIdx.x + 1, acc[1]);
+        gst(p.partials + 3 * blockIdx.x + 2, acc[2]);
     }
 }
 
@@ -1960,8 +1970,8 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
         tf[k] = tau_factor(c, dl, dm);
         w[k] = pv[k] = zero4;
         if (q < V4) {
-            st_vox4(s_lds, q, zero4);
-            st_vox4(x_lds, q, zero4);
+            st_vox4_lds(s_lds, q, zero4);
+            st_vox4_lds(x_lds, q, zero4);
         }
     }
     stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
@@ -1986,10 +1996,10 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
         for (int k = 0; k < Q4; ++k) {
             const int q = tid + kPlaneThreads * k;
             if (q >= V4) continue;
-            vox4 sv = ld_vox4(s_lds, q), xv = ld_vox4(x_lds, q);
+            vox4 sv = ld_vox4_lds(s_lds, q), xv = ld_vox4_lds(x_lds, q);
             plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv, xv, tf[k], bufr);
-            st_vox4(s_lds, q, sv);
-            st_vox4(x_lds, q, xv);
+            st_vox4_lds(s_lds, q, sv);
+            st_vox4_lds(x_lds, q, xv);
         }
         updated = updated || upd;
         plane_fwd_fft<NP>(buf, pl, tw_l, tw_m, [&](int idx, cx v) { spec.stc(plane_off + tile_off(idx), v); }, tid);
@@ -2065,7 +2075,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
             const int q = threadIdx.x + kPlaneThreads * k;
             if (q >= V4) continue;
             st_vox4(p.r32 + base, q, r[k]);
-            st_vox4(p.corr + base, q, ld_vox4(x_lds, q));
+            st_vox4(p.corr + base, q, ld_vox4_lds(x_lds, q));
         }
     }
 }
@@ -2442,13 +2452,19 @@ template <class K> static void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024) MGK_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+template <int NN, bool BATCH, int PU> static void launch_fwd_plane_pu(const LsLaunch& p, int flags, hipStream_t s) {
+    static const bool lds_ok = (allow_lds(k_fwd_plane<NN, BATCH, PU>, plane_lds_bytes<NN>()), true);
+    (void)lds_ok;
+    hipLaunchKernelGGL((k_fwd_plane<NN, BATCH, PU>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads), plane_lds_bytes<NN>(),
+                       s, p.h, p.P, flags);
+}
+
 template <bool BATCH> static void launch_fwd_plane(const LsLaunch& p, int flags, hipStream_t s) {
+    const bool one_row = 6 * p.g.nf <= kPlaneThreads;
 #define MGK_CASE(NN)                                                                                  \
     if (p.g.nm == NN) {                                                                               \
-        static const bool lds_ok = (allow_lds(k_fwd_plane<NN, BATCH>, plane_lds_bytes<NN>()), true);  \
-        (void)lds_ok;                                                                                 \
-        hipLaunchKernelGGL((k_fwd_plane<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads),       \
-                           plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
+        if (one_row) launch_fwd_plane_pu<NN, BATCH, 1>(p, flags, s);                                  \
+        else launch_fwd_plane_pu<NN, BATCH, PlaneDims<NN>::PU>(p, flags, s);                          \
         return;                                                                                       \
     }
     MGK_PLANE_CASES(MGK_CASE)
