@@ -299,9 +299,11 @@ int mgk_time_ls_pass(mgk_ctx* ctx, int pass, int fp32, int reps, double* avg_ms)
 /* Persistent CG cycles (no counterpart in the reference, whose
  * spectral_solver_3d runs the basic scheme, linear_elastic_solver.hpp:494-581):
  * on an FP32+refine context whose grid is cubic with 64-point planes, each
- * refinement cycle's CG steps run as one cooperative launch that keeps the CG
- * vectors on chip (iterates equal to the separately launched plane passes,
- * bit for bit; the environment variable MGK_LS_CYCLE=0 turns it off).
+ * refinement cycle's CG steps run as one launch whose blocks are all resident
+ * and keep the CG vectors on chip (iterates equal to the separately launched
+ * plane passes, bit for bit; the environment variable MGK_LS_CYCLE=0 turns it
+ * off).  One such cycle runs at a time per process: a context's solve waits
+ * while another context's cycle is in flight.
  * mgk_cycle_supported: 1 when this context's solves use it (the grid runs
  * it and the autotuner, or mgk_set_tuning, kept it on).
  * mgk_cycle_timer(on): measurement helper; resets the counters and, when on,

@@ -37,6 +37,17 @@ int fail(int code, const std::string& msg) {
 // that make them, and every graph capture, hold this lock.
 std::recursive_mutex g_device_mu;
 
+// A persistent CG cycle's blocks wait for each other at grid barriers, so all
+// of them must be resident at once.  It is an ordinary launch (a cooperative
+// launch cost about 22 us per event at 64^3 in stream synchronisation around
+// it); the device holds its blocks (checked once with the occupancy API), and
+// this lock keeps a second cycle from another context from competing for the
+// same slots: a thread holds it from a cycle's launch until its stream has
+// finished the round.  Kernels of other streams do not wait on a cycle, so
+// blocks they hold free up; a barrier that still waits past its poll limit
+// fails the solve with an error instead of hanging.
+std::mutex g_cycle_mu;
+
 // Flips of a cascade round uploaded and applied per launch (mgk_apply_flips).
 constexpr int kFlipChunk = 64;
 
@@ -418,6 +429,7 @@ struct mgk_ctx {
         int enqueued = 0;   // FP64 iterations (refinement cycles) enqueued
         bool final_done = false;
         int inner_before = 0;   // CG steps counted before this batch
+        std::unique_lock<std::mutex> cycle_lock(g_cycle_mu, std::defer_lock);   // held while a cycle is in flight
         if (max_iter > 0) {
             int batch = std::min(max_iter, std::max(refine ? 1 : 2, predicted));
             while (true) {
@@ -430,6 +442,7 @@ struct mgk_ctx {
                     if (refine) {
                         const int J = (k == 0) ? inner_per_outer : inner_per_outer2;
                         if (p.h.gbar) {
+                            if (!cycle_lock.owns_lock()) cycle_lock.lock();
                             if (timed) MGK_HIP_CHECK(hipEventRecord(cyc_ev0, stream));
                             ls_enqueue_cycle(p, J, stream);   // the whole cycle in one launch
                             if (timed) MGK_HIP_CHECK(hipEventRecord(cyc_ev1, stream));
@@ -449,6 +462,7 @@ struct mgk_ctx {
                                              hipMemcpyDeviceToHost, stream));
                 kctl_pending = false;
                 MGK_HIP_CHECK(hipStreamSynchronize(stream));
+                if (cycle_lock.owns_lock()) cycle_lock.unlock();
                 if (hctl->cg_err) throw HipError("persistent CG cycle: grid barrier timed out");
                 if (timed && hctl->inner > inner_before) {   // the cycle ran (not converged before it)
                     float ms = 0.f;

@@ -152,7 +152,8 @@ template <int K> __device__ __forceinline__ void reduce_partials(const double* p
 }
 
 // ---------------------------------------------------------------------------
-// Device-coherent accesses and the grid barrier of a cooperative launch.
+// Device-coherent accesses and the grid barrier of a persistent launch (all
+// blocks resident at once).
 // The blocks of one persistent launch exchange the spectrum and the reduction
 // partials through buffer loads and stores with the sc1 cache-policy bit
 // (device scope): they bypass the XCD-private L2 caches, so a grid barrier
@@ -1886,8 +1887,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
 }
 
 // ---------------------------------------------------------------------------
-// Persistent CG cycle (one context on a cubic plane grid).  One cooperative
-// launch runs all CG steps of a refinement cycle.  Block x is (component,
+// Persistent CG cycle (one context on a cubic plane grid).  One launch whose
+// blocks are all resident runs all CG steps of a refinement cycle.  Block x is (component,
 // plane) as in the plane passes; its CG vectors r, w, p, s and the iterate x
 // stay in registers for the whole cycle, so a step moves only the spectrum
 // (out and back in once per phase) and the reduction partials, through
@@ -2503,7 +2504,6 @@ template <int NP> static bool cycle_fits() {
         int dev = 0, per_cu = 0;
         hipDeviceProp_t pr;
         bool ok = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess &&
-                  pr.cooperativeLaunch &&
                   hipFuncSetAttribute((const void*)k_cg_cycle<NP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)cycle_lds_bytes<NP>()) == hipSuccess &&
                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_cg_cycle<NP>, kPlaneThreads,
@@ -2523,14 +2523,15 @@ bool ls_cycle_supported(const Geom& g) {
     return false;
 }
 
+// An ordinary launch whose blocks the device holds at once (cycle_fits); the
+// caller keeps a second cycle from running beside it (one cycle in flight per
+// process, see g_cycle_mu).
 void ls_enqueue_cycle(const LsLaunch& p, int J, hipStream_t s) {
     if (p.batch || !p.h.gbar) throw ArgError("persistent CG cycles run on a single context");
-    LsParams h = p.h;
-    void* args[] = {&h, &J};
 #define MGK_CASE(NN)                                                                                           \
     if (p.g.nm == NN) {                                                                                        \
-        MGK_HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_cg_cycle<NN>, dim3(6 * NN), dim3(kPlaneThreads), \
-                                                 args, cycle_lds_bytes<NN>(), s));                             \
+        hipLaunchKernelGGL((k_cg_cycle<NN>), dim3(6 * NN), dim3(kPlaneThreads), cycle_lds_bytes<NN>(), s, p.h, J); \
+        MGK_HIP_CHECK(hipGetLastError());                                                                      \
         return;                                                                                                \
     }
     MGK_CYCLE_CASES(MGK_CASE)
@@ -2538,16 +2539,6 @@ void ls_enqueue_cycle(const LsLaunch& p, int J, hipStream_t s) {
     throw UnsupportedError("persistent CG cycle: grid not supported");
 }
 
-// Zeroes the grid barrier words (a cycle launched without the FP64
-// iteration's reduction before it: kernel timing).
-__global__ void k_gbar_reset(unsigned* w) {
-    for (int i = threadIdx.x; i < kBarWords; i += blockDim.x) w[i] = 0u;
-}
-
-void ls_enqueue_gbar_reset(const LsLaunch& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_gbar_reset, dim3(1), dim3(256), 0, s, p.h.gbar);
-    MGK_HIP_CHECK(hipGetLastError());
-}
 
 #define MGK_PLANE64_CASES(X) X(16) X(32) X(48) X(64)
 

@@ -96,9 +96,9 @@ long ls_gbar_offset(const Geom& g);
 // whose 6 n blocks the device holds at once (MGK_LS_CYCLE=0 turns them off).
 bool ls_cycle_supported(const Geom& g);
 // All CG steps of one refinement cycle (at most J, as J enqueued plane-pass
-// steps would run) in one cooperative launch.
+// steps would run) in one launch whose blocks are all resident; the caller
+// runs at most one cycle at a time in the process.
 void ls_enqueue_cycle(const LsLaunch& l, int J, hipStream_t s);
-void ls_enqueue_gbar_reset(const LsLaunch& l, hipStream_t s);
 void ls_enqueue_start(const LsLaunch& l, int start, hipStream_t s);
 void ls_enqueue_iteration(const LsLaunch& l, int mode, int flags, hipStream_t s);
 void ls_enqueue_final(const LsLaunch& l, hipStream_t s);
