@@ -453,8 +453,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--probe", action="store_true", help="only the launches of the counter runs")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
-                    help="c2: the headline line (with a bounded c4 sample); c3 / c4: the full batch / serving runs")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c3-shape"], default="c2",
+                    help="c2: the headline line (with a bounded c4 sample); c3 / c4: the full batch / serving runs; "
+                         "c3-shape: the c3 trajectories of one grid (--shape) as one batch, one thread")
+    ap.add_argument("--shape", default="48x48x48", help="grid of the c3-shape workload")
+    ap.add_argument("--seconds", type=float, default=3.0, help="timed seconds of the c3-shape workload")
     ap.add_argument("--serving-requests", type=int, default=48, help="requests of the bounded c4 sample")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="leave out the full c3 batch run of the default line")
@@ -473,6 +476,8 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     from mgkmc_cpp_amd import abi, configs
+    if a.workload == "c3-shape":
+        return main_c3_shape(a)
     if a.workload in ("c3", "c4"):
         return main_many(a)
     E, nu = configs.material(SHAPE, 4.0, 42)
@@ -557,6 +562,54 @@ def main():
     }
     if rank == 0:
         print(json.dumps(out), flush=True)
+
+
+def main_c3_shape(a):
+    """The c3 trajectories of one grid advanced as one batch from one thread
+    (no other shape beside it): set-up, one untimed event step, then event
+    steps of 8 events for about `a.seconds`.  Their single-trajectory
+    FP32+refine rate (the first trajectory alone, 100 events) is reported
+    beside the batch's."""
+    from mgkmc_cpp_amd import SimulationBatch, configs
+    from mgkmc_cpp_amd.serving import request_simulation
+    shape = tuple(int(x) for x in a.shape.split("x"))
+    reqs = [r for r in configs.c3_requests(256, events=2000) if r.shape == shape]
+    if not reqs:
+        sys.exit(f"no c3 trajectory has the grid {a.shape}")
+    print(f"[bench] c3-shape {a.shape}: {len(reqs)} trajectories", file=sys.stderr, flush=True)
+    sims = [request_simulation(r) for r in reqs]
+    for s in sims:
+        s.elastic_run()
+    one = sims[0]
+    one.event_steps(5)
+    t0 = time.perf_counter()
+    st = one.event_steps(100)
+    single = st.kmc_events / (time.perf_counter() - t0)
+    batch = SimulationBatch(sims)
+    batch.event_steps(8)
+    n = rounds = fp64 = fp32 = solves = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < a.seconds:
+        stats = batch.event_steps(8)
+        n += sum(int(x.kmc_events) for x in stats)
+        fp64 += sum(int(x.ls_iters) for x in stats)
+        fp32 += sum(int(x.ls_iters_fp32) for x in stats)
+        solves += sum(int(x.ls_solves) for x in stats)
+        rounds += 1
+    dt = time.perf_counter() - t0
+    batch.close()
+    for s in sims:
+        s.close()
+    out = {"metric": "aggregate KMC events/s of one grid's c3 trajectories in one batch", "value": round(n / dt, 2),
+           "unit": "events/s", "n_gpus": 1, "steps": n, "warmup": 8, "ms_per_step": round(1e3 * dt / max(n, 1), 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64-refine",
+           "data": "synthetic",
+           "config": {"workload": f"c3 trajectories on {a.shape} (seed 7 draw), one batch of {len(sims)}, "
+                                  "8 events per step, FP32+refine, inputs drawn on the device"},
+           "trajectories": len(sims), "single_trajectory_events_per_s": round(single, 2),
+           "ms_per_batch_step": round(1e3 * dt / max(rounds, 1), 3),
+           "ls_iters_per_event": {"fp64": round(fp64 / max(solves, 1), 2), "fp32": round(fp32 / max(solves, 1), 2)}}
+    print(json.dumps(out), flush=True)
 
 
 def main_many(a):
