@@ -1171,10 +1171,12 @@ __device__ __forceinline__ CgDecision cg_step_update(Ctl* ctl, const double (&to
 // reads it as contiguously as the interleaved layout.  Block x is (c, f) =
 // (x / nf, x % nf).
 // ---------------------------------------------------------------------------
-#ifndef MGK_PLANE_THREADS
-#define MGK_PLANE_THREADS 512
-#endif
-constexpr int kPlaneThreads = MGK_PLANE_THREADS;   // lanes per plane block (a build-time tuning knob)
+// Lanes per plane block: 256 up to 32-point planes (a 32-point plane's 256
+// four-voxel groups one per lane; 512 lanes left half of them idle in the
+// vector loops: a 32^3 batch of 46 ran 17.1 k against 22.1 k events/s), else
+// 512 (48-point planes: 5586 against 4910 events/s with 576 lanes, one group
+// per lane; the persistent cycle's pass C runs on the 512 lanes of its blocks).
+template <int NP> constexpr int plane_threads() { return NP <= 32 ? 256 : 512; }
 
 // Four consecutive voxels of one row: the plane passes move the per-voxel
 // CG vectors in 16-byte accesses (planes hold a multiple of 64 voxels, and
@@ -1192,13 +1194,14 @@ template <int NP> constexpr size_t plane_lds_bytes() {
 
 // Sizes of one (component, plane) block's work.
 template <int NP> struct PlaneDims {
+    static constexpr int T = plane_threads<NP>();                                // lanes per block
     static constexpr int NL = NP, NM = NP, NLH = NL / 2 + 1, NLP = padded<NL>(), NMP = padded<NM>();
     static constexpr int VOX = NM * NL;
     static constexpr int V4 = VOX / 4;                                          // groups of four voxels
-    static constexpr int Q4 = (V4 + kPlaneThreads - 1) / kPlaneThreads;         // groups per lane
+    static constexpr int Q4 = (V4 + T - 1) / T;         // groups per lane
     static constexpr int NS = NM * NLH;                                         // half-spectrum points
-    static constexpr int SPT = (NS + kPlaneThreads - 1) / kPlaneThreads;        // points per lane
-    static constexpr int PU = (6 * 256 + kPlaneThreads - 1) / kPlaneThreads;    // partial rows per lane (nf <= 256)
+    static constexpr int SPT = (NS + T - 1) / T;        // points per lane
+    static constexpr int PU = (6 * 256 + T - 1) / T;    // partial rows per lane (nf <= 256)
     static constexpr int QC = Q4 > 2 ? 2 : Q4;                                  // groups a lane holds at once
 };
 
@@ -1260,9 +1263,9 @@ __device__ __forceinline__ void plane_fwd_fft(cplx<SR>* buf, cplx<SR>* pl, const
     using cx = cplx<SR>;
     constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
     __syncthreads();
-    lds_fft_lane<NL, -1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+    lds_fft_lane<NL, -1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, PlaneDims<NP>::T);
     // split each packed row pair into the two half spectra, transposed
-    for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+    for (int idx = tid; idx < (NM / 2) * NLH; idx += PlaneDims<NP>::T) {
         const int pr = idx / NLH, kb = idx % NLH;
         const cx z = buf[pr * NLP + kb];
         const cx zm = buf[pr * NLP + (NL - kb) % NL];
@@ -1270,15 +1273,15 @@ __device__ __forceinline__ void plane_fwd_fft(cplx<SR>* buf, cplx<SR>* pl, const
         pl[kb * NMP + 2 * pr + 1] = {(SR)0.5 * (z.y + zm.y), (SR)-0.5 * (z.x - zm.x)};
     }
     __syncthreads();
-    lds_fft_lane<NM, -1, SR>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
-    for (int idx = tid; idx < NM * NLH; idx += kPlaneThreads) {
+    lds_fft_lane<NM, -1, SR>(pl, NLH, NMP, 1, tw_m, tid, PlaneDims<NP>::T);
+    for (int idx = tid; idx < NM * NLH; idx += PlaneDims<NP>::T) {
         const int m = idx / NLH, kb = idx % NLH;
         st(idx, pl[kb * NMP + m]);
     }
 }
 
 // Inverse in-plane transforms: the plane's half spectrum (point
-// idx = threadIdx.x + kPlaneThreads*k in sp[k]) into LDS, c2c along M, c2r
+// idx = threadIdx.x + PlaneDims<NP>::T*k in sp[k]) into LDS, c2c along M, c2r
 // along L; eps_tilde_c is left in buf's packed-row layout.
 template <int NP, typename SR>
 __device__ __forceinline__ void plane_inv_fft_staged(cplx<SR>* buf, cplx<SR>* pl, const cplx<SR>* tw_l,
@@ -1292,7 +1295,7 @@ __device__ __forceinline__ void plane_inv_fft(const cplx<SR> (&sp)[PlaneDims<NP>
     constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
 #pragma unroll
     for (int k = 0; k < PD::SPT; ++k) {
-        const int idx = tid + kPlaneThreads * k;
+        const int idx = tid + PlaneDims<NP>::T * k;
         if (idx < NM * NLH) pl[(idx % NLH) * NMP + idx / NLH] = sp[k];
     }
     plane_inv_fft_staged<NP, SR>(buf, pl, tw_l, tw_m, tid);
@@ -1307,9 +1310,9 @@ __device__ __forceinline__ void plane_inv_fft_staged(cplx<SR>* buf, cplx<SR>* pl
     using cx = cplx<SR>;
     constexpr int NL = PD::NL, NM = PD::NM, NLH = PD::NLH, NLP = PD::NLP, NMP = PD::NMP;
     __syncthreads();
-    lds_fft_lane<NM, +1, SR>(pl, NLH, NMP, 1, tw_m, tid, kPlaneThreads);
+    lds_fft_lane<NM, +1, SR>(pl, NLH, NMP, 1, tw_m, tid, PlaneDims<NP>::T);
     // pack row pairs into complex lines for the c2r along L
-    for (int idx = tid; idx < (NM / 2) * NLH; idx += kPlaneThreads) {
+    for (int idx = tid; idx < (NM / 2) * NLH; idx += PlaneDims<NP>::T) {
         const int pr = idx / NLH, kb = idx % NLH;
         const cx A = pl[kb * NMP + 2 * pr], Bv = pl[kb * NMP + 2 * pr + 1];
         cx* line = buf + pr * NLP;
@@ -1321,7 +1324,7 @@ __device__ __forceinline__ void plane_inv_fft_staged(cplx<SR>* buf, cplx<SR>* pl
         }
     }
     __syncthreads();
-    lds_fft_lane<NL, +1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, kPlaneThreads);
+    lds_fft_lane<NL, +1, SR>(buf, NM / 2, NLP, 1, tw_l, tid, PlaneDims<NP>::T);
 }
 
 // w_c = r_c - eps_tilde_c (eps_tilde in buf, as plane_inv_fft leaves it) and
@@ -1336,7 +1339,7 @@ __device__ __forceinline__ void plane_w_partials(const float* bufr, int c, doubl
     acc[0] = acc[1] = acc[2] = 0.0;
 #pragma unroll
     for (int k = 0; k < PD::Q4; ++k) {
-        const int q = tid + kPlaneThreads * k;
+        const int q = tid + PlaneDims<NP>::T * k;
         if (q >= PD::V4) continue;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -1369,12 +1372,12 @@ __device__ __forceinline__ void plane_w_partials_stream(const float* bufr, int c
         vox4 r[QC];
 #pragma unroll
         for (int k = 0; k < QC; ++k) {
-            const int q = tid + kPlaneThreads * (ch * QC + k);
+            const int q = tid + PlaneDims<NP>::T * (ch * QC + k);
             r[k] = ld_vox4(rg, q < PD::V4 ? q : 0);
         }
 #pragma unroll
         for (int k = 0; k < QC; ++k) {
-            const int q = tid + kPlaneThreads * (ch * QC + k);
+            const int q = tid + PlaneDims<NP>::T * (ch * QC + k);
             if (q >= PD::V4) continue;
             vox4 w;
 #pragma unroll
@@ -1397,16 +1400,16 @@ __device__ __forceinline__ void plane_w_partials_stream(const float* bufr, int c
 }
 
 // The folded reduction's per-lane sums of the previous step's partial rows
-// (row b = threadIdx.x + kPlaneThreads*u, loaded into part[.][u]); a
+// (row b = threadIdx.x + T*u, loaded into part[.][u]); a
 // block_sum of tot follows.
-template <int PU>
+template <int PU, int T>
 __device__ __forceinline__ void plane_fold_sum(const double (&part)[3][PU], int nbp, double (&tot)[3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         double v = 0.0;
 #pragma unroll
         for (int u = 0; u < PU; ++u)
-            if ((int)threadIdx.x + kPlaneThreads * u < nbp) v += part[k][u];
+            if ((int)threadIdx.x + T * u < nbp) v += part[k][u];
         tot[k] = v;
     }
 }
@@ -1444,7 +1447,7 @@ __device__ __forceinline__ vox4 r0_from_cart(const LsParams& p, int m, long base
 // blocks share a CU: the batched instantiation took 134 with three rows and
 // ran one block per CU.  A 128-point plane's LDS holds one block per CU.
 template <int NP, bool BATCH, int PU>
-__global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(NP > 64 ? 1 : 4))) void k_fwd_plane(
+__global__ __launch_bounds__(PlaneDims<NP>::T) __attribute__((amdgpu_waves_per_eu(NP > 64 ? 1 : 4))) void k_fwd_plane(
     const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     using PD = PlaneDims<NP>;
@@ -1477,7 +1480,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(N
     if (upd) {
 #pragma unroll
         for (int u = 0; u < PU; ++u) {
-            const int b = threadIdx.x + kPlaneThreads * u;
+            const int b = threadIdx.x + PlaneDims<NP>::T * u;
 #pragma unroll
             for (int k = 0; k < 3; ++k) part[k][u] = gld(p.partials + 3 * (b < nbp ? b : 0) + k);
         }
@@ -1495,7 +1498,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(N
     auto load_chunk = [&](int ch) {
 #pragma unroll
         for (int k = 0; k < QC; ++k) {
-            const int q = threadIdx.x + kPlaneThreads * (ch * QC + k);
+            const int q = threadIdx.x + PlaneDims<NP>::T * (ch * QC + k);
             const int qq = q < V4 ? q : 0;
             r[k] = r_cart ? r0_from_cart(p, c, (long)f * VOX, qq) : ld_vox4(p.r32 + base, qq);
             dm[k] = ld_vox4(p.dmu32 + (long)f * VOX, qq);
@@ -1517,7 +1520,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(N
     float al = (float)p.ctl->cg_alpha, be = (float)p.ctl->cg_beta;
     if (upd) {
         double tot[3];
-        plane_fold_sum<PU>(part, nbp, tot);
+        plane_fold_sum<PU, PD::T>(part, nbp, tot);
         block_sum<3>(tot, red);
         if (threadIdx.x == 0) {
             const bool probe = flags & LS_FLAG_PROBE;
@@ -1537,7 +1540,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(N
         if (ch > 0) load_chunk(ch);
 #pragma unroll
         for (int k = 0; k < QC; ++k) {
-            const int q = threadIdx.x + kPlaneThreads * (ch * QC + k);
+            const int q = threadIdx.x + PlaneDims<NP>::T * (ch * QC + k);
             if (q >= V4) continue;
             plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv[k], xv[k], tau_factor(c, dl[k], dm[k]),
                                  bufr);
@@ -1555,7 +1558,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(N
 }
 
 template <int NP, bool BATCH>
-__global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+__global__ __launch_bounds__(PlaneDims<NP>::T) void k_inv_plane(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     using PD = PlaneDims<NP>;
     constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, SPT = PD::SPT;
@@ -1575,7 +1578,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
     cx sp[SPT];
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
-        const int idx = threadIdx.x + kPlaneThreads * k;
+        const int idx = threadIdx.x + PlaneDims<NP>::T * k;
         sp[k] = gld(spec + tile_off(idx < PD::NS ? idx : 0));
     }
     double acc[3];
@@ -1584,7 +1587,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         vox4 r[Q4], w[Q4];
 #pragma unroll
         for (int k = 0; k < Q4; ++k) {
-            const int q = threadIdx.x + kPlaneThreads * k;
+            const int q = threadIdx.x + PlaneDims<NP>::T * k;
             r[k] = ld_vox4(p.r32 + base, q < V4 ? q : 0);
         }
         stage_twiddles<float>(tw_l, reinterpret_cast<const cx*>(p.tw_l32), NL);
@@ -1594,7 +1597,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane(const LsParams p_ar
         plane_w_partials<NP>(reinterpret_cast<const float*>(buf), c, p.lam0, p.mu0, r, w, acc, (int)threadIdx.x);
 #pragma unroll
         for (int k = 0; k < Q4; ++k) {
-            const int q = threadIdx.x + kPlaneThreads * k;
+            const int q = threadIdx.x + PlaneDims<NP>::T * k;
             if (q < V4) st_vox4(p.w32 + base, q, w[k]);
         }
     } else {
@@ -1645,9 +1648,9 @@ template <int NP> constexpr size_t plane64_lds_bytes() {
 // loads stay unconditional.  eps' = eps + warm-start shift (+ x + r).
 template <int NP> struct Plane64Loop {
     static constexpr int VOX = PlaneDims<NP>::VOX;
-    static constexpr int K = (VOX + kPlaneThreads - 1) / kPlaneThreads;   // voxels per lane
+    static constexpr int K = (VOX + PlaneDims<NP>::T - 1) / PlaneDims<NP>::T;   // voxels per lane
     static constexpr int U = K < 4 ? K : 4;                               // voxels per chunk
-    __device__ static int voxel(int k) { return (int)threadIdx.x + kPlaneThreads * k; }
+    __device__ static int voxel(int k) { return (int)threadIdx.x + PlaneDims<NP>::T * k; }
 };
 
 constexpr double kRsqrt2 = 0.70710678118654752440;
@@ -1797,7 +1800,7 @@ __device__ __forceinline__ void plane64_update(const LsParams& p, int c, int f, 
 }
 
 template <int NP, bool BATCH>
-__global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+__global__ __launch_bounds__(PlaneDims<NP>::T) void k_fwd_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     using PD = PlaneDims<NP>;
     constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX;
@@ -1830,7 +1833,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_plane64(const LsParams p_
 }
 
 template <int NP, bool BATCH>
-__global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
+__global__ __launch_bounds__(PlaneDims<NP>::T) void k_inv_plane64(const LsParams p_arg, const LsParams* __restrict__ P, int flags) {
     const LsParams& p = param_sel<BATCH>(p_arg, P);
     using PD = PlaneDims<NP>;
     constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, SPT = PD::SPT;
@@ -1848,7 +1851,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
         cx sp[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
-            const int idx = threadIdx.x + kPlaneThreads * k;
+            const int idx = threadIdx.x + PlaneDims<NP>::T * k;
             sp[k] = gld(spec + tile_off(idx < PD::NS ? idx : 0));
         }
         stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
@@ -1860,7 +1863,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_inv_plane64(const LsParams p_
         // there: the bins go straight to LDS instead (same placement).
         stage_twiddles<double>(tw_l, reinterpret_cast<const cx*>(p.tw_l64), NL);
         stage_twiddles<double>(tw_m, reinterpret_cast<const cx*>(p.tw_m64), NM);
-        for (int idx = threadIdx.x; idx < PD::NS; idx += kPlaneThreads)
+        for (int idx = threadIdx.x; idx < PD::NS; idx += PlaneDims<NP>::T)
             pl[(idx % PD::NLH) * PD::NMP + idx / PD::NLH] = gld(spec + tile_off(idx));
         plane_inv_fft_staged<NP, double>(buf, pl, tw_l, tw_m, (int)threadIdx.x);
     }
@@ -1928,13 +1931,13 @@ template <int NP> constexpr size_t cycle_lds_bytes() {
 }
 
 template <int NP>
-__global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_cycle(const LsParams p, int J) {
+__global__ __launch_bounds__(PlaneDims<NP>::T) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_cycle(const LsParams p, int J) {
     using PD = PlaneDims<NP>;
     constexpr int NF = NP;
     constexpr int NL = PD::NL, NM = PD::NM, NLP = PD::NLP, VOX = PD::VOX, V4 = PD::V4, Q4 = PD::Q4, SPT = PD::SPT,
                   PU = PD::PU;
     constexpr int G = gamma_points<float, NF, 6>();
-    static_assert(gamma_threads<float, NF, 6>() == kPlaneThreads, "pass C runs on the plane blocks' lanes");
+    static_assert(gamma_threads<float, NF, 6>() == PlaneDims<NP>::T, "pass C runs on the plane blocks' lanes");
     using cx = cplx<float>;
     Ctl* ctl = p.ctl;
     // Every block reads the control block before its first barrier arrival;
@@ -1963,7 +1966,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
     vox4 r[Q4], w[Q4], pv[Q4], tf[Q4];
 #pragma unroll
     for (int k = 0; k < Q4; ++k) {
-        const int q = threadIdx.x + kPlaneThreads * k;
+        const int q = threadIdx.x + PlaneDims<NP>::T * k;
         const int qq = q < V4 ? q : 0;
         r[k] = p.r0_cart ? r0_from_cart(p, c, (long)f * VOX, qq) : ld_vox4(p.r32 + base, qq);
         const vox4 dm = ld_vox4(p.dmu32 + (long)f * VOX, qq);
@@ -1995,7 +1998,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
         int tid = lane_opaque();
 #pragma unroll
         for (int k = 0; k < Q4; ++k) {
-            const int q = tid + kPlaneThreads * k;
+            const int q = tid + PlaneDims<NP>::T * k;
             if (q >= V4) continue;
             vox4 sv = ld_vox4_lds(s_lds, q), xv = ld_vox4_lds(x_lds, q);
             plane_update_tau<NP>(q, upd, upd1, al, be, r[k], w[k], pv[k], sv, xv, tf[k], bufr);
@@ -2014,7 +2017,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
             const CohLine column{spec, ln.active ? xi0 * 6 + ln.l : 0, nxi * 6};
             cx v[NF / FftPlan<NF>::threads];
             line_gather<NF, float>(column, ln.t, ln.active, v);
-            gamma_core<float, NF, 6, true>(v, ln, column, tw_f, buf, xi0, g, p.lam0, p.mu0, tid, kPlaneThreads);
+            gamma_core<float, NF, 6, true>(v, ln, column, tw_f, buf, xi0, g, p.lam0, p.mu0, tid, PlaneDims<NP>::T);
         }
         if (!(ok = grid_sync(p.gbar, ++epoch, &s_flag))) break;
 
@@ -2023,7 +2026,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
         cx sp[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
-            const int idx = tid + kPlaneThreads * k;
+            const int idx = tid + PlaneDims<NP>::T * k;
             sp[k] = spec.ldc(plane_off + tile_off(idx < PD::NS ? idx : 0));
         }
         plane_inv_fft<NP>(sp, buf, pl, tw_l, tw_m, tid);
@@ -2044,11 +2047,11 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
             double part[3][PU];
 #pragma unroll
             for (int u = 0; u < PU; ++u) {
-                const int b = tid + kPlaneThreads * u;
+                const int b = tid + PlaneDims<NP>::T * u;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) part[k][u] = parts.ldd(3 * (b < nbp ? b : 0) + k);
             }
-            plane_fold_sum<PU>(part, nbp, tot);
+            plane_fold_sum<PU, PD::T>(part, nbp, tot);
             block_sum<3>(tot, red);
         }
         if (threadIdx.x == 0) {
@@ -2073,7 +2076,7 @@ __global__ __launch_bounds__(kPlaneThreads) __attribute__((amdgpu_waves_per_eu(4
     if (updated) {
 #pragma unroll
         for (int k = 0; k < Q4; ++k) {
-            const int q = threadIdx.x + kPlaneThreads * k;
+            const int q = threadIdx.x + PlaneDims<NP>::T * k;
             if (q >= V4) continue;
             st_vox4(p.r32 + base, q, r[k]);
             st_vox4(p.corr + base, q, ld_vox4_lds(x_lds, q));
@@ -2456,15 +2459,14 @@ template <class K> static void allow_lds(K kernel, size_t bytes) {
 template <int NN, bool BATCH, int PU> static void launch_fwd_plane_pu(const LsLaunch& p, int flags, hipStream_t s) {
     static const bool lds_ok = (allow_lds(k_fwd_plane<NN, BATCH, PU>, plane_lds_bytes<NN>()), true);
     (void)lds_ok;
-    hipLaunchKernelGGL((k_fwd_plane<NN, BATCH, PU>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads), plane_lds_bytes<NN>(),
+    hipLaunchKernelGGL((k_fwd_plane<NN, BATCH, PU>), dim3(6 * p.g.nf, p.B), dim3(PlaneDims<NN>::T), plane_lds_bytes<NN>(),
                        s, p.h, p.P, flags);
 }
 
 template <bool BATCH> static void launch_fwd_plane(const LsLaunch& p, int flags, hipStream_t s) {
-    const bool one_row = 6 * p.g.nf <= kPlaneThreads;
 #define MGK_CASE(NN)                                                                                  \
     if (p.g.nm == NN) {                                                                               \
-        if (one_row) launch_fwd_plane_pu<NN, BATCH, 1>(p, flags, s);                                  \
+        if (6 * p.g.nf <= PlaneDims<NN>::T) launch_fwd_plane_pu<NN, BATCH, 1>(p, flags, s);          \
         else launch_fwd_plane_pu<NN, BATCH, PlaneDims<NN>::PU>(p, flags, s);                          \
         return;                                                                                       \
     }
@@ -2479,7 +2481,7 @@ template <bool BATCH> static void launch_inv_plane(const LsLaunch& p, int flags,
     if (p.g.nm == NN) {                                                                               \
         static const bool lds_ok = (allow_lds(k_inv_plane<NN, BATCH>, plane_lds_bytes<NN>()), true);  \
         (void)lds_ok;                                                                                 \
-        hipLaunchKernelGGL((k_inv_plane<NN, BATCH>), dim3(nb, p.B), dim3(kPlaneThreads),              \
+        hipLaunchKernelGGL((k_inv_plane<NN, BATCH>), dim3(nb, p.B), dim3(PlaneDims<NN>::T),              \
                            plane_lds_bytes<NN>(), s, p.h, p.P, flags);                               \
         if (reduce)                                                                                   \
             hipLaunchKernelGGL((k_ls_reduce<LS_CG, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb, flags); \
@@ -2506,7 +2508,7 @@ template <int NP> static bool cycle_fits() {
         bool ok = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess &&
                   hipFuncSetAttribute((const void*)k_cg_cycle<NP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)cycle_lds_bytes<NP>()) == hipSuccess &&
-                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_cg_cycle<NP>, kPlaneThreads,
+                  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_cg_cycle<NP>, PlaneDims<NP>::T,
                                                                cycle_lds_bytes<NP>()) == hipSuccess;
         ok = ok && (long)per_cu * pr.multiProcessorCount >= 6L * NP;
         state.store(ok ? 1 : 2);
@@ -2530,7 +2532,7 @@ void ls_enqueue_cycle(const LsLaunch& p, int J, hipStream_t s) {
     if (p.batch || !p.h.gbar) throw ArgError("persistent CG cycles run on a single context");
 #define MGK_CASE(NN)                                                                                           \
     if (p.g.nm == NN) {                                                                                        \
-        hipLaunchKernelGGL((k_cg_cycle<NN>), dim3(6 * NN), dim3(kPlaneThreads), cycle_lds_bytes<NN>(), s, p.h, J); \
+        hipLaunchKernelGGL((k_cg_cycle<NN>), dim3(6 * NN), dim3(PlaneDims<NN>::T), cycle_lds_bytes<NN>(), s, p.h, J); \
         MGK_HIP_CHECK(hipGetLastError());                                                                      \
         return;                                                                                                \
     }
@@ -2582,7 +2584,7 @@ template <bool BATCH> static void launch_fwd_plane64(const LsLaunch& p, int flag
     if (p.g.nm == NN) {                                                                               \
         static const bool lds_ok = (allow_lds(k_fwd_plane64<NN, BATCH>, plane64_lds_bytes<NN>()), true); \
         (void)lds_ok;                                                                                 \
-        hipLaunchKernelGGL((k_fwd_plane64<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(kPlaneThreads),     \
+        hipLaunchKernelGGL((k_fwd_plane64<NN, BATCH>), dim3(6 * p.g.nf, p.B), dim3(PlaneDims<NN>::T),     \
                            plane64_lds_bytes<NN>(), s, p.h, p.P, flags);                             \
         return;                                                                                       \
     }
@@ -2597,7 +2599,7 @@ template <bool BATCH> static void launch_inv_plane64(const LsLaunch& p, int flag
     if (p.g.nm == NN) {                                                                               \
         static const bool lds_ok = (allow_lds(k_inv_plane64<NN, BATCH>, plane64_lds_bytes<NN>()), true); \
         (void)lds_ok;                                                                                 \
-        hipLaunchKernelGGL((k_inv_plane64<NN, BATCH>), dim3(nb, p.B), dim3(kPlaneThreads),            \
+        hipLaunchKernelGGL((k_inv_plane64<NN, BATCH>), dim3(nb, p.B), dim3(PlaneDims<NN>::T),            \
                            plane64_lds_bytes<NN>(), s, p.h, p.P, flags);                             \
         if (reduce)                                                                                   \
             hipLaunchKernelGGL((k_ls_reduce<LS_FULL, BATCH>), dim3(1, p.B), dim3(256), 0, s, p.h, p.P, nb, flags); \
