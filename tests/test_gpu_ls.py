@@ -111,8 +111,10 @@ def test_fp32_refine_error_bound():
     assert err < 1e-6
 
 
+# (64, 32, 32): 32-point planes run 256 lanes per block, and the previous
+# step's 6*64 partials take more than one row per lane in the folded reduction
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (48, 48, 48), (64, 64, 64), (16, 48, 32), (32, 16, 16),
-                                   (16, 128, 128)])
+                                   (16, 128, 128), (64, 32, 32)])
 def test_fp32_refine_plane_and_line_passes(shape, monkeypatch):
     # Square nm = nl planes run the CG steps as plane passes (two launches per
     # step fewer); MGK_LS_PLANE=0 forces the five-pass pipeline.  Both are

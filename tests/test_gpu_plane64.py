@@ -21,7 +21,7 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
-SHAPES = [(16, 16, 16), (32, 32, 32), (48, 48, 48), (64, 64, 64), (32, 48, 48), (16, 64, 64)]
+SHAPES = [(16, 16, 16), (32, 32, 32), (48, 48, 48), (64, 64, 64), (32, 48, 48), (16, 64, 64), (64, 32, 32)]
 
 
 def _field(shape, seed, cov=0.15, mean=85e9):
