@@ -1,7 +1,9 @@
-"""Persistent CG cycles (one cooperative launch per refinement cycle, 64^3
-planes) against the separately launched plane passes they replace: the same
-iterates bit for bit, in a cold LS solve with several refinement cycles and in
-a KMC event trajectory; the in-situ cycle timer counts what the solves ran."""
+"""Persistent CG cycles (one launch per refinement cycle whose blocks are all
+resident, 64^3 planes) against the separately launched plane passes they
+replace: the same iterates bit for bit, in a cold LS solve with several
+refinement cycles and in a KMC event trajectory; the in-situ cycle timer
+counts what the solves ran; contexts that run cycles in several threads, or
+beside a batch on another stream, give their sequential results."""
 import numpy as np
 import pytest
 
@@ -85,3 +87,96 @@ def test_cycle_timer_counts_the_cycles():
     cycle_timer(ctx.handle, False)
     assert cycle_timer_read(ctx.handle) == (0.0, 0, 0)
     ctx.close()
+
+
+def _run_events(sim, n):
+    e = np.zeros((3, 3))
+    e[0, 0] = 0.01
+    sim.set_macro(e, 0.0)
+    sim.elastic_run()
+    sim.event_steps(n)
+    return ([(x.flat_idx, x.time, x.ls_iters) for x in sim.events()],
+            [sim.get_field(w) for w in (abi.FIELD_EPS, abi.FIELD_SIG, abi.FIELD_EPS_PLASTIC)])
+
+
+def _same(a, b):
+    assert a[0] == b[0]
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("beside", ["cycle", "batch"])
+def test_cycles_beside_other_work_equal_sequential_runs(beside):
+    # One persistent cycle is in flight per process: a second context's cycle
+    # waits for it instead of competing for the same CU slots.  Work of other
+    # streams (a batch of 48^3 trajectories) only delays a cycle's blocks.
+    import threading
+    from mgkmc_cpp_amd import SimulationBatch
+    E, nu = configs.material(SHAPE, 4, 42)
+    cfg = configs.sim_config(SHAPE, precision=abi.MGK_FP32_REFINE, seed=42)
+    ref = Simulation(cfg, E, nu)
+    set_tuning(ref.device_context_handle(), cycle=1)
+    assert cycle_supported(ref.device_context_handle()) == 1
+    want = _run_events(ref, 8)
+    ref.close()
+    if beside == "cycle":
+        E2, nu2 = configs.material(SHAPE, 4, 7)
+        cfg2 = configs.sim_config(SHAPE, precision=abi.MGK_FP32_REFINE, seed=7)
+        ref2 = Simulation(cfg2, E2, nu2)
+        set_tuning(ref2.device_context_handle(), cycle=1)
+        want2 = _run_events(ref2, 8)
+        ref2.close()
+    else:
+        shape2 = (48, 48, 48)
+        cfgs = [configs.sim_config(shape2, precision=abi.MGK_FP32_REFINE, seed=s) for s in (3, 4)]
+        mats = [configs.material(shape2, 4, s) for s in (3, 4)]
+        alone = [Simulation(c, *m) for c, m in zip(cfgs, mats)]
+        for a in alone:
+            a.elastic_run()
+            a.event_steps(6)
+        want2 = [([x.flat_idx for x in a.events()], a.get_field(abi.FIELD_SIG)) for a in alone]
+        for a in alone:
+            a.close()
+    sim = Simulation(cfg, E, nu)
+    set_tuning(sim.device_context_handle(), cycle=1)
+    got, errs = {}, []
+
+    def first():
+        try:
+            got[1] = _run_events(sim, 8)
+        except BaseException as e:   # reported below
+            errs.append(e)
+
+    def second():
+        try:
+            if beside == "cycle":
+                sim2 = Simulation(cfg2, E2, nu2)
+                set_tuning(sim2.device_context_handle(), cycle=1)
+                got[2] = _run_events(sim2, 8)
+                sim2.close()
+            else:
+                sims = [Simulation(c, *m) for c, m in zip(cfgs, mats)]
+                for x in sims:
+                    x.elastic_run()
+                batch = SimulationBatch(sims)
+                batch.event_steps(6)
+                got[2] = [([x.flat_idx for x in y.events()], y.get_field(abi.FIELD_SIG)) for y in sims]
+                batch.close()
+                for x in sims:
+                    x.close()
+        except BaseException as e:   # reported below
+            errs.append(e)
+
+    ts = [threading.Thread(target=first), threading.Thread(target=second)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    sim.close()
+    assert not errs, errs
+    _same(got[1], want)
+    if beside == "cycle":
+        _same(got[2], want2)
+    else:
+        for (gi, gs), (wi, ws) in zip(got[2], want2):
+            assert gi == wi and np.array_equal(gs, ws)
