@@ -205,8 +205,10 @@ constexpr unsigned kSpinLimit = 1u << 21;      // polls before a barrier gives u
 // Every block of the launch calls this with the same epoch (1, 2, ...).  A
 // lane's stores are performed before its block arrives (s_waitcnt vmcnt(0)),
 // and the data exchanged across the barrier moves through CohBuf accesses.
-// Returns false on a timeout (the error flag is set and every later barrier
-// returns false at once), so every block leaves a failed launch.
+// Lane 0 arrives with no load ahead of its atomic, and each poll loads the
+// group's release word and the error flag together (one round trip per
+// poll).  Returns false once the error flag is set, by this block on a
+// timeout or by another block, so every block leaves a failed launch.
 __device__ __forceinline__ bool grid_sync(unsigned* w, unsigned epoch, int* s_flag) {
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     __syncthreads();
@@ -215,24 +217,24 @@ __device__ __forceinline__ bool grid_sync(unsigned* w, unsigned epoch, int* s_fl
         const unsigned ng = nb < 8 ? nb : 8u;
         const unsigned x = blockIdx.x % 8;
         const unsigned nx = nb / 8 + (x < nb % 8 ? 1u : 0u);
-        int err = __hip_atomic_load(w + 32 * 17, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-        if (!err) {
-            const unsigned old = __hip_atomic_fetch_add(w + 32 * (1 + x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old + 1 == epoch * nx) {
-                const unsigned top = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (top + 1 == epoch * ng)
-                    for (unsigned q = 0; q < ng; ++q)
-                        __hip_atomic_store(w + 32 * (9 + q), epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int err = 0;
+        const unsigned old = __hip_atomic_fetch_add(w + 32 * (1 + x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == epoch * nx) {
+            const unsigned top = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (top + 1 == epoch * ng)
+                for (unsigned q = 0; q < ng; ++q)
+                    __hip_atomic_store(w + 32 * (9 + q), epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (unsigned spins = 0;; ++spins) {
+            const unsigned rel = __hip_atomic_load(w + 32 * (9 + x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned bad = __hip_atomic_load(w + 32 * 17, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (bad != 0 || (rel < epoch && spins >= kSpinLimit)) {
+                __hip_atomic_store(w + 32 * 17, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                err = 1;
+                break;
             }
-            unsigned spins = 0;
-            while (__hip_atomic_load(w + 32 * (9 + x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                if (++spins > kSpinLimit || __hip_atomic_load(w + 32 * 17, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    __hip_atomic_store(w + 32 * 17, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    err = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+            if (rel >= epoch) break;
+            __builtin_amdgcn_s_sleep(1);
         }
         *s_flag = err;
     }

@@ -7,6 +7,8 @@
 * apply_flip agrees bit-exactly;
 * whole trajectories of the drop-in host driver reproduce the oracle's event
   sequence, with stress within 1e-10."""
+import math
+
 import numpy as np
 import pytest
 
@@ -75,6 +77,38 @@ def test_rates_and_selection_fed_oracle_state(dims, directional):
         want = idx_o[po.select_sequential(rates_o, u2 * total_o)]
         got, q = ctx.select(u2)
         assert got == want, (u2, got, want)
+
+
+@pytest.mark.parametrize("catalog,directional", [(abi.CATALOG_FIXED, 0), (abi.CATALOG_RANDOM, 1)])
+def test_rates_and_selection_fed_oracle_state_64cube(catalog, directional):
+    # 3.1 M modes: the rate pass runs four 1024-mode rate blocks per
+    # workgroup (shared catalog staged in LDS, or per-voxel catalog rows)
+    c = _cfg(dims=3, nx=64, ny=64, nz=64, M=12, softening_directional=directional, instability_kmc=1,
+             catalog_mode=catalog)
+    N = c.nx * c.ny * c.nz
+    E, nu = _field(N)
+    osim = _evolved_oracle(c, E, nu, events=2)
+    ctx = _device_from_oracle(osim, c, E, nu)
+    _, _, t = osim.macro()
+    osim.update_barriers()
+    total_o, rates_o, idx_o = osim.rates()
+    total_d, n_unst, n_incl = ctx.update_rates(t)
+    assert n_incl == len(rates_o) == N * c.M
+    assert _rel(ctx.get_field(abi.FIELD_Q), osim.get_field(abi.FIELD_Q)) < 1e-14
+    # The oracle's left-to-right total (the reference's order) carries up to
+    # n u of rounding over 3.1 M terms: once the running sum is large, small
+    # rates fall below its half ulp (1.3e-11 of the total here).  The
+    # device's blocked sum stays within 1e-12 of the exactly rounded sum,
+    # and selection allows for the n u difference (its ambiguity bound).
+    exact = math.fsum(rates_o)
+    assert abs(total_o - exact) <= N * c.M * 2.0 ** -53 * exact
+    assert abs(total_d - exact) <= 1e-12 * exact
+    rng = np.random.default_rng(12)
+    for u2 in rng.random(300):
+        want = idx_o[po.select_sequential(rates_o, u2 * total_o)]
+        got, q = ctx.select(u2)
+        assert got == want, (u2, got, want)
+    ctx.close()
 
 
 @pytest.mark.parametrize("kw", [
