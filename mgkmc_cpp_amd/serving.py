@@ -49,6 +49,7 @@ class Request:
     finished: Optional[float] = None
     events_done: int = 0
     footprint: int = 0
+    stopped: bool = False              # ended short of its events: an attempt without rate (R not > 0)
 
     @property
     def latency(self):
@@ -216,6 +217,7 @@ class _Worker(threading.Thread):
                 stopped = made == 0 and int(st.cascades) == 0
                 if sl.req.events_done >= sl.req.events or stopped:
                     sl.req.finished = t
+                    sl.req.stopped = sl.req.events_done < sl.req.events
                     if srv.on_complete is not None:
                         srv.on_complete(sl.req, sl.sim)
                     sl.parked = True
@@ -337,6 +339,7 @@ def summary(requests, wall_s=None):
     t_start = min((r.arrival for r in requests), default=0.0)
     span = wall_s if wall_s is not None else (t_end - t_start)
     return dict(requests=len(requests), completed=int(lat.size), events=int(ev),
+                stopped_short=sum(1 for r in requests if r.stopped),
                 p50_latency_s=float(np.percentile(lat, 50)) if lat.size else None,
                 p99_latency_s=float(np.percentile(lat, 99)) if lat.size else None,
                 events_per_s=ev / span if span > 0 else None, span_s=span)
