@@ -2544,11 +2544,7 @@ void ls_enqueue_cycle(const LsLaunch& p, int J, hipStream_t s) {
 }
 
 
-// 48-point planes are left out: under the serving loop's c3 workload their
-// FP64 plane passes left non-finite strain in about one 48^3 trajectory per
-// 250 k events (some runs none, some three), with no cause found in the
-// kernels; with the five-pass FP64 iterations on those grids no run did.
-#define MGK_PLANE64_CASES(X) X(16) X(32) X(64)
+#define MGK_PLANE64_CASES(X) X(16) X(32) X(48) X(64)
 
 // FP64 pass C on the tiled spectrum needs whole tiles per block.
 static bool plane64_gamma_tiles(int nf) {
