@@ -183,151 +183,97 @@ __global__ __launch_bounds__(kRateThreads) void k_rates(const KmcParams p_arg, c
     rate_block_totals(p, lsum, lun, linc);
 }
 
-// The same pass for M >= kStageMinM, RB consecutive rate blocks per
-// workgroup.  The workgroup's voxels (at most RB kRateBlock / M + 2) are
-// staged in LDS once -- stress, previous direction, softening base and its
-// exponential -- so a lane loads only its modes' Q0 (and catalog rows when
-// they are per voxel).  A rate block's modes sit on the same lanes and are
-// summed in the same order for every RB, so Q and the block sums do not
-// depend on it; a larger RB puts more loads in flight per lane and runs the
-// grid in fewer rounds of resident workgroups (three per CU at six waves per
-// SIMD).  Dynamic LDS: nvmax doubles (at least the voxel count) for each
-// stress and direction component, g_base and exp(-g_base).
-template <int D, int C, bool BATCH, int RB>
-__global__ __launch_bounds__(kRateThreads) __attribute__((amdgpu_waves_per_eu(6)))
-void k_rates_staged(const KmcParams p_arg, const KmcParams* __restrict__ PK, double time,
-                    const double* __restrict__ times, int nvmax) {
+// The same pass for M >= kStageMinM: the block's voxels (at most
+// kRateBlock / M + 2) are staged in LDS once -- stress, previous direction,
+// softening base and its exponential -- so a lane loads only its modes' Q0
+// (and catalog rows when they are per voxel).  Same arithmetic, same bits.
+template <int D, int C, bool BATCH>
+__global__ __launch_bounds__(kRateThreads) void k_rates_staged(const KmcParams p_arg, const KmcParams* __restrict__ PK,
+                                                               double time, const double* __restrict__ times) {
     const KmcParams& p = param_sel<BATCH>(p_arg, PK);
     if (BATCH && p.skip) return;   // inactive batch row: Q and the block sums stay as they are
     if (BATCH) time = times[blockIdx.y];
-    constexpr int KB = kRateBlock / kRateThreads;   // modes per lane and rate block
-    constexpr int K = KB * RB;
-    constexpr int NW = kRateThreads / 64;
+    constexpr int K = kRateBlock / kRateThreads;
+    constexpr int NV = kRateBlock / kStageMinM + 2;
+    static_assert(NV <= kRateThreads, "one staged voxel per lane");
     const Geom g = p.g;
     const KmcPhys ph = p.ph;
     const long nmodes = g.N * ph.M;
-    const long base = (long)blockIdx.x * (RB * kRateBlock);
-    extern __shared__ double s_vox[];
-    double* s_sig = s_vox;              // [C][nvmax]
-    double* s_pd = s_sig + C * nvmax;   // [C][nvmax]
-    double* s_gb = s_pd + C * nvmax;
-    double* s_eg = s_gb + nvmax;
+    const long base = (long)blockIdx.x * kRateBlock;
+    __shared__ double s_sig[C][NV], s_pd[C][NV], s_gb[NV], s_eg[NV];
     __shared__ double s_cat[C][kStageCat];
-    __shared__ double sd[RB][NW];
-    __shared__ int su[RB][NW], si[RB][NW];
 
     const unsigned M = (unsigned)ph.M;
     const long v0 = (long)((unsigned)base / M);
-    const long last = (base + RB * kRateBlock < nmodes ? base + RB * kRateBlock : nmodes) - 1;
+    const long last = (base + kRateBlock < nmodes ? base + kRateBlock : nmodes) - 1;
     const long vend = (long)((unsigned)last / M);
     const int nv = (int)(vend - v0 + 1);
     const bool cat_lds = p.cat_shared && ph.M <= kStageCat;   // M*C <= kRateThreads entries
-    // a lane's mode indices are recomputed where used, so that only loaded
-    // values are held
-    const auto mode_of = [&](int k) {
-        const long i = base + k * kRateThreads + threadIdx.x;
-        return i < nmodes ? i : base;
-    };
-    // per-voxel inputs, one voxel per lane and round
-    for (int j0 = 0; j0 < nv; j0 += kRateThreads) {
-        const int j = j0 + (int)threadIdx.x;
-        const long v = v0 + (j < nv ? j : 0);
-        double gp = p.soft[v], gt = p.soft[g.N + v], tl = p.last_t[v], sv[C], pv[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) sv[c] = p.sig[c * g.N + v];
-        if (ph.directional) {
-#pragma unroll
-            for (int c = 0; c < C; ++c) pv[c] = p.prev_dir[c * g.N + v];
-        }
-        if (j < nv) {
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                s_sig[c * nvmax + j] = sv[c];
-                if (ph.directional) s_pd[c * nvmax + j] = pv[c];
-            }
-            const double g_base = voxel_gbase(ph, time, gp, gt, tl);
-            s_gb[j] = g_base;
-            s_eg[j] = ph.directional ? 0.0 : exp(-g_base);
-        }
-    }
-    // per-mode inputs, after the staging (it then needs no registers for
-    // them: six waves per SIMD): Q0, and per-voxel catalog rows here for one
-    // rate block per workgroup, at their use for several
+    // per-mode inputs
+    long mi[K];
+    int vj[K];
+    bool ok[K];
     double q0[K], cat[K][C];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const long mi = mode_of(k);
-        q0[k] = p.Q0[mi];
-        if constexpr (RB == 1) {
-            if (!p.cat_shared) {
+        const long i = base + k * kRateThreads + threadIdx.x;
+        ok[k] = i < nmodes;
+        mi[k] = ok[k] ? i : base;
+        const long v = (long)((unsigned)mi[k] / M);
+        vj[k] = (int)(v - v0);
+        q0[k] = p.Q0[mi[k]];
+        if (!p.cat_shared) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * nmodes + mi];
-            } else if (!cat_lds) {
-                const long v = (long)((unsigned)mi / M);
+            for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * nmodes + mi[k]];
+        } else if (!cat_lds) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * ph.M + (mi - v * ph.M)];
-            }
+            for (int c = 0; c < C; ++c) cat[k][c] = p.cat[c * ph.M + (mi[k] - v * ph.M)];
         }
     }
-    {
-        const int j = threadIdx.x;
-        if (cat_lds && j < ph.M * C) s_cat[j / ph.M][j % ph.M] = p.cat[j];   // [C][M] order
+    // per-voxel inputs, one voxel per lane
+    const int j = threadIdx.x;
+    const long v = v0 + (j < nv ? j : 0);
+    double gp = p.soft[v], gt = p.soft[g.N + v], tl = p.last_t[v], sv[C], pv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) sv[c] = p.sig[c * g.N + v];
+    if (ph.directional) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) pv[c] = p.prev_dir[c * g.N + v];
     }
+    double cv = 0.0;
+    if (cat_lds && j < ph.M * C) cv = p.cat[j];   // [C][M] order
+    if (j < nv) {
+        const double g_base = voxel_gbase(ph, time, gp, gt, tl);
+        s_gb[j] = g_base;
+        s_eg[j] = ph.directional ? 0.0 : exp(-g_base);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            s_sig[c][j] = sv[c];
+            if (ph.directional) s_pd[c][j] = pv[c];
+        }
+    }
+    if (cat_lds && j < ph.M * C) s_cat[j / ph.M][j % ph.M] = cv;
     __syncthreads();
 
-    // per rate block, once the lane's modes of it are done: wave sums, then
-    // (below) the wave totals in wave order
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double lsum = 0.0;
     int lun = 0, linc = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (base + k * kRateThreads + (long)threadIdx.x < nmodes) {
-            const long mi = mode_of(k);
-            const int jj = (int)((long)((unsigned)mi / M) - v0);
-            const int m = (int)(mi - (v0 + jj) * (long)ph.M);
-            double cr[C];
-            if (!cat_lds) {
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    if constexpr (RB == 1) cr[c] = cat[k][c];
-                    else cr[c] = p.cat_shared ? p.cat[c * ph.M + m] : p.cat[c * nmodes + mi];
-                }
-            }
-            const double q = cat_lds
-                ? mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c * nvmax + jj]; },
-                                     [&](int c) { return s_cat[c][m]; }, [&](int c) { return s_pd[c * nvmax + jj]; })
-                : mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c * nvmax + jj]; },
-                                     [&](int c) { return cr[c]; }, [&](int c) { return s_pd[c * nvmax + jj]; });
-            p.Q[mi] = q;
-            bool incl;
-            const double rt = mode_rate(ph, q, incl);
-            if (incl) { lsum += rt; linc++; }
-            if (q < ph.thr) lun++;
-        }
-        if (k % KB == KB - 1) {
-            const double t = wsum(lsum);
-            const int u = wsum(lun), n = wsum(linc);
-            if (lane == 0) { sd[k / KB][wid] = t; su[k / KB][wid] = u; si[k / KB][wid] = n; }
-            lsum = 0.0;
-            lun = 0;
-            linc = 0;
-        }
+        if (!ok[k]) continue;
+        const int jj = vj[k];
+        const int m = (int)(mi[k] - (v0 + jj) * (long)ph.M);
+        const double q = cat_lds
+            ? mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c][jj]; },
+                                 [&](int c) { return s_cat[c][m]; }, [&](int c) { return s_pd[c][jj]; })
+            : mode_barrier<D, C>(ph, s_gb[jj], s_eg[jj], q0[k], [&](int c) { return s_sig[c][jj]; },
+                                 [&](int c) { return cat[k][c]; }, [&](int c) { return s_pd[c][jj]; });
+        p.Q[mi[k]] = q;
+        bool incl;
+        const double r = mode_rate(ph, q, incl);
+        if (incl) { lsum += r; linc++; }
+        if (q < ph.thr) lun++;
     }
-    __syncthreads();
-    if ((int)threadIdx.x < RB) {
-        const int r = threadIdx.x;
-        const long b = (long)blockIdx.x * RB + r;
-        if (b * kRateBlock < nmodes) {
-            double t = 0.0;
-            int tu = 0, ti = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) { t += sd[r][w]; tu += su[r][w]; ti += si[r][w]; }
-            p.bsum[b] = t;
-            p.bunst[b] = tu;
-            p.binc[b] = ti;
-        }
-    }
+    rate_block_totals(p, lsum, lun, linc);
 }
 
 // Inclusive wave scan (lane order) of a double and two ints.
@@ -1189,42 +1135,20 @@ __global__ __launch_bounds__(1024) void k_event_chain(const KmcParams p_arg, con
 // ---------------------------------------------------------------------------
 int kmc_nblocks(long nmodes) { return (int)((nmodes + kRateBlock - 1) / kRateBlock); }
 
-// Rate blocks per workgroup of the staged rate pass: 4 while the launch keeps
-// at least 512 workgroups (two per CU) and the staged voxels fit 48 KB of
-// LDS, else 1.  nvmax: the staged voxels' bound, RB kRateBlock / M + 2.
-static int rate_blocks_per_group(const KmcLaunch& l, int nb, int C, int& nvmax) {
-    const int nv4 = 4 * kRateBlock / l.M + 2;
-    if ((long)l.B * ((nb + 3) / 4) >= 512 && (size_t)nv4 * (2 * C + 2) * sizeof(double) <= 48 * 1024) {
-        nvmax = nv4;
-        return 4;
-    }
-    nvmax = kRateBlock / l.M + 2;
-    return 1;
-}
-
-template <int D, int C, bool BATCH>
-static void enqueue_rates_staged(const KmcLaunch& l, int nb, double time, const double* times, hipStream_t s) {
-    int nvmax = 0;
-    const int rb = rate_blocks_per_group(l, nb, C, nvmax);
-    const size_t lds = (size_t)nvmax * (2 * C + 2) * sizeof(double);
-    if (rb == 4)
-        hipLaunchKernelGGL((k_rates_staged<D, C, BATCH, 4>), dim3((nb + 3) / 4, l.B), dim3(kRateThreads), lds, s, l.h,
-                           l.P, time, times, nvmax);
-    else
-        hipLaunchKernelGGL((k_rates_staged<D, C, BATCH, 1>), dim3(nb, l.B), dim3(kRateThreads), lds, s, l.h, l.P, time,
-                           times, nvmax);
-}
-
 template <bool BATCH> static void enqueue_rates_t(const KmcLaunch& l, double time, const double* times, hipStream_t s,
                                                   bool with_scan) {
     const int nb = kmc_nblocks(l.g.N * l.M);
     const bool staged = l.M >= kStageMinM;
     if (l.g.dims == 2) {
-        if (staged) enqueue_rates_staged<2, 3, BATCH>(l, nb, time, times, s);
-        else hipLaunchKernelGGL((k_rates<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        if (staged)
+            hipLaunchKernelGGL((k_rates_staged<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        else
+            hipLaunchKernelGGL((k_rates<2, 3, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
     } else {
-        if (staged) enqueue_rates_staged<3, 6, BATCH>(l, nb, time, times, s);
-        else hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        if (staged)
+            hipLaunchKernelGGL((k_rates_staged<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
+        else
+            hipLaunchKernelGGL((k_rates<3, 6, BATCH>), dim3(nb, l.B), dim3(kRateThreads), 0, s, l.h, l.P, time, times);
     }
     if (with_scan) hipLaunchKernelGGL(k_rates_scan<BATCH>, dim3(1, l.B), dim3(1024), 0, s, l.h, l.P, nb);
 }

@@ -81,8 +81,8 @@ def test_rates_and_selection_fed_oracle_state(dims, directional):
 
 @pytest.mark.parametrize("catalog,directional", [(abi.CATALOG_FIXED, 0), (abi.CATALOG_RANDOM, 1)])
 def test_rates_and_selection_fed_oracle_state_64cube(catalog, directional):
-    # 3.1 M modes: the rate pass runs four 1024-mode rate blocks per
-    # workgroup (shared catalog staged in LDS, or per-voxel catalog rows)
+    # 3.1 M modes in 3072 rate blocks (shared catalog staged in LDS, or
+    # per-voxel catalog rows)
     c = _cfg(dims=3, nx=64, ny=64, nz=64, M=12, softening_directional=directional, instability_kmc=1,
              catalog_mode=catalog)
     N = c.nx * c.ny * c.nz
